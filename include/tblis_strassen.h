@@ -1,0 +1,201 @@
+/*
+ * tblis_strassen.h -- C-ABI of the MI355X (gfx950) Strassen tensor contraction library.
+ *
+ * Drop-in surface for the Strassen tensor-contraction path of flame/tblis-strassen:
+ *   - the tblis_tensor struct, byte-identical to src/util/basic_types.h:341-419
+ *     (len_type = stride_type = ptrdiff_t, label_type = char, the configure defaults of
+ *     configure.ac:77-103);
+ *   - stra_tblis_tensor_mult        replaces src/stra_iface/3t/stra_mult.h:17-20       (1-level Strassen)
+ *   - stra_tblis_tensor_mult_2level replaces src/stra_iface/3t/stra_mult_2level.h:17-20 (2-level Strassen)
+ *   - tblis_tensor_mult             replaces src/iface/3t/mult.h:17-20                 (classical, L=0)
+ * The three are declared extern "C" inside namespace tblis, as in the reference, so their unmangled
+ * names are the ABI.  Semantics: C = alpha_A*alpha_B * A.B + alpha_C * C, with alpha in tblis_tensor.scalar;
+ * on return C->scalar = 1 and C->conj = 0 (src/stra_iface/3t/stra_mult.cxx:251-252).
+ *
+ * Tensor data may live in device memory (HBM, the fast path) or in host memory (staged through HBM).
+ * The impl (ABC / AB / Naive) of the drop-in symbols is the process-global tblis::internal::impl
+ * (src/internal/3t/mult.cxx:26); C callers use mistra_set_impl().
+ *
+ * Everything prefixed mistra_ is an extension without a reference counterpart: explicit level, impl,
+ * precision and HIP stream, error codes instead of void, batching, autotuning and adjoints.
+ */
+#ifndef TBLIS_STRASSEN_AMD_H
+#define TBLIS_STRASSEN_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+namespace tblis {
+extern "C" {
+#endif
+
+typedef ptrdiff_t len_type;     /* src/util/basic_types.h:61, configure default */
+typedef ptrdiff_t stride_type;  /* src/util/basic_types.h:62, configure default */
+typedef char label_type;        /* src/util/basic_types.h:63, configure default */
+
+/* src/util/basic_types.h:52-58 */
+typedef enum
+{
+    TYPE_SINGLE   = 0,
+    TYPE_FLOAT    = TYPE_SINGLE,
+    TYPE_DOUBLE   = 1,
+    TYPE_SCOMPLEX = 2,
+    TYPE_DCOMPLEX = 3
+} type_t;
+
+/* Opaque in this library: a thread communicator (tci_comm) and a CPU kernel configuration in the
+ * reference.  Both are accepted for ABI compatibility and ignored: the GPU grid replaces the thread
+ * team and the kernel table replaces the CPU configuration. */
+typedef struct tci_comm tblis_comm;
+typedef struct tblis_config_s tblis_config;
+
+/* src/util/basic_types.h:341-419 (same member order, alignment and size: 56 bytes on LP64) */
+typedef struct tblis_tensor
+{
+    type_t type;
+    int conj;
+    char scalar[16] __attribute__((__aligned__(8)));
+    void* data;
+    unsigned ndim;
+    len_type* len;
+    stride_type* stride;
+} tblis_tensor;
+
+/* src/util/thread.h:8 -- a one-thread communicator; here only a distinguishable non-null pointer. */
+extern const tblis_comm* const tblis_single;
+
+/* src/stra_iface/3t/stra_mult.h:17-20: one-level Strassen tensor contraction. */
+void stra_tblis_tensor_mult(const tblis_comm* comm, const tblis_config* cfg,
+                            const tblis_tensor* A, const label_type* idx_A,
+                            const tblis_tensor* B, const label_type* idx_B,
+                                  tblis_tensor* C, const label_type* idx_C);
+
+/* src/stra_iface/3t/stra_mult_2level.h:17-20: two-level Strassen tensor contraction. */
+void stra_tblis_tensor_mult_2level(const tblis_comm* comm, const tblis_config* cfg,
+                                   const tblis_tensor* A, const label_type* idx_A,
+                                   const tblis_tensor* B, const label_type* idx_B,
+                                         tblis_tensor* C, const label_type* idx_C);
+
+/* src/iface/3t/mult.h:17-20: classical (L=0) tensor contraction. */
+void tblis_tensor_mult(const tblis_comm* comm, const tblis_config* cfg,
+                       const tblis_tensor* A, const label_type* idx_A,
+                       const tblis_tensor* B, const label_type* idx_B,
+                             tblis_tensor* C, const label_type* idx_C);
+
+/* ------------------------------------------------------------------------------------------------
+ * Extensions (no reference counterpart).
+ * ------------------------------------------------------------------------------------------------ */
+
+/* Implementation selector; numeric values follow tblis::internal::impl_t (src/internal/3t/mult.hpp:14). */
+enum
+{
+    MISTRA_IMPL_ABC   = 0,  /* BLIS_BASED: sums fused into the LDS gather, multi-C epilogue   */
+    MISTRA_IMPL_BLAS  = 1,  /* BLAS_BASED: treated as ABC on the GPU                         */
+    MISTRA_IMPL_REF   = 2,  /* REFERENCE: brute-force classical contraction kernel           */
+    MISTRA_IMPL_NAIVE = 3,  /* STRA_NAIVE: explicit S=sum(A), T=sum(B), plain GEMM, combine   */
+    MISTRA_IMPL_AB    = 4,  /* STRA_AB: fused sums, products staged in HBM, one combine pass  */
+    MISTRA_IMPL_AUTO  = -1  /* let the autotuner choose level, impl and tile shape            */
+};
+
+/* Arithmetic of the leaf products. */
+enum
+{
+    MISTRA_PREC_NATIVE     = 0, /* FP64 MFMA for double, FP32 MFMA for float                          */
+    MISTRA_PREC_BF16X3     = 1, /* double only: hi/lo BF16 split, 3 products, FP32 accumulate          */
+    MISTRA_PREC_OZAKI_BF16 = 2, /* double only: Ozaki error-free BF16 slicing, exact FP32 chunks, FP64 sum */
+    MISTRA_PREC_FP8_BLOCK  = 3  /* double/float: FP8 E4M3 with one scale per 128-long K block          */
+};
+
+/* Flags for mistra_tensor_mult. */
+enum
+{
+    MISTRA_FLAG_REF_FRINGE = 1u << 0, /* drop the rows/cols/k that the 2^L split truncates, exactly as
+                                         src/matrix/stra_tensor_view.hpp:246-247 does (default: computed) */
+    MISTRA_FLAG_SYNC       = 1u << 1  /* synchronize the stream before returning                          */
+};
+
+/* Error codes (0 = success). */
+enum
+{
+    MISTRA_OK              = 0,
+    MISTRA_ERR_ARG         = -1, /* invalid argument (types, shapes, labels)           */
+    MISTRA_ERR_UNSUPPORTED = -2, /* valid but unsupported combination (e.g. complex)   */
+    MISTRA_ERR_HIP         = -3, /* a HIP runtime call failed                          */
+    MISTRA_ERR_NOMEM       = -4  /* device workspace allocation failed                 */
+};
+
+/* Explicit-everything contraction: level L in {0,1,2}, impl as above, precision as above,
+ * stream is a hipStream_t (NULL = the per-thread default stream). Returns 0 or a negative code. */
+int mistra_tensor_mult(const tblis_tensor* A, const label_type* idx_A,
+                       const tblis_tensor* B, const label_type* idx_B,
+                             tblis_tensor* C, const label_type* idx_C,
+                       int level, int impl, int precision, unsigned flags, void* stream);
+
+/* Thread-local description of the last error of this thread ("" if none). */
+const char* mistra_last_error(void);
+
+/* Process-global impl used by the drop-in symbols (same variable as tblis::internal::impl). */
+int mistra_set_impl(int impl);
+int mistra_get_impl(void);
+
+/* Release the per-device workspace arena and the plan cache. */
+int mistra_release_workspace(void);
+
+/* Kernel timing hook: when enabled, the library records HIP events around the launches of the
+ * dominant (leaf-product) kernel on the stream it launches on; read back the summed time in ms,
+ * the number of launches and the algorithmic flops of those launches, then reset. */
+int mistra_kernel_timing(int enable);
+int mistra_kernel_timing_read(double* total_ms, long long* launches, double* flops, int reset);
+
+/* Synthetic data: fill n elements (type TYPE_FLOAT or TYPE_DOUBLE) at device pointer ptr with
+ * uniform [-1,1) values from splitmix64(seed*2^32 + i); if scale_exp_max > 0, multiply each by
+ * 2^u with u uniform-integer in [-scale_exp_max, scale_exp_max] (a second splitmix64 stream). */
+int mistra_fill_uniform(int type, void* ptr, long long n, unsigned long long seed,
+                        int scale_exp_max, void* stream);
+
+/* Plain device copy of nbytes (a calibration kernel of known traffic for counter-based profiling). */
+int mistra_copy_bytes(void* dst, const void* src, long long nbytes, void* stream);
+
+/* Relative Frobenius distance ||X - Y||_F / ||Y||_F between two device tensors of the same
+ * dense element count n (type TYPE_FLOAT or TYPE_DOUBLE; reduction in FP64 on the device). */
+int mistra_rel_error(int type, const void* X, const void* Y, long long n, double* out, void* stream);
+
+/* ---- Batching: many independent contractions in one persistent launch ------------------------ */
+typedef struct mistra_contraction
+{
+    const tblis_tensor* A; const label_type* idx_A;
+    const tblis_tensor* B; const label_type* idx_B;
+          tblis_tensor* C; const label_type* idx_C;
+    int level;   /* -1 = autotuner's choice */
+    int impl;    /* -1 = autotuner's choice (only ABC is available inside a batch) */
+} mistra_contraction;
+
+/* Run count contractions (same element type; device data only) as one persistent launch.
+ * Entries must not write overlapping C memory. */
+int mistra_batch_mult(int count, const mistra_contraction* entries, unsigned flags, void* stream);
+
+/* ---- Adjoints of C = alpha*A.B (+beta*C) ------------------------------------------------------ */
+/* dA[idx_A] = alpha * sum dC[idx_C] . B[idx_B]  and  dB[idx_B] = alpha * sum A[idx_A] . dC[idx_C],
+ * each computed as a Strassen contraction of the given level/impl/precision (dA, dB overwritten
+ * when beta_grad == 0, accumulated into otherwise). Either gradient pointer may be NULL. */
+int mistra_tensor_mult_backward(const tblis_tensor* A, const label_type* idx_A,
+                                const tblis_tensor* B, const label_type* idx_B,
+                                const tblis_tensor* dC, const label_type* idx_C,
+                                tblis_tensor* dA, tblis_tensor* dB,
+                                int level, int impl, int precision, void* stream);
+
+/* ---- Autotuner --------------------------------------------------------------------------------- */
+/* Choose level, impl and tile shape for a contraction; if measure != 0 time the candidates on the
+ * device (on scratch copies of C), else use the analytic model. Results are cached per shape. */
+int mistra_autotune(const tblis_tensor* A, const label_type* idx_A,
+                    const tblis_tensor* B, const label_type* idx_B,
+                    const tblis_tensor* C, const label_type* idx_C,
+                    int measure, int* level_out, int* impl_out, int* tile_out, void* stream);
+
+#ifdef __cplusplus
+} /* extern "C" */
+} /* namespace tblis */
+#endif
+
+#endif /* TBLIS_STRASSEN_AMD_H */

@@ -1,0 +1,35 @@
+/*
+ * tblis_strassen_debug.h -- introspection of the host plan, for tests.  No reference counterpart:
+ * it exposes the index analysis that src/stra_iface/3t/stra_mult.cxx:35-180 and
+ * src/stra_internal/3t/stra_mult.cxx:336-407 perform internally, without touching the device.
+ */
+#ifndef TBLIS_STRASSEN_AMD_DEBUG_H
+#define TBLIS_STRASSEN_AMD_DEBUG_H
+
+#include "tblis_strassen.h"
+
+#ifdef __cplusplus
+namespace tblis {
+extern "C" {
+#endif
+
+/* dims[0..12] = m, n, k, nbatch, pure, transposed, ms, ns, ks, nprod (core products), nfringe,
+ *               a_mfast, b_nfast.
+ * scat[6*cap]  = a_m, a_k, b_k, b_n, c_m, c_n (in the plan's own orientation), -1 padded.
+ * prods[cap_prods*40] per product (core then fringe): nA, nB, nC, first_mask, then 4 x (roff, coff,
+ *               coeff, base) for A, B and C (-1 padded).  Returns the number of products or < 0. */
+int mistra_debug_plan(const tblis_tensor* A, const label_type* idx_A,
+                      const tblis_tensor* B, const label_type* idx_B,
+                      const tblis_tensor* C, const label_type* idx_C,
+                      int level, unsigned flags, long long* dims, long long* scat, long long cap,
+                      double* prods, int cap_prods);
+
+/* Run strides of a scatter vector (see run_strides in the plan code); out has n entries. */
+int mistra_debug_run_strides(const long long* scat, long long n, int run, long long* out);
+
+#ifdef __cplusplus
+} /* extern "C" */
+} /* namespace tblis */
+#endif
+
+#endif

@@ -1,0 +1,295 @@
+// HBM-bound helper kernels of the Strassen tensor contraction (gfx950):
+//   * operand sums S_p = sum_i c_i A_i for the Naive impl (the gather-sum `add` of
+//     src/stra_internal/3t/tensor2matrix.hpp:821-825 used by straprim_naive2,
+//     src/stra_internal/3t/straprim_common.hpp:70-157);
+//   * the combine C_i = beta*C_i + alpha*sum_p coeff_{p,i} M_p for the AB and Naive impls (the
+//     scatter-add of src/stra_internal/3t/tensor2matrix.hpp:827-830, here done once for all products);
+//   * synthetic data, a calibration copy, a relative-error reduction and C = beta*C.
+#include <hip/hip_runtime.h>
+
+#include "launch.hpp"
+
+namespace mistra {
+
+// ---- operand sums ---------------------------------------------------------------------------------
+
+template <typename T, bool ROWS_FAST>
+__global__ __launch_bounds__(256) void operand_sums_kernel(SumArgs a)
+{
+    constexpr int TR = 64, TC = 64;
+    __shared__ T tile[TC][TR + 1];
+    const int tiles_r = int((a.rows + TR - 1) / TR);
+    const int tr = blockIdx.x % tiles_r, tc = blockIdx.x / tiles_r;
+    const int p = blockIdx.y;
+    const ProdDesc& P = a.prods[p];
+    const int nt = a.which == 0 ? P.nA : P.nB;
+    const SubRef* terms = a.which == 0 ? P.a : P.b;
+    const T* X = static_cast<const T*>(a.X) + a.zoff;
+    const int tid = threadIdx.x;
+    const int64_t r0 = int64_t(tr) * TR, c0 = int64_t(tc) * TC;
+
+    // read phase: lanes along the unit-stride dimension of the source
+#pragma unroll
+    for (int e = 0; e < (TR * TC) / 256; e++)
+    {
+        const int lin = tid + e * 256;
+        const int rr = ROWS_FAST ? lin % TR : lin / TC;
+        const int cc = ROWS_FAST ? lin / TR : lin % TC;
+        const int64_t r = r0 + rr, c = c0 + cc;
+        T v = T(0);
+        if (r < a.rows && c < a.cols)
+        {
+#pragma unroll
+            for (int t = 0; t < kMaxTerms; t++)
+                if (t < nt)
+                    v += T(terms[t].coeff) * X[terms[t].base + a.rs[terms[t].roff + r] + a.cs[terms[t].coff + c]];
+        }
+        tile[cc][rr] = v;
+    }
+    __syncthreads();
+    // write phase: dense column-major, rows fastest
+    T* out = static_cast<T*>(a.out) + int64_t(p) * a.rows * a.cols;
+#pragma unroll
+    for (int e = 0; e < (TR * TC) / 256; e++)
+    {
+        const int lin = tid + e * 256;
+        const int rr = lin % TR, cc = lin / TR;
+        const int64_t r = r0 + rr, c = c0 + cc;
+        if (r < a.rows && c < a.cols) out[c * a.rows + r] = tile[cc][rr];
+    }
+}
+
+hipError_t launch_operand_sums(int elem_size, const SumArgs& a, hipStream_t s)
+{
+    const int64_t tiles = ((a.rows + 63) / 64) * ((a.cols + 63) / 64);
+    if (tiles == 0 || a.nprod == 0) return hipSuccess;
+    dim3 grid(unsigned(tiles), unsigned(a.nprod));
+    if (elem_size == 8)
+    {
+        if (a.rows_fast) hipLaunchKernelGGL((operand_sums_kernel<double, true>), grid, dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((operand_sums_kernel<double, false>), grid, dim3(256), 0, s, a);
+    }
+    else
+    {
+        if (a.rows_fast) hipLaunchKernelGGL((operand_sums_kernel<float, true>), grid, dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((operand_sums_kernel<float, false>), grid, dim3(256), 0, s, a);
+    }
+    return hipGetLastError();
+}
+
+// ---- combine ----------------------------------------------------------------------------------------
+
+template <typename T>
+__global__ __launch_bounds__(256) void combine_kernel(CombineArgs a)
+{
+    const int s = blockIdx.y;
+    const int64_t z = blockIdx.z;
+    const int64_t nel = a.ms * a.ns;
+    const int tb = a.term_begin[s], te = a.term_begin[s + 1];
+    T* C = static_cast<T*>(a.C) + (a.batch_c ? a.batch_c[z] : 0);
+    const T* M = static_cast<const T*>(a.M) + z * int64_t(a.nprod) * nel;
+    const int64_t roff = a.sub_roff[s], coff = a.sub_coff[s];
+    for (int64_t e = int64_t(blockIdx.x) * 256 + threadIdx.x; e < nel; e += int64_t(gridDim.x) * 256)
+    {
+        const int64_t r = e % a.ms, c = e / a.ms;
+        double acc = 0.0;
+        for (int t = tb; t < te; t++) acc += a.terms[t].coeff * double(M[int64_t(a.terms[t].prod) * nel + e]);
+        T* ptr = C + a.c_m[roff + r] + a.c_n[coff + c];
+        const double v = a.alpha * acc;
+        *ptr = a.beta == 0.0 ? T(v) : T(a.beta * double(*ptr) + v);
+    }
+}
+
+hipError_t launch_combine(int elem_size, const CombineArgs& a, hipStream_t s)
+{
+    const int64_t nel = a.ms * a.ns;
+    if (nel == 0 || a.nsub == 0 || a.nbatch == 0) return hipSuccess;
+    int64_t blocks = (nel + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    dim3 grid(unsigned(blocks), unsigned(a.nsub), unsigned(a.nbatch));
+    if (elem_size == 8) hipLaunchKernelGGL((combine_kernel<double>), grid, dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((combine_kernel<float>), grid, dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+// ---- synthetic data ----------------------------------------------------------------------------------
+
+__device__ __forceinline__ uint64_t splitmix64(uint64_t x)
+{
+    uint64_t z = x + 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void fill_uniform_kernel(T* p, int64_t n, uint64_t seed, int emax)
+{
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256)
+    {
+        const uint64_t key = (seed << 32) + uint64_t(i);
+        double v = double(splitmix64(key) >> 11) * 0x1.0p-53 * 2.0 - 1.0;
+        if (emax > 0)
+        {
+            const uint64_t u = splitmix64(key ^ 0xA5A5A5A5A5A5A5A5ull);
+            const int e = int(u % uint64_t(2 * emax + 1)) - emax;
+            v = ldexp(v, e);
+        }
+        p[i] = T(v);
+    }
+}
+
+hipError_t launch_fill_uniform(int elem_size, void* ptr, int64_t n, uint64_t seed, int emax, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    if (elem_size == 8)
+        hipLaunchKernelGGL(fill_uniform_kernel<double>, dim3(unsigned(blocks)), dim3(256), 0, s,
+                           static_cast<double*>(ptr), n, seed, emax);
+    else
+        hipLaunchKernelGGL(fill_uniform_kernel<float>, dim3(unsigned(blocks)), dim3(256), 0, s,
+                           static_cast<float*>(ptr), n, seed, emax);
+    return hipGetLastError();
+}
+
+// ---- copy ---------------------------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void copy16_kernel(uint4* dst, const uint4* src, int64_t n16)
+{
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n16; i += int64_t(gridDim.x) * 256)
+        dst[i] = src[i];
+}
+
+__global__ void copy1_kernel(unsigned char* dst, const unsigned char* src, int64_t n)
+{
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
+}
+
+hipError_t launch_copy_bytes(void* dst, const void* src, int64_t nbytes, hipStream_t s)
+{
+    const int64_t n16 = nbytes / 16;
+    if (n16 > 0)
+    {
+        int64_t blocks = (n16 + 255) / 256;
+        if (blocks > 8192) blocks = 8192;
+        hipLaunchKernelGGL(copy16_kernel, dim3(unsigned(blocks)), dim3(256), 0, s, static_cast<uint4*>(dst),
+                           static_cast<const uint4*>(src), n16);
+    }
+    const int64_t tail = nbytes - n16 * 16;
+    if (tail > 0)
+        hipLaunchKernelGGL(copy1_kernel, dim3(1), dim3(64), 0, s, static_cast<unsigned char*>(dst) + n16 * 16,
+                           static_cast<const unsigned char*>(src) + n16 * 16, tail);
+    return hipGetLastError();
+}
+
+// ---- relative error -------------------------------------------------------------------------------------
+
+constexpr int kErrBlocks = 1024;
+
+template <typename T>
+__global__ __launch_bounds__(256) void sq_partial_kernel(const T* X, const T* Y, int64_t n, double* part)
+{
+    __shared__ double s0[256], s1[256];
+    double d2 = 0.0, y2 = 0.0;
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256)
+    {
+        const double x = double(X[i]), y = double(Y[i]);
+        d2 += (x - y) * (x - y);
+        y2 += y * y;
+    }
+    s0[threadIdx.x] = d2;
+    s1[threadIdx.x] = y2;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1)
+    {
+        if (int(threadIdx.x) < w)
+        {
+            s0[threadIdx.x] += s0[threadIdx.x + w];
+            s1[threadIdx.x] += s1[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+    {
+        part[2 * blockIdx.x] = s0[0];
+        part[2 * blockIdx.x + 1] = s1[0];
+    }
+}
+
+__global__ __launch_bounds__(256) void sq_final_kernel(const double* part, double* out)
+{
+    __shared__ double s0[256], s1[256];
+    double a = 0.0, b = 0.0;
+    for (int i = threadIdx.x; i < kErrBlocks; i += 256)
+    {
+        a += part[2 * i];
+        b += part[2 * i + 1];
+    }
+    s0[threadIdx.x] = a;
+    s1[threadIdx.x] = b;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1)
+    {
+        if (int(threadIdx.x) < w)
+        {
+            s0[threadIdx.x] += s0[threadIdx.x + w];
+            s1[threadIdx.x] += s1[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+    {
+        out[0] = s0[0];
+        out[1] = s1[0];
+    }
+}
+
+// `out` must hold 2 + 2*kErrBlocks doubles: the result pair followed by the per-block partials.
+hipError_t launch_rel_error(int elem_size, const void* X, const void* Y, int64_t n, double* out, hipStream_t s)
+{
+    double* part = out + 2;
+    if (elem_size == 8)
+        hipLaunchKernelGGL(sq_partial_kernel<double>, dim3(kErrBlocks), dim3(256), 0, s,
+                           static_cast<const double*>(X), static_cast<const double*>(Y), n, part);
+    else
+        hipLaunchKernelGGL(sq_partial_kernel<float>, dim3(kErrBlocks), dim3(256), 0, s,
+                           static_cast<const float*>(X), static_cast<const float*>(Y), n, part);
+    hipLaunchKernelGGL(sq_final_kernel, dim3(1), dim3(256), 0, s, part, out);
+    return hipGetLastError();
+}
+
+// ---- C = beta*C over a strided tensor ---------------------------------------------------------------------
+
+template <typename T>
+__global__ __launch_bounds__(256) void scale_kernel(T* C, StridedDims d, int64_t n, double beta)
+{
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256)
+    {
+        int64_t rem = i, off = 0;
+        for (int k = 0; k < d.ndim; k++)
+        {
+            off += (rem % d.len[k]) * d.stride[k];
+            rem /= d.len[k];
+        }
+        C[off] = beta == 0.0 ? T(0) : T(beta * double(C[off]));
+    }
+}
+
+hipError_t launch_scale(int elem_size, void* C, const StridedDims& d, double beta, hipStream_t s)
+{
+    int64_t n = 1;
+    for (int k = 0; k < d.ndim; k++) n *= d.len[k];
+    if (n <= 0) return hipSuccess;
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    if (elem_size == 8)
+        hipLaunchKernelGGL(scale_kernel<double>, dim3(unsigned(blocks)), dim3(256), 0, s, static_cast<double*>(C),
+                           d, n, beta);
+    else
+        hipLaunchKernelGGL(scale_kernel<float>, dim3(unsigned(blocks)), dim3(256), 0, s, static_cast<float*>(C), d,
+                           n, beta);
+    return hipGetLastError();
+}
+
+}  // namespace mistra

@@ -1,0 +1,62 @@
+// Host-plan introspection for tests (include/tblis_strassen_debug.h); never touches the device.
+#include <cstring>
+
+#include "driver.hpp"
+#include "tblis_strassen_debug.h"
+
+namespace tblis {
+extern "C" {
+
+int mistra_debug_plan(const tblis_tensor* A, const label_type* idx_A, const tblis_tensor* B, const label_type* idx_B,
+                      const tblis_tensor* C, const label_type* idx_C, int level, unsigned flags, long long* dims,
+                      long long* scat, long long cap, double* prods, int cap_prods)
+{
+    mistra::Plan p;
+    mistra::Status st = mistra::build_plan_host(A, idx_A, B, idx_B, C, idx_C, level, mistra::IMPL_ABC,
+                                                mistra::TILE_128, flags, p);
+    if (!st.ok()) return st.code;
+    const long long d[13] = {p.m, p.n, p.k, p.nbatch, p.general ? 0 : 1, p.transposed, p.ms, p.ns, p.ks,
+                             p.nprod, (long long)p.fringe.size(), p.a_mfast, p.b_nfast};
+    std::memcpy(dims, d, sizeof(d));
+    const std::vector<int64_t>* v[6] = {&p.h_a_m, &p.h_a_k, &p.h_b_k, &p.h_b_n, &p.h_c_m, &p.h_c_n};
+    for (int i = 0; i < 6; i++)
+        for (long long j = 0; j < cap; j++)
+            scat[i * cap + j] = j < (long long)v[i]->size() ? (*v[i])[size_t(j)] : -1;
+    const int np = int(p.h_prods.size());
+    for (int q = 0; q < np && q < cap_prods; q++)
+    {
+        const mistra::ProdDesc& pd = p.h_prods[size_t(q)];
+        double* o = prods + q * 40;
+        o[0] = pd.nA;
+        o[1] = pd.nB;
+        o[2] = pd.nC;
+        o[3] = pd.first_mask;
+        const mistra::SubRef* refs[3] = {pd.a, pd.b, pd.c};
+        const int cnt[3] = {pd.nA, pd.nB, pd.nC};
+        for (int s = 0; s < 3; s++)
+            for (int t = 0; t < 4; t++)
+            {
+                double* r = o + 4 + s * 12 + t * 3;
+                if (t < cnt[s])
+                {
+                    r[0] = double(refs[s][t].roff);
+                    r[1] = double(refs[s][t].coff);
+                    r[2] = refs[s][t].coeff;
+                }
+                else
+                    r[0] = r[1] = -1, r[2] = 0;
+            }
+    }
+    return np;
+}
+
+int mistra_debug_run_strides(const long long* scat, long long n, int run, long long* out)
+{
+    std::vector<int64_t> v(scat, scat + n);
+    std::vector<int64_t> r = mistra::run_strides(v, run);
+    for (long long i = 0; i < n; i++) out[i] = r[size_t(i)];
+    return 0;
+}
+
+}  // extern "C"
+}  // namespace tblis

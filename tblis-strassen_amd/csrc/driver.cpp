@@ -1,0 +1,575 @@
+// Host driver: plan construction and caching, launch sequencing per level and impl.
+//
+// Reference call structure being replaced (src/stra_internal/3t/):
+//   stra_mult dispatcher          stra_mult.cxx:1701-1846, stra_mult_2level.cxx:4434-4578
+//   ABC, 7 products serialized    stra_mult.cxx:308-605 (L=1), stra_mult_2level.cxx:241-1486 (L=2)
+//   Naive (explicit S, T, M)      stra_mult.cxx:630-951, straprim_common.hpp:70-157
+//   AB (staged M, scatter-add)    stra_mult.cxx:976-1285, straprim_common.hpp:160-204
+// Here a level/impl becomes: one ABC launch that runs every product per tile; or, for AB, one launch
+// computing all products into HBM staging plus one combine pass; or, for Naive, two operand-sum
+// passes, one dense GEMM launch and one combine pass.  The rows/cols/k that the 2^L split leaves over
+// (dropped by the reference, src/matrix/stra_tensor_view.hpp:246-247) are computed by classical
+// passes unless MISTRA_FLAG_REF_FRINGE asks for the reference's truncation.
+#include "driver.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <list>
+#include <map>
+#include <mutex>
+#include <sstream>
+
+namespace mistra {
+
+namespace {
+
+std::mutex g_plan_mutex;
+std::list<std::pair<std::string, std::shared_ptr<Plan>>> g_plans;  // LRU, most recent first
+constexpr size_t kMaxPlans = 64;
+
+#define HIP_TRY(expr)                                                                             \
+    do                                                                                            \
+    {                                                                                             \
+        hipError_t e_ = (expr);                                                                   \
+        if (e_ != hipSuccess)                                                                     \
+            return Status::error(MISTRA_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+void append_key(std::string& key, const tblis::tblis_tensor* t, const label_type* idx)
+{
+    std::ostringstream os;
+    os << t->ndim << ':';
+    for (unsigned i = 0; i < t->ndim; i++) os << idx[i] << t->len[i] << ',' << t->stride[i] << ';';
+    key += os.str();
+}
+
+bool unit(const std::vector<int64_t>& v)
+{
+    return v.size() > 1 && (v[1] - v[0] == 1 || v[1] - v[0] == -1);
+}
+
+template <typename V>
+void permute_group(IndexGroup& g, const std::vector<unsigned>& order)
+{
+    IndexGroup out;
+    for (unsigned i : order)
+    {
+        out.idx.push_back(g.idx[i]);
+        out.len.push_back(g.len[i]);
+        if (!g.sA.empty()) out.sA.push_back(g.sA[i]);
+        if (!g.sB.empty()) out.sB.push_back(g.sB[i]);
+        if (!g.sC.empty()) out.sC.push_back(g.sC[i]);
+    }
+    g = std::move(out);
+}
+
+ProdDesc make_desc(const Product& pr, int64_t d, int64_t ms, int64_t ns, int64_t ks, unsigned& touched)
+{
+    ProdDesc pd;
+    std::memset(&pd, 0, sizeof(pd));
+    pd.nA = int(pr.A.size());
+    pd.nB = int(pr.B.size());
+    pd.nC = int(pr.C.size());
+    for (size_t i = 0; i < pr.A.size(); i++)
+        pd.a[i] = SubRef{0, (pr.A[i].subid / d) * ms, (pr.A[i].subid % d) * ks, pr.A[i].coeff};
+    for (size_t i = 0; i < pr.B.size(); i++)
+        pd.b[i] = SubRef{0, (pr.B[i].subid / d) * ks, (pr.B[i].subid % d) * ns, pr.B[i].coeff};
+    for (size_t i = 0; i < pr.C.size(); i++)
+    {
+        pd.c[i] = SubRef{0, (pr.C[i].subid / d) * ms, (pr.C[i].subid % d) * ns, pr.C[i].coeff};
+        if (!(touched & (1u << pr.C[i].subid)))
+        {
+            pd.first_mask |= 1 << int(i);
+            touched |= 1u << pr.C[i].subid;
+        }
+    }
+    return pd;
+}
+
+ProdDesc single_desc(int64_t a_r, int64_t a_c, int64_t b_r, int64_t b_c, int64_t c_r, int64_t c_c, bool first)
+{
+    ProdDesc pd;
+    std::memset(&pd, 0, sizeof(pd));
+    pd.nA = pd.nB = pd.nC = 1;
+    pd.a[0] = SubRef{0, a_r, a_c, 1.0};
+    pd.b[0] = SubRef{0, b_r, b_c, 1.0};
+    pd.c[0] = SubRef{0, c_r, c_c, 1.0};
+    pd.first_mask = first ? 1 : 0;
+    return pd;
+}
+
+size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+}  // namespace
+
+Plan::~Plan()
+{
+    if (dmem) (void)hipFree(dmem);
+}
+
+Status build_plan_host(const tblis::tblis_tensor* A, const label_type* idx_A, const tblis::tblis_tensor* B,
+                       const label_type* idx_B, const tblis::tblis_tensor* C, const label_type* idx_C,
+                       int level, int impl, int tile, unsigned flags, Plan& p)
+{
+    if (level < 0 || level > 2) return Status::error(MISTRA_ERR_ARG, "level must be 0, 1 or 2");
+    if (A->type != B->type || A->type != C->type)
+        return Status::error(MISTRA_ERR_ARG, "A, B and C must have the same type");
+    if (A->type != tblis::TYPE_DOUBLE && A->type != tblis::TYPE_FLOAT)
+        return Status::error(MISTRA_ERR_UNSUPPORTED, "only TYPE_FLOAT and TYPE_DOUBLE are supported");
+    if (A->conj || B->conj || C->conj)
+        return Status::error(MISTRA_ERR_UNSUPPORTED, "conjugation is not supported for real types");
+
+    p.elem = A->type == tblis::TYPE_DOUBLE ? 8 : 4;
+    p.level = level;
+    p.impl = impl;
+    p.tile = tile;
+    p.flags = flags;
+
+    Grouping g;
+    std::string err;
+    if (!group_indices(*A, idx_A, *B, idx_B, *C, idx_C, g, err)) return Status::error(MISTRA_ERR_ARG, err);
+    p.grouping = g;
+    if (!g.pure())
+    {
+        p.general = true;
+        return Status();
+    }
+
+    // src/stra_internal/3t/stra_mult.cxx:336-356: AC by (C, A) strides, BC by (C, B), AB by (A, B).
+    permute_group<int>(g.AC, sort_by_stride(g.AC.sC, g.AC.sA));
+    permute_group<int>(g.BC, sort_by_stride(g.BC.sC, g.BC.sB));
+    permute_group<int>(g.AB, sort_by_stride(g.AB.sA, g.AB.sB));
+
+    std::vector<int64_t> a_m = scatter(g.AC.len, g.AC.sA), c_m = scatter(g.AC.len, g.AC.sC);
+    std::vector<int64_t> a_k = scatter(g.AB.len, g.AB.sA), b_k = scatter(g.AB.len, g.AB.sB);
+    std::vector<int64_t> b_n = scatter(g.BC.len, g.BC.sB), c_n = scatter(g.BC.len, g.BC.sC);
+    p.h_batch_a = scatter(g.ABC.len, g.ABC.sA);
+    p.h_batch_b = scatter(g.ABC.len, g.ABC.sB);
+    p.h_batch_c = scatter(g.ABC.len, g.ABC.sC);
+    p.nbatch = int64_t(p.h_batch_c.size());
+
+    // If C's column index is the unit-stride one, compute C^T = B^T A^T so that the MFMA output
+    // (4 consecutive rows per lane) lands on contiguous memory (src/stra_internal/3t/stra_mult.cxx:436-444).
+    if (unit(c_n) && !unit(c_m))
+    {
+        p.transposed = true;
+        std::swap(a_m, b_n);  // A' = B^T : rows = n, cols = k
+        std::swap(a_k, b_k);  // B' = A^T : rows = k, cols = m
+        std::swap(c_m, c_n);
+        std::swap(p.h_batch_a, p.h_batch_b);
+    }
+    p.m = int64_t(a_m.size());
+    p.n = int64_t(b_n.size());
+    p.k = int64_t(a_k.size());
+    p.a_mfast = unit(a_m) || !unit(a_k);
+    p.b_nfast = unit(b_n) || !unit(b_k);
+
+    p.d = int64_t(1) << level;
+    p.ms = p.m / p.d;
+    p.ns = p.n / p.d;
+    p.ks = p.k / p.d;
+    p.core = level > 0 && p.ms > 0 && p.ns > 0 && p.ks > 0;
+    if (level > 0 && !p.core && (flags & MISTRA_FLAG_REF_FRINGE))
+    {
+        // the reference computes nothing when a leaf extent truncates to zero
+        p.h_prods.clear();
+        p.nprod = 0;
+    }
+    else if (!p.core)
+    {
+        p.d = 1;
+        p.ms = p.m;
+        p.ns = p.n;
+        p.ks = p.k;
+        p.h_prods = {single_desc(0, 0, 0, 0, 0, 0, true)};
+        p.nprod = 1;
+    }
+    else
+    {
+        unsigned touched = 0;
+        for (const Product& pr : products_for_level(level))
+            p.h_prods.push_back(make_desc(pr, p.d, p.ms, p.ns, p.ks, touched));
+        p.nprod = int(p.h_prods.size());
+        if (!(flags & MISTRA_FLAG_REF_FRINGE))
+        {
+            const int64_t dm = p.d * p.ms, dn = p.d * p.ns, dk = p.d * p.ks;
+            if (dm < p.m)
+            {
+                p.fringe.push_back(Pass{p.m - dm, p.n, p.k, int(p.h_prods.size())});
+                p.h_prods.push_back(single_desc(dm, 0, 0, 0, dm, 0, true));
+            }
+            if (dn < p.n)
+            {
+                p.fringe.push_back(Pass{dm, p.n - dn, p.k, int(p.h_prods.size())});
+                p.h_prods.push_back(single_desc(0, 0, 0, dn, 0, dn, true));
+            }
+            if (dk < p.k)
+            {
+                p.fringe.push_back(Pass{dm, dn, p.k - dk, int(p.h_prods.size())});
+                p.h_prods.push_back(single_desc(0, dk, dk, 0, 0, 0, false));
+            }
+        }
+        // combine tables for AB / Naive: per C sub-block, the products feeding it in product order
+        p.nsub = int(p.d * p.d);
+        p.h_term_begin.assign(size_t(p.nsub) + 1, 0);
+        for (int s = 0; s < p.nsub; s++)
+        {
+            p.h_sub_roff.push_back((s / p.d) * p.ms);
+            p.h_sub_coff.push_back((s % p.d) * p.ns);
+            p.h_term_begin[size_t(s)] = int(p.h_terms.size());
+            const auto& prods = products_for_level(level);
+            for (size_t q = 0; q < prods.size(); q++)
+                for (const SubTerm& t : prods[q].C)
+                    if (t.subid == s) p.h_terms.push_back(CombineTerm{int(q), t.coeff});
+        }
+        p.h_term_begin[size_t(p.nsub)] = int(p.h_terms.size());
+        for (int q = 0; q < p.nprod; q++)
+        {
+            ProdDesc pd;
+            std::memset(&pd, 0, sizeof(pd));
+            pd.nA = pd.nB = 1;
+            pd.a[0] = SubRef{int64_t(q) * p.ms * p.ks, 0, 0, 1.0};
+            pd.b[0] = SubRef{int64_t(q) * p.ks * p.ns, 0, 0, 1.0};
+            p.h_dense_prods.push_back(pd);
+        }
+    }
+    p.h_a_m = std::move(a_m);
+    p.h_a_k = std::move(a_k);
+    p.h_b_k = std::move(b_k);
+    p.h_b_n = std::move(b_n);
+    p.h_c_m = std::move(c_m);
+    p.h_c_n = std::move(c_n);
+    return Status();
+}
+
+namespace {
+
+Status upload(Plan& p)
+{
+    const int TK = tile_kdepth(p.tile, p.elem);
+    std::vector<int64_t> a_krun = run_strides(p.h_a_k, TK), b_krun = run_strides(p.h_b_k, TK);
+    std::vector<int64_t> s_m, s_k, t_k, t_n, s_krun, t_krun;
+    if (p.core)
+    {
+        for (int64_t i = 0; i < p.ms; i++) s_m.push_back(i);
+        for (int64_t i = 0; i < p.ks; i++) s_k.push_back(i * p.ms);
+        for (int64_t i = 0; i < p.ks; i++) t_k.push_back(i);
+        for (int64_t i = 0; i < p.ns; i++) t_n.push_back(i * p.ks);
+        s_krun = run_strides(s_k, TK);
+        t_krun = run_strides(t_k, TK);
+    }
+
+    struct Item
+    {
+        const void* src;
+        size_t bytes;
+        size_t off;
+    };
+    std::vector<Item> items;
+    size_t total = 0;
+    auto add = [&](const void* src, size_t bytes) {
+        items.push_back(Item{src, bytes, total});
+        total += align256(bytes > 0 ? bytes : 8);
+        return items.size() - 1;
+    };
+    auto i_a_m = add(p.h_a_m.data(), p.h_a_m.size() * 8), i_a_k = add(p.h_a_k.data(), p.h_a_k.size() * 8),
+         i_a_kr = add(a_krun.data(), a_krun.size() * 8), i_b_k = add(p.h_b_k.data(), p.h_b_k.size() * 8),
+         i_b_kr = add(b_krun.data(), b_krun.size() * 8), i_b_n = add(p.h_b_n.data(), p.h_b_n.size() * 8),
+         i_c_m = add(p.h_c_m.data(), p.h_c_m.size() * 8), i_c_n = add(p.h_c_n.data(), p.h_c_n.size() * 8),
+         i_ba = add(p.h_batch_a.data(), p.h_batch_a.size() * 8),
+         i_bb = add(p.h_batch_b.data(), p.h_batch_b.size() * 8),
+         i_bc = add(p.h_batch_c.data(), p.h_batch_c.size() * 8),
+         i_pr = add(p.h_prods.data(), p.h_prods.size() * sizeof(ProdDesc)),
+         i_sr = add(p.h_sub_roff.data(), p.h_sub_roff.size() * 8),
+         i_sc = add(p.h_sub_coff.data(), p.h_sub_coff.size() * 8),
+         i_tb = add(p.h_term_begin.data(), p.h_term_begin.size() * sizeof(int)),
+         i_te = add(p.h_terms.data(), p.h_terms.size() * sizeof(CombineTerm)),
+         i_sm = add(s_m.data(), s_m.size() * 8), i_sk = add(s_k.data(), s_k.size() * 8),
+         i_skr = add(s_krun.data(), s_krun.size() * 8), i_tk = add(t_k.data(), t_k.size() * 8),
+         i_tkr = add(t_krun.data(), t_krun.size() * 8), i_tn = add(t_n.data(), t_n.size() * 8),
+         i_dp = add(p.h_dense_prods.data(), p.h_dense_prods.size() * sizeof(ProdDesc));
+
+    std::vector<unsigned char> staging(total, 0);
+    for (const Item& it : items)
+        if (it.bytes) std::memcpy(staging.data() + it.off, it.src, it.bytes);
+    void* d = nullptr;
+    hipError_t e = hipMalloc(&d, total);
+    if (e != hipSuccess) return Status::error(MISTRA_ERR_NOMEM, std::string("plan allocation: ") + hipGetErrorString(e));
+    e = hipMemcpy(d, staging.data(), total, hipMemcpyHostToDevice);
+    if (e != hipSuccess)
+    {
+        (void)hipFree(d);
+        return Status::error(MISTRA_ERR_HIP, std::string("plan upload: ") + hipGetErrorString(e));
+    }
+    p.dmem = d;
+    auto at = [&](size_t i) { return reinterpret_cast<unsigned char*>(d) + items[i].off; };
+    p.a_m = reinterpret_cast<const int64_t*>(at(i_a_m));
+    p.a_k = reinterpret_cast<const int64_t*>(at(i_a_k));
+    p.a_krun = reinterpret_cast<const int64_t*>(at(i_a_kr));
+    p.b_k = reinterpret_cast<const int64_t*>(at(i_b_k));
+    p.b_krun = reinterpret_cast<const int64_t*>(at(i_b_kr));
+    p.b_n = reinterpret_cast<const int64_t*>(at(i_b_n));
+    p.c_m = reinterpret_cast<const int64_t*>(at(i_c_m));
+    p.c_n = reinterpret_cast<const int64_t*>(at(i_c_n));
+    p.batch_a = reinterpret_cast<const int64_t*>(at(i_ba));
+    p.batch_b = reinterpret_cast<const int64_t*>(at(i_bb));
+    p.batch_c = reinterpret_cast<const int64_t*>(at(i_bc));
+    p.prods = reinterpret_cast<const ProdDesc*>(at(i_pr));
+    p.sub_roff = reinterpret_cast<const int64_t*>(at(i_sr));
+    p.sub_coff = reinterpret_cast<const int64_t*>(at(i_sc));
+    p.term_begin = reinterpret_cast<const int*>(at(i_tb));
+    p.terms = reinterpret_cast<const CombineTerm*>(at(i_te));
+    p.s_m = reinterpret_cast<const int64_t*>(at(i_sm));
+    p.s_k = reinterpret_cast<const int64_t*>(at(i_sk));
+    p.s_krun = reinterpret_cast<const int64_t*>(at(i_skr));
+    p.t_k = reinterpret_cast<const int64_t*>(at(i_tk));
+    p.t_krun = reinterpret_cast<const int64_t*>(at(i_tkr));
+    p.t_n = reinterpret_cast<const int64_t*>(at(i_tn));
+    p.dense_prods = reinterpret_cast<const ProdDesc*>(at(i_dp));
+    return Status();
+}
+
+}  // namespace
+
+Status get_plan(const tblis::tblis_tensor* A, const label_type* idx_A, const tblis::tblis_tensor* B,
+                const label_type* idx_B, const tblis::tblis_tensor* C, const label_type* idx_C, int level,
+                int impl, int tile, unsigned flags, std::shared_ptr<Plan>& out)
+{
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    std::ostringstream ks;
+    ks << dev << '|' << int(A->type) << '|' << level << '|' << impl << '|' << tile << '|'
+       << (flags & MISTRA_FLAG_REF_FRINGE) << '|';
+    std::string key = ks.str();
+    append_key(key, A, idx_A);
+    append_key(key, B, idx_B);
+    append_key(key, C, idx_C);
+    {
+        std::lock_guard<std::mutex> lk(g_plan_mutex);
+        for (auto it = g_plans.begin(); it != g_plans.end(); ++it)
+            if (it->first == key)
+            {
+                out = it->second;
+                g_plans.splice(g_plans.begin(), g_plans, it);
+                return Status();
+            }
+    }
+    auto p = std::make_shared<Plan>();
+    Status st = build_plan_host(A, idx_A, B, idx_B, C, idx_C, level, impl, tile, flags, *p);
+    if (!st.ok()) return st;
+    if (!p->general)
+    {
+        st = upload(*p);
+        if (!st.ok()) return st;
+    }
+    std::lock_guard<std::mutex> lk(g_plan_mutex);
+    g_plans.emplace_front(key, p);
+    while (g_plans.size() > kMaxPlans) g_plans.pop_back();
+    out = p;
+    return Status();
+}
+
+void release_all()
+{
+    std::lock_guard<std::mutex> lk(g_plan_mutex);
+    g_plans.clear();
+}
+
+namespace {
+
+GemmArgs base_args(const Plan& p, const Call& c)
+{
+    GemmArgs g;
+    std::memset(&g, 0, sizeof(g));
+    g.A = c.A;
+    g.B = c.B;
+    g.C = c.C;
+    g.a_m = p.a_m;
+    g.a_k = p.a_k;
+    g.a_krun = p.a_krun;
+    g.b_k = p.b_k;
+    g.b_krun = p.b_krun;
+    g.b_n = p.b_n;
+    g.c_m = p.c_m;
+    g.c_n = p.c_n;
+    g.alpha = c.alpha;
+    g.beta = c.beta;
+    g.nbatch = p.nbatch;
+    g.batch_a = p.nbatch > 1 ? p.batch_a : nullptr;
+    g.batch_b = p.nbatch > 1 ? p.batch_b : nullptr;
+    g.batch_c = p.nbatch > 1 ? p.batch_c : nullptr;
+    if (p.nbatch == 1)
+    {
+        g.A = static_cast<const char*>(c.A) + p.h_batch_a[0] * p.elem;
+        g.B = static_cast<const char*>(c.B) + p.h_batch_b[0] * p.elem;
+        g.C = static_cast<char*>(c.C) + p.h_batch_c[0] * p.elem;
+    }
+    return g;
+}
+
+double leaf_flops(const Plan& p, int nprod, int64_t ms, int64_t ns, int64_t ks)
+{
+    return 2.0 * double(nprod) * double(ms) * double(ns) * double(ks) * double(p.nbatch);
+}
+
+}  // namespace
+
+Status run_plan(const Plan& p, const Call& c, hipStream_t stream)
+{
+    if (p.general) return Status::error(MISTRA_ERR_UNSUPPORTED, "contractions with A-only, B-only or C-only indices are not supported yet");
+    if (p.nprod == 0) return Status();
+    GemmArgs g = base_args(p, c);
+
+    if (!p.core || p.impl == IMPL_ABC || p.impl == IMPL_BLAS || p.impl == IMPL_REF)
+    {
+        g.ms = p.ms;
+        g.ns = p.ns;
+        g.ks = p.ks;
+        g.prods = p.prods;
+        g.nprod = p.nprod;
+        timing_begin(stream);
+        HIP_TRY(launch_leaf_gemm(p.elem, EPI_MULTI_C, p.tile, g, p.a_mfast, p.b_nfast, stream));
+        timing_end(stream, leaf_flops(p, p.nprod, p.ms, p.ns, p.ks));
+    }
+    else if (p.impl == IMPL_AB)
+    {
+        const size_t mbytes = size_t(p.nbatch) * size_t(p.nprod) * size_t(p.ms) * size_t(p.ns) * size_t(p.elem);
+        void* M = nullptr;
+        HIP_TRY(hipMallocAsync(&M, mbytes, stream));
+        g.ms = p.ms;
+        g.ns = p.ns;
+        g.ks = p.ks;
+        g.prods = p.prods;
+        g.nprod = p.nprod;
+        g.M = M;
+        timing_begin(stream);
+        hipError_t e = launch_leaf_gemm(p.elem, EPI_STORE, p.tile, g, p.a_mfast, p.b_nfast, stream);
+        timing_end(stream, leaf_flops(p, p.nprod, p.ms, p.ns, p.ks));
+        if (e == hipSuccess)
+        {
+            CombineArgs ca;
+            std::memset(&ca, 0, sizeof(ca));
+            ca.C = g.C;
+            ca.M = M;
+            ca.c_m = p.c_m;
+            ca.c_n = p.c_n;
+            ca.ms = p.ms;
+            ca.ns = p.ns;
+            ca.nsub = p.nsub;
+            ca.sub_roff = p.sub_roff;
+            ca.sub_coff = p.sub_coff;
+            ca.term_begin = p.term_begin;
+            ca.terms = p.terms;
+            ca.nprod = p.nprod;
+            ca.alpha = c.alpha;
+            ca.beta = c.beta;
+            ca.nbatch = p.nbatch;
+            ca.batch_c = g.batch_c;
+            e = launch_combine(p.elem, ca, stream);
+        }
+        hipError_t f = hipFreeAsync(M, stream);
+        HIP_TRY(e);
+        HIP_TRY(f);
+    }
+    else  // IMPL_NAIVE
+    {
+        const size_t sbytes = size_t(p.nprod) * size_t(p.ms) * size_t(p.ks) * size_t(p.elem);
+        const size_t tbytes = size_t(p.nprod) * size_t(p.ks) * size_t(p.ns) * size_t(p.elem);
+        const size_t mbytes = size_t(p.nprod) * size_t(p.ms) * size_t(p.ns) * size_t(p.elem);
+        void *S = nullptr, *T = nullptr, *M = nullptr;
+        HIP_TRY(hipMallocAsync(&S, sbytes, stream));
+        HIP_TRY(hipMallocAsync(&T, tbytes, stream));
+        HIP_TRY(hipMallocAsync(&M, mbytes, stream));
+        hipError_t e = hipSuccess;
+        for (int64_t z = 0; z < p.nbatch && e == hipSuccess; z++)
+        {
+            SumArgs sa;
+            std::memset(&sa, 0, sizeof(sa));
+            sa.X = c.A;
+            sa.rs = p.a_m;
+            sa.cs = p.a_k;
+            sa.rows = p.ms;
+            sa.cols = p.ks;
+            sa.prods = p.prods;
+            sa.nprod = p.nprod;
+            sa.which = 0;
+            sa.rows_fast = p.a_mfast;
+            sa.out = S;
+            sa.zoff = p.h_batch_a[size_t(z)];
+            e = launch_operand_sums(p.elem, sa, stream);
+            if (e != hipSuccess) break;
+            sa.X = c.B;
+            sa.rs = p.b_k;
+            sa.cs = p.b_n;
+            sa.rows = p.ks;
+            sa.cols = p.ns;
+            sa.which = 1;
+            sa.rows_fast = !p.b_nfast;
+            sa.out = T;
+            sa.zoff = p.h_batch_b[size_t(z)];
+            e = launch_operand_sums(p.elem, sa, stream);
+            if (e != hipSuccess) break;
+
+            GemmArgs gd;
+            std::memset(&gd, 0, sizeof(gd));
+            gd.A = S;
+            gd.B = T;
+            gd.a_m = p.s_m;
+            gd.a_k = p.s_k;
+            gd.a_krun = p.s_krun;
+            gd.b_k = p.t_k;
+            gd.b_krun = p.t_krun;
+            gd.b_n = p.t_n;
+            gd.ms = p.ms;
+            gd.ns = p.ns;
+            gd.ks = p.ks;
+            gd.prods = p.dense_prods;
+            gd.nprod = p.nprod;
+            gd.nbatch = 1;
+            gd.M = M;
+            timing_begin(stream);
+            e = launch_leaf_gemm(p.elem, EPI_STORE, p.tile, gd, true, false, stream);
+            timing_end(stream, 2.0 * double(p.nprod) * double(p.ms) * double(p.ns) * double(p.ks));
+            if (e != hipSuccess) break;
+
+            CombineArgs ca;
+            std::memset(&ca, 0, sizeof(ca));
+            ca.C = static_cast<char*>(c.C) + p.h_batch_c[size_t(z)] * p.elem;
+            ca.M = M;
+            ca.c_m = p.c_m;
+            ca.c_n = p.c_n;
+            ca.ms = p.ms;
+            ca.ns = p.ns;
+            ca.nsub = p.nsub;
+            ca.sub_roff = p.sub_roff;
+            ca.sub_coff = p.sub_coff;
+            ca.term_begin = p.term_begin;
+            ca.terms = p.terms;
+            ca.nprod = p.nprod;
+            ca.alpha = c.alpha;
+            ca.beta = c.beta;
+            ca.nbatch = 1;
+            e = launch_combine(p.elem, ca, stream);
+        }
+        hipError_t f1 = hipFreeAsync(S, stream), f2 = hipFreeAsync(T, stream), f3 = hipFreeAsync(M, stream);
+        HIP_TRY(e);
+        HIP_TRY(f1);
+        HIP_TRY(f2);
+        HIP_TRY(f3);
+    }
+
+    // classical passes over the remainder of the 2^L split
+    for (const Pass& ps : p.fringe)
+    {
+        GemmArgs f = base_args(p, c);
+        f.ms = ps.ms;
+        f.ns = ps.ns;
+        f.ks = ps.ks;
+        f.prods = p.prods + ps.prod_index;
+        f.nprod = 1;
+        HIP_TRY(launch_leaf_gemm(p.elem, EPI_MULTI_C, TILE_64, f, p.a_mfast, p.b_nfast, stream));
+    }
+    return Status();
+}
+
+}  // namespace mistra

@@ -1,0 +1,123 @@
+// Host driver of the Strassen tensor contraction: plans (device-resident scatter vectors and product
+// tables, built once per contraction shape) and their execution on a HIP stream.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+#include "launch.hpp"
+#include "plan.hpp"
+
+namespace mistra {
+
+// Status carrier: code 0 = ok, else a MISTRA_ERR_* code with a message.
+struct Status
+{
+    int code = 0;
+    std::string msg;
+    bool ok() const { return code == 0; }
+    static Status error(int c, std::string m) { return Status{c, std::move(m)}; }
+};
+
+// Implementation choice, numbered as tblis::internal::impl_t.
+enum Impl
+{
+    IMPL_ABC = 0,
+    IMPL_BLAS = 1,
+    IMPL_REF = 2,
+    IMPL_NAIVE = 3,
+    IMPL_AB = 4
+};
+
+struct Pass  // one classical launch over a sub-range (the fringe the 2^L split leaves over)
+{
+    int64_t ms, ns, ks;
+    int prod_index;  // into the device product array
+};
+
+struct Plan
+{
+    int elem = 8;  // element size in bytes
+    int level = 0;
+    int impl = IMPL_ABC;
+    int tile = TILE_128;
+    unsigned flags = 0;
+    int64_t m = 0, n = 0, k = 0, nbatch = 1;
+    int64_t d = 1, ms = 0, ns = 0, ks = 0;  // split factor and leaf extents
+    bool transposed = false;
+    bool a_mfast = true, b_nfast = true;
+    bool general = false;   // A-only / B-only / C-only indices present (handled by the reduce path)
+    bool core = false;      // Strassen core (level >= 1 with non-empty leaves) present
+
+    // host copies of the index data (kept for the reduce path and for tests)
+    std::vector<int64_t> h_a_m, h_a_k, h_b_k, h_b_n, h_c_m, h_c_n, h_batch_a, h_batch_b, h_batch_c;
+    std::vector<ProdDesc> h_prods;  // [0, nprod) core products, then one per fringe pass
+    int nprod = 0;
+    std::vector<Pass> fringe;
+
+    // combine tables (AB / Naive)
+    int nsub = 0;
+    std::vector<int64_t> h_sub_roff, h_sub_coff;
+    std::vector<int> h_term_begin;
+    std::vector<CombineTerm> h_terms;
+
+    // Naive: dense products over S_p (ms x ks) and T_p (ks x ns)
+    std::vector<ProdDesc> h_dense_prods;
+
+    // device mirror (one allocation)
+    void* dmem = nullptr;
+    const int64_t *a_m = nullptr, *a_k = nullptr, *a_krun = nullptr, *b_k = nullptr, *b_krun = nullptr,
+                  *b_n = nullptr, *c_m = nullptr, *c_n = nullptr;
+    const int64_t *batch_a = nullptr, *batch_b = nullptr, *batch_c = nullptr;
+    const ProdDesc* prods = nullptr;
+    const int64_t *sub_roff = nullptr, *sub_coff = nullptr;
+    const int* term_begin = nullptr;
+    const CombineTerm* terms = nullptr;
+    const int64_t *s_m = nullptr, *s_k = nullptr, *s_krun = nullptr, *t_k = nullptr, *t_krun = nullptr,
+                  *t_n = nullptr;
+    const ProdDesc* dense_prods = nullptr;
+
+    // general (non-pure) contractions: reduce/broadcast descriptors
+    Grouping grouping;
+
+    ~Plan();
+};
+
+struct Call
+{
+    const void* A;
+    const void* B;
+    void* C;
+    double alpha, beta;
+};
+
+// Build (or fetch from the cache) the plan of a contraction.
+Status get_plan(const tblis::tblis_tensor* A, const label_type* idx_A, const tblis::tblis_tensor* B,
+                const label_type* idx_B, const tblis::tblis_tensor* C, const label_type* idx_C, int level,
+                int impl, int tile, unsigned flags, std::shared_ptr<Plan>& out);
+
+// Build a plan without touching the device (index analysis and tables only; for tests and tuning).
+Status build_plan_host(const tblis::tblis_tensor* A, const label_type* idx_A, const tblis::tblis_tensor* B,
+                       const label_type* idx_B, const tblis::tblis_tensor* C, const label_type* idx_C,
+                       int level, int impl, int tile, unsigned flags, Plan& p);
+
+// Execute a plan on device pointers.
+Status run_plan(const Plan& p, const Call& c, hipStream_t stream);
+
+// Whole-call entry used by every C-ABI symbol.
+Status tensor_mult(const tblis::tblis_tensor* A, const label_type* idx_A, const tblis::tblis_tensor* B,
+                   const label_type* idx_B, tblis::tblis_tensor* C, const label_type* idx_C, int level, int impl,
+                   int precision, unsigned flags, hipStream_t stream, bool synchronous);
+
+// Kernel timing hook of the dominant leaf kernel.
+void timing_begin(hipStream_t s);
+void timing_end(hipStream_t s, double flops);
+
+void release_all();
+
+}  // namespace mistra

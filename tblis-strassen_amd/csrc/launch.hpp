@@ -1,0 +1,85 @@
+// Host-callable launchers of the gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "kernels.hpp"
+
+namespace mistra {
+
+enum TileCfg
+{
+    TILE_128 = 0,  // 128x128 workgroup tile (2x2 waves of 64x64)
+    TILE_64 = 1    // 64x64 workgroup tile (2x2 waves of 32x32), for small or skinny leaves
+};
+
+int tile_extent(int tile);
+int tile_kdepth(int tile, int elem_size);
+
+// Leaf GEMM over scatter vectors with fused operand sums (leaf_gemm.hip).
+hipError_t launch_leaf_gemm(int elem_size, int epi, int tile, GemmArgs g, bool a_mfast, bool b_nfast,
+                            hipStream_t stream);
+
+// Explicit operand sums (Naive impl): for product p and its terms t,
+//   out[p*rows*cols + r + c*rows] = sum_t coeff_t * X[base_t + rs[roff_t + r] + cs[coff_t + c]]
+// `which` selects the A (0) or B (1) terms of each ProdDesc.
+struct SumArgs
+{
+    const void* X;
+    const int64_t* rs;
+    const int64_t* cs;
+    int64_t rows, cols;
+    const ProdDesc* prods;
+    int nprod;
+    int which;
+    bool rows_fast;  // true if rs has unit stride (read along rows), else read along cols
+    void* out;
+    int64_t zoff;  // batch offset added to X
+};
+hipError_t launch_operand_sums(int elem_size, const SumArgs& a, hipStream_t stream);
+
+// Combine staged products into C sub-blocks:
+//   C[c_m[roff_s + r] + c_n[coff_s + c]] = beta*C + alpha * sum_{(p,coef) in list_s} coef*M_p[r + c*ms]
+struct CombineTerm
+{
+    int prod;
+    double coeff;
+};
+struct CombineArgs
+{
+    void* C;
+    const void* M;
+    const int64_t* c_m;
+    const int64_t* c_n;
+    int64_t ms, ns;
+    int nsub;                   // number of C sub-blocks (4^L)
+    const int64_t* sub_roff;    // [nsub]
+    const int64_t* sub_coff;    // [nsub]
+    const int* term_begin;      // [nsub+1] into terms
+    const CombineTerm* terms;
+    int nprod;
+    double alpha, beta;
+    int64_t nbatch;
+    const int64_t* batch_c;
+};
+hipError_t launch_combine(int elem_size, const CombineArgs& a, hipStream_t stream);
+
+// Elementwise helpers.
+hipError_t launch_fill_uniform(int elem_size, void* ptr, int64_t n, uint64_t seed, int scale_exp_max,
+                               hipStream_t stream);
+hipError_t launch_copy_bytes(void* dst, const void* src, int64_t nbytes, hipStream_t stream);
+// partial sums of (x-y)^2 and y^2 into out[0..1] (device doubles, zeroed by the kernel launch itself)
+hipError_t launch_rel_error(int elem_size, const void* X, const void* Y, int64_t n, double* out,
+                            hipStream_t stream);
+// C = beta*C (beta == 0: set to zero) over a strided tensor of up to 8 dims.
+struct StridedDims
+{
+    int ndim;
+    int64_t len[8];
+    int64_t stride[8];
+};
+hipError_t launch_scale(int elem_size, void* C, const StridedDims& d, double beta, hipStream_t stream);
+
+}  // namespace mistra

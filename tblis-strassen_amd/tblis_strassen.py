@@ -1,0 +1,216 @@
+"""ctypes binding of libtblis_strassen_amd.so (the C-ABI in include/tblis_strassen.h).
+
+Mirrors the reference's operator interface for the Strassen tensor-contraction path:
+    stra_mult(alpha, A, idx_A, B, idx_B, beta, C, idx_C)         src/stra_iface/3t/stra_mult.h:26-36
+    stra_mult_2level(alpha, A, idx_A, B, idx_B, beta, C, idx_C)  src/stra_iface/3t/stra_mult_2level.h:26-36
+    mult(alpha, A, idx_A, B, idx_B, beta, C, idx_C)              src/iface/3t/mult.h:26-36
+    set_impl / get_impl                                         tblis::internal::impl, src/internal/3t/mult.cxx:26
+with index strings as str, tensors as torch tensors (device or host) or numpy arrays (host).  Strides
+are taken from the tensor object in elements, as tblis_tensor expects.  The HIP library must be built:
+loading fails loudly otherwise (there is no CPU fallback).
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtblis_strassen_amd.so")
+
+TYPE_FLOAT, TYPE_DOUBLE = 0, 1
+IMPL = {"abc": 0, "blis": 0, "blas": 1, "ref": 2, "naive": 3, "ab": 4, "auto": -1}
+PREC = {"native": 0, "bf16x3": 1, "ozaki_bf16": 2, "fp8_block": 3}
+FLAG_REF_FRINGE = 1
+FLAG_SYNC = 2
+
+
+class MistraError(RuntimeError):
+    pass
+
+
+class TblisTensor(ctypes.Structure):
+    """Byte image of tblis_tensor (src/util/basic_types.h:341-419)."""
+
+    _fields_ = [
+        ("type", ctypes.c_int),
+        ("conj", ctypes.c_int),
+        ("scalar", ctypes.c_char * 16),
+        ("data", ctypes.c_void_p),
+        ("ndim", ctypes.c_uint),
+        ("len", ctypes.POINTER(ctypes.c_ssize_t)),
+        ("stride", ctypes.POINTER(ctypes.c_ssize_t)),
+    ]
+
+
+class Contraction(ctypes.Structure):
+    _fields_ = [
+        ("A", ctypes.POINTER(TblisTensor)), ("idx_A", ctypes.c_char_p),
+        ("B", ctypes.POINTER(TblisTensor)), ("idx_B", ctypes.c_char_p),
+        ("C", ctypes.POINTER(TblisTensor)), ("idx_C", ctypes.c_char_p),
+        ("level", ctypes.c_int), ("impl", ctypes.c_int),
+    ]
+
+
+assert ctypes.sizeof(TblisTensor) == 56 and TblisTensor.data.offset == 24 and TblisTensor.len.offset == 40
+
+_lib = None
+
+
+def load(path=None):
+    """Load the HIP library (idempotent).  Raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    p = path or os.environ.get("MISTRA_LIB", LIB_PATH)
+    if not os.path.exists(p):
+        raise MistraError(f"{p} is missing: build it with `python tblis-strassen_amd/build.py`")
+    lib = ctypes.CDLL(p, mode=ctypes.RTLD_GLOBAL)
+    T = ctypes.POINTER(TblisTensor)
+    s = ctypes.c_char_p
+    for name in ("stra_tblis_tensor_mult", "stra_tblis_tensor_mult_2level", "tblis_tensor_mult"):
+        f = getattr(lib, name)
+        f.argtypes = [ctypes.c_void_p, ctypes.c_void_p, T, s, T, s, T, s]
+        f.restype = None
+    lib.mistra_tensor_mult.argtypes = [T, s, T, s, T, s, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                       ctypes.c_uint, ctypes.c_void_p]
+    lib.mistra_tensor_mult.restype = ctypes.c_int
+    lib.mistra_last_error.restype = ctypes.c_char_p
+    lib.mistra_set_impl.argtypes = [ctypes.c_int]
+    lib.mistra_get_impl.restype = ctypes.c_int
+    lib.mistra_kernel_timing.argtypes = [ctypes.c_int]
+    lib.mistra_kernel_timing_read.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_longlong),
+                                              ctypes.POINTER(ctypes.c_double), ctypes.c_int]
+    lib.mistra_fill_uniform.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_longlong, ctypes.c_ulonglong,
+                                        ctypes.c_int, ctypes.c_void_p]
+    lib.mistra_copy_bytes.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_longlong, ctypes.c_void_p]
+    lib.mistra_rel_error.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_longlong,
+                                     ctypes.POINTER(ctypes.c_double), ctypes.c_void_p]
+    lib.mistra_batch_mult.argtypes = [ctypes.c_int, ctypes.POINTER(Contraction), ctypes.c_uint, ctypes.c_void_p]
+    lib.mistra_tensor_mult_backward.argtypes = [T, s, T, s, T, s, T, T, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                ctypes.c_void_p]
+    lib.mistra_autotune.argtypes = [T, s, T, s, T, s, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                    ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), ctypes.c_void_p]
+    _lib = lib
+    return lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise MistraError(f"error {rc}: {_lib.mistra_last_error().decode()}")
+
+
+def _dtype_code(x):
+    name = str(x.dtype)
+    if name.endswith("float64"):
+        return TYPE_DOUBLE
+    if name.endswith("float32"):
+        return TYPE_FLOAT
+    raise MistraError(f"unsupported dtype {x.dtype}")
+
+
+def _shape_strides_ptr(x):
+    """(shape, strides in elements, data pointer) of a torch tensor or numpy array."""
+    if hasattr(x, "data_ptr"):  # torch
+        return list(x.shape), list(x.stride()), x.data_ptr()
+    import numpy as np
+
+    assert isinstance(x, np.ndarray)
+    item = x.dtype.itemsize
+    return list(x.shape), [s // item for s in x.strides], x.ctypes.data
+
+
+def make_tensor(x, scalar=1.0):
+    """Build a tblis_tensor over x (keeps its len/stride arrays alive on the struct)."""
+    shape, strides, ptr = _shape_strides_ptr(x)
+    t = TblisTensor()
+    t.type = _dtype_code(x)
+    t.conj = 0
+    buf = (ctypes.c_double(scalar) if t.type == TYPE_DOUBLE else ctypes.c_float(scalar))
+    ctypes.memmove(t.scalar, ctypes.addressof(buf), ctypes.sizeof(buf))
+    t.data = ptr
+    t.ndim = len(shape)
+    t._len = (ctypes.c_ssize_t * max(1, len(shape)))(*shape)
+    t._stride = (ctypes.c_ssize_t * max(1, len(strides)))(*strides)
+    t.len = ctypes.cast(t._len, ctypes.POINTER(ctypes.c_ssize_t))
+    t.stride = ctypes.cast(t._stride, ctypes.POINTER(ctypes.c_ssize_t))
+    return t
+
+
+def _stream_ptr(stream):
+    if stream is None:
+        return None
+    if hasattr(stream, "cuda_stream"):
+        return stream.cuda_stream
+    return int(stream)
+
+
+def tensor_mult(alpha, A, idx_A, B, idx_B, beta, C, idx_C, level=1, impl="abc", precision="native",
+                stream=None, flags=0):
+    """Extension entry point: explicit level / impl / precision / stream (asynchronous on the stream)."""
+    lib = load()
+    tA, tB, tC = make_tensor(A, alpha), make_tensor(B, 1.0), make_tensor(C, beta)
+    impl_c = IMPL[impl] if isinstance(impl, str) else int(impl)
+    prec_c = PREC[precision] if isinstance(precision, str) else int(precision)
+    _check(lib.mistra_tensor_mult(ctypes.byref(tA), idx_A.encode(), ctypes.byref(tB), idx_B.encode(),
+                                  ctypes.byref(tC), idx_C.encode(), level, impl_c, prec_c, flags,
+                                  _stream_ptr(stream)))
+
+
+def _dropin(symbol, alpha, A, idx_A, B, idx_B, beta, C, idx_C):
+    lib = load()
+    tA, tB, tC = make_tensor(A, alpha), make_tensor(B, 1.0), make_tensor(C, beta)
+    getattr(lib, symbol)(None, None, ctypes.byref(tA), idx_A.encode(), ctypes.byref(tB), idx_B.encode(),
+                         ctypes.byref(tC), idx_C.encode())
+    msg = lib.mistra_last_error().decode()  # the void symbols report through the thread-local message
+    if msg:
+        raise MistraError(msg)
+
+
+def stra_mult(alpha, A, idx_A, B, idx_B, beta, C, idx_C):
+    """Drop-in stra_tblis_tensor_mult (1-level; impl from set_impl). Synchronous."""
+    _dropin("stra_tblis_tensor_mult", alpha, A, idx_A, B, idx_B, beta, C, idx_C)
+
+
+def stra_mult_2level(alpha, A, idx_A, B, idx_B, beta, C, idx_C):
+    """Drop-in stra_tblis_tensor_mult_2level. Synchronous."""
+    _dropin("stra_tblis_tensor_mult_2level", alpha, A, idx_A, B, idx_B, beta, C, idx_C)
+
+
+def mult(alpha, A, idx_A, B, idx_B, beta, C, idx_C):
+    """Drop-in tblis_tensor_mult (classical, L=0). Synchronous."""
+    _dropin("tblis_tensor_mult", alpha, A, idx_A, B, idx_B, beta, C, idx_C)
+
+
+def set_impl(impl):
+    lib = load()
+    _check(lib.mistra_set_impl(IMPL[impl] if isinstance(impl, str) else int(impl)))
+
+
+def get_impl():
+    return load().mistra_get_impl()
+
+
+def kernel_timing(enable=True):
+    _check(load().mistra_kernel_timing(1 if enable else 0))
+
+
+def kernel_timing_read(reset=True):
+    """(total ms, launches, flops) of the timed leaf-kernel launches since the last reset."""
+    ms, n, fl = ctypes.c_double(), ctypes.c_longlong(), ctypes.c_double()
+    _check(load().mistra_kernel_timing_read(ctypes.byref(ms), ctypes.byref(n), ctypes.byref(fl), 1 if reset else 0))
+    return ms.value, n.value, fl.value
+
+
+def fill_uniform(x, seed, emax=0, stream=None):
+    """Device fill with the library's splitmix64 uniform[-1,1) generator (x: contiguous torch tensor)."""
+    _check(load().mistra_fill_uniform(_dtype_code(x), x.data_ptr(), x.numel(), seed, emax, _stream_ptr(stream)))
+
+
+def copy_bytes(dst, src, nbytes, stream=None):
+    _check(load().mistra_copy_bytes(dst.data_ptr(), src.data_ptr(), nbytes, _stream_ptr(stream)))
+
+
+def rel_error(X, Y, stream=None):
+    """||X-Y||_F / ||Y||_F over two contiguous device tensors of equal size (FP64 reduction on device)."""
+    out = ctypes.c_double()
+    _check(load().mistra_rel_error(_dtype_code(X), X.data_ptr(), Y.data_ptr(), X.numel(), ctypes.byref(out),
+                                   _stream_ptr(stream)))
+    return out.value

@@ -1,0 +1,186 @@
+"""CPU-only checks of the HIP library's host side: it loads, exports every symbol the headers declare,
+and its index analysis / Strassen plan agree with the oracle (no device calls are made here)."""
+import ctypes
+import glob
+import os
+import re
+
+import numpy as np
+import pytest
+
+import cases
+import oracle
+import tblis_strassen as ts
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared_symbols():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        text = re.sub(r"//[^\n]*", "", text)
+        for m in re.finditer(r"^\s*(?:const\s+)?(?:void|int|char)\s*\*?\s*(\w+)\s*\(", text, flags=re.M):
+            names.add(m.group(1))
+        if re.search(r"extern\s+const\s+tblis_comm\s*\*\s*const\s+tblis_single", text):
+            names.add("tblis_single")
+    return sorted(names)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ts.load()
+    names = _declared_symbols()
+    assert "stra_tblis_tensor_mult" in names and "mistra_tensor_mult" in names and len(names) > 15
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_cxx_impl_global_is_exported():
+    # tblis::internal::impl keeps its C++ name for drop-in users of the reference's global
+    lib = ts.load()
+    assert hasattr(lib, "_ZN5tblis8internal4implE")
+    old = ts.get_impl()
+    ts.set_impl("naive")
+    assert ts.get_impl() == 3 and ctypes.c_int.in_dll(lib, "_ZN5tblis8internal4implE").value == 3
+    ts.set_impl(old)
+
+
+def _debug_plan(case, level, flags=0, strides=None, cap=8192):
+    lib = ts.load()
+    T = ctypes.POINTER(ts.TblisTensor)
+    lib.mistra_debug_plan.restype = ctypes.c_int
+    lib.mistra_debug_plan.argtypes = [T, ctypes.c_char_p, T, ctypes.c_char_p, T, ctypes.c_char_p, ctypes.c_int,
+                                      ctypes.c_uint, ctypes.POINTER(ctypes.c_longlong),
+                                      ctypes.POINTER(ctypes.c_longlong), ctypes.c_longlong,
+                                      ctypes.POINTER(ctypes.c_double), ctypes.c_int]
+    tensors = []
+    for x in "ABC":
+        shape = case[f"shape_{x}"]
+        st = strides[x] if strides else cases.colmajor_strides(shape)
+        t = ts.TblisTensor()
+        t.type = ts.TYPE_DOUBLE
+        t.ndim = len(shape)
+        t._len = (ctypes.c_ssize_t * max(1, len(shape)))(*shape)
+        t._stride = (ctypes.c_ssize_t * max(1, len(shape)))(*st)
+        t.len = ctypes.cast(t._len, ctypes.POINTER(ctypes.c_ssize_t))
+        t.stride = ctypes.cast(t._stride, ctypes.POINTER(ctypes.c_ssize_t))
+        t.data = None
+        tensors.append(t)
+    dims = (ctypes.c_longlong * 13)()
+    scat = (ctypes.c_longlong * (6 * cap))()
+    prods = (ctypes.c_double * (64 * 40))()
+    np_ = lib.mistra_debug_plan(ctypes.byref(tensors[0]), case["idx_A"].encode(), ctypes.byref(tensors[1]),
+                                case["idx_B"].encode(), ctypes.byref(tensors[2]), case["idx_C"].encode(), level, flags,
+                                dims, scat, cap, prods, 64)
+    return np_, list(dims), [np.array(scat[i * cap:(i + 1) * cap]) for i in range(6)], np.array(prods[:np_ * 40] if np_ > 0 else [])
+
+
+PLAN_CASES = [
+    dict(m=(6,), n=(5,), k=(7,)),
+    dict(m=(4, 4), n=(4, 4), k=(4, 4)),
+    dict(m=(2, 2, 4), n=(2, 2, 4), k=(2, 2, 4)),
+    dict(m=(3, 5), n=(2, 7), k=(9,)),
+    dict(m=(4,), n=(8,), k=(4,), batch=(3,)),
+]
+
+
+@pytest.mark.parametrize("ci", range(len(PLAN_CASES)))
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_index_analysis_matches_oracle(ci, seed):
+    case = cases.make_case(1000 * seed + ci, **PLAN_CASES[ci])
+    nprod, dims, vecs, _ = _debug_plan(case, 0)
+    assert nprod == 1
+    m, n, k, nb, pure, transposed = dims[:6]
+    sA, sB, sC = (cases.colmajor_strides(case[f"shape_{x}"]) for x in "ABC")
+    (odims, ovecs) = oracle.describe(case["shape_A"], sA, case["idx_A"], case["shape_B"], sB, case["idx_B"],
+                                     case["shape_C"], sC, case["idx_C"])
+    am, ak, bk, bn, cm, cn = ovecs
+    if transposed:
+        want = [bn, bk, ak, am, cn, cm]
+        assert (m, n, k) == (odims[1], odims[0], odims[2])
+    else:
+        want = [am, ak, bk, bn, cm, cn]
+        assert (m, n, k) == (odims[0], odims[1], odims[2])
+    assert nb == odims[3] and pure == 1
+    for got, w in zip(vecs, want):
+        assert np.array_equal(got[:len(w)], w)
+
+
+@pytest.mark.parametrize("level", [1, 2])
+def test_product_table_matches_oracle(level):
+    case = cases.make_case(7, m=(16,), n=(16,), k=(16,))
+    nprod, dims, _, prods = _debug_plan(case, level)
+    assert nprod == 7 ** level and dims[10] == 0  # no fringe passes
+    d = 2 ** level
+    ms, ns, ks = dims[6:9]
+    table = oracle.products(level)
+    touched = set()
+    for q, (A, B, C) in enumerate(table):
+        o = prods[q * 40:(q + 1) * 40]
+        assert (o[0], o[1], o[2]) == (len(A), len(B), len(C))
+        for si, (terms, ext_r, ext_c) in enumerate(((A, ms, ks), (B, ks, ns), (C, ms, ns))):
+            for t, (sub, coef) in enumerate(terms):
+                r = o[4 + si * 12 + t * 3:4 + si * 12 + t * 3 + 3]
+                assert (r[0], r[1], r[2]) == ((sub // d) * ext_r, (sub % d) * ext_c, coef)
+        mask = 0
+        for t, (sub, _) in enumerate(C):
+            if sub not in touched:
+                mask |= 1 << t
+                touched.add(sub)
+        assert int(o[3]) == mask
+
+
+@pytest.mark.parametrize("level", [1, 2])
+def test_fringe_passes_cover_remainder(level):
+    case = cases.make_case(9, m=(13,), n=(10,), k=(7,))
+    nprod, dims, _, prods = _debug_plan(case, level)
+    d = 2 ** level
+    m, n, k = dims[:3]
+    ms, ns, ks, ncore, nfringe = dims[6:11]
+    assert (ms, ns, ks) == (m // d, n // d, k // d)
+    assert nfringe == sum([m % d != 0, n % d != 0, k % d != 0])
+    _, dims_rf, _, _ = _debug_plan(case, level, flags=ts.FLAG_REF_FRINGE)
+    assert dims_rf[10] == 0
+
+
+def test_transpose_when_c_is_row_major():
+    # C stored with its n index unit-stride -> the plan computes C^T = B^T A^T
+    case = cases.make_case(3, m=(8,), n=(8,), k=(8,), permute=False)
+    strides = {"A": [1, 8], "B": [1, 8], "C": [8, 1]}
+    _, dims, vecs, _ = _debug_plan(case, 1, strides=strides)
+    assert dims[5] == 1
+    assert np.array_equal(vecs[4][:8], np.arange(8))  # rows of C^T are unit-stride
+
+
+def test_run_strides_bruteforce():
+    lib = ts.load()
+    rng = np.random.default_rng(0)
+    for trial in range(50):
+        n = int(rng.integers(1, 60))
+        run = int(rng.choice([4, 16, 32]))
+        # piecewise-affine vectors, like the scatter of a multi-index group
+        L0 = int(rng.integers(1, 9))
+        s0, s1 = int(rng.integers(1, 5)), int(rng.integers(5, 50))
+        v = np.array([(p % L0) * s0 + (p // L0) * s1 for p in range(n)], dtype=np.int64)
+        if trial % 5 == 0:
+            v = rng.permutation(v)
+        out = (ctypes.c_longlong * n)()
+        lib.mistra_debug_run_strides.argtypes = [ctypes.POINTER(ctypes.c_longlong), ctypes.c_longlong, ctypes.c_int,
+                                                 ctypes.POINTER(ctypes.c_longlong)]
+        lib.mistra_debug_run_strides(v.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)), n, run, out)
+        for p in range(n):
+            cnt = min(run, n - p)
+            if cnt <= 1:
+                continue
+            d = v[p + 1] - v[p]
+            affine = d != 0 and all(v[p + j] == v[p] + j * d for j in range(cnt))
+            assert out[p] == (d if affine else 0), (trial, p)
+
+
+def test_label_length_mismatch_is_an_error():
+    case = cases.make_case(1, m=(4,), n=(4,), k=(4,), permute=False)
+    case = dict(case)
+    case["shape_B"] = [5, 4]  # k has length 4 in A, 5 in B
+    nprod, _, _, _ = _debug_plan(case, 1)
+    assert nprod == -1  # MISTRA_ERR_ARG
