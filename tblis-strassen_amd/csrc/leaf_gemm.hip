@@ -29,6 +29,8 @@ struct Mfma<double>
     {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
+    // output element v of a lane sits in row lane/16 + 4*v, column lane%16 (measured on gfx950)
+    static __device__ __forceinline__ int row(int lane, int v) { return (lane >> 4) + 4 * v; }
 };
 
 template <>
@@ -39,6 +41,8 @@ struct Mfma<float>
     {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
+    // output element v of a lane sits in row 4*(lane/16) + v, column lane%16 (measured on gfx950)
+    static __device__ __forceinline__ int row(int lane, int v) { return 4 * (lane >> 4) + v; }
 };
 
 // Tile index -> (tm, tn).  Workgroups are dispatched round-robin over the 8 XCDs (bid % 8), so the
@@ -248,7 +252,7 @@ __global__ __launch_bounds__(256) void leaf_gemm_kernel(GemmArgs g)
             __syncthreads();
         }
 
-        // Epilogue.  MFMA 16x16 output layout: lane holds rows 4*(lane/16)+v, v<4, of column lane%16.
+        // Epilogue: element v of a lane's 16x16 block is (Mfma<T>::row(lane, v), lane % 16).
         if (EPI == EPI_MULTI_C)
         {
             const double alpha = g.alpha, beta = g.beta;
@@ -273,7 +277,7 @@ __global__ __launch_bounds__(256) void leaf_gemm_kernel(GemmArgs g)
 #pragma unroll
                             for (int v = 0; v < 4; v++)
                             {
-                                const int64_t m = row0 + wm0 + i * 16 + 4 * (lane >> 4) + v;
+                                const int64_t m = row0 + wm0 + i * 16 + Mfma<T>::row(lane, v);
                                 if (m >= ms) continue;
                                 T* ptr = Cg + g.c_m[cr.roff + m] + con;
                                 const T val = coef * acc[i][j][v];
@@ -300,7 +304,7 @@ __global__ __launch_bounds__(256) void leaf_gemm_kernel(GemmArgs g)
 #pragma unroll
                     for (int v = 0; v < 4; v++)
                     {
-                        const int64_t m = row0 + wm0 + i * 16 + 4 * (lane >> 4) + v;
+                        const int64_t m = row0 + wm0 + i * 16 + Mfma<T>::row(lane, v);
                         if (m < ms) Mp[n * ms + m] = acc[i][j][v];
                     }
             }

@@ -124,7 +124,8 @@ def make_tensor(x, scalar=1.0):
     t.type = _dtype_code(x)
     t.conj = 0
     buf = (ctypes.c_double(scalar) if t.type == TYPE_DOUBLE else ctypes.c_float(scalar))
-    ctypes.memmove(t.scalar, ctypes.addressof(buf), ctypes.sizeof(buf))
+    # (t.scalar would be a bytes copy: write through the struct's own address)
+    ctypes.memmove(ctypes.addressof(t) + TblisTensor.scalar.offset, ctypes.addressof(buf), ctypes.sizeof(buf))
     t.data = ptr
     t.ndim = len(shape)
     t._len = (ctypes.c_ssize_t * max(1, len(shape)))(*shape)

@@ -184,3 +184,15 @@ def test_label_length_mismatch_is_an_error():
     case["shape_B"] = [5, 4]  # k has length 4 in A, 5 in B
     nprod, _, _, _ = _debug_plan(case, 1)
     assert nprod == -1  # MISTRA_ERR_ARG
+
+
+def test_binding_writes_scalars_and_strides():
+    # tblis_tensor.scalar holds alpha/beta as the tensor's own type (src/util/basic_types.h:341-419)
+    x = np.zeros((3, 4), dtype=np.float64, order="F")
+    t = ts.make_tensor(x, 2.5)
+    raw = bytes(t)
+    assert np.frombuffer(raw[8:16], dtype=np.float64)[0] == 2.5
+    assert t.type == ts.TYPE_DOUBLE and t.ndim == 2 and [t.len[0], t.len[1]] == [3, 4]
+    assert [t.stride[0], t.stride[1]] == [1, 3]
+    t32 = ts.make_tensor(np.zeros((2, 2), dtype=np.float32), -0.5)
+    assert np.frombuffer(bytes(t32)[8:12], dtype=np.float32)[0] == -0.5
