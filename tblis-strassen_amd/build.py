@@ -20,7 +20,11 @@ SOURCES = [
     "driver.cpp",
     "api.cpp",
     "debug.cpp",
-    "leaf_gemm.hip",
+    "leaf_gemm.cpp",
+    "leaf_f64_multi.hip",
+    "leaf_f64_store.hip",
+    "leaf_f32_multi.hip",
+    "leaf_f32_store.hip",
     "aux_kernels.hip",
 ]
 

@@ -190,7 +190,7 @@ Status tensor_mult(const tblis::tblis_tensor* A, const label_type* idx_A, const 
     std::shared_ptr<Plan> plan;
     if (!empty_tensor(C) && alpha != 0.0 && !empty_tensor(A) && !empty_tensor(B))
     {
-        st = get_plan(A, idx_A, B, idx_B, C, idx_C, level, impl, TILE_128, flags, plan);
+        st = get_plan(A, idx_A, B, idx_B, C, idx_C, level, impl, -1, flags, plan);
         if (!st.ok()) return st;
     }
     else if (A->type != C->type || B->type != C->type)

@@ -13,7 +13,7 @@ int mistra_debug_plan(const tblis_tensor* A, const label_type* idx_A, const tbli
 {
     mistra::Plan p;
     mistra::Status st = mistra::build_plan_host(A, idx_A, B, idx_B, C, idx_C, level, mistra::IMPL_ABC,
-                                                mistra::TILE_128, flags, p);
+                                                -1, flags, p);
     if (!st.ok()) return st.code;
     const long long d[13] = {p.m, p.n, p.k, p.nbatch, p.general ? 0 : 1, p.transposed, p.ms, p.ns, p.ks,
                              p.nprod, (long long)p.fringe.size(), p.a_mfast, p.b_nfast};

@@ -13,6 +13,7 @@
 #include "driver.hpp"
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <list>
 #include <map>
@@ -100,6 +101,96 @@ ProdDesc single_desc(int64_t a_r, int64_t a_c, int64_t b_r, int64_t b_c, int64_t
 
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
+// Workgroup tile of the leaf kernel, per launch kind.  MISTRA_TILE=64|128 overrides (for tuning).
+// Measured on MI355X at m=n=k=8192 (FP64): launches with the multi-C epilogue (classical and ABC) and
+// the AB staging launch run best with 64x64 tiles (fewer registers: 3-4 waves per SIMD); the dense
+// Naive GEMM over contiguous S_p / T_p with 128x128 tiles.
+int forced_tile()
+{
+    static const int forced = [] {
+        const char* e = std::getenv("MISTRA_TILE");
+        if (!e) return -1;
+        return std::atoi(e) == 64 ? int(TILE_64) : std::atoi(e) == 128 ? int(TILE_128) : -1;
+    }();
+    return forced;
+}
+
+int choose_tile(const Plan& p)
+{
+    if (forced_tile() >= 0) return forced_tile();
+    (void)p;
+    return TILE_64;
+}
+
+int dense_tile(const Plan& p)
+{
+    if (forced_tile() >= 0) return forced_tile();
+    return p.ms * p.ns >= int64_t(1024) * 1024 ? TILE_128 : TILE_64;
+}
+
+// A launch may use the fast-K kernel when, for every A term and every B term of its products, the
+// K-run starting at each tile (term offset + j*TK) is affine with one stride common to all A terms
+// (and one common to all B terms), and every element offset stays within 31 bits so that per-element
+// byte offsets fit a 32-bit register.
+KFast kfast_check(const std::vector<ProdDesc>& prods, int first, int count, int64_t ks, int TK,
+                  const std::vector<int64_t>& a_krun, const std::vector<int64_t>& b_krun, int64_t a_span,
+                  int64_t b_span, int elem)
+{
+    KFast kf;
+    const int64_t limit = (int64_t(1) << 31) / elem;
+    if (a_span < 0 || b_span < 0 || a_span >= limit || b_span >= limit || ks <= 0) return kf;
+    int64_t sa = 0, sb = 0;
+    for (int q = first; q < first + count; q++)
+    {
+        const ProdDesc& pd = prods[size_t(q)];
+        for (int t = 0; t < pd.nA; t++)
+            for (int64_t k0 = 0; k0 < ks; k0 += TK)
+            {
+                const int64_t r = a_krun[size_t(pd.a[t].coff + k0)];
+                if (r == 0 || (sa != 0 && r != sa)) return kf;
+                sa = r;
+            }
+        for (int t = 0; t < pd.nB; t++)
+            for (int64_t k0 = 0; k0 < ks; k0 += TK)
+            {
+                const int64_t r = b_krun[size_t(pd.b[t].roff + k0)];
+                if (r == 0 || (sb != 0 && r != sb)) return kf;
+                sb = r;
+            }
+    }
+    // (TK-1) extra strides: the last elements of a tile are addressed from its first
+    if ((TK - 1) * (sa < 0 ? -sa : sa) >= limit - a_span || (TK - 1) * (sb < 0 ? -sb : sb) >= limit - b_span)
+        return kf;
+    kf.on = sa > 0 && sb > 0;
+    kf.sa = sa;
+    kf.sb = sb;
+    return kf;
+}
+
+// Largest element offset of an operand addressed through (rows, cols) scatter vectors, or -1 if any
+// entry is negative (negative strides are served by the gathered-K kernel).
+int64_t operand_span(const std::vector<int64_t>& rs, const std::vector<int64_t>& cs, const std::vector<int64_t>& batch)
+{
+    int64_t mr = 0, mc = 0, mb = 0;
+    for (int64_t v : rs)
+    {
+        if (v < 0) return -1;
+        mr = std::max(mr, v);
+    }
+    for (int64_t v : cs)
+    {
+        if (v < 0) return -1;
+        mc = std::max(mc, v);
+    }
+    for (int64_t v : batch)
+    {
+        if (v < 0) return -1;
+        mb = std::max(mb, v);
+    }
+    (void)mb;  // batch offsets are added to the 64-bit base pointer, not to the 32-bit offsets
+    return mr + mc;
+}
+
 }  // namespace
 
 Plan::~Plan()
@@ -169,6 +260,7 @@ Status build_plan_host(const tblis::tblis_tensor* A, const label_type* idx_A, co
     p.ns = p.n / p.d;
     p.ks = p.k / p.d;
     p.core = level > 0 && p.ms > 0 && p.ns > 0 && p.ks > 0;
+    if (p.tile < 0) p.tile = choose_tile(p);
     if (level > 0 && !p.core && (flags & MISTRA_FLAG_REF_FRINGE))
     {
         // the reference computes nothing when a leaf extent truncates to zero
@@ -248,6 +340,22 @@ Status upload(Plan& p)
 {
     const int TK = tile_kdepth(p.tile, p.elem);
     std::vector<int64_t> a_krun = run_strides(p.h_a_k, TK), b_krun = run_strides(p.h_b_k, TK);
+    {
+        const int64_t a_span = operand_span(p.h_a_m, p.h_a_k, p.h_batch_a);
+        const int64_t b_span = operand_span(p.h_b_k, p.h_b_n, p.h_batch_b);
+        p.kf_core = kfast_check(p.h_prods, 0, p.nprod, p.ks, TK, a_krun, b_krun, a_span, b_span, p.elem);
+        for (Pass& ps : p.fringe)
+            ps.kf = kfast_check(p.h_prods, ps.prod_index, 1, ps.ks, TK, a_krun, b_krun, a_span, b_span, p.elem);
+        if (p.core)
+        {
+            // dense S_p (ms x ks, column-major) and T_p (ks x ns): K strides ms and 1
+            const int64_t limit = (int64_t(1) << 31) / p.elem;
+            const int64_t s_span = int64_t(p.nprod) * p.ms * p.ks, t_span = int64_t(p.nprod) * p.ks * p.ns;
+            p.kf_dense.on = s_span + TK * p.ms < limit && t_span + TK < limit;
+            p.kf_dense.sa = p.ms;
+            p.kf_dense.sb = 1;
+        }
+    }
     std::vector<int64_t> s_m, s_k, t_k, t_n, s_krun, t_krun;
     if (p.core)
     {
@@ -287,11 +395,12 @@ Status upload(Plan& p)
          i_sm = add(s_m.data(), s_m.size() * 8), i_sk = add(s_k.data(), s_k.size() * 8),
          i_skr = add(s_krun.data(), s_krun.size() * 8), i_tk = add(t_k.data(), t_k.size() * 8),
          i_tkr = add(t_krun.data(), t_krun.size() * 8), i_tn = add(t_n.data(), t_n.size() * 8),
-         i_dp = add(p.h_dense_prods.data(), p.h_dense_prods.size() * sizeof(ProdDesc));
+         i_dp = add(p.h_dense_prods.data(), p.h_dense_prods.size() * sizeof(ProdDesc)),
+         i_zero = add(nullptr, 256);
 
     std::vector<unsigned char> staging(total, 0);
     for (const Item& it : items)
-        if (it.bytes) std::memcpy(staging.data() + it.off, it.src, it.bytes);
+        if (it.bytes && it.src) std::memcpy(staging.data() + it.off, it.src, it.bytes);
     void* d = nullptr;
     hipError_t e = hipMalloc(&d, total);
     if (e != hipSuccess) return Status::error(MISTRA_ERR_NOMEM, std::string("plan allocation: ") + hipGetErrorString(e));
@@ -326,6 +435,7 @@ Status upload(Plan& p)
     p.t_krun = reinterpret_cast<const int64_t*>(at(i_tkr));
     p.t_n = reinterpret_cast<const int64_t*>(at(i_tn));
     p.dense_prods = reinterpret_cast<const ProdDesc*>(at(i_dp));
+    p.zero = at(i_zero);
     return Status();
 }
 
@@ -394,6 +504,7 @@ GemmArgs base_args(const Plan& p, const Call& c)
     g.c_n = p.c_n;
     g.alpha = c.alpha;
     g.beta = c.beta;
+    g.zero = p.zero;
     g.nbatch = p.nbatch;
     g.batch_a = p.nbatch > 1 ? p.batch_a : nullptr;
     g.batch_b = p.nbatch > 1 ? p.batch_b : nullptr;
@@ -425,10 +536,18 @@ Status run_plan(const Plan& p, const Call& c, hipStream_t stream)
         g.ms = p.ms;
         g.ns = p.ns;
         g.ks = p.ks;
-        g.prods = p.prods;
-        g.nprod = p.nprod;
+        g.ka_stride = p.kf_core.sa;
+        g.kb_stride = p.kf_core.sb;
+        g.nprod = 1;
+        const int terms = max_terms(p.h_prods.data(), p.nprod);
         timing_begin(stream);
-        HIP_TRY(launch_leaf_gemm(p.elem, EPI_MULTI_C, p.tile, g, p.a_mfast, p.b_nfast, stream));
+        // the products in table order, one launch each: stream order serializes their C updates
+        for (int q = 0; q < p.nprod; q++)
+        {
+            g.prods = p.prods + q;
+            HIP_TRY(launch_leaf_gemm(p.elem, EPI_MULTI_C, p.tile, terms, p.kf_core.on, g, p.a_mfast, p.b_nfast,
+                                     stream));
+        }
         timing_end(stream, leaf_flops(p, p.nprod, p.ms, p.ns, p.ks));
     }
     else if (p.impl == IMPL_AB)
@@ -442,8 +561,11 @@ Status run_plan(const Plan& p, const Call& c, hipStream_t stream)
         g.prods = p.prods;
         g.nprod = p.nprod;
         g.M = M;
+        g.ka_stride = p.kf_core.sa;
+        g.kb_stride = p.kf_core.sb;
         timing_begin(stream);
-        hipError_t e = launch_leaf_gemm(p.elem, EPI_STORE, p.tile, g, p.a_mfast, p.b_nfast, stream);
+        hipError_t e = launch_leaf_gemm(p.elem, EPI_STORE, p.tile, max_terms(p.h_prods.data(), p.nprod),
+                                        p.kf_core.on, g, p.a_mfast, p.b_nfast, stream);
         timing_end(stream, leaf_flops(p, p.nprod, p.ms, p.ns, p.ks));
         if (e == hipSuccess)
         {
@@ -527,8 +649,11 @@ Status run_plan(const Plan& p, const Call& c, hipStream_t stream)
             gd.nprod = p.nprod;
             gd.nbatch = 1;
             gd.M = M;
+            gd.zero = p.zero;
+            gd.ka_stride = p.kf_dense.sa;
+            gd.kb_stride = p.kf_dense.sb;
             timing_begin(stream);
-            e = launch_leaf_gemm(p.elem, EPI_STORE, p.tile, gd, true, false, stream);
+            e = launch_leaf_gemm(p.elem, EPI_STORE, dense_tile(p), 1, p.kf_dense.on, gd, true, false, stream);
             timing_end(stream, 2.0 * double(p.nprod) * double(p.ms) * double(p.ns) * double(p.ks));
             if (e != hipSuccess) break;
 
@@ -567,7 +692,9 @@ Status run_plan(const Plan& p, const Call& c, hipStream_t stream)
         f.ks = ps.ks;
         f.prods = p.prods + ps.prod_index;
         f.nprod = 1;
-        HIP_TRY(launch_leaf_gemm(p.elem, EPI_MULTI_C, TILE_64, f, p.a_mfast, p.b_nfast, stream));
+        f.ka_stride = ps.kf.sa;
+        f.kb_stride = ps.kf.sb;
+        HIP_TRY(launch_leaf_gemm(p.elem, EPI_MULTI_C, TILE_64, 1, ps.kf.on, f, p.a_mfast, p.b_nfast, stream));
     }
     return Status();
 }

@@ -34,10 +34,18 @@ enum Impl
     IMPL_AB = 4
 };
 
+// Fast-K eligibility of one launch (see kfast_check in driver.cpp).
+struct KFast
+{
+    bool on = false;
+    int64_t sa = 0, sb = 0;  // common K-run strides of A and B
+};
+
 struct Pass  // one classical launch over a sub-range (the fringe the 2^L split leaves over)
 {
     int64_t ms, ns, ks;
     int prod_index;  // into the device product array
+    KFast kf;
 };
 
 struct Plan
@@ -69,6 +77,9 @@ struct Plan
     // Naive: dense products over S_p (ms x ks) and T_p (ks x ns)
     std::vector<ProdDesc> h_dense_prods;
 
+    // fast-K eligibility of the core launch and of the Naive dense launch
+    KFast kf_core, kf_dense;
+
     // device mirror (one allocation)
     void* dmem = nullptr;
     const int64_t *a_m = nullptr, *a_k = nullptr, *a_krun = nullptr, *b_k = nullptr, *b_krun = nullptr,
@@ -81,6 +92,7 @@ struct Plan
     const int64_t *s_m = nullptr, *s_k = nullptr, *s_krun = nullptr, *t_k = nullptr, *t_krun = nullptr,
                   *t_n = nullptr;
     const ProdDesc* dense_prods = nullptr;
+    const void* zero = nullptr;  // 256 zeroed bytes
 
     // general (non-pure) contractions: reduce/broadcast descriptors
     Grouping grouping;

@@ -62,6 +62,11 @@ struct GemmArgs
     const int64_t* batch_c;
     // staging buffer of EPI_STORE
     void* M;
+    // a few zeroed bytes in device memory, read by lanes whose position lies outside the leaf
+    const void* zero;
+    // fast K mode (chosen by the host per launch): every term's K-tiles are affine with these strides
+    // and every element offset fits in 31 bits
+    int64_t ka_stride, kb_stride;
 };
 
 }  // namespace mistra

@@ -18,9 +18,13 @@ enum TileCfg
 int tile_extent(int tile);
 int tile_kdepth(int tile, int elem_size);
 
-// Leaf GEMM over scatter vectors with fused operand sums (leaf_gemm.hip).
-hipError_t launch_leaf_gemm(int elem_size, int epi, int tile, GemmArgs g, bool a_mfast, bool b_nfast,
-                            hipStream_t stream);
+// Leaf GEMM over scatter vectors with fused operand sums (leaf_gemm_kernel.hpp); `terms` is the
+// largest number of sub-blocks in any operand sum of the launch (1, 2 or 4).
+// `kfast`: every term's K-tiles are affine with g.ka_stride / g.kb_stride (host-checked, see
+// kfast_check in the driver); otherwise the gathered-K variant runs.
+hipError_t launch_leaf_gemm(int elem_size, int epi, int tile, int terms, bool kfast, GemmArgs g, bool a_mfast,
+                            bool b_nfast, hipStream_t stream);
+int max_terms(const ProdDesc* prods, int nprod);
 
 // Explicit operand sums (Naive impl): for product p and its terms t,
 //   out[p*rows*cols + r + c*rows] = sum_t coeff_t * X[base_t + rs[roff_t + r] + cs[coff_t + c]]

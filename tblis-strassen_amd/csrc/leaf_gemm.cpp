@@ -1,0 +1,54 @@
+// Host side of the leaf GEMM: grid shape and the per-type dispatch (kernels in leaf_f64.hip / leaf_f32.hip).
+#include "launch.hpp"
+
+namespace mistra {
+
+#define MISTRA_DECLARE_LEAF(name)                                                                     \
+    hipError_t name(int tile, int terms, bool kfast, const GemmArgs& g, bool a_mfast, bool b_nfast, hipStream_t s, \
+                    dim3 grid);
+MISTRA_DECLARE_LEAF(launch_leaf_f64_multi)
+MISTRA_DECLARE_LEAF(launch_leaf_f64_store)
+MISTRA_DECLARE_LEAF(launch_leaf_f32_multi)
+MISTRA_DECLARE_LEAF(launch_leaf_f32_store)
+#undef MISTRA_DECLARE_LEAF
+
+int tile_extent(int tile) { return tile == TILE_64 ? 64 : 128; }
+
+int tile_kdepth(int tile, int elem_size)
+{
+    (void)tile;
+    return elem_size == 8 ? 16 : 32;
+}
+
+int max_terms(const ProdDesc* prods, int nprod)
+{
+    int t = 1;
+    for (int p = 0; p < nprod; p++)
+    {
+        if (prods[p].nA > t) t = prods[p].nA;
+        if (prods[p].nB > t) t = prods[p].nB;
+    }
+    return t;
+}
+
+hipError_t launch_leaf_gemm(int elem_size, int epi, int tile, int terms, bool kfast, GemmArgs g, bool a_mfast,
+                            bool b_nfast, hipStream_t stream)
+{
+    if (!kfast) tile = TILE_64;  // the gathered-K variant exists with the 64x64 tile only
+    const int te = tile_extent(tile);
+    g.tiles_m = int((g.ms + te - 1) / te);
+    g.tiles_n = int((g.ns + te - 1) / te);
+    if (g.tiles_m == 0 || g.tiles_n == 0 || g.nbatch == 0 || g.nprod == 0) return hipSuccess;
+    if (g.ks <= 0) return hipErrorInvalidValue;  // callers route empty K to a scale of C
+    // one product per workgroup row: grid.y indexes the launch's products; a multi-C launch carries
+    // a single product (products that share C sub-blocks must run one after the other)
+    if (epi == EPI_MULTI_C && g.nprod != 1) return hipErrorInvalidValue;
+    dim3 grid(unsigned(g.tiles_m) * unsigned(g.tiles_n), unsigned(g.nprod), unsigned(g.nbatch));
+    if (elem_size == 8)
+        return epi == EPI_STORE ? launch_leaf_f64_store(tile, terms, kfast, g, a_mfast, b_nfast, stream, grid)
+                                : launch_leaf_f64_multi(tile, terms, kfast, g, a_mfast, b_nfast, stream, grid);
+    return epi == EPI_STORE ? launch_leaf_f32_store(tile, terms, kfast, g, a_mfast, b_nfast, stream, grid)
+                            : launch_leaf_f32_multi(tile, terms, kfast, g, a_mfast, b_nfast, stream, grid);
+}
+
+}  // namespace mistra

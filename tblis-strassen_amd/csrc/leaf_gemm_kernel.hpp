@@ -1,0 +1,532 @@
+// Leaf-product GEMM kernel template of the Strassen tensor contraction, for gfx950 (CDNA4) MFMA.
+// Instantiated per element type in leaf_f64.hip / leaf_f32.hip.
+//
+// One kernel serves every level and impl:
+//   * operands are gathered through device scatter vectors (the reference's block-scatter matrices,
+//     src/matrix/stra_block_scatter_matrix.hpp) and the Strassen operand sums sum_i c_i A_i are
+//     formed while staging the K-tile into LDS -- the fused packing of
+//     src/stra_nodes/stra_packm.hpp:367-683 and the stra_pack_*_ukr of src/kernels/3m/gemm.hpp:505-2227;
+//   * the register tile is accumulated with v_mfma_f64_16x16x4_f64 (or the f32 form for float);
+//   * EPI_MULTI_C scatters the product into every C sub-block it feeds, C_i += alpha*coeff_i*M (the
+//     multi-C micro-kernel of src/stra_nodes/stra_gemm_ukr.hpp:626-930), applying beta exactly once per
+//     C sub-block; EPI_STORE writes the product to a staging buffer (AB and Naive impls).
+// ABC issues the 7^L products as consecutive launches on one stream (the serialized products of
+// src/stra_internal/3t/stra_mult.cxx:445-592), so no two workgroups of a launch write the same C
+// element and no atomics are needed.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+#include "launch.hpp"
+
+namespace mistra {
+
+template <typename T>
+struct Mfma;
+
+template <>
+struct Mfma<double>
+{
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 run(double a, double b, v4 c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    // output element v of a lane sits in row lane/16 + 4*v, column lane%16 (measured on gfx950)
+    static __device__ __forceinline__ int row(int lane, int v) { return (lane >> 4) + 4 * v; }
+};
+
+template <>
+struct Mfma<float>
+{
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 run(float a, float b, v4 c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    // output element v of a lane sits in row 4*(lane/16) + v, column lane%16 (measured on gfx950)
+    static __device__ __forceinline__ int row(int lane, int v) { return 4 * (lane >> 4) + v; }
+};
+
+// Tile index -> (tm, tn).  Workgroups are dispatched round-robin over the 8 XCDs (bid % 8), so the
+// first remap gives each XCD a contiguous range of tiles; inside the range tiles are visited in
+// groups of kGroupM tile-rows so the XCD's L2 holds a few A row-panels and B column-panels.
+__device__ __forceinline__ void tile_coords(int bid, int tiles_m, int tiles_n, int& tm, int& tn)
+{
+    constexpr int kXcd = 8, kGroupM = 4;
+    const int tiles = tiles_m * tiles_n;
+    const int per = tiles / kXcd;
+    int tile = bid < per * kXcd ? (bid % kXcd) * per + bid / kXcd : bid;
+    const int in_group = kGroupM * tiles_n;
+    const int group = tile / in_group;
+    const int first_m = group * kGroupM;
+    const int gsz = min(tiles_m - first_m, kGroupM);
+    const int r = tile - group * in_group;
+    tm = first_m + r % gsz;
+    tn = r / gsz;
+}
+
+// Kernel-start inputs (batch offsets) are read through the constant address space: with a uniform
+// address that becomes a scalar load into SGPRs.  Inside the K loop scalar loads are avoided: they
+// share the lgkmcnt counter with LDS reads and return out of order, so any wait for an LDS read would
+// also wait for them.
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(4))) T* cptr(const T* p)
+{
+    return (const __attribute__((address_space(4))) T*)(p);
+}
+
+__device__ __forceinline__ int64_t lmin(int64_t a, int64_t b) { return a < b ? a : b; }
+
+// Lane `l`'s copy of a 64-bit value, in a scalar register.
+__device__ __forceinline__ int64_t readlane64(int64_t x, int l)
+{
+    const int lo = __builtin_amdgcn_readlane(int(uint64_t(x) & 0xffffffffu), l);
+    const int hi = __builtin_amdgcn_readlane(int(uint64_t(x) >> 32), l);
+    return int64_t((uint64_t(uint32_t(hi)) << 32) | uint64_t(uint32_t(lo)));
+}
+
+template <typename T, int TM, int TN, int TK, int TERMS, bool A_MFAST, bool B_NFAST, int EPI, bool KFAST>
+__global__ __launch_bounds__(256) void leaf_gemm_kernel(GemmArgs g)
+{
+    constexpr int NT = 256;
+    constexpr int WM = TM / 2, WN = TN / 2;  // 2x2 waves
+    constexpr int FM = WM / 16, FN = WN / 16;
+    constexpr int LDA = TM + 1, LDB = TN + 1;  // +1 element: conflict-free K-fast LDS stores
+    constexpr int EA = TM * TK / NT, EB = TN * TK / NT;
+    constexpr int RA = A_MFAST ? 1 : EA, RB = B_NFAST ? 1 : EB;  // row (col) positions per thread
+    static_assert(EA >= 1 && EB >= 1, "tile too small for 256 threads");
+    static_assert(NT % TM == 0 && NT % TN == 0 && NT % TK == 0, "loader layout");
+    typedef typename Mfma<T>::v4 v4;
+
+    __shared__ T sA[2][TK * LDA];
+    __shared__ T sB[2][TK * LDB];
+    __shared__ int64_t ccol[EPI == EPI_MULTI_C ? kMaxTerms : 1][TN];  // C column offsets per target
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int wm0 = (wave & 1) * WM;
+    const int wn0 = (wave >> 1) * WN;
+
+    int tm, tn;
+    tile_coords(blockIdx.x, g.tiles_m, g.tiles_n, tm, tn);
+    const int64_t row0 = int64_t(tm) * TM;
+    const int64_t col0 = int64_t(tn) * TN;
+    const int64_t ms = g.ms, ns = g.ns, ks = g.ks;
+
+    const int64_t z = blockIdx.z;
+    const T* __restrict__ Ag = static_cast<const T*>(g.A) + (g.batch_a ? cptr(g.batch_a)[z] : 0);
+    const T* __restrict__ Bg = static_cast<const T*>(g.B) + (g.batch_b ? cptr(g.batch_b)[z] : 0);
+    T* __restrict__ Cg = static_cast<T*>(g.C) + (g.batch_c ? cptr(g.batch_c)[z] : 0);
+    const T* __restrict__ zero = static_cast<const T*>(g.zero);
+
+    // Loader positions inside the tile.  A: M-fast -> consecutive lanes walk m (coalesced when m has
+    // unit stride); else lanes walk k.  B likewise with n.
+    const int a_m = A_MFAST ? tid % TM : tid / TK;
+    const int a_k = A_MFAST ? tid / TM : tid % TK;
+    constexpr int A_MSTEP = A_MFAST ? 0 : NT / TK;
+    constexpr int A_KSTEP = A_MFAST ? NT / TM : 0;
+    const int b_n = B_NFAST ? tid % TN : tid / TK;
+    const int b_k = B_NFAST ? tid / TN : tid % TK;
+    constexpr int B_NSTEP = B_NFAST ? 0 : NT / TK;
+    constexpr int B_KSTEP = B_NFAST ? NT / TN : 0;
+
+    // One product per workgroup: blockIdx.y selects it among the launch's products.  (ABC launches
+    // one product at a time, in stream order, so that products updating the same C sub-block are
+    // serialized; a loop over products inside the kernel costs registers in the main loop.)
+    {
+        const int p = blockIdx.y;
+        const ProdDesc& P = g.prods[p];
+        const int nA = __builtin_amdgcn_readfirstlane(P.nA), nB = __builtin_amdgcn_readfirstlane(P.nB);
+
+        // Row offsets of A (per sub-block) and column offsets of B: fixed for the whole K loop.
+        // Positions outside the leaf are clamped to a valid one and flagged; such lanes later read
+        // the zero block, so every load is issued unconditionally and its value used as is.
+        int64_t aro[TERMS][RA], bco[TERMS][RB];
+        bool arow_ok[RA], bcol_ok[RB];
+        T acoef[TERMS], bcoef[TERMS];
+#pragma unroll
+        for (int e = 0; e < RA; e++) arow_ok[e] = row0 + a_m + e * A_MSTEP < ms;
+#pragma unroll
+        for (int e = 0; e < RB; e++) bcol_ok[e] = col0 + b_n + e * B_NSTEP < ns;
+#pragma unroll
+        for (int a = 0; a < TERMS; a++)
+        {
+            const bool on = a < nA;
+            acoef[a] = on ? T(P.a[a].coeff) : T(0);
+            const int64_t base = on ? P.a[a].base : 0, roff = on ? P.a[a].roff : 0;
+#pragma unroll
+            for (int e = 0; e < RA; e++)
+                aro[a][e] = base + g.a_m[roff + lmin(row0 + a_m + e * A_MSTEP, ms - 1)];
+        }
+#pragma unroll
+        for (int b = 0; b < TERMS; b++)
+        {
+            const bool on = b < nB;
+            bcoef[b] = on ? T(P.b[b].coeff) : T(0);
+            const int64_t base = on ? P.b[b].base : 0, coff = on ? P.b[b].coff : 0;
+#pragma unroll
+            for (int e = 0; e < RB; e++)
+                bco[b][e] = base + g.b_n[coff + lmin(col0 + b_n + e * B_NSTEP, ns - 1)];
+        }
+
+        // Fast mode (KFAST, decided per launch by the host): every term's K runs are affine with the same
+        // stride at every tile start (g.ka_stride / g.kb_stride) and all offsets are non-negative and
+        // below 2^31 elements.  Each element's byte offset relative to its tile's first K offset is
+        // then fixed for the whole product: full K-tiles load from (uniform base + 32-bit offset) with
+        // no per-element arithmetic.
+        constexpr bool kfast = KFAST;
+        uint32_t aoff[TERMS][EA], boff[TERMS][EB];
+        if (kfast)
+        {
+#pragma unroll
+            for (int a = 0; a < TERMS; a++)
+#pragma unroll
+                for (int e = 0; e < EA; e++)
+                    aoff[a][e] = uint32_t((aro[a][A_MFAST ? 0 : e] + int64_t(a_k + e * A_KSTEP) * g.ka_stride) *
+                                          int64_t(sizeof(T)));
+#pragma unroll
+            for (int b = 0; b < TERMS; b++)
+#pragma unroll
+                for (int e = 0; e < EB; e++)
+                    boff[b][e] = uint32_t((bco[b][B_NFAST ? 0 : e] + int64_t(b_k + e * B_KSTEP) * g.kb_stride) *
+                                          int64_t(sizeof(T)));
+        }
+
+        // K-run info of a K-tile.  Lane t < TERMS holds the block-scatter stride of A term t's run
+        // starting at the tile (0 = not affine), lane TERMS+t the offset of its first element, lanes
+        // 2*TERMS+t and 3*TERMS+t the same for B.  One vector load per tile fetches all of it, one tile
+        // ahead of use (vector loads retire in order, so waiting for a tile's data never waits for
+        // it), and v_readlane moves each entry to a scalar register when it is used.  (Loads from a
+        // uniform address would become scalar loads, whose out-of-order lgkmcnt would make every LDS
+        // wait of the MFMA block wait for them too.)
+        const int kw = lane < 4 * TERMS ? lane / TERMS : 3;
+        const int kt_term = lane % TERMS;
+        const int64_t kpos = kw < 2 ? (kt_term < nA ? P.a[kt_term].coff : 0) : (kt_term < nB ? P.b[kt_term].roff : 0);
+        const int64_t* ksrc = kw == 0 ? g.a_krun : kw == 1 ? g.a_k : kw == 2 ? g.b_krun : g.b_k;
+        int64_t kinfo = 0;
+        auto fetch_kinfo = [&](int64_t k0) { kinfo = ksrc[kpos + k0]; };
+
+        T ra[TERMS][EA], rb[TERMS][EB];
+
+        // Issue the global loads of one K-tile (raw values of every sub-block; they are summed after
+        // the MFMA work of the previous tile, so the loads stay in flight across it).  The affine and
+        // gathered forms are whole uniform branches, so no per-element control flow (and no wait at
+        // its join) is generated.
+        auto load_tile = [&](int64_t k0) {
+            if (kfast)
+            {
+                // a partial last tile reads the zero block past the leaf's K extent
+                const int klim = int(lmin(ks - 1 - k0, TK - 1));
+                const bool full = klim == TK - 1;
+#pragma unroll
+                for (int a = 0; a < TERMS; a++)
+                {
+                    if (a < nA)
+                    {
+                        const char* base = reinterpret_cast<const char*>(Ag + readlane64(kinfo, TERMS + a));
+#pragma unroll
+                        for (int e = 0; e < EA; e++)
+                            ra[a][e] = *((full || a_k + e * A_KSTEP <= klim) ? reinterpret_cast<const T*>(base + aoff[a][e])
+                                                                             : zero);
+                    }
+                }
+#pragma unroll
+                for (int b = 0; b < TERMS; b++)
+                {
+                    if (b < nB)
+                    {
+                        const char* base = reinterpret_cast<const char*>(Bg + readlane64(kinfo, 3 * TERMS + b));
+#pragma unroll
+                        for (int e = 0; e < EB; e++)
+                            rb[b][e] = *((full || b_k + e * B_KSTEP <= klim) ? reinterpret_cast<const T*>(base + boff[b][e])
+                                                                             : zero);
+                    }
+                }
+                return;
+            }
+            const int klim = int(lmin(ks - 1 - k0, TK - 1));  // last valid k inside this tile
+            // All run info first (its load is a tile old, so nothing newer is waited for), then one
+            // uniform branch: every term affine (the common case) or the general per-term code.
+            int64_t arun[TERMS], ac0[TERMS], brun[TERMS], br0[TERMS];
+            bool affine = true;
+#pragma unroll
+            for (int a = 0; a < TERMS; a++)
+            {
+                arun[a] = readlane64(kinfo, a);
+                ac0[a] = readlane64(kinfo, TERMS + a);
+                affine = affine && (a >= nA || arun[a] != 0);
+            }
+#pragma unroll
+            for (int b = 0; b < TERMS; b++)
+            {
+                brun[b] = readlane64(kinfo, 2 * TERMS + b);
+                br0[b] = readlane64(kinfo, 3 * TERMS + b);
+                affine = affine && (b >= nB || brun[b] != 0);
+            }
+            if (affine)
+            {
+#pragma unroll
+                for (int a = 0; a < TERMS; a++)
+                {
+                    if (a < nA)
+                    {
+#pragma unroll
+                        for (int e = 0; e < EA; e++)
+                        {
+                            const int kk = a_k + e * A_KSTEP;
+                            const int kc = kk < klim ? kk : klim;
+                            const bool ok = arow_ok[A_MFAST ? 0 : e] && kk <= klim;
+                            ra[a][e] = *(ok ? Ag + aro[a][A_MFAST ? 0 : e] + ac0[a] + int64_t(kc) * arun[a] : zero);
+                        }
+                    }
+                }
+#pragma unroll
+                for (int b = 0; b < TERMS; b++)
+                {
+                    if (b < nB)
+                    {
+#pragma unroll
+                        for (int e = 0; e < EB; e++)
+                        {
+                            const int kk = b_k + e * B_KSTEP;
+                            const int kc = kk < klim ? kk : klim;
+                            const bool ok = bcol_ok[B_NFAST ? 0 : e] && kk <= klim;
+                            rb[b][e] = *(ok ? Bg + bco[b][B_NFAST ? 0 : e] + br0[b] + int64_t(kc) * brun[b] : zero);
+                        }
+                    }
+                }
+                return;
+            }
+#pragma unroll
+            for (int a = 0; a < TERMS; a++)
+            {
+                if (a < nA)
+                {
+                    const int64_t run = arun[a];
+                    if (run != 0)
+                    {
+                        const int64_t c0 = ac0[a];
+#pragma unroll
+                        for (int e = 0; e < EA; e++)
+                        {
+                            const int kk = a_k + e * A_KSTEP;
+                            const int kc = kk < klim ? kk : klim;
+                            const bool ok = arow_ok[A_MFAST ? 0 : e] && kk <= klim;
+                            ra[a][e] = *(ok ? Ag + aro[a][A_MFAST ? 0 : e] + c0 + int64_t(kc) * run : zero);
+                        }
+                    }
+                    else
+                    {
+                        const int64_t kb = readlane64(kpos, TERMS + a) + k0;
+                        int64_t co[EA];
+#pragma unroll
+                        for (int e = 0; e < EA; e++)
+                        {
+                            const int kk = a_k + e * A_KSTEP;
+                            co[e] = g.a_k[kb + (kk < klim ? kk : klim)];
+                        }
+#pragma unroll
+                        for (int e = 0; e < EA; e++)
+                        {
+                            const bool ok = arow_ok[A_MFAST ? 0 : e] && a_k + e * A_KSTEP <= klim;
+                            ra[a][e] = *(ok ? Ag + aro[a][A_MFAST ? 0 : e] + co[e] : zero);
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int b = 0; b < TERMS; b++)
+            {
+                if (b < nB)
+                {
+                    const int64_t run = brun[b];
+                    if (run != 0)
+                    {
+                        const int64_t r0 = br0[b];
+#pragma unroll
+                        for (int e = 0; e < EB; e++)
+                        {
+                            const int kk = b_k + e * B_KSTEP;
+                            const int kc = kk < klim ? kk : klim;
+                            const bool ok = bcol_ok[B_NFAST ? 0 : e] && kk <= klim;
+                            rb[b][e] = *(ok ? Bg + bco[b][B_NFAST ? 0 : e] + r0 + int64_t(kc) * run : zero);
+                        }
+                    }
+                    else
+                    {
+                        const int64_t kb = readlane64(kpos, 3 * TERMS + b) + k0;
+                        int64_t ro[EB];
+#pragma unroll
+                        for (int e = 0; e < EB; e++)
+                        {
+                            const int kk = b_k + e * B_KSTEP;
+                            ro[e] = g.b_k[kb + (kk < klim ? kk : klim)];
+                        }
+#pragma unroll
+                        for (int e = 0; e < EB; e++)
+                        {
+                            const bool ok = bcol_ok[B_NFAST ? 0 : e] && b_k + e * B_KSTEP <= klim;
+                            rb[b][e] = *(ok ? Bg + bco[b][B_NFAST ? 0 : e] + ro[e] : zero);
+                        }
+                    }
+                }
+            }
+        };
+
+        // Form the Strassen sums in registers and store the K-tile to LDS as [k][m] / [k][n].
+        auto store_tile = [&](int buf) {
+#pragma unroll
+            for (int e = 0; e < EA; e++)
+            {
+                T v = acoef[0] * ra[0][e];
+#pragma unroll
+                for (int a = 1; a < TERMS; a++)
+                    if (a < nA) v += acoef[a] * ra[a][e];
+                sA[buf][(a_k + e * A_KSTEP) * LDA + a_m + e * A_MSTEP] = arow_ok[A_MFAST ? 0 : e] ? v : T(0);
+            }
+#pragma unroll
+            for (int e = 0; e < EB; e++)
+            {
+                T v = bcoef[0] * rb[0][e];
+#pragma unroll
+                for (int b = 1; b < TERMS; b++)
+                    if (b < nB) v += bcoef[b] * rb[b][e];
+                sB[buf][(b_k + e * B_KSTEP) * LDB + b_n + e * B_NSTEP] = bcol_ok[B_NFAST ? 0 : e] ? v : T(0);
+            }
+        };
+
+        v4 acc[FM][FN];
+#pragma unroll
+        for (int i = 0; i < FM; i++)
+#pragma unroll
+            for (int j = 0; j < FN; j++) acc[i][j] = v4{0, 0, 0, 0};
+
+        const int64_t nkt = (ks + TK - 1) / TK;
+        fetch_kinfo(0);
+        load_tile(0);
+        if (nkt > 1) fetch_kinfo(TK);
+        store_tile(0);
+        __syncthreads();
+
+        for (int64_t kt = 0; kt < nkt; kt++)
+        {
+            const int cur = int(kt & 1);
+            if (kt + 1 < nkt)
+            {
+                load_tile((kt + 1) * TK);
+                if (kt + 2 < nkt) fetch_kinfo((kt + 2) * TK);
+            }
+            const T* __restrict__ la = sA[cur];
+            const T* __restrict__ lb = sB[cur];
+#pragma unroll
+            for (int k4 = 0; k4 < TK / 4; k4++)
+            {
+                const int kr = k4 * 4 + (lane >> 4);
+                T af[FM], bf[FN];
+#pragma unroll
+                for (int i = 0; i < FM; i++) af[i] = la[kr * LDA + wm0 + i * 16 + (lane & 15)];
+#pragma unroll
+                for (int j = 0; j < FN; j++) bf[j] = lb[kr * LDB + wn0 + j * 16 + (lane & 15)];
+#pragma unroll
+                for (int i = 0; i < FM; i++)
+#pragma unroll
+                    for (int j = 0; j < FN; j++) acc[i][j] = Mfma<T>::run(af[i], bf[j], acc[i][j]);
+            }
+            if (kt + 1 < nkt) store_tile(cur ^ 1);
+            __syncthreads();
+        }
+
+        // Epilogue: element v of a lane's 16x16 block is (Mfma<T>::row(lane, v), lane % 16).
+        int n_pos[FN];
+        bool n_ok[FN];
+#pragma unroll
+        for (int j = 0; j < FN; j++)
+        {
+            const int64_t n = col0 + wn0 + j * 16 + (lane & 15);
+            n_ok[j] = n < ns;
+            n_pos[j] = int(lmin(n, ns - 1));
+        }
+        if (EPI == EPI_MULTI_C)
+        {
+            // Multi-C update through LDS: for each 16-row block i, the waves write their accumulators
+            // (straight from the accumulator registers) into a 32 x TN staging tile, then all 256
+            // threads update every C target from it, consecutive threads on consecutive rows (C's
+            // unit-stride dimension after the plan's orientation).  Keeps the accumulators out of the
+            // VGPRs, so the main loop keeps the store epilogue's register budget.
+            constexpr int LDE = TN + 1;
+            constexpr int RROWS = 32, CSTEP = NT / RROWS, CPER = TN / CSTEP;
+            static_assert(2 * 16 * LDE <= 2 * TK * (LDA + LDB), "staging tile must fit the K buffers");
+            T* stage = sA[0];  // the K-tile buffers are free after the final barrier of the K loop
+            const double alpha = g.alpha, beta = g.beta;
+            const int nC = P.nC;
+            const int first_mask = P.first_mask;
+            // column offsets of every target, for this tile
+            for (int x = tid; x < nC * TN; x += NT)
+            {
+                const int c = x / TN, col = x % TN;
+                ccol[c][col] = P.c[c].base + g.c_n[P.c[c].coff + lmin(col0 + col, ns - 1)];
+            }
+            const int rl = tid % RROWS, cg = tid / RROWS;
+#pragma unroll 1
+            for (int i = 0; i < FM; i++)
+            {
+                // static accumulator indices (a uniform branch per block) keep acc in registers
+#pragma unroll
+                for (int ii = 0; ii < FM; ii++)
+                    if (ii == i)
+                    {
+#pragma unroll
+                        for (int j = 0; j < FN; j++)
+#pragma unroll
+                            for (int v = 0; v < 4; v++)
+                                stage[((wave & 1) * 16 + Mfma<T>::row(lane, v)) * LDE + wn0 + j * 16 + (lane & 15)] =
+                                    acc[ii][j][v];
+                    }
+                __syncthreads();
+                const int64_t grow = row0 + (rl / 16) * WM + i * 16 + (rl % 16);
+                const bool rok = grow < ms;
+                const int64_t growc = lmin(grow, ms - 1);
+#pragma unroll 1
+                for (int c = 0; c < nC; c++)
+                {
+                    const T coef = T(alpha * P.c[c].coeff);
+                    const bool first = (first_mask >> c) & 1;
+                    const bool read_c = !(first && beta == 0.0);
+                    const T bet = first ? T(beta) : T(1);
+                    T* Crow = Cg + g.c_m[P.c[c].roff + growc];
+                    T old[CPER];
+#pragma unroll
+                    for (int e = 0; e < CPER; e++) old[e] = read_c ? Crow[ccol[c][cg + e * CSTEP]] : T(0);
+#pragma unroll
+                    for (int e = 0; e < CPER; e++)
+                    {
+                        const int col = cg + e * CSTEP;
+                        const T val = coef * stage[rl * LDE + col];
+                        const T out = read_c ? bet * old[e] + val : val;
+                        if (rok && col0 + col < ns) Crow[ccol[c][col]] = out;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        else
+        {
+            T* Mp = static_cast<T*>(g.M) + (z * g.nprod + p) * ms * ns;
+#pragma unroll
+            for (int i = 0; i < FM; i++)
+#pragma unroll
+                for (int v = 0; v < 4; v++)
+                {
+                    const int64_t m = row0 + wm0 + i * 16 + Mfma<T>::row(lane, v);
+#pragma unroll
+                    for (int j = 0; j < FN; j++)
+                        if (m < ms && n_ok[j]) Mp[int64_t(n_pos[j]) * ms + m] = acc[i][j][v];
+                }
+        }
+    }
+}
+
+}  // namespace mistra
