@@ -540,15 +540,15 @@ Status run_plan(const Plan& p, const Call& c, hipStream_t stream)
         g.kb_stride = p.kf_core.sb;
         g.nprod = 1;
         const int terms = max_terms(p.h_prods.data(), p.nprod);
-        timing_begin(stream);
         // the products in table order, one launch each: stream order serializes their C updates
         for (int q = 0; q < p.nprod; q++)
         {
             g.prods = p.prods + q;
+            timing_begin(stream);
             HIP_TRY(launch_leaf_gemm(p.elem, EPI_MULTI_C, p.tile, terms, p.kf_core.on, g, p.a_mfast, p.b_nfast,
                                      stream));
+            timing_end(stream, leaf_flops(p, 1, p.ms, p.ns, p.ks));
         }
-        timing_end(stream, leaf_flops(p, p.nprod, p.ms, p.ns, p.ks));
     }
     else if (p.impl == IMPL_AB)
     {
