@@ -128,67 +128,69 @@ int dense_tile(const Plan& p)
     return p.ms * p.ns >= int64_t(1024) * 1024 ? TILE_128 : TILE_64;
 }
 
-// A launch may use the fast-K kernel when, for every A term and every B term of its products, the
-// K-run starting at each tile (term offset + j*TK) is affine with one stride common to all A terms
-// (and one common to all B terms), and every element offset stays within 31 bits so that per-element
-// byte offsets fit a 32-bit register.
-KFast kfast_check(const std::vector<ProdDesc>& prods, int first, int count, int64_t ks, int TK,
-                  const std::vector<int64_t>& a_krun, const std::vector<int64_t>& b_krun, int64_t a_span,
-                  int64_t b_span, int elem)
+// Whether the sub-block at `off` (length len) of scatter vector v is a translate of the one at 0:
+// v[off + i] - v[off] == v[i] - v[0] for every i < len.
+bool is_translate(const std::vector<int64_t>& v, int64_t off, int64_t len)
+{
+    if (off == 0) return true;
+    const int64_t d = v[size_t(off)] - v[0];
+    for (int64_t i = 0; i < len; i++)
+        if (v[size_t(off + i)] - v[size_t(i)] != d) return false;
+    return true;
+}
+
+// A launch may use the fast-K kernel body when (1) every sum term's sub-block is a translate of the
+// first sub-block in its operand's row and column scatter vectors (so a term is the first sub-block
+// shifted by a uniform offset, which the fast product table stores as its base), (2) the first
+// sub-block's K runs are affine at every K-tile start, with one stride for A and one for B, and (3) the
+// per-element offsets inside a K-tile fit 31 bits.  All C targets must be translates as well.
+KFast kfast_check(const Plan& p, const std::vector<ProdDesc>& prods, int first, int count, int64_t ms, int64_t ns,
+                  int64_t ks, int TK, const std::vector<int64_t>& a_krun, const std::vector<int64_t>& b_krun)
 {
     KFast kf;
-    const int64_t limit = (int64_t(1) << 31) / elem;
-    if (a_span < 0 || b_span < 0 || a_span >= limit || b_span >= limit || ks <= 0) return kf;
-    int64_t sa = 0, sb = 0;
+    if (ks <= 0 || ms <= 0 || ns <= 0) return kf;
     for (int q = first; q < first + count; q++)
     {
         const ProdDesc& pd = prods[size_t(q)];
         for (int t = 0; t < pd.nA; t++)
-            for (int64_t k0 = 0; k0 < ks; k0 += TK)
-            {
-                const int64_t r = a_krun[size_t(pd.a[t].coff + k0)];
-                if (r == 0 || (sa != 0 && r != sa)) return kf;
-                sa = r;
-            }
+            if (!is_translate(p.h_a_m, pd.a[t].roff, ms) || !is_translate(p.h_a_k, pd.a[t].coff, ks)) return kf;
         for (int t = 0; t < pd.nB; t++)
-            for (int64_t k0 = 0; k0 < ks; k0 += TK)
-            {
-                const int64_t r = b_krun[size_t(pd.b[t].roff + k0)];
-                if (r == 0 || (sb != 0 && r != sb)) return kf;
-                sb = r;
-            }
+            if (!is_translate(p.h_b_k, pd.b[t].roff, ks) || !is_translate(p.h_b_n, pd.b[t].coff, ns)) return kf;
+        for (int t = 0; t < pd.nC; t++)
+            if (!is_translate(p.h_c_m, pd.c[t].roff, ms) || !is_translate(p.h_c_n, pd.c[t].coff, ns)) return kf;
     }
-    // (TK-1) extra strides: the last elements of a tile are addressed from its first
-    if ((TK - 1) * (sa < 0 ? -sa : sa) >= limit - a_span || (TK - 1) * (sb < 0 ? -sb : sb) >= limit - b_span)
-        return kf;
-    kf.on = sa > 0 && sb > 0;
+    const int64_t sa = a_krun[0], sb = b_krun[0];
+    if (sa <= 0 || sb <= 0) return kf;
+    for (int64_t k0 = 0; k0 < ks; k0 += TK)
+        if (a_krun[size_t(k0)] != sa || b_krun[size_t(k0)] != sb) return kf;
+    const int64_t limit = (int64_t(1) << 31) / p.elem;
+    int64_t am = 0, bn = 0;
+    for (int64_t i = 0; i < ms; i++)
+    {
+        if (p.h_a_m[size_t(i)] < 0) return kf;
+        am = std::max(am, p.h_a_m[size_t(i)]);
+    }
+    for (int64_t i = 0; i < ns; i++)
+    {
+        if (p.h_b_n[size_t(i)] < 0) return kf;
+        bn = std::max(bn, p.h_b_n[size_t(i)]);
+    }
+    if (am + (TK - 1) * sa >= limit || bn + (TK - 1) * sb >= limit) return kf;
+    kf.on = true;
     kf.sa = sa;
     kf.sb = sb;
     return kf;
 }
 
-// Largest element offset of an operand addressed through (rows, cols) scatter vectors, or -1 if any
-// entry is negative (negative strides are served by the gathered-K kernel).
-int64_t operand_span(const std::vector<int64_t>& rs, const std::vector<int64_t>& cs, const std::vector<int64_t>& batch)
+// The fast body's product table: each term's base becomes its uniform shift from the first sub-block.
+ProdDesc shifted(const Plan& p, const ProdDesc& pd)
 {
-    int64_t mr = 0, mc = 0, mb = 0;
-    for (int64_t v : rs)
-    {
-        if (v < 0) return -1;
-        mr = std::max(mr, v);
-    }
-    for (int64_t v : cs)
-    {
-        if (v < 0) return -1;
-        mc = std::max(mc, v);
-    }
-    for (int64_t v : batch)
-    {
-        if (v < 0) return -1;
-        mb = std::max(mb, v);
-    }
-    (void)mb;  // batch offsets are added to the 64-bit base pointer, not to the 32-bit offsets
-    return mr + mc;
+    auto d = [](const std::vector<int64_t>& v, int64_t off) { return v[size_t(off)] - v[0]; };
+    ProdDesc f = pd;
+    for (int t = 0; t < pd.nA; t++) f.a[t].base = pd.a[t].base + d(p.h_a_m, pd.a[t].roff) + d(p.h_a_k, pd.a[t].coff);
+    for (int t = 0; t < pd.nB; t++) f.b[t].base = pd.b[t].base + d(p.h_b_k, pd.b[t].roff) + d(p.h_b_n, pd.b[t].coff);
+    for (int t = 0; t < pd.nC; t++) f.c[t].base = pd.c[t].base + d(p.h_c_m, pd.c[t].roff) + d(p.h_c_n, pd.c[t].coff);
+    return f;
 }
 
 }  // namespace
@@ -340,21 +342,18 @@ Status upload(Plan& p)
 {
     const int TK = tile_kdepth(p.tile, p.elem);
     std::vector<int64_t> a_krun = run_strides(p.h_a_k, TK), b_krun = run_strides(p.h_b_k, TK);
+    p.kf_core = kfast_check(p, p.h_prods, 0, p.nprod, p.ms, p.ns, p.ks, TK, a_krun, b_krun);
+    for (Pass& ps : p.fringe)
+        ps.kf = kfast_check(p, p.h_prods, ps.prod_index, 1, ps.ms, ps.ns, ps.ks, TK, a_krun, b_krun);
+    std::vector<ProdDesc> fast_prods;
+    for (const ProdDesc& pd : p.h_prods) fast_prods.push_back(shifted(p, pd));
+    if (p.core)
     {
-        const int64_t a_span = operand_span(p.h_a_m, p.h_a_k, p.h_batch_a);
-        const int64_t b_span = operand_span(p.h_b_k, p.h_b_n, p.h_batch_b);
-        p.kf_core = kfast_check(p.h_prods, 0, p.nprod, p.ks, TK, a_krun, b_krun, a_span, b_span, p.elem);
-        for (Pass& ps : p.fringe)
-            ps.kf = kfast_check(p.h_prods, ps.prod_index, 1, ps.ks, TK, a_krun, b_krun, a_span, b_span, p.elem);
-        if (p.core)
-        {
-            // dense S_p (ms x ks, column-major) and T_p (ks x ns): K strides ms and 1
-            const int64_t limit = (int64_t(1) << 31) / p.elem;
-            const int64_t s_span = int64_t(p.nprod) * p.ms * p.ks, t_span = int64_t(p.nprod) * p.ks * p.ns;
-            p.kf_dense.on = s_span + TK * p.ms < limit && t_span + TK < limit;
-            p.kf_dense.sa = p.ms;
-            p.kf_dense.sb = 1;
-        }
+        // dense S_p (ms x ks, column-major) and T_p (ks x ns): K strides ms and 1, terms at base offsets
+        const int64_t limit = (int64_t(1) << 31) / p.elem;
+        p.kf_dense.on = (p.ms - 1) + (TK - 1) * p.ms < limit && (p.ns - 1) * p.ks + (TK - 1) < limit;
+        p.kf_dense.sa = p.ms;
+        p.kf_dense.sb = 1;
     }
     std::vector<int64_t> s_m, s_k, t_k, t_n, s_krun, t_krun;
     if (p.core)
@@ -388,6 +387,7 @@ Status upload(Plan& p)
          i_bb = add(p.h_batch_b.data(), p.h_batch_b.size() * 8),
          i_bc = add(p.h_batch_c.data(), p.h_batch_c.size() * 8),
          i_pr = add(p.h_prods.data(), p.h_prods.size() * sizeof(ProdDesc)),
+         i_prf = add(fast_prods.data(), fast_prods.size() * sizeof(ProdDesc)),
          i_sr = add(p.h_sub_roff.data(), p.h_sub_roff.size() * 8),
          i_sc = add(p.h_sub_coff.data(), p.h_sub_coff.size() * 8),
          i_tb = add(p.h_term_begin.data(), p.h_term_begin.size() * sizeof(int)),
@@ -424,6 +424,7 @@ Status upload(Plan& p)
     p.batch_b = reinterpret_cast<const int64_t*>(at(i_bb));
     p.batch_c = reinterpret_cast<const int64_t*>(at(i_bc));
     p.prods = reinterpret_cast<const ProdDesc*>(at(i_pr));
+    p.prods_fast = reinterpret_cast<const ProdDesc*>(at(i_prf));
     p.sub_roff = reinterpret_cast<const int64_t*>(at(i_sr));
     p.sub_coff = reinterpret_cast<const int64_t*>(at(i_sc));
     p.term_begin = reinterpret_cast<const int*>(at(i_tb));
@@ -543,7 +544,7 @@ Status run_plan(const Plan& p, const Call& c, hipStream_t stream)
         // the products in table order, one launch each: stream order serializes their C updates
         for (int q = 0; q < p.nprod; q++)
         {
-            g.prods = p.prods + q;
+            g.prods = (p.kf_core.on ? p.prods_fast : p.prods) + q;
             timing_begin(stream);
             HIP_TRY(launch_leaf_gemm(p.elem, EPI_MULTI_C, p.tile, terms, p.kf_core.on, g, p.a_mfast, p.b_nfast,
                                      stream));
@@ -558,7 +559,7 @@ Status run_plan(const Plan& p, const Call& c, hipStream_t stream)
         g.ms = p.ms;
         g.ns = p.ns;
         g.ks = p.ks;
-        g.prods = p.prods;
+        g.prods = p.kf_core.on ? p.prods_fast : p.prods;
         g.nprod = p.nprod;
         g.M = M;
         g.ka_stride = p.kf_core.sa;
@@ -690,7 +691,7 @@ Status run_plan(const Plan& p, const Call& c, hipStream_t stream)
         f.ms = ps.ms;
         f.ns = ps.ns;
         f.ks = ps.ks;
-        f.prods = p.prods + ps.prod_index;
+        f.prods = (ps.kf.on ? p.prods_fast : p.prods) + ps.prod_index;
         f.nprod = 1;
         f.ka_stride = ps.kf.sa;
         f.kb_stride = ps.kf.sb;

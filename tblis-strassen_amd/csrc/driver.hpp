@@ -86,6 +86,7 @@ struct Plan
                   *b_n = nullptr, *c_m = nullptr, *c_n = nullptr;
     const int64_t *batch_a = nullptr, *batch_b = nullptr, *batch_c = nullptr;
     const ProdDesc* prods = nullptr;
+    const ProdDesc* prods_fast = nullptr;  // bases = term shifts (fast kernel body)
     const int64_t *sub_roff = nullptr, *sub_coff = nullptr;
     const int* term_begin = nullptr;
     const CombineTerm* terms = nullptr;

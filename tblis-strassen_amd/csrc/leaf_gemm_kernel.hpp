@@ -133,329 +133,17 @@ __global__ __launch_bounds__(256) void leaf_gemm_kernel(GemmArgs g)
     constexpr int B_NSTEP = B_NFAST ? 0 : NT / TK;
     constexpr int B_KSTEP = B_NFAST ? NT / TN : 0;
 
-    // One product per workgroup: blockIdx.y selects it among the launch's products.  (ABC launches
-    // one product at a time, in stream order, so that products updating the same C sub-block are
-    // serialized; a loop over products inside the kernel costs registers in the main loop.)
-    {
-        const int p = blockIdx.y;
-        const ProdDesc& P = g.prods[p];
-        const int nA = __builtin_amdgcn_readfirstlane(P.nA), nB = __builtin_amdgcn_readfirstlane(P.nB);
-
-        // Row offsets of A (per sub-block) and column offsets of B: fixed for the whole K loop.
-        // Positions outside the leaf are clamped to a valid one and flagged; such lanes later read
-        // the zero block, so every load is issued unconditionally and its value used as is.
-        int64_t aro[TERMS][RA], bco[TERMS][RB];
-        bool arow_ok[RA], bcol_ok[RB];
-        T acoef[TERMS], bcoef[TERMS];
-#pragma unroll
-        for (int e = 0; e < RA; e++) arow_ok[e] = row0 + a_m + e * A_MSTEP < ms;
-#pragma unroll
-        for (int e = 0; e < RB; e++) bcol_ok[e] = col0 + b_n + e * B_NSTEP < ns;
-#pragma unroll
-        for (int a = 0; a < TERMS; a++)
-        {
-            const bool on = a < nA;
-            acoef[a] = on ? T(P.a[a].coeff) : T(0);
-            const int64_t base = on ? P.a[a].base : 0, roff = on ? P.a[a].roff : 0;
-#pragma unroll
-            for (int e = 0; e < RA; e++)
-                aro[a][e] = base + g.a_m[roff + lmin(row0 + a_m + e * A_MSTEP, ms - 1)];
-        }
-#pragma unroll
-        for (int b = 0; b < TERMS; b++)
-        {
-            const bool on = b < nB;
-            bcoef[b] = on ? T(P.b[b].coeff) : T(0);
-            const int64_t base = on ? P.b[b].base : 0, coff = on ? P.b[b].coff : 0;
-#pragma unroll
-            for (int e = 0; e < RB; e++)
-                bco[b][e] = base + g.b_n[coff + lmin(col0 + b_n + e * B_NSTEP, ns - 1)];
-        }
-
-        // Fast mode (KFAST, decided per launch by the host): every term's K runs are affine with the same
-        // stride at every tile start (g.ka_stride / g.kb_stride) and all offsets are non-negative and
-        // below 2^31 elements.  Each element's byte offset relative to its tile's first K offset is
-        // then fixed for the whole product: full K-tiles load from (uniform base + 32-bit offset) with
-        // no per-element arithmetic.
-        constexpr bool kfast = KFAST;
-        uint32_t aoff[TERMS][EA], boff[TERMS][EB];
-        if (kfast)
-        {
-#pragma unroll
-            for (int a = 0; a < TERMS; a++)
-#pragma unroll
-                for (int e = 0; e < EA; e++)
-                    aoff[a][e] = uint32_t((aro[a][A_MFAST ? 0 : e] + int64_t(a_k + e * A_KSTEP) * g.ka_stride) *
-                                          int64_t(sizeof(T)));
-#pragma unroll
-            for (int b = 0; b < TERMS; b++)
-#pragma unroll
-                for (int e = 0; e < EB; e++)
-                    boff[b][e] = uint32_t((bco[b][B_NFAST ? 0 : e] + int64_t(b_k + e * B_KSTEP) * g.kb_stride) *
-                                          int64_t(sizeof(T)));
-        }
-
-        // K-run info of a K-tile.  Lane t < TERMS holds the block-scatter stride of A term t's run
-        // starting at the tile (0 = not affine), lane TERMS+t the offset of its first element, lanes
-        // 2*TERMS+t and 3*TERMS+t the same for B.  One vector load per tile fetches all of it, one tile
-        // ahead of use (vector loads retire in order, so waiting for a tile's data never waits for
-        // it), and v_readlane moves each entry to a scalar register when it is used.  (Loads from a
-        // uniform address would become scalar loads, whose out-of-order lgkmcnt would make every LDS
-        // wait of the MFMA block wait for them too.)
-        const int kw = lane < 4 * TERMS ? lane / TERMS : 3;
-        const int kt_term = lane % TERMS;
-        const int64_t kpos = kw < 2 ? (kt_term < nA ? P.a[kt_term].coff : 0) : (kt_term < nB ? P.b[kt_term].roff : 0);
-        const int64_t* ksrc = kw == 0 ? g.a_krun : kw == 1 ? g.a_k : kw == 2 ? g.b_krun : g.b_k;
-        int64_t kinfo = 0;
-        auto fetch_kinfo = [&](int64_t k0) { kinfo = ksrc[kpos + k0]; };
-
-        T ra[TERMS][EA], rb[TERMS][EB];
-
-        // Issue the global loads of one K-tile (raw values of every sub-block; they are summed after
-        // the MFMA work of the previous tile, so the loads stay in flight across it).  The affine and
-        // gathered forms are whole uniform branches, so no per-element control flow (and no wait at
-        // its join) is generated.
-        auto load_tile = [&](int64_t k0) {
-            if (kfast)
-            {
-                // a partial last tile reads the zero block past the leaf's K extent
-                const int klim = int(lmin(ks - 1 - k0, TK - 1));
-                const bool full = klim == TK - 1;
-#pragma unroll
-                for (int a = 0; a < TERMS; a++)
-                {
-                    if (a < nA)
-                    {
-                        const char* base = reinterpret_cast<const char*>(Ag + readlane64(kinfo, TERMS + a));
-#pragma unroll
-                        for (int e = 0; e < EA; e++)
-                            ra[a][e] = *((full || a_k + e * A_KSTEP <= klim) ? reinterpret_cast<const T*>(base + aoff[a][e])
-                                                                             : zero);
-                    }
-                }
-#pragma unroll
-                for (int b = 0; b < TERMS; b++)
-                {
-                    if (b < nB)
-                    {
-                        const char* base = reinterpret_cast<const char*>(Bg + readlane64(kinfo, 3 * TERMS + b));
-#pragma unroll
-                        for (int e = 0; e < EB; e++)
-                            rb[b][e] = *((full || b_k + e * B_KSTEP <= klim) ? reinterpret_cast<const T*>(base + boff[b][e])
-                                                                             : zero);
-                    }
-                }
-                return;
-            }
-            const int klim = int(lmin(ks - 1 - k0, TK - 1));  // last valid k inside this tile
-            // All run info first (its load is a tile old, so nothing newer is waited for), then one
-            // uniform branch: every term affine (the common case) or the general per-term code.
-            int64_t arun[TERMS], ac0[TERMS], brun[TERMS], br0[TERMS];
-            bool affine = true;
-#pragma unroll
-            for (int a = 0; a < TERMS; a++)
-            {
-                arun[a] = readlane64(kinfo, a);
-                ac0[a] = readlane64(kinfo, TERMS + a);
-                affine = affine && (a >= nA || arun[a] != 0);
-            }
-#pragma unroll
-            for (int b = 0; b < TERMS; b++)
-            {
-                brun[b] = readlane64(kinfo, 2 * TERMS + b);
-                br0[b] = readlane64(kinfo, 3 * TERMS + b);
-                affine = affine && (b >= nB || brun[b] != 0);
-            }
-            if (affine)
-            {
-#pragma unroll
-                for (int a = 0; a < TERMS; a++)
-                {
-                    if (a < nA)
-                    {
-#pragma unroll
-                        for (int e = 0; e < EA; e++)
-                        {
-                            const int kk = a_k + e * A_KSTEP;
-                            const int kc = kk < klim ? kk : klim;
-                            const bool ok = arow_ok[A_MFAST ? 0 : e] && kk <= klim;
-                            ra[a][e] = *(ok ? Ag + aro[a][A_MFAST ? 0 : e] + ac0[a] + int64_t(kc) * arun[a] : zero);
-                        }
-                    }
-                }
-#pragma unroll
-                for (int b = 0; b < TERMS; b++)
-                {
-                    if (b < nB)
-                    {
-#pragma unroll
-                        for (int e = 0; e < EB; e++)
-                        {
-                            const int kk = b_k + e * B_KSTEP;
-                            const int kc = kk < klim ? kk : klim;
-                            const bool ok = bcol_ok[B_NFAST ? 0 : e] && kk <= klim;
-                            rb[b][e] = *(ok ? Bg + bco[b][B_NFAST ? 0 : e] + br0[b] + int64_t(kc) * brun[b] : zero);
-                        }
-                    }
-                }
-                return;
-            }
-#pragma unroll
-            for (int a = 0; a < TERMS; a++)
-            {
-                if (a < nA)
-                {
-                    const int64_t run = arun[a];
-                    if (run != 0)
-                    {
-                        const int64_t c0 = ac0[a];
-#pragma unroll
-                        for (int e = 0; e < EA; e++)
-                        {
-                            const int kk = a_k + e * A_KSTEP;
-                            const int kc = kk < klim ? kk : klim;
-                            const bool ok = arow_ok[A_MFAST ? 0 : e] && kk <= klim;
-                            ra[a][e] = *(ok ? Ag + aro[a][A_MFAST ? 0 : e] + c0 + int64_t(kc) * run : zero);
-                        }
-                    }
-                    else
-                    {
-                        const int64_t kb = readlane64(kpos, TERMS + a) + k0;
-                        int64_t co[EA];
-#pragma unroll
-                        for (int e = 0; e < EA; e++)
-                        {
-                            const int kk = a_k + e * A_KSTEP;
-                            co[e] = g.a_k[kb + (kk < klim ? kk : klim)];
-                        }
-#pragma unroll
-                        for (int e = 0; e < EA; e++)
-                        {
-                            const bool ok = arow_ok[A_MFAST ? 0 : e] && a_k + e * A_KSTEP <= klim;
-                            ra[a][e] = *(ok ? Ag + aro[a][A_MFAST ? 0 : e] + co[e] : zero);
-                        }
-                    }
-                }
-            }
-#pragma unroll
-            for (int b = 0; b < TERMS; b++)
-            {
-                if (b < nB)
-                {
-                    const int64_t run = brun[b];
-                    if (run != 0)
-                    {
-                        const int64_t r0 = br0[b];
-#pragma unroll
-                        for (int e = 0; e < EB; e++)
-                        {
-                            const int kk = b_k + e * B_KSTEP;
-                            const int kc = kk < klim ? kk : klim;
-                            const bool ok = bcol_ok[B_NFAST ? 0 : e] && kk <= klim;
-                            rb[b][e] = *(ok ? Bg + bco[b][B_NFAST ? 0 : e] + r0 + int64_t(kc) * run : zero);
-                        }
-                    }
-                    else
-                    {
-                        const int64_t kb = readlane64(kpos, 3 * TERMS + b) + k0;
-                        int64_t ro[EB];
-#pragma unroll
-                        for (int e = 0; e < EB; e++)
-                        {
-                            const int kk = b_k + e * B_KSTEP;
-                            ro[e] = g.b_k[kb + (kk < klim ? kk : klim)];
-                        }
-#pragma unroll
-                        for (int e = 0; e < EB; e++)
-                        {
-                            const bool ok = bcol_ok[B_NFAST ? 0 : e] && b_k + e * B_KSTEP <= klim;
-                            rb[b][e] = *(ok ? Bg + bco[b][B_NFAST ? 0 : e] + ro[e] : zero);
-                        }
-                    }
-                }
-            }
-        };
-
-        // Form the Strassen sums in registers and store the K-tile to LDS as [k][m] / [k][n].
-        auto store_tile = [&](int buf) {
-#pragma unroll
-            for (int e = 0; e < EA; e++)
-            {
-                T v = acoef[0] * ra[0][e];
-#pragma unroll
-                for (int a = 1; a < TERMS; a++)
-                    if (a < nA) v += acoef[a] * ra[a][e];
-                sA[buf][(a_k + e * A_KSTEP) * LDA + a_m + e * A_MSTEP] = arow_ok[A_MFAST ? 0 : e] ? v : T(0);
-            }
-#pragma unroll
-            for (int e = 0; e < EB; e++)
-            {
-                T v = bcoef[0] * rb[0][e];
-#pragma unroll
-                for (int b = 1; b < TERMS; b++)
-                    if (b < nB) v += bcoef[b] * rb[b][e];
-                sB[buf][(b_k + e * B_KSTEP) * LDB + b_n + e * B_NSTEP] = bcol_ok[B_NFAST ? 0 : e] ? v : T(0);
-            }
-        };
-
-        v4 acc[FM][FN];
-#pragma unroll
-        for (int i = 0; i < FM; i++)
-#pragma unroll
-            for (int j = 0; j < FN; j++) acc[i][j] = v4{0, 0, 0, 0};
-
-        const int64_t nkt = (ks + TK - 1) / TK;
-        fetch_kinfo(0);
-        load_tile(0);
-        if (nkt > 1) fetch_kinfo(TK);
-        store_tile(0);
-        __syncthreads();
-
-        for (int64_t kt = 0; kt < nkt; kt++)
-        {
-            const int cur = int(kt & 1);
-            if (kt + 1 < nkt)
-            {
-                load_tile((kt + 1) * TK);
-                if (kt + 2 < nkt) fetch_kinfo((kt + 2) * TK);
-            }
-            const T* __restrict__ la = sA[cur];
-            const T* __restrict__ lb = sB[cur];
-#pragma unroll
-            for (int k4 = 0; k4 < TK / 4; k4++)
-            {
-                const int kr = k4 * 4 + (lane >> 4);
-                T af[FM], bf[FN];
-#pragma unroll
-                for (int i = 0; i < FM; i++) af[i] = la[kr * LDA + wm0 + i * 16 + (lane & 15)];
-#pragma unroll
-                for (int j = 0; j < FN; j++) bf[j] = lb[kr * LDB + wn0 + j * 16 + (lane & 15)];
-#pragma unroll
-                for (int i = 0; i < FM; i++)
-#pragma unroll
-                    for (int j = 0; j < FN; j++) acc[i][j] = Mfma<T>::run(af[i], bf[j], acc[i][j]);
-            }
-            if (kt + 1 < nkt) store_tile(cur ^ 1);
-            __syncthreads();
-        }
-
-        // Epilogue: element v of a lane's 16x16 block is (Mfma<T>::row(lane, v), lane % 16).
-        int n_pos[FN];
-        bool n_ok[FN];
-#pragma unroll
-        for (int j = 0; j < FN; j++)
-        {
-            const int64_t n = col0 + wn0 + j * 16 + (lane & 15);
-            n_ok[j] = n < ns;
-            n_pos[j] = int(lmin(n, ns - 1));
-        }
-        if (EPI == EPI_MULTI_C)
+    // Epilogue, shared by both bodies.  Element v of a lane's 16x16 accumulator block is
+    // (Mfma<T>::row(lane, v), lane % 16).  `translate`: C targets are shifts (ProdDesc base) of the first
+    // C sub-block, else they are addressed through (roff, coff) into the scatter vectors.
+    auto epilogue = [&](const ProdDesc& P, int p, auto& acc, bool translate) {
+        if constexpr (EPI == EPI_MULTI_C)
         {
             // Multi-C update through LDS: for each 16-row block i, the waves write their accumulators
             // (straight from the accumulator registers) into a 32 x TN staging tile, then all 256
             // threads update every C target from it, consecutive threads on consecutive rows (C's
             // unit-stride dimension after the plan's orientation).  Keeps the accumulators out of the
-            // VGPRs, so the main loop keeps the store epilogue's register budget.
+            // VGPRs, so the main loop keeps a small register budget.
             constexpr int LDE = TN + 1;
             constexpr int RROWS = 32, CSTEP = NT / RROWS, CPER = TN / CSTEP;
             static_assert(2 * 16 * LDE <= 2 * TK * (LDA + LDB), "staging tile must fit the K buffers");
@@ -463,11 +151,11 @@ __global__ __launch_bounds__(256) void leaf_gemm_kernel(GemmArgs g)
             const double alpha = g.alpha, beta = g.beta;
             const int nC = P.nC;
             const int first_mask = P.first_mask;
-            // column offsets of every target, for this tile
-            for (int x = tid; x < nC * TN; x += NT)
+            // column offsets of every target, for this tile (one shared row in translate mode)
+            for (int x = tid; x < (translate ? 1 : nC) * TN; x += NT)
             {
                 const int c = x / TN, col = x % TN;
-                ccol[c][col] = P.c[c].base + g.c_n[P.c[c].coff + lmin(col0 + col, ns - 1)];
+                ccol[c][col] = g.c_n[(translate ? 0 : P.c[c].coff) + lmin(col0 + col, ns - 1)];
             }
             const int rl = tid % RROWS, cg = tid / RROWS;
 #pragma unroll 1
@@ -489,6 +177,7 @@ __global__ __launch_bounds__(256) void leaf_gemm_kernel(GemmArgs g)
                 const int64_t grow = row0 + (rl / 16) * WM + i * 16 + (rl % 16);
                 const bool rok = grow < ms;
                 const int64_t growc = lmin(grow, ms - 1);
+                const int64_t rbase = translate ? g.c_m[growc] : 0;
 #pragma unroll 1
                 for (int c = 0; c < nC; c++)
                 {
@@ -496,17 +185,18 @@ __global__ __launch_bounds__(256) void leaf_gemm_kernel(GemmArgs g)
                     const bool first = (first_mask >> c) & 1;
                     const bool read_c = !(first && beta == 0.0);
                     const T bet = first ? T(beta) : T(1);
-                    T* Crow = Cg + g.c_m[P.c[c].roff + growc];
+                    T* Crow = Cg + P.c[c].base + (translate ? rbase : g.c_m[P.c[c].roff + growc]);
+                    const int64_t* cc = ccol[translate ? 0 : c];
                     T old[CPER];
 #pragma unroll
-                    for (int e = 0; e < CPER; e++) old[e] = read_c ? Crow[ccol[c][cg + e * CSTEP]] : T(0);
+                    for (int e = 0; e < CPER; e++) old[e] = read_c ? Crow[cc[cg + e * CSTEP]] : T(0);
 #pragma unroll
                     for (int e = 0; e < CPER; e++)
                     {
                         const int col = cg + e * CSTEP;
                         const T val = coef * stage[rl * LDE + col];
                         const T out = read_c ? bet * old[e] + val : val;
-                        if (rok && col0 + col < ns) Crow[ccol[c][col]] = out;
+                        if (rok && col0 + col < ns) Crow[cc[col]] = out;
                     }
                 }
                 __syncthreads();
@@ -514,6 +204,8 @@ __global__ __launch_bounds__(256) void leaf_gemm_kernel(GemmArgs g)
         }
         else
         {
+            (void)P;
+            (void)translate;
             T* Mp = static_cast<T*>(g.M) + (z * g.nprod + p) * ms * ns;
 #pragma unroll
             for (int i = 0; i < FM; i++)
@@ -523,8 +215,415 @@ __global__ __launch_bounds__(256) void leaf_gemm_kernel(GemmArgs g)
                     const int64_t m = row0 + wm0 + i * 16 + Mfma<T>::row(lane, v);
 #pragma unroll
                     for (int j = 0; j < FN; j++)
-                        if (m < ms && n_ok[j]) Mp[int64_t(n_pos[j]) * ms + m] = acc[i][j][v];
+                    {
+                        const int64_t n = col0 + wn0 + j * 16 + (lane & 15);
+                        if (m < ms && n < ns) Mp[n * ms + m] = acc[i][j][v];
+                    }
                 }
+        }
+    };
+
+    if constexpr (KFAST)
+    {
+        // ---- fast body -------------------------------------------------------------------------------
+        // Chosen by the host (kfast_check) when every sum term's sub-block is a translate of the first
+        // sub-block in all three scatter vectors of its operand, and the K runs of that first sub-block
+        // are affine at every K-tile start with stride g.ka_stride (A) / g.kb_stride (B).  Then term t of
+        // product p sits at a uniform shift (ProdDesc base, precomputed on the host) from the first
+        // sub-block, every element's byte offset within a K-tile is the same for all terms and all
+        // K-tiles, and a K-tile load is: uniform base (shift + the tile's first K offset) + that offset.
+        const int p = blockIdx.y;
+        const ProdDesc& P = g.prods[p];
+        const int nA = __builtin_amdgcn_readfirstlane(P.nA), nB = __builtin_amdgcn_readfirstlane(P.nB);
+
+        bool arow_ok[RA], bcol_ok[RB];
+        uint32_t aoff[EA], boff[EB];
+#pragma unroll
+        for (int e = 0; e < RA; e++) arow_ok[e] = row0 + a_m + e * A_MSTEP < ms;
+#pragma unroll
+        for (int e = 0; e < RB; e++) bcol_ok[e] = col0 + b_n + e * B_NSTEP < ns;
+        {
+            int64_t ar[RA], bc[RB];
+#pragma unroll
+            for (int e = 0; e < RA; e++) ar[e] = g.a_m[lmin(row0 + a_m + e * A_MSTEP, ms - 1)];
+#pragma unroll
+            for (int e = 0; e < RB; e++) bc[e] = g.b_n[lmin(col0 + b_n + e * B_NSTEP, ns - 1)];
+#pragma unroll
+            for (int e = 0; e < EA; e++)
+                aoff[e] = uint32_t((ar[A_MFAST ? 0 : e] + int64_t(a_k + e * A_KSTEP) * g.ka_stride) * int64_t(sizeof(T)));
+#pragma unroll
+            for (int e = 0; e < EB; e++)
+                boff[e] = uint32_t((bc[B_NFAST ? 0 : e] + int64_t(b_k + e * B_KSTEP) * g.kb_stride) * int64_t(sizeof(T)));
+        }
+        T acoef[TERMS], bcoef[TERMS];
+        int64_t ash[TERMS], bsh[TERMS];
+#pragma unroll
+        for (int t = 0; t < TERMS; t++)
+        {
+            acoef[t] = t < nA ? T(P.a[t].coeff) : T(0);
+            bcoef[t] = t < nB ? T(P.b[t].coeff) : T(0);
+            ash[t] = t < nA ? P.a[t].base : 0;
+            bsh[t] = t < nB ? P.b[t].base : 0;
+        }
+
+        // first K offsets of a tile (lane 0: A, lane 1: B), one vector load a tile ahead of use
+        int64_t kinfo = 0;
+        auto fetch_kinfo = [&](int64_t k0) { kinfo = (lane == 1 ? g.b_k : g.a_k)[k0]; };
+
+        // One sum term in flight at a time: the K loop runs in TERMS phases; phase t issues the loads
+        // of term t of the next K-tile, runs 1/TERMS of the current tile's MFMA work, then adds the
+        // scaled term into the next tile's LDS buffer (term 0 stores, later terms read-modify-write
+        // their own elements, so no barrier is needed between phases).  Register cost is that of a
+        // single-term GEMM whatever the number of terms.
+        T ra[EA], rb[EB];
+        auto issue = [&](int t, int64_t k0) {
+            const int64_t c0 = readlane64(kinfo, 0), r0 = readlane64(kinfo, 1);
+            const int klim = int(lmin(ks - 1 - k0, TK - 1));
+            const bool full = klim == TK - 1;  // a partial last tile reads the zero block past ks
+            if (t < nA)
+            {
+                const char* base = reinterpret_cast<const char*>(Ag + ash[t] + c0);
+#pragma unroll
+                for (int e = 0; e < EA; e++)
+                    ra[e] = *((full || a_k + e * A_KSTEP <= klim) ? reinterpret_cast<const T*>(base + aoff[e]) : zero);
+            }
+            if (t < nB)
+            {
+                const char* base = reinterpret_cast<const char*>(Bg + bsh[t] + r0);
+#pragma unroll
+                for (int e = 0; e < EB; e++)
+                    rb[e] = *((full || b_k + e * B_KSTEP <= klim) ? reinterpret_cast<const T*>(base + boff[e]) : zero);
+            }
+        };
+        auto put = [&](int t, int buf) {
+            if (t < nA)
+            {
+#pragma unroll
+                for (int e = 0; e < EA; e++)
+                {
+                    T& dst = sA[buf][(a_k + e * A_KSTEP) * LDA + a_m + e * A_MSTEP];
+                    const T v = arow_ok[A_MFAST ? 0 : e] ? acoef[t] * ra[e] : T(0);
+                    dst = t == 0 ? v : dst + v;
+                }
+            }
+            if (t < nB)
+            {
+#pragma unroll
+                for (int e = 0; e < EB; e++)
+                {
+                    T& dst = sB[buf][(b_k + e * B_KSTEP) * LDB + b_n + e * B_NSTEP];
+                    const T v = bcol_ok[B_NFAST ? 0 : e] ? bcoef[t] * rb[e] : T(0);
+                    dst = t == 0 ? v : dst + v;
+                }
+            }
+        };
+
+        v4 acc[FM][FN];
+#pragma unroll
+        for (int i = 0; i < FM; i++)
+#pragma unroll
+            for (int j = 0; j < FN; j++) acc[i][j] = v4{0, 0, 0, 0};
+
+        constexpr int K4 = TK / 4;
+        constexpr int PH = TERMS < K4 ? TERMS : K4;  // phases per K-tile
+        static_assert(TERMS <= K4 && K4 % PH == 0, "one phase per term");
+        const int64_t nkt = (ks + TK - 1) / TK;
+        fetch_kinfo(0);
+#pragma unroll
+        for (int t = 0; t < TERMS; t++)
+        {
+            issue(t, 0);
+            put(t, 0);
+        }
+        if (nkt > 1) fetch_kinfo(TK);
+        __syncthreads();
+        for (int64_t kt = 0; kt < nkt; kt++)
+        {
+            const int cur = int(kt & 1);
+            const bool more = kt + 1 < nkt;
+            const T* __restrict__ la = sA[cur];
+            const T* __restrict__ lb = sB[cur];
+#pragma unroll
+            for (int ph = 0; ph < PH; ph++)
+            {
+                if (more) issue(ph, (kt + 1) * TK);
+                if (ph == PH - 1 && kt + 2 < nkt) fetch_kinfo((kt + 2) * TK);
+#pragma unroll
+                for (int k4 = ph * (K4 / PH); k4 < (ph + 1) * (K4 / PH); k4++)
+                {
+                    const int kr = k4 * 4 + (lane >> 4);
+                    T af[FM], bf[FN];
+#pragma unroll
+                    for (int i = 0; i < FM; i++) af[i] = la[kr * LDA + wm0 + i * 16 + (lane & 15)];
+#pragma unroll
+                    for (int j = 0; j < FN; j++) bf[j] = lb[kr * LDB + wn0 + j * 16 + (lane & 15)];
+#pragma unroll
+                    for (int i = 0; i < FM; i++)
+#pragma unroll
+                        for (int j = 0; j < FN; j++) acc[i][j] = Mfma<T>::run(af[i], bf[j], acc[i][j]);
+                }
+                if (more) put(ph, cur ^ 1);
+            }
+            __syncthreads();
+        }
+        epilogue(P, p, acc, true);
+        return;
+    }
+    else
+    {
+    // One product per workgroup: blockIdx.y selects it among the launch's products.  (ABC launches
+        // one product at a time, in stream order, so that products updating the same C sub-block are
+        // serialized; a loop over products inside the kernel costs registers in the main loop.)
+        {
+            const int p = blockIdx.y;
+            const ProdDesc& P = g.prods[p];
+            const int nA = __builtin_amdgcn_readfirstlane(P.nA), nB = __builtin_amdgcn_readfirstlane(P.nB);
+    
+            // Row offsets of A (per sub-block) and column offsets of B: fixed for the whole K loop.
+            // Positions outside the leaf are clamped to a valid one and flagged; such lanes later read
+            // the zero block, so every load is issued unconditionally and its value used as is.
+            int64_t aro[TERMS][RA], bco[TERMS][RB];
+            bool arow_ok[RA], bcol_ok[RB];
+            T acoef[TERMS], bcoef[TERMS];
+    #pragma unroll
+            for (int e = 0; e < RA; e++) arow_ok[e] = row0 + a_m + e * A_MSTEP < ms;
+    #pragma unroll
+            for (int e = 0; e < RB; e++) bcol_ok[e] = col0 + b_n + e * B_NSTEP < ns;
+    #pragma unroll
+            for (int a = 0; a < TERMS; a++)
+            {
+                const bool on = a < nA;
+                acoef[a] = on ? T(P.a[a].coeff) : T(0);
+                const int64_t base = on ? P.a[a].base : 0, roff = on ? P.a[a].roff : 0;
+    #pragma unroll
+                for (int e = 0; e < RA; e++)
+                    aro[a][e] = base + g.a_m[roff + lmin(row0 + a_m + e * A_MSTEP, ms - 1)];
+            }
+    #pragma unroll
+            for (int b = 0; b < TERMS; b++)
+            {
+                const bool on = b < nB;
+                bcoef[b] = on ? T(P.b[b].coeff) : T(0);
+                const int64_t base = on ? P.b[b].base : 0, coff = on ? P.b[b].coff : 0;
+    #pragma unroll
+                for (int e = 0; e < RB; e++)
+                    bco[b][e] = base + g.b_n[coff + lmin(col0 + b_n + e * B_NSTEP, ns - 1)];
+            }
+    
+            // K-run info of a K-tile.  Lane t < TERMS holds the block-scatter stride of A term t's run
+            // starting at the tile (0 = not affine), lane TERMS+t the offset of its first element, lanes
+            // 2*TERMS+t and 3*TERMS+t the same for B.  One vector load per tile fetches all of it, one tile
+            // ahead of use (vector loads retire in order, so waiting for a tile's data never waits for
+            // it), and v_readlane moves each entry to a scalar register when it is used.  (Loads from a
+            // uniform address would become scalar loads, whose out-of-order lgkmcnt would make every LDS
+            // wait of the MFMA block wait for them too.)
+            const int kw = lane < 4 * TERMS ? lane / TERMS : 3;
+            const int kt_term = lane % TERMS;
+            const int64_t kpos = kw < 2 ? (kt_term < nA ? P.a[kt_term].coff : 0) : (kt_term < nB ? P.b[kt_term].roff : 0);
+            const int64_t* ksrc = kw == 0 ? g.a_krun : kw == 1 ? g.a_k : kw == 2 ? g.b_krun : g.b_k;
+            int64_t kinfo = 0;
+            auto fetch_kinfo = [&](int64_t k0) { kinfo = ksrc[kpos + k0]; };
+    
+            T ra[TERMS][EA], rb[TERMS][EB];
+    
+            // Issue the global loads of one K-tile (raw values of every sub-block; they are summed after
+            // the MFMA work of the previous tile, so the loads stay in flight across it).  The affine and
+            // gathered forms are whole uniform branches, so no per-element control flow (and no wait at
+            // its join) is generated.
+            auto load_tile = [&](int64_t k0) {
+                const int klim = int(lmin(ks - 1 - k0, TK - 1));  // last valid k inside this tile
+                // All run info first (its load is a tile old, so nothing newer is waited for), then one
+                // uniform branch: every term affine (the common case) or the general per-term code.
+                int64_t arun[TERMS], ac0[TERMS], brun[TERMS], br0[TERMS];
+                bool affine = true;
+    #pragma unroll
+                for (int a = 0; a < TERMS; a++)
+                {
+                    arun[a] = readlane64(kinfo, a);
+                    ac0[a] = readlane64(kinfo, TERMS + a);
+                    affine = affine && (a >= nA || arun[a] != 0);
+                }
+    #pragma unroll
+                for (int b = 0; b < TERMS; b++)
+                {
+                    brun[b] = readlane64(kinfo, 2 * TERMS + b);
+                    br0[b] = readlane64(kinfo, 3 * TERMS + b);
+                    affine = affine && (b >= nB || brun[b] != 0);
+                }
+                if (affine)
+                {
+    #pragma unroll
+                    for (int a = 0; a < TERMS; a++)
+                    {
+                        if (a < nA)
+                        {
+    #pragma unroll
+                            for (int e = 0; e < EA; e++)
+                            {
+                                const int kk = a_k + e * A_KSTEP;
+                                const int kc = kk < klim ? kk : klim;
+                                const bool ok = arow_ok[A_MFAST ? 0 : e] && kk <= klim;
+                                ra[a][e] = *(ok ? Ag + aro[a][A_MFAST ? 0 : e] + ac0[a] + int64_t(kc) * arun[a] : zero);
+                            }
+                        }
+                    }
+    #pragma unroll
+                    for (int b = 0; b < TERMS; b++)
+                    {
+                        if (b < nB)
+                        {
+    #pragma unroll
+                            for (int e = 0; e < EB; e++)
+                            {
+                                const int kk = b_k + e * B_KSTEP;
+                                const int kc = kk < klim ? kk : klim;
+                                const bool ok = bcol_ok[B_NFAST ? 0 : e] && kk <= klim;
+                                rb[b][e] = *(ok ? Bg + bco[b][B_NFAST ? 0 : e] + br0[b] + int64_t(kc) * brun[b] : zero);
+                            }
+                        }
+                    }
+                    return;
+                }
+    #pragma unroll
+                for (int a = 0; a < TERMS; a++)
+                {
+                    if (a < nA)
+                    {
+                        const int64_t run = arun[a];
+                        if (run != 0)
+                        {
+                            const int64_t c0 = ac0[a];
+    #pragma unroll
+                            for (int e = 0; e < EA; e++)
+                            {
+                                const int kk = a_k + e * A_KSTEP;
+                                const int kc = kk < klim ? kk : klim;
+                                const bool ok = arow_ok[A_MFAST ? 0 : e] && kk <= klim;
+                                ra[a][e] = *(ok ? Ag + aro[a][A_MFAST ? 0 : e] + c0 + int64_t(kc) * run : zero);
+                            }
+                        }
+                        else
+                        {
+                            const int64_t kb = readlane64(kpos, TERMS + a) + k0;
+                            int64_t co[EA];
+    #pragma unroll
+                            for (int e = 0; e < EA; e++)
+                            {
+                                const int kk = a_k + e * A_KSTEP;
+                                co[e] = g.a_k[kb + (kk < klim ? kk : klim)];
+                            }
+    #pragma unroll
+                            for (int e = 0; e < EA; e++)
+                            {
+                                const bool ok = arow_ok[A_MFAST ? 0 : e] && a_k + e * A_KSTEP <= klim;
+                                ra[a][e] = *(ok ? Ag + aro[a][A_MFAST ? 0 : e] + co[e] : zero);
+                            }
+                        }
+                    }
+                }
+    #pragma unroll
+                for (int b = 0; b < TERMS; b++)
+                {
+                    if (b < nB)
+                    {
+                        const int64_t run = brun[b];
+                        if (run != 0)
+                        {
+                            const int64_t r0 = br0[b];
+    #pragma unroll
+                            for (int e = 0; e < EB; e++)
+                            {
+                                const int kk = b_k + e * B_KSTEP;
+                                const int kc = kk < klim ? kk : klim;
+                                const bool ok = bcol_ok[B_NFAST ? 0 : e] && kk <= klim;
+                                rb[b][e] = *(ok ? Bg + bco[b][B_NFAST ? 0 : e] + r0 + int64_t(kc) * run : zero);
+                            }
+                        }
+                        else
+                        {
+                            const int64_t kb = readlane64(kpos, 3 * TERMS + b) + k0;
+                            int64_t ro[EB];
+    #pragma unroll
+                            for (int e = 0; e < EB; e++)
+                            {
+                                const int kk = b_k + e * B_KSTEP;
+                                ro[e] = g.b_k[kb + (kk < klim ? kk : klim)];
+                            }
+    #pragma unroll
+                            for (int e = 0; e < EB; e++)
+                            {
+                                const bool ok = bcol_ok[B_NFAST ? 0 : e] && b_k + e * B_KSTEP <= klim;
+                                rb[b][e] = *(ok ? Bg + bco[b][B_NFAST ? 0 : e] + ro[e] : zero);
+                            }
+                        }
+                    }
+                }
+            };
+    
+            // Form the Strassen sums in registers and store the K-tile to LDS as [k][m] / [k][n].
+            auto store_tile = [&](int buf) {
+    #pragma unroll
+                for (int e = 0; e < EA; e++)
+                {
+                    T v = acoef[0] * ra[0][e];
+    #pragma unroll
+                    for (int a = 1; a < TERMS; a++)
+                        if (a < nA) v += acoef[a] * ra[a][e];
+                    sA[buf][(a_k + e * A_KSTEP) * LDA + a_m + e * A_MSTEP] = arow_ok[A_MFAST ? 0 : e] ? v : T(0);
+                }
+    #pragma unroll
+                for (int e = 0; e < EB; e++)
+                {
+                    T v = bcoef[0] * rb[0][e];
+    #pragma unroll
+                    for (int b = 1; b < TERMS; b++)
+                        if (b < nB) v += bcoef[b] * rb[b][e];
+                    sB[buf][(b_k + e * B_KSTEP) * LDB + b_n + e * B_NSTEP] = bcol_ok[B_NFAST ? 0 : e] ? v : T(0);
+                }
+            };
+    
+            v4 acc[FM][FN];
+    #pragma unroll
+            for (int i = 0; i < FM; i++)
+    #pragma unroll
+                for (int j = 0; j < FN; j++) acc[i][j] = v4{0, 0, 0, 0};
+    
+            const int64_t nkt = (ks + TK - 1) / TK;
+            fetch_kinfo(0);
+            load_tile(0);
+            if (nkt > 1) fetch_kinfo(TK);
+            store_tile(0);
+            __syncthreads();
+    
+            for (int64_t kt = 0; kt < nkt; kt++)
+            {
+                const int cur = int(kt & 1);
+                if (kt + 1 < nkt)
+                {
+                    load_tile((kt + 1) * TK);
+                    if (kt + 2 < nkt) fetch_kinfo((kt + 2) * TK);
+                }
+                const T* __restrict__ la = sA[cur];
+                const T* __restrict__ lb = sB[cur];
+    #pragma unroll
+                for (int k4 = 0; k4 < TK / 4; k4++)
+                {
+                    const int kr = k4 * 4 + (lane >> 4);
+                    T af[FM], bf[FN];
+    #pragma unroll
+                    for (int i = 0; i < FM; i++) af[i] = la[kr * LDA + wm0 + i * 16 + (lane & 15)];
+    #pragma unroll
+                    for (int j = 0; j < FN; j++) bf[j] = lb[kr * LDB + wn0 + j * 16 + (lane & 15)];
+    #pragma unroll
+                    for (int i = 0; i < FM; i++)
+    #pragma unroll
+                        for (int j = 0; j < FN; j++) acc[i][j] = Mfma<T>::run(af[i], bf[j], acc[i][j]);
+                }
+                if (kt + 1 < nkt) store_tile(cur ^ 1);
+                __syncthreads();
+            }
+    
+            epilogue(P, p, acc, false);
         }
     }
 }
