@@ -79,6 +79,35 @@ __device__ __forceinline__ const __attribute__((address_space(4))) T* cptr(const
 
 __device__ __forceinline__ int64_t lmin(int64_t a, int64_t b) { return a < b ? a : b; }
 
+// Raw buffer loads for the fast body: a uniform base (the SGPR resource) plus a 32-bit lane offset,
+// one instruction per element.  An offset at or beyond kOob reads 0 (the resource's range check),
+// which zero-fills rows/columns past the tile edge and K past ks without branches or selects.
+constexpr uint32_t kOob = 0x80000000u;
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* base)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, int(kOob), 0x00020000);
+}
+// kOob when k > klim, else 0 (OR-ed into an in-range offset, which is below kOob)
+__device__ __forceinline__ uint32_t past(int k, int klim) { return uint32_t(klim - k) & kOob; }
+template <typename T>
+__device__ __forceinline__ T bload(__amdgpu_buffer_rsrc_t r, uint32_t off);
+template <>
+__device__ __forceinline__ double bload<double>(__amdgpu_buffer_rsrc_t r, uint32_t off)
+{
+    auto x = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
+    double d;
+    __builtin_memcpy(&d, &x, sizeof(d));
+    return d;
+}
+template <>
+__device__ __forceinline__ float bload<float>(__amdgpu_buffer_rsrc_t r, uint32_t off)
+{
+    auto x = __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0);
+    float f;
+    __builtin_memcpy(&f, &x, sizeof(f));
+    return f;
+}
+
 // Lane `l`'s copy of a 64-bit value, in a scalar register.
 __device__ __forceinline__ int64_t readlane64(int64_t x, int l)
 {
@@ -236,24 +265,34 @@ __global__ __launch_bounds__(256) void leaf_gemm_kernel(GemmArgs g)
         const ProdDesc& P = g.prods[p];
         const int nA = __builtin_amdgcn_readfirstlane(P.nA), nB = __builtin_amdgcn_readfirstlane(P.nB);
 
-        bool arow_ok[RA], bcol_ok[RB];
+        // Per-lane byte offsets of the K-tile elements, relative to the tile's first K offset; rows
+        // (columns) past the tile edge get an out-of-range offset and read 0.
         uint32_t aoff[EA], boff[EB];
-#pragma unroll
-        for (int e = 0; e < RA; e++) arow_ok[e] = row0 + a_m + e * A_MSTEP < ms;
-#pragma unroll
-        for (int e = 0; e < RB; e++) bcol_ok[e] = col0 + b_n + e * B_NSTEP < ns;
         {
             int64_t ar[RA], bc[RB];
+            bool aok[RA], bok[RB];
 #pragma unroll
-            for (int e = 0; e < RA; e++) ar[e] = g.a_m[lmin(row0 + a_m + e * A_MSTEP, ms - 1)];
+            for (int e = 0; e < RA; e++)
+            {
+                aok[e] = row0 + a_m + e * A_MSTEP < ms;
+                ar[e] = g.a_m[lmin(row0 + a_m + e * A_MSTEP, ms - 1)];
+            }
 #pragma unroll
-            for (int e = 0; e < RB; e++) bc[e] = g.b_n[lmin(col0 + b_n + e * B_NSTEP, ns - 1)];
+            for (int e = 0; e < RB; e++)
+            {
+                bok[e] = col0 + b_n + e * B_NSTEP < ns;
+                bc[e] = g.b_n[lmin(col0 + b_n + e * B_NSTEP, ns - 1)];
+            }
 #pragma unroll
             for (int e = 0; e < EA; e++)
-                aoff[e] = uint32_t((ar[A_MFAST ? 0 : e] + int64_t(a_k + e * A_KSTEP) * g.ka_stride) * int64_t(sizeof(T)));
+                aoff[e] = aok[A_MFAST ? 0 : e]
+                              ? uint32_t((ar[A_MFAST ? 0 : e] + int64_t(a_k + e * A_KSTEP) * g.ka_stride) * int64_t(sizeof(T)))
+                              : kOob;
 #pragma unroll
             for (int e = 0; e < EB; e++)
-                boff[e] = uint32_t((bc[B_NFAST ? 0 : e] + int64_t(b_k + e * B_KSTEP) * g.kb_stride) * int64_t(sizeof(T)));
+                boff[e] = bok[B_NFAST ? 0 : e]
+                              ? uint32_t((bc[B_NFAST ? 0 : e] + int64_t(b_k + e * B_KSTEP) * g.kb_stride) * int64_t(sizeof(T)))
+                              : kOob;
         }
         T acoef[TERMS], bcoef[TERMS];
         int64_t ash[TERMS], bsh[TERMS];
@@ -279,20 +318,34 @@ __global__ __launch_bounds__(256) void leaf_gemm_kernel(GemmArgs g)
         auto issue = [&](int t, int64_t k0) {
             const int64_t c0 = readlane64(kinfo, 0), r0 = readlane64(kinfo, 1);
             const int klim = int(lmin(ks - 1 - k0, TK - 1));
-            const bool full = klim == TK - 1;  // a partial last tile reads the zero block past ks
+            const bool full = klim == TK - 1;  // a partial last tile reads 0 past ks
             if (t < nA)
             {
-                const char* base = reinterpret_cast<const char*>(Ag + ash[t] + c0);
+                const __amdgpu_buffer_rsrc_t r = make_rsrc(Ag + ash[t] + c0);
+                if (full)
+                {
 #pragma unroll
-                for (int e = 0; e < EA; e++)
-                    ra[e] = *((full || a_k + e * A_KSTEP <= klim) ? reinterpret_cast<const T*>(base + aoff[e]) : zero);
+                    for (int e = 0; e < EA; e++) ra[e] = bload<T>(r, aoff[e]);
+                }
+                else
+                {
+#pragma unroll
+                    for (int e = 0; e < EA; e++) ra[e] = bload<T>(r, aoff[e] | past(a_k + e * A_KSTEP, klim));
+                }
             }
             if (t < nB)
             {
-                const char* base = reinterpret_cast<const char*>(Bg + bsh[t] + r0);
+                const __amdgpu_buffer_rsrc_t r = make_rsrc(Bg + bsh[t] + r0);
+                if (full)
+                {
 #pragma unroll
-                for (int e = 0; e < EB; e++)
-                    rb[e] = *((full || b_k + e * B_KSTEP <= klim) ? reinterpret_cast<const T*>(base + boff[e]) : zero);
+                    for (int e = 0; e < EB; e++) rb[e] = bload<T>(r, boff[e]);
+                }
+                else
+                {
+#pragma unroll
+                    for (int e = 0; e < EB; e++) rb[e] = bload<T>(r, boff[e] | past(b_k + e * B_KSTEP, klim));
+                }
             }
         };
         auto put = [&](int t, int buf) {
@@ -302,7 +355,7 @@ __global__ __launch_bounds__(256) void leaf_gemm_kernel(GemmArgs g)
                 for (int e = 0; e < EA; e++)
                 {
                     T& dst = sA[buf][(a_k + e * A_KSTEP) * LDA + a_m + e * A_MSTEP];
-                    const T v = arow_ok[A_MFAST ? 0 : e] ? acoef[t] * ra[e] : T(0);
+                    const T v = acoef[t] * ra[e];
                     dst = t == 0 ? v : dst + v;
                 }
             }
@@ -312,7 +365,7 @@ __global__ __launch_bounds__(256) void leaf_gemm_kernel(GemmArgs g)
                 for (int e = 0; e < EB; e++)
                 {
                     T& dst = sB[buf][(b_k + e * B_KSTEP) * LDB + b_n + e * B_NSTEP];
-                    const T v = bcol_ok[B_NFAST ? 0 : e] ? bcoef[t] * rb[e] : T(0);
+                    const T v = bcoef[t] * rb[e];
                     dst = t == 0 ? v : dst + v;
                 }
             }
