@@ -115,11 +115,14 @@ int forced_tile()
     return forced;
 }
 
+// 128 x 128 register tiles once a leaf has enough of them to fill the GPU (2 per CU at 256 CUs);
+// smaller leaves take 64 x 64 tiles (4 per CU).
 int choose_tile(const Plan& p)
 {
     if (forced_tile() >= 0) return forced_tile();
-    (void)p;
-    return TILE_64;
+    const int64_t ms = p.core ? p.ms : p.m, ns = p.core ? p.ns : p.n;
+    const int64_t t128 = ((ms + 127) / 128) * ((ns + 127) / 128) * p.nbatch;
+    return t128 >= 256 ? TILE_128 : TILE_64;
 }
 
 int dense_tile(const Plan& p)
@@ -338,6 +341,54 @@ Status build_plan_host(const tblis::tblis_tensor* A, const label_type* idx_A, co
 
 namespace {
 
+// Greedy launch stages for ABC: product q goes to the first stage after the last stage of every earlier
+// product that shares a C sub-block with it.  Products of one stage write disjoint C sub-blocks, and
+// each C sub-block receives its updates in product-table order, so the result is bitwise the serial one.
+void make_stages(Plan& p, std::vector<int>& order)
+{
+    std::vector<std::pair<int64_t, std::pair<int64_t, int64_t>>> keys;
+    std::vector<int> last;
+    std::vector<int> stage(size_t(p.nprod), 0);
+    int nst = 0;
+    for (int q = 0; q < p.nprod; q++)
+    {
+        const ProdDesc& pd = p.h_prods[size_t(q)];
+        int st = 0;
+        std::vector<size_t> ids;
+        for (int t = 0; t < pd.nC; t++)
+        {
+            const auto key = std::make_pair(pd.c[t].base, std::make_pair(pd.c[t].roff, pd.c[t].coff));
+            size_t i = 0;
+            while (i < keys.size() && keys[i] != key) i++;
+            if (i == keys.size())
+            {
+                keys.push_back(key);
+                last.push_back(-1);
+            }
+            ids.push_back(i);
+            st = std::max(st, last[i] + 1);
+        }
+        for (size_t i : ids) last[i] = st;
+        stage[size_t(q)] = st;
+        nst = std::max(nst, st + 1);
+    }
+    order.clear();
+    p.h_stage_begin.assign(1, 0);
+    p.h_stage_terms.clear();
+    for (int st = 0; st < nst; st++)
+    {
+        int terms = 1;
+        for (int q = 0; q < p.nprod; q++)
+            if (stage[size_t(q)] == st)
+            {
+                order.push_back(q);
+                terms = std::max(terms, max_terms(&p.h_prods[size_t(q)], 1));
+            }
+        p.h_stage_begin.push_back(int(order.size()));
+        p.h_stage_terms.push_back(terms);
+    }
+}
+
 Status upload(Plan& p)
 {
     const int TK = tile_kdepth(p.tile, p.elem);
@@ -347,6 +398,14 @@ Status upload(Plan& p)
         ps.kf = kfast_check(p, p.h_prods, ps.prod_index, 1, ps.ms, ps.ns, ps.ks, TK, a_krun, b_krun);
     std::vector<ProdDesc> fast_prods;
     for (const ProdDesc& pd : p.h_prods) fast_prods.push_back(shifted(p, pd));
+    std::vector<int> order;
+    make_stages(p, order);
+    std::vector<ProdDesc> staged, fast_staged;
+    for (int q : order)
+    {
+        staged.push_back(p.h_prods[size_t(q)]);
+        fast_staged.push_back(fast_prods[size_t(q)]);
+    }
     if (p.core)
     {
         // dense S_p (ms x ks, column-major) and T_p (ks x ns): K strides ms and 1, terms at base offsets
@@ -388,6 +447,8 @@ Status upload(Plan& p)
          i_bc = add(p.h_batch_c.data(), p.h_batch_c.size() * 8),
          i_pr = add(p.h_prods.data(), p.h_prods.size() * sizeof(ProdDesc)),
          i_prf = add(fast_prods.data(), fast_prods.size() * sizeof(ProdDesc)),
+         i_pst = add(staged.data(), staged.size() * sizeof(ProdDesc)),
+         i_pfst = add(fast_staged.data(), fast_staged.size() * sizeof(ProdDesc)),
          i_sr = add(p.h_sub_roff.data(), p.h_sub_roff.size() * 8),
          i_sc = add(p.h_sub_coff.data(), p.h_sub_coff.size() * 8),
          i_tb = add(p.h_term_begin.data(), p.h_term_begin.size() * sizeof(int)),
@@ -425,6 +486,8 @@ Status upload(Plan& p)
     p.batch_c = reinterpret_cast<const int64_t*>(at(i_bc));
     p.prods = reinterpret_cast<const ProdDesc*>(at(i_pr));
     p.prods_fast = reinterpret_cast<const ProdDesc*>(at(i_prf));
+    p.prods_staged = reinterpret_cast<const ProdDesc*>(at(i_pst));
+    p.prods_fast_staged = reinterpret_cast<const ProdDesc*>(at(i_pfst));
     p.sub_roff = reinterpret_cast<const int64_t*>(at(i_sr));
     p.sub_coff = reinterpret_cast<const int64_t*>(at(i_sc));
     p.term_begin = reinterpret_cast<const int*>(at(i_tb));
@@ -539,16 +602,16 @@ Status run_plan(const Plan& p, const Call& c, hipStream_t stream)
         g.ks = p.ks;
         g.ka_stride = p.kf_core.sa;
         g.kb_stride = p.kf_core.sb;
-        g.nprod = 1;
-        const int terms = max_terms(p.h_prods.data(), p.nprod);
-        // the products in table order, one launch each: stream order serializes their C updates
-        for (int q = 0; q < p.nprod; q++)
+        // one launch per stage (products with disjoint C targets); stream order serializes the stages
+        for (size_t st = 0; st + 1 < p.h_stage_begin.size(); st++)
         {
-            g.prods = (p.kf_core.on ? p.prods_fast : p.prods) + q;
+            const int b = p.h_stage_begin[st], n = p.h_stage_begin[st + 1] - b;
+            g.prods = (p.kf_core.on ? p.prods_fast_staged : p.prods_staged) + b;
+            g.nprod = n;
             timing_begin(stream);
-            HIP_TRY(launch_leaf_gemm(p.elem, EPI_MULTI_C, p.tile, terms, p.kf_core.on, g, p.a_mfast, p.b_nfast,
-                                     stream));
-            timing_end(stream, leaf_flops(p, 1, p.ms, p.ns, p.ks));
+            HIP_TRY(launch_leaf_gemm(p.elem, EPI_MULTI_C, p.tile, p.h_stage_terms[st], p.kf_core.on, g, p.a_mfast,
+                                     p.b_nfast, stream));
+            timing_end(stream, leaf_flops(p, n, p.ms, p.ns, p.ks));
         }
     }
     else if (p.impl == IMPL_AB)

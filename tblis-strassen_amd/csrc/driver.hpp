@@ -77,6 +77,12 @@ struct Plan
     // Naive: dense products over S_p (ms x ks) and T_p (ks x ns)
     std::vector<ProdDesc> h_dense_prods;
 
+    // ABC launch stages: products [stage_begin[s], stage_begin[s + 1]) of the staged tables run in one
+    // launch; their C targets are disjoint, and every C sub-block sees its products in table order
+    std::vector<int> h_stage_begin, h_stage_terms;
+    const ProdDesc* prods_staged = nullptr;
+    const ProdDesc* prods_fast_staged = nullptr;
+
     // fast-K eligibility of the core launch and of the Naive dense launch
     KFast kf_core, kf_dense;
 

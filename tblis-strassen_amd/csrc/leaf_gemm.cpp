@@ -40,9 +40,8 @@ hipError_t launch_leaf_gemm(int elem_size, int epi, int tile, int terms, bool kf
     g.tiles_n = int((g.ns + te - 1) / te);
     if (g.tiles_m == 0 || g.tiles_n == 0 || g.nbatch == 0 || g.nprod == 0) return hipSuccess;
     if (g.ks <= 0) return hipErrorInvalidValue;  // callers route empty K to a scale of C
-    // one product per workgroup row: grid.y indexes the launch's products; a multi-C launch carries
-    // a single product (products that share C sub-blocks must run one after the other)
-    if (epi == EPI_MULTI_C && g.nprod != 1) return hipErrorInvalidValue;
+    // grid.y indexes the launch's products; the products of a multi-C launch must write disjoint C
+    // sub-blocks (the caller groups them so), since their read-modify-write updates are not ordered
     dim3 grid(unsigned(g.tiles_m) * unsigned(g.tiles_n), unsigned(g.nprod), unsigned(g.nbatch));
     if (elem_size == 8)
         return epi == EPI_STORE ? launch_leaf_f64_store(tile, terms, kfast, g, a_mfast, b_nfast, stream, grid)

@@ -17,6 +17,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "kernels.hpp"
 #include "launch.hpp"
 
@@ -79,6 +81,15 @@ __device__ __forceinline__ const __attribute__((address_space(4))) T* cptr(const
 
 __device__ __forceinline__ int64_t lmin(int64_t a, int64_t b) { return a < b ? a : b; }
 
+// Multi-term products in the fast body: with one register set the loads of each term land during
+// 1/TERMS of a tile's MFMA work; with two sets each load gets two phases.  Measured on gfx950 (128 x 128
+// tile, FP64): one set is faster for 2 terms (the register file is nearly full with two), two sets for 4.
+template <int TERMS>
+constexpr bool two_sets()
+{
+    return TERMS >= 4;
+}
+
 // Raw buffer loads for the fast body: a uniform base (the SGPR resource) plus a 32-bit lane offset,
 // one instruction per element.  An offset at or beyond kOob reads 0 (the resource's range check),
 // which zero-fills rows/columns past the tile edge and K past ks without branches or selects.
@@ -117,7 +128,7 @@ __device__ __forceinline__ int64_t readlane64(int64_t x, int l)
 }
 
 template <typename T, int TM, int TN, int TK, int TERMS, bool A_MFAST, bool B_NFAST, int EPI, bool KFAST>
-__global__ __launch_bounds__(256) void leaf_gemm_kernel(GemmArgs g)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void leaf_gemm_kernel(GemmArgs g)
 {
     constexpr int NT = 256;
     constexpr int WM = TM / 2, WN = TN / 2;  // 2x2 waves
@@ -305,57 +316,49 @@ __global__ __launch_bounds__(256) void leaf_gemm_kernel(GemmArgs g)
             bsh[t] = t < nB ? P.b[t].base : 0;
         }
 
-        // first K offsets of a tile (lane 0: A, lane 1: B), one vector load a tile ahead of use
+        const int64_t nkt = (ks + TK - 1) / TK;
+        // First K offsets of a tile (lane 0: A, lane 1: B): one vector load, read into SGPRs with
+        // v_readlane two tiles before the tile is loaded.
         int64_t kinfo = 0;
-        auto fetch_kinfo = [&](int64_t k0) { kinfo = (lane == 1 ? g.b_k : g.a_k)[k0]; };
-
-        // One sum term in flight at a time: the K loop runs in TERMS phases; phase t issues the loads
-        // of term t of the next K-tile, runs 1/TERMS of the current tile's MFMA work, then adds the
-        // scaled term into the next tile's LDS buffer (term 0 stores, later terms read-modify-write
-        // their own elements, so no barrier is needed between phases).  Register cost is that of a
-        // single-term GEMM whatever the number of terms.
-        T ra[EA], rb[EB];
-        auto issue = [&](int t, int64_t k0) {
-            const int64_t c0 = readlane64(kinfo, 0), r0 = readlane64(kinfo, 1);
-            const int klim = int(lmin(ks - 1 - k0, TK - 1));
-            const bool full = klim == TK - 1;  // a partial last tile reads 0 past ks
-            if (t < nA)
-            {
-                const __amdgpu_buffer_rsrc_t r = make_rsrc(Ag + ash[t] + c0);
-                if (full)
-                {
-#pragma unroll
-                    for (int e = 0; e < EA; e++) ra[e] = bload<T>(r, aoff[e]);
-                }
-                else
-                {
-#pragma unroll
-                    for (int e = 0; e < EA; e++) ra[e] = bload<T>(r, aoff[e] | past(a_k + e * A_KSTEP, klim));
-                }
-            }
-            if (t < nB)
-            {
-                const __amdgpu_buffer_rsrc_t r = make_rsrc(Bg + bsh[t] + r0);
-                if (full)
-                {
-#pragma unroll
-                    for (int e = 0; e < EB; e++) rb[e] = bload<T>(r, boff[e]);
-                }
-                else
-                {
-#pragma unroll
-                    for (int e = 0; e < EB; e++) rb[e] = bload<T>(r, boff[e] | past(b_k + e * B_KSTEP, klim));
-                }
-            }
+        auto fetch_kinfo = [&](int64_t kt) { kinfo = (lane == 1 ? g.b_k : g.a_k)[lmin(kt, nkt - 1) * TK]; };
+        struct KOff
+        {
+            int64_t a, b;
         };
-        auto put = [&](int t, int buf) {
+        auto read_kinfo = [&]() { return KOff{readlane64(kinfo, 0), readlane64(kinfo, 1)}; };
+
+        // Load jobs: job (tile kt, term t) reads term t of K-tile kt into one of two register sets;
+        // the K loop runs TERMS phases per tile, and phase q issues job q + TERMS + 1 and adds job
+        // q + TERMS (issued one phase earlier) into the next tile's LDS buffer -- term 0 stores, later
+        // terms read-modify-write their own elements, so phases need no barrier.  Every load has two
+        // phases of MFMA work to land in, and the register cost is two single-term sets whatever
+        // the number of terms.
+        constexpr bool kTwoSets = two_sets<TERMS>();
+        T ra[kTwoSets ? 2 : 1][EA], rb[kTwoSets ? 2 : 1][EB];
+        // Every issue executes its loads (a job past the last tile or a term the product lacks reads
+        // out of range, i.e. zeros with no memory traffic): the load count between any two points of
+        // the loop is then the same on every path, so each wait covers exactly the loads it needs
+        // (a skipped load on one path would make the waits count conservatively, down to vmcnt(0)).
+        auto issue = [&](auto set, int t, int64_t kt, KOff ko) {
+            constexpr int S = decltype(set)::value;
+            const int klim = kt < nkt ? int(lmin(ks - 1 - kt * TK, TK - 1)) : -1;  // k > klim reads 0
+            const int kla = t < nA ? klim : -1, klb = t < nB ? klim : -1;
+            const __amdgpu_buffer_rsrc_t ra_ = make_rsrc(Ag + ash[t] + ko.a);
+#pragma unroll
+            for (int e = 0; e < EA; e++) ra[S][e] = bload<T>(ra_, aoff[e] | past(a_k + e * A_KSTEP, kla));
+            const __amdgpu_buffer_rsrc_t rb_ = make_rsrc(Bg + bsh[t] + ko.b);
+#pragma unroll
+            for (int e = 0; e < EB; e++) rb[S][e] = bload<T>(rb_, boff[e] | past(b_k + e * B_KSTEP, klb));
+        };
+        auto put = [&](auto set, int t, int buf) {
+            constexpr int S = decltype(set)::value;
             if (t < nA)
             {
 #pragma unroll
                 for (int e = 0; e < EA; e++)
                 {
                     T& dst = sA[buf][(a_k + e * A_KSTEP) * LDA + a_m + e * A_MSTEP];
-                    const T v = acoef[t] * ra[e];
+                    const T v = acoef[t] * ra[S][e];
                     dst = t == 0 ? v : dst + v;
                 }
             }
@@ -365,11 +368,13 @@ __global__ __launch_bounds__(256) void leaf_gemm_kernel(GemmArgs g)
                 for (int e = 0; e < EB; e++)
                 {
                     T& dst = sB[buf][(b_k + e * B_KSTEP) * LDB + b_n + e * B_NSTEP];
-                    const T v = bcoef[t] * rb[e];
+                    const T v = bcoef[t] * rb[S][e];
                     dst = t == 0 ? v : dst + v;
                 }
             }
         };
+        typedef std::integral_constant<int, 0> S0;
+        typedef std::integral_constant<int, kTwoSets ? 1 : 0> S1;
 
         v4 acc[FM][FN];
 #pragma unroll
@@ -378,46 +383,101 @@ __global__ __launch_bounds__(256) void leaf_gemm_kernel(GemmArgs g)
             for (int j = 0; j < FN; j++) acc[i][j] = v4{0, 0, 0, 0};
 
         constexpr int K4 = TK / 4;
-        constexpr int PH = TERMS < K4 ? TERMS : K4;  // phases per K-tile
-        static_assert(TERMS <= K4 && K4 % PH == 0, "one phase per term");
-        const int64_t nkt = (ks + TK - 1) / TK;
-        fetch_kinfo(0);
+        static_assert(TERMS <= K4 && K4 % TERMS == 0, "one phase per term");
+        constexpr int KPH = K4 / TERMS;  // k4 steps per phase
+        auto mfma_steps = [&](const T* __restrict__ la, const T* __restrict__ lb, int k4_begin, int k4_end) {
 #pragma unroll
-        for (int t = 0; t < TERMS; t++)
-        {
-            issue(t, 0);
-            put(t, 0);
-        }
-        if (nkt > 1) fetch_kinfo(TK);
-        __syncthreads();
-        for (int64_t kt = 0; kt < nkt; kt++)
-        {
-            const int cur = int(kt & 1);
-            const bool more = kt + 1 < nkt;
-            const T* __restrict__ la = sA[cur];
-            const T* __restrict__ lb = sB[cur];
-#pragma unroll
-            for (int ph = 0; ph < PH; ph++)
+            for (int k4 = k4_begin; k4 < k4_end; k4++)
             {
-                if (more) issue(ph, (kt + 1) * TK);
-                if (ph == PH - 1 && kt + 2 < nkt) fetch_kinfo((kt + 2) * TK);
+                const int kr = k4 * 4 + (lane >> 4);
+                T af[FM], bf[FN];
 #pragma unroll
-                for (int k4 = ph * (K4 / PH); k4 < (ph + 1) * (K4 / PH); k4++)
-                {
-                    const int kr = k4 * 4 + (lane >> 4);
-                    T af[FM], bf[FN];
+                for (int i = 0; i < FM; i++) af[i] = la[kr * LDA + wm0 + i * 16 + (lane & 15)];
 #pragma unroll
-                    for (int i = 0; i < FM; i++) af[i] = la[kr * LDA + wm0 + i * 16 + (lane & 15)];
+                for (int j = 0; j < FN; j++) bf[j] = lb[kr * LDB + wn0 + j * 16 + (lane & 15)];
 #pragma unroll
-                    for (int j = 0; j < FN; j++) bf[j] = lb[kr * LDB + wn0 + j * 16 + (lane & 15)];
+                for (int i = 0; i < FM; i++)
 #pragma unroll
-                    for (int i = 0; i < FM; i++)
-#pragma unroll
-                        for (int j = 0; j < FN; j++) acc[i][j] = Mfma<T>::run(af[i], bf[j], acc[i][j]);
-                }
-                if (more) put(ph, cur ^ 1);
+                    for (int j = 0; j < FN; j++) acc[i][j] = Mfma<T>::run(af[i], bf[j], acc[i][j]);
             }
+        };
+
+        if constexpr (!kTwoSets)
+        {
+            // One register set: phase t of tile kt issues term t of tile kt + 1, runs 1/TERMS of the
+            // tile's MFMA work and adds the term into the next LDS buffer.
+            fetch_kinfo(0);
+            {
+                const KOff k0 = read_kinfo();
+#pragma unroll
+                for (int t = 0; t < TERMS; t++)
+                {
+                    issue(S0(), t, 0, k0);
+                    put(S0(), t, 0);
+                }
+            }
+            fetch_kinfo(1);
             __syncthreads();
+            for (int64_t kt = 0; kt < nkt; kt++)
+            {
+                const int cur = int(kt & 1);
+                const KOff kn = read_kinfo();
+                fetch_kinfo(kt + 2);
+#pragma unroll
+                for (int ph = 0; ph < TERMS; ph++)
+                {
+                    issue(S0(), ph, kt + 1, kn);
+                    mfma_steps(sA[cur], sB[cur], ph * KPH, (ph + 1) * KPH);
+                    if (kt + 1 < nkt) put(S0(), ph, cur ^ 1);
+                }
+                __syncthreads();
+            }
+        }
+        else
+        {
+            // prologue: tile 0 straight into buffer 0, job (1, 0) in flight, tile 2's offsets requested
+            fetch_kinfo(0);
+            KOff kcur = read_kinfo();
+#pragma unroll
+            for (int t = 0; t < TERMS; t++)
+            {
+                issue(S0(), t, 0, kcur);
+                put(S0(), t, 0);
+            }
+            fetch_kinfo(1);
+            kcur = read_kinfo();
+            fetch_kinfo(2);  // before the job's loads, as in the loop, so the first wait for it is exact
+            issue(S0(), 0, 1, kcur);  // job TERMS: set TERMS & 1 = 0
+            __syncthreads();
+            static_assert(TERMS % 2 == 0, "register-set choice must not depend on the tile parity");
+            for (int64_t kt = 0; kt < nkt; kt++)
+            {
+                const int cur = int(kt & 1);
+                const KOff knext = read_kinfo();
+                fetch_kinfo(kt + 3);
+#pragma unroll
+                for (int ph = 0; ph < TERMS; ph++)
+                {
+                    // phase q = kt * TERMS + ph issues job q + TERMS + 1 into set (ph + 1) & 1 --
+                    // (kt + 1, ph + 1), or (kt + 2, 0) in the last phase -- and puts job q + TERMS =
+                    // (kt + 1, ph) from set ph & 1
+                    if (ph + 1 < TERMS)
+                    {
+                        if ((ph & 1) == 0) issue(S1(), ph + 1, kt + 1, kcur);
+                        else issue(S0(), ph + 1, kt + 1, kcur);
+                    }
+                    else
+                        issue(S0(), 0, kt + 2, knext);
+                    mfma_steps(sA[cur], sB[cur], ph * KPH, (ph + 1) * KPH);
+                    if (kt + 1 < nkt)
+                    {
+                        if ((ph & 1) == 0) put(S0(), ph, cur ^ 1);
+                        else put(S1(), ph, cur ^ 1);
+                    }
+                }
+                kcur = knext;
+                __syncthreads();
+            }
         }
         epilogue(P, p, acc, true);
         return;
