@@ -350,6 +350,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void l
 #pragma unroll
             for (int e = 0; e < EB; e++) rb[S][e] = bload<T>(rb_, boff[e] | past(b_k + e * B_KSTEP, klb));
         };
+        // Term sums: term 0 stores its scaled values into the LDS buffer, later terms read-modify-write
+        // their own elements (a running sum in registers measured slower: it costs the registers that
+        // hide load latency).
         auto put = [&](auto set, int t, int buf) {
             constexpr int S = decltype(set)::value;
             if (t < nA)
