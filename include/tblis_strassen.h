@@ -176,9 +176,12 @@ typedef struct mistra_contraction
 int mistra_batch_mult(int count, const mistra_contraction* entries, unsigned flags, void* stream);
 
 /* ---- Adjoints of C = alpha*A.B (+beta*C) ------------------------------------------------------ */
-/* dA[idx_A] = alpha * sum dC[idx_C] . B[idx_B]  and  dB[idx_B] = alpha * sum A[idx_A] . dC[idx_C],
- * each computed as a Strassen contraction of the given level/impl/precision (dA, dB overwritten
- * when beta_grad == 0, accumulated into otherwise). Either gradient pointer may be NULL. */
+/* dA[idx_A] = alpha * sum dC[idx_C] . B[idx_B] + beta_A * dA  and
+ * dB[idx_B] = alpha * sum A[idx_A] . dC[idx_C] + beta_B * dB, with alpha = scalar(A)*scalar(B)*scalar(dC)
+ * (the forward alpha times a scale of dC) and beta_A, beta_B the scalars of dA, dB (0 overwrites).
+ * Each is computed as a Strassen contraction of the given level/impl/precision, asynchronously on
+ * stream; the scalars of dA and dB are reset to 1 as for any C. Either gradient pointer may be NULL.
+ * No counterpart in the reference (its library has no adjoint entry point). */
 int mistra_tensor_mult_backward(const tblis_tensor* A, const label_type* idx_A,
                                 const tblis_tensor* B, const label_type* idx_B,
                                 const tblis_tensor* dC, const label_type* idx_C,

@@ -101,6 +101,21 @@ double scalar_as_double(const tblis::tblis_tensor* t)
     return t->type == tblis::TYPE_DOUBLE ? scalar_of<double>(t) : double(scalar_of<float>(t));
 }
 
+}  // namespace
+
+void set_scalar(tblis::tblis_tensor* t, double v)
+{
+    if (t->type == tblis::TYPE_DOUBLE)
+        std::memcpy(t->scalar, &v, sizeof(v));
+    else
+    {
+        const float f = float(v);
+        std::memcpy(t->scalar, &f, sizeof(f));
+    }
+}
+
+namespace {
+
 void set_scalar_one(tblis::tblis_tensor* t)
 {
     if (t->type == tblis::TYPE_DOUBLE)
@@ -420,11 +435,33 @@ int mistra_batch_mult(int, const mistra_contraction*, unsigned, void*)
     return fail(Status::error(MISTRA_ERR_UNSUPPORTED, "batching not available in this build"));
 }
 
-int mistra_tensor_mult_backward(const tblis_tensor*, const label_type*, const tblis_tensor*, const label_type*,
-                                const tblis_tensor*, const label_type*, tblis_tensor*, tblis_tensor*, int, int, int,
-                                void*)
+int mistra_tensor_mult_backward(const tblis_tensor* A, const label_type* idx_A, const tblis_tensor* B,
+                                const label_type* idx_B, const tblis_tensor* dC, const label_type* idx_C,
+                                tblis_tensor* dA, tblis_tensor* dB, int level, int impl, int precision, void* stream)
 {
-    return fail(Status::error(MISTRA_ERR_UNSUPPORTED, "adjoints not available in this build"));
+    // The adjoints of a contraction are contractions over the same labels with the roles exchanged:
+    // dA[idx_A] = alpha * dC[idx_C] . B[idx_B] (summed over the labels of B not in A), and
+    // dB[idx_B] = alpha * A[idx_A] . dC[idx_C]; alpha = scalar(A) * scalar(B) * scalar(dC).
+    mistra::g_last_error.clear();
+    if (!A || !B || !dC) return fail(Status::error(MISTRA_ERR_ARG, "null tensor"));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const double aA = mistra::scalar_as_double(A), aB = mistra::scalar_as_double(B),
+                 aC = mistra::scalar_as_double(dC);
+    if (dA)
+    {
+        tblis_tensor X = *dC, Y = *B;
+        mistra::set_scalar(&X, aA * aC);
+        Status st = mistra::tensor_mult(&X, idx_C, &Y, idx_B, dA, idx_A, level, impl, precision, 0, s, false);
+        if (!st.ok()) return fail(st);
+    }
+    if (dB)
+    {
+        tblis_tensor X = *A, Y = *dC;
+        mistra::set_scalar(&Y, aB * aC);
+        Status st = mistra::tensor_mult(&X, idx_A, &Y, idx_C, dB, idx_B, level, impl, precision, 0, s, false);
+        if (!st.ok()) return fail(st);
+    }
+    return 0;
 }
 
 int mistra_autotune(const tblis_tensor*, const label_type*, const tblis_tensor*, const label_type*,

@@ -155,6 +155,23 @@ def tensor_mult(alpha, A, idx_A, B, idx_B, beta, C, idx_C, level=1, impl="abc", 
                                   _stream_ptr(stream)))
 
 
+def tensor_mult_backward(alpha, A, idx_A, B, idx_B, dC, idx_C, dA=None, dB=None, beta=0.0, level=1, impl="abc",
+                         precision="native", stream=None):
+    """Adjoints of C = alpha*A.B: dA = alpha*dC.B + beta*dA and dB = alpha*A.dC + beta*dB over the same
+    labels, each as a Strassen contraction (asynchronous on the stream).  dA or dB may be None."""
+    lib = load()
+    tA, tB, tC = make_tensor(A, alpha), make_tensor(B, 1.0), make_tensor(dC, 1.0)
+    gA = make_tensor(dA, beta) if dA is not None else None
+    gB = make_tensor(dB, beta) if dB is not None else None
+    impl_c = IMPL[impl] if isinstance(impl, str) else int(impl)
+    prec_c = PREC[precision] if isinstance(precision, str) else int(precision)
+    _check(lib.mistra_tensor_mult_backward(ctypes.byref(tA), idx_A.encode(), ctypes.byref(tB), idx_B.encode(),
+                                           ctypes.byref(tC), idx_C.encode(),
+                                           ctypes.byref(gA) if gA is not None else None,
+                                           ctypes.byref(gB) if gB is not None else None, level, impl_c, prec_c,
+                                           _stream_ptr(stream)))
+
+
 def _dropin(symbol, alpha, A, idx_A, B, idx_B, beta, C, idx_C):
     lib = load()
     tA, tB, tC = make_tensor(A, alpha), make_tensor(B, 1.0), make_tensor(C, beta)
