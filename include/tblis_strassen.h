@@ -62,6 +62,17 @@ typedef struct tblis_tensor
     stride_type* stride;
 } tblis_tensor;
 
+/* src/util/basic_types.h:264-272 (same member order and alignment: 64 bytes on LP64). */
+typedef struct tblis_matrix
+{
+    type_t type;
+    int conj;
+    char scalar[16] __attribute__((__aligned__(8)));
+    void* data;
+    len_type m, n;
+    stride_type rs, cs;
+} tblis_matrix;
+
 /* src/util/thread.h:8 -- a one-thread communicator; here only a distinguishable non-null pointer. */
 extern const tblis_comm* const tblis_single;
 
@@ -160,6 +171,16 @@ int mistra_copy_bytes(void* dst, const void* src, long long nbytes, void* stream
 /* Relative Frobenius distance ||X - Y||_F / ||Y||_F between two device tensors of the same
  * dense element count n (type TYPE_FLOAT or TYPE_DOUBLE; reduction in FP64 on the device). */
 int mistra_rel_error(int type, const void* X, const void* Y, long long n, double* out, void* stream);
+
+/* ---- Matrix Strassen (src/stra_iface/3m/stra_mult.h:17-27) ------------------------------------ */
+/* C = alpha*A.B + beta*C for strided matrices (alpha = scalar(A)*scalar(B), beta = scalar(C)), one
+ * Strassen level: ABC, Naive and AB respectively. Synchronous; errors via mistra_last_error(). */
+void stra_tblis_matrix_mult(const tblis_comm* comm, const tblis_config* cfg,
+                            const tblis_matrix* A, const tblis_matrix* B, tblis_matrix* C);
+void stra_tblis_matrix_mult_naive(const tblis_comm* comm, const tblis_config* cfg,
+                                  const tblis_matrix* A, const tblis_matrix* B, tblis_matrix* C);
+void stra_tblis_matrix_mult_ab(const tblis_comm* comm, const tblis_config* cfg,
+                               const tblis_matrix* A, const tblis_matrix* B, tblis_matrix* C);
 
 /* ---- Batching: many independent contractions in one persistent launch ------------------------ */
 typedef struct mistra_contraction

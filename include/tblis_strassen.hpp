@@ -175,6 +175,51 @@ void stra_mult_ex(T alpha, const ViewA& A, const label_type* idx_A, const ViewB&
     if (rc != MISTRA_OK) throw std::runtime_error(std::string("mistra_tensor_mult: ") + mistra_last_error());
 }
 
+// Matrix Strassen, src/stra_iface/3m/stra_mult.h:29-100: any view with data(), length(d), stride(d).
+namespace detail {
+template <typename T, typename View>
+tblis_matrix make_matrix(T alpha, const View& v)
+{
+    tblis_matrix t;
+    t.type = type_tag<T>::value;
+    t.conj = 0;
+    for (auto& c : t.scalar) c = 0;
+    *reinterpret_cast<T*>(t.scalar) = alpha;
+    t.data = const_cast<void*>(static_cast<const void*>(v.data()));
+    t.m = v.length(0);
+    t.n = v.length(1);
+    t.rs = v.stride(0);
+    t.cs = v.stride(1);
+    return t;
+}
+template <typename T, typename VA, typename VB, typename VC>
+void matrix_call(void (*fn)(const tblis_comm*, const tblis_config*, const tblis_matrix*, const tblis_matrix*,
+                            tblis_matrix*),
+                 T alpha, const VA& A, const VB& B, T beta, const VC& C)
+{
+    tblis_matrix A_s = make_matrix(alpha, A), B_s = make_matrix(T(1), B), C_s = make_matrix(beta, C);
+    fn(nullptr, nullptr, &A_s, &B_s, &C_s);
+    const char* err = mistra_last_error();
+    if (err && *err) throw std::runtime_error(err);
+}
+}  // namespace detail
+
+template <typename T, typename VA, typename VB, typename VC>
+void stra_mult(T alpha, const VA& A, const VB& B, T beta, const VC& C)
+{
+    detail::matrix_call(stra_tblis_matrix_mult, alpha, A, B, beta, C);
+}
+template <typename T, typename VA, typename VB, typename VC>
+void stra_mult_naive(T alpha, const VA& A, const VB& B, T beta, const VC& C)
+{
+    detail::matrix_call(stra_tblis_matrix_mult_naive, alpha, A, B, beta, C);
+}
+template <typename T, typename VA, typename VB, typename VC>
+void stra_mult_ab(T alpha, const VA& A, const VB& B, T beta, const VC& C)
+{
+    detail::matrix_call(stra_tblis_matrix_mult_ab, alpha, A, B, beta, C);
+}
+
 // Extension: adjoints dA = alpha * dC . B^T and dB = alpha * A^T . dC as Strassen contractions.
 template <typename T, typename ViewA, typename ViewB, typename ViewC, typename ViewGA, typename ViewGB>
 void stra_mult_backward(T alpha, const ViewA& A, const label_type* idx_A, const ViewB& B, const label_type* idx_B,

@@ -188,6 +188,205 @@ void timing_end(hipStream_t s, double flops)
     g_cur_start = nullptr;
 }
 
+namespace {
+
+struct Dim
+{
+    char label;
+    int64_t len, stride;
+};
+
+// Repeated labels of one tensor address its diagonal: one dimension with the summed stride
+// (src/util/tensor.hpp:305-342).
+Status merged_dims(const tblis::tblis_tensor* t, const label_type* idx, std::vector<Dim>& out)
+{
+    out.clear();
+    for (unsigned i = 0; i < t->ndim; i++)
+    {
+        bool found = false;
+        for (Dim& d : out)
+            if (d.label == idx[i])
+            {
+                if (d.len != t->len[i]) return Status::error(MISTRA_ERR_ARG, "repeated label with different lengths");
+                d.stride += t->stride[i];
+                found = true;
+            }
+        if (!found) out.push_back(Dim{idx[i], t->len[i], t->stride[i]});
+    }
+    if (out.size() > 8) return Status::error(MISTRA_ERR_UNSUPPORTED, "more than 8 distinct labels in one tensor");
+    return Status();
+}
+
+bool has_label(const std::vector<Dim>& v, char l)
+{
+    for (const Dim& d : v)
+        if (d.label == l) return true;
+    return false;
+}
+
+// A dense (first index fastest) temporary with the given labels and lengths, as a tblis_tensor.
+struct Temp
+{
+    std::string labels;
+    std::vector<ptrdiff_t> len, stride;
+    void* data = nullptr;
+    tblis::tblis_tensor t;
+};
+
+void make_temp(Temp& x, const std::vector<Dim>& dims, int type)
+{
+    x.labels.clear();
+    x.len.clear();
+    x.stride.clear();
+    int64_t st = 1;
+    for (const Dim& d : dims)
+    {
+        x.labels.push_back(d.label);
+        x.len.push_back(d.len);
+        x.stride.push_back(st);
+        st *= d.len;
+    }
+    std::memset(&x.t, 0, sizeof(x.t));
+    x.t.type = tblis::type_t(type);
+    x.t.ndim = unsigned(dims.size());
+    x.t.len = x.len.data();
+    x.t.stride = x.stride.data();
+}
+
+int64_t volume(const std::vector<Dim>& dims)
+{
+    int64_t n = 1;
+    for (const Dim& d : dims) n *= d.len;
+    return n;
+}
+
+}  // namespace
+
+Status tensor_mult(const tblis::tblis_tensor* A, const label_type* idx_A, const tblis::tblis_tensor* B,
+                   const label_type* idx_B, tblis::tblis_tensor* C, const label_type* idx_C, int level, int impl,
+                   int precision, unsigned flags, hipStream_t stream, bool synchronous);
+
+// General index classes, the classical fallback of the reference (stra_mult_blas,
+// src/stra_internal/3t/stra_mult.cxx:1311-1382): labels only in A (only in B) are summed out of A
+// (B) into dense temporaries, the remaining pure contraction runs through the Strassen path with the
+// requested level and impl, and labels only in C receive the result by broadcast: C = R + beta*C.
+Status general_mult(const tblis::tblis_tensor* A, const label_type* idx_A, const void* dA,
+                    const tblis::tblis_tensor* B, const label_type* idx_B, const void* dB,
+                    const tblis::tblis_tensor* C, const label_type* idx_C, void* dC, int level, int impl,
+                    int precision, unsigned flags, double alpha, double beta, int elem, hipStream_t stream)
+{
+    std::vector<Dim> a, b, c;
+    Status st = merged_dims(A, idx_A, a);
+    if (st.ok()) st = merged_dims(B, idx_B, b);
+    if (st.ok()) st = merged_dims(C, idx_C, c);
+    if (!st.ok()) return st;
+    std::vector<void*> temps;
+    auto release = [&]() {
+        for (void* p : temps) (void)hipFreeAsync(p, stream);
+    };
+    auto alloc = [&](size_t bytes, void** out) -> Status {
+        hipError_t e = hipMallocAsync(out, bytes ? bytes : 1, stream);
+        if (e != hipSuccess) return Status::error(MISTRA_ERR_NOMEM, std::string("general path: ") + hipGetErrorString(e));
+        temps.push_back(*out);
+        return Status();
+    };
+
+    // operand X with its one-operand labels summed out
+    auto prepare = [&](const std::vector<Dim>& x, const std::vector<Dim>& o1, const std::vector<Dim>& o2,
+                       const void* data, Temp& out) -> Status {
+        std::vector<Dim> keep, red;
+        for (const Dim& d : x) (has_label(o1, d.label) || has_label(o2, d.label) ? keep : red).push_back(d);
+        make_temp(out, keep, C->type);
+        if (red.empty())
+        {
+            // no reduction: view the operand itself with its merged labels
+            for (size_t i = 0; i < keep.size(); i++) out.stride[i] = keep[i].stride;
+            out.data = const_cast<void*>(data);
+        }
+        else
+        {
+            Status s2 = alloc(size_t(volume(keep)) * elem, &out.data);
+            if (!s2.ok()) return s2;
+            ReduceArgs ra;
+            std::memset(&ra, 0, sizeof(ra));
+            ra.out.ndim = int(keep.size());
+            for (size_t i = 0; i < keep.size(); i++)
+            {
+                ra.out.len[i] = keep[i].len;
+                ra.out.stride[i] = keep[i].stride;
+            }
+            ra.red.ndim = int(red.size());
+            for (size_t i = 0; i < red.size(); i++)
+            {
+                ra.red.len[i] = red[i].len;
+                ra.red.stride[i] = red[i].stride;
+            }
+            hipError_t e = launch_reduce(elem, data, out.data, ra, stream);
+            if (e != hipSuccess) return Status::error(MISTRA_ERR_HIP, hipGetErrorString(e));
+        }
+        out.t.data = out.data;
+        return Status();
+    };
+    Temp ta, tb, tc;
+    st = prepare(a, b, c, dA, ta);
+    if (st.ok()) st = prepare(b, a, c, dB, tb);
+    if (!st.ok())
+    {
+        release();
+        return st;
+    }
+    set_scalar(&ta.t, alpha);
+    set_scalar(&tb.t, 1.0);
+
+    std::vector<Dim> keep;
+    bool conly = false;
+    for (const Dim& d : c)
+    {
+        if (has_label(a, d.label) || has_label(b, d.label))
+            keep.push_back(d);
+        else
+            conly = true;
+    }
+    make_temp(tc, keep, C->type);
+    if (!conly)
+    {
+        for (size_t i = 0; i < keep.size(); i++) tc.stride[i] = keep[i].stride;
+        tc.t.data = dC;
+        set_scalar(&tc.t, beta);
+        st = tensor_mult(&ta.t, ta.labels.c_str(), &tb.t, tb.labels.c_str(), &tc.t, tc.labels.c_str(), level, impl,
+                         precision, flags & ~unsigned(MISTRA_FLAG_SYNC), stream, false);
+    }
+    else
+    {
+        st = alloc(size_t(volume(keep)) * elem, &tc.data);
+        if (st.ok())
+        {
+            tc.t.data = tc.data;
+            set_scalar(&tc.t, 0.0);
+            st = tensor_mult(&ta.t, ta.labels.c_str(), &tb.t, tb.labels.c_str(), &tc.t, tc.labels.c_str(), level,
+                             impl, precision, flags & ~unsigned(MISTRA_FLAG_SYNC), stream, false);
+        }
+        if (st.ok())
+        {
+            BcastArgs ba;
+            std::memset(&ba, 0, sizeof(ba));
+            ba.c.ndim = int(c.size());
+            for (size_t i = 0; i < c.size(); i++)
+            {
+                ba.c.len[i] = c[i].len;
+                ba.c.stride[i] = c[i].stride;
+                for (size_t j = 0; j < keep.size(); j++)
+                    if (keep[j].label == c[i].label) ba.r_stride[i] = tc.stride[j];
+            }
+            ba.beta = beta;
+            hipError_t e = launch_bcast_add(elem, tc.data, dC, ba, stream);
+            if (e != hipSuccess) st = Status::error(MISTRA_ERR_HIP, hipGetErrorString(e));
+        }
+    }
+    release();
+    return st;
+}
+
 Status tensor_mult(const tblis::tblis_tensor* A, const label_type* idx_A, const tblis::tblis_tensor* B,
                    const label_type* idx_B, tblis::tblis_tensor* C, const label_type* idx_C, int level, int impl,
                    int precision, unsigned flags, hipStream_t stream, bool synchronous)
@@ -269,9 +468,15 @@ Status tensor_mult(const tblis::tblis_tensor* A, const label_type* idx_A, const 
         }
         else
         {
-            Call call{dA, dB, dC, alpha, beta};
-            if (plan->transposed) std::swap(call.A, call.B);
-            st = run_plan(*plan, call, stream);
+            if (plan->general)
+                st = general_mult(A, idx_A, dA, B, idx_B, dB, C, idx_C, dC, level, impl, precision, flags, alpha, beta,
+                                  elem, stream);
+            else
+            {
+                Call call{dA, dB, dC, alpha, beta};
+                if (plan->transposed) std::swap(call.A, call.B);
+                st = run_plan(*plan, call, stream);
+            }
         }
         if (!st.ok())
         {
@@ -333,6 +538,82 @@ void tblis_tensor_mult(const tblis_comm*, const tblis_config*, const tblis_tenso
     Status s = mistra::tensor_mult(A, idx_A, B, idx_B, C, idx_C, 0, mistra::IMPL_ABC, MISTRA_PREC_NATIVE, 0, nullptr,
                                    true);
     if (!s.ok()) fail(s);
+}
+
+}  // extern "C"
+}  // namespace tblis
+
+namespace mistra {
+namespace {
+
+// The matrix entry points are contractions C[ij] = A[ik] B[kj] with two-dimensional views
+// (src/stra_iface/3m/stra_mult.cxx: 1-level Strassen, impl by symbol).
+void matrix_mult(const tblis::tblis_matrix* A, const tblis::tblis_matrix* B, tblis::tblis_matrix* C, int impl)
+{
+    g_last_error.clear();
+    if (!A || !B || !C)
+    {
+        fail(Status::error(MISTRA_ERR_ARG, "null matrix"));
+        return;
+    }
+    if (A->m != C->m || B->n != C->n || A->n != B->m)
+    {
+        fail(Status::error(MISTRA_ERR_ARG, "matrix dimensions do not conform"));
+        return;
+    }
+    if (A->conj || B->conj || C->conj)
+    {
+        fail(Status::error(MISTRA_ERR_UNSUPPORTED, "conjugation is not supported"));
+        return;
+    }
+    ptrdiff_t la[2] = {A->m, A->n}, sa[2] = {A->rs, A->cs};
+    ptrdiff_t lb[2] = {B->m, B->n}, sb[2] = {B->rs, B->cs};
+    ptrdiff_t lc[2] = {C->m, C->n}, sc[2] = {C->rs, C->cs};
+    tblis::tblis_tensor tA, tB, tC;
+    for (auto* x : {&tA, &tB, &tC}) std::memset(x, 0, sizeof(*x));
+    auto fill = [](tblis::tblis_tensor& t, const tblis::tblis_matrix* m, ptrdiff_t* len, ptrdiff_t* str) {
+        t.type = m->type;
+        std::memcpy(t.scalar, m->scalar, sizeof(t.scalar));
+        t.data = m->data;
+        t.ndim = 2;
+        t.len = len;
+        t.stride = str;
+    };
+    fill(tA, A, la, sa);
+    fill(tB, B, lb, sb);
+    fill(tC, C, lc, sc);
+    Status st = tensor_mult(&tA, "ik", &tB, "kj", &tC, "ij", 1, impl, MISTRA_PREC_NATIVE, 0, nullptr, true);
+    if (!st.ok())
+    {
+        fail(st);
+        return;
+    }
+    std::memcpy(C->scalar, tC.scalar, sizeof(C->scalar));
+    C->conj = 0;
+}
+
+}  // namespace
+}  // namespace mistra
+
+namespace tblis {
+extern "C" {
+
+void stra_tblis_matrix_mult(const tblis_comm*, const tblis_config*, const tblis_matrix* A, const tblis_matrix* B,
+                            tblis_matrix* C)
+{
+    mistra::matrix_mult(A, B, C, mistra::IMPL_ABC);
+}
+
+void stra_tblis_matrix_mult_naive(const tblis_comm*, const tblis_config*, const tblis_matrix* A,
+                                  const tblis_matrix* B, tblis_matrix* C)
+{
+    mistra::matrix_mult(A, B, C, mistra::IMPL_NAIVE);
+}
+
+void stra_tblis_matrix_mult_ab(const tblis_comm*, const tblis_config*, const tblis_matrix* A, const tblis_matrix* B,
+                               tblis_matrix* C)
+{
+    mistra::matrix_mult(A, B, C, mistra::IMPL_AB);
 }
 
 int mistra_tensor_mult(const tblis_tensor* A, const label_type* idx_A, const tblis_tensor* B, const label_type* idx_B,

@@ -292,4 +292,85 @@ hipError_t launch_scale(int elem_size, void* C, const StridedDims& d, double bet
     return hipGetLastError();
 }
 
+// ---- general index classes -----------------------------------------------------------------------------
+
+template <typename T>
+__global__ __launch_bounds__(256) void reduce_kernel(const T* __restrict__ src, T* __restrict__ dst, ReduceArgs a,
+                                                     int64_t n_out, int64_t n_red)
+{
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n_out; i += int64_t(gridDim.x) * 256)
+    {
+        int64_t rem = i, base = 0;
+        for (int k = 0; k < a.out.ndim; k++)
+        {
+            base += (rem % a.out.len[k]) * a.out.stride[k];
+            rem /= a.out.len[k];
+        }
+        double sum = 0.0;
+        for (int64_t j = 0; j < n_red; j++)
+        {
+            int64_t r = j, off = base;
+            for (int k = 0; k < a.red.ndim; k++)
+            {
+                off += (r % a.red.len[k]) * a.red.stride[k];
+                r /= a.red.len[k];
+            }
+            sum += double(src[off]);
+        }
+        dst[i] = T(sum);
+    }
+}
+
+hipError_t launch_reduce(int elem_size, const void* src, void* dst, const ReduceArgs& a, hipStream_t s)
+{
+    int64_t n_out = 1, n_red = 1;
+    for (int k = 0; k < a.out.ndim; k++) n_out *= a.out.len[k];
+    for (int k = 0; k < a.red.ndim; k++) n_red *= a.red.len[k];
+    if (n_out <= 0) return hipSuccess;
+    int64_t blocks = (n_out + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    if (elem_size == 8)
+        hipLaunchKernelGGL(reduce_kernel<double>, dim3(unsigned(blocks)), dim3(256), 0, s,
+                           static_cast<const double*>(src), static_cast<double*>(dst), a, n_out, n_red);
+    else
+        hipLaunchKernelGGL(reduce_kernel<float>, dim3(unsigned(blocks)), dim3(256), 0, s,
+                           static_cast<const float*>(src), static_cast<float*>(dst), a, n_out, n_red);
+    return hipGetLastError();
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void bcast_add_kernel(const T* __restrict__ R, T* __restrict__ C, BcastArgs a,
+                                                        int64_t n)
+{
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256)
+    {
+        int64_t rem = i, off = 0, roff = 0;
+        for (int k = 0; k < a.c.ndim; k++)
+        {
+            const int64_t x = rem % a.c.len[k];
+            rem /= a.c.len[k];
+            off += x * a.c.stride[k];
+            roff += x * a.r_stride[k];
+        }
+        const double r = double(R[roff]);
+        C[off] = a.beta == 0.0 ? T(r) : T(r + a.beta * double(C[off]));
+    }
+}
+
+hipError_t launch_bcast_add(int elem_size, const void* R, void* C, const BcastArgs& a, hipStream_t s)
+{
+    int64_t n = 1;
+    for (int k = 0; k < a.c.ndim; k++) n *= a.c.len[k];
+    if (n <= 0) return hipSuccess;
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    if (elem_size == 8)
+        hipLaunchKernelGGL(bcast_add_kernel<double>, dim3(unsigned(blocks)), dim3(256), 0, s,
+                           static_cast<const double*>(R), static_cast<double*>(C), a, n);
+    else
+        hipLaunchKernelGGL(bcast_add_kernel<float>, dim3(unsigned(blocks)), dim3(256), 0, s,
+                           static_cast<const float*>(R), static_cast<float*>(C), a, n);
+    return hipGetLastError();
+}
+
 }  // namespace mistra

@@ -86,4 +86,22 @@ struct StridedDims
 };
 hipError_t launch_scale(int elem_size, void* C, const StridedDims& d, double beta, hipStream_t stream);
 
+// General index classes (the classical fallback of src/stra_internal/3t/stra_mult.cxx:1311-1382):
+// dst (dense, first index fastest over `out`) = sum over the `red` dims of src.
+struct ReduceArgs
+{
+    StridedDims out;  // lengths and source strides of the kept dims
+    StridedDims red;  // lengths and source strides of the summed dims
+};
+hipError_t launch_reduce(int elem_size, const void* src, void* dst, const ReduceArgs& a, hipStream_t stream);
+
+// C = R + beta*C over every element of C; R is read with stride 0 along C's C-only dims.
+struct BcastArgs
+{
+    StridedDims c;  // lengths and strides of C
+    int64_t r_stride[8];
+    double beta;
+};
+hipError_t launch_bcast_add(int elem_size, const void* R, void* C, const BcastArgs& a, hipStream_t stream);
+
 }  // namespace mistra

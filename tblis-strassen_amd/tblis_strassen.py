@@ -26,6 +26,21 @@ class MistraError(RuntimeError):
     pass
 
 
+class TblisMatrix(ctypes.Structure):
+    """Byte image of tblis_matrix (src/util/basic_types.h:264-272)."""
+
+    _fields_ = [
+        ("type", ctypes.c_int),
+        ("conj", ctypes.c_int),
+        ("scalar", ctypes.c_char * 16),
+        ("data", ctypes.c_void_p),
+        ("m", ctypes.c_ssize_t),
+        ("n", ctypes.c_ssize_t),
+        ("rs", ctypes.c_ssize_t),
+        ("cs", ctypes.c_ssize_t),
+    ]
+
+
 class TblisTensor(ctypes.Structure):
     """Byte image of tblis_tensor (src/util/basic_types.h:341-419)."""
 
@@ -170,6 +185,37 @@ def tensor_mult_backward(alpha, A, idx_A, B, idx_B, dC, idx_C, dA=None, dB=None,
                                            ctypes.byref(gA) if gA is not None else None,
                                            ctypes.byref(gB) if gB is not None else None, level, impl_c, prec_c,
                                            _stream_ptr(stream)))
+
+
+def make_matrix(x, scalar=1.0):
+    """tblis_matrix over a 2-d torch tensor or numpy array (strides in elements)."""
+    shape, strides, ptr = _shape_strides_ptr(x)
+    if len(shape) != 2:
+        raise MistraError("a matrix needs two dimensions")
+    t = TblisMatrix()
+    t.type = _dtype_code(x)
+    t.conj = 0
+    buf = (ctypes.c_double(scalar) if t.type == TYPE_DOUBLE else ctypes.c_float(scalar))
+    ctypes.memmove(ctypes.addressof(t) + TblisMatrix.scalar.offset, ctypes.addressof(buf), ctypes.sizeof(buf))
+    t.data = ptr
+    t.m, t.n = shape
+    t.rs, t.cs = strides
+    return t
+
+
+def matrix_mult(alpha, A, B, beta, C, impl="abc"):
+    """Drop-in stra_tblis_matrix_mult{,_naive,_ab}: C = alpha*A.B + beta*C, one Strassen level. Synchronous."""
+    lib = load()
+    symbol = {"abc": "stra_tblis_matrix_mult", "naive": "stra_tblis_matrix_mult_naive",
+              "ab": "stra_tblis_matrix_mult_ab"}[impl]
+    tA, tB, tC = make_matrix(A, alpha), make_matrix(B, 1.0), make_matrix(C, beta)
+    fn = getattr(lib, symbol)
+    fn.argtypes = [ctypes.c_void_p, ctypes.c_void_p] + [ctypes.POINTER(TblisMatrix)] * 3
+    fn.restype = None
+    fn(None, None, ctypes.byref(tA), ctypes.byref(tB), ctypes.byref(tC))
+    msg = lib.mistra_last_error().decode()
+    if msg:
+        raise MistraError(msg)
 
 
 def _dropin(symbol, alpha, A, idx_A, B, idx_B, beta, C, idx_C):

@@ -196,3 +196,27 @@ def test_binding_writes_scalars_and_strides():
     assert [t.stride[0], t.stride[1]] == [1, 3]
     t32 = ts.make_tensor(np.zeros((2, 2), dtype=np.float32), -0.5)
     assert np.frombuffer(bytes(t32)[8:12], dtype=np.float32)[0] == -0.5
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    # the ctypes images used by the binding and the C structs of include/tblis_strassen.h agree byte for byte
+    src = tmp_path / "layout.c"
+    src.write_text(
+        '#include <stdio.h>\n#include <stddef.h>\n#include "tblis_strassen.h"\n'
+        "int main(void) {\n"
+        '  printf("%zu %zu %zu %zu %zu %zu\\n", sizeof(tblis_tensor), offsetof(tblis_tensor, scalar),\n'
+        "         offsetof(tblis_tensor, data), offsetof(tblis_tensor, ndim), offsetof(tblis_tensor, len),\n"
+        "         offsetof(tblis_tensor, stride));\n"
+        '  printf("%zu %zu %zu %zu %zu\\n", sizeof(tblis_matrix), offsetof(tblis_matrix, scalar),\n'
+        "         offsetof(tblis_matrix, data), offsetof(tblis_matrix, m), offsetof(tblis_matrix, cs));\n"
+        "  return 0; }\n")
+    exe = tmp_path / "layout"
+    import subprocess
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    T, M = ts.TblisTensor, ts.TblisMatrix
+    assert [int(x) for x in out[0].split()] == [ctypes.sizeof(T), T.scalar.offset, T.data.offset, T.ndim.offset,
+                                                T.len.offset, T.stride.offset]
+    assert [int(x) for x in out[1].split()] == [ctypes.sizeof(M), M.scalar.offset, M.data.offset, M.m.offset,
+                                                M.cs.offset]
+    assert ctypes.sizeof(T) == 56 and ctypes.sizeof(M) == 64
