@@ -20,6 +20,7 @@ SOURCES = [
     "driver.cpp",
     "api.cpp",
     "debug.cpp",
+    "autotune.cpp",
     "leaf_gemm.cpp",
     "leaf_f64_multi.hip",
     "leaf_f64_store.hip",

@@ -262,10 +262,6 @@ int64_t volume(const std::vector<Dim>& dims)
 
 }  // namespace
 
-Status tensor_mult(const tblis::tblis_tensor* A, const label_type* idx_A, const tblis::tblis_tensor* B,
-                   const label_type* idx_B, tblis::tblis_tensor* C, const label_type* idx_C, int level, int impl,
-                   int precision, unsigned flags, hipStream_t stream, bool synchronous);
-
 // General index classes, the classical fallback of the reference (stra_mult_blas,
 // src/stra_internal/3t/stra_mult.cxx:1311-1382): labels only in A (only in B) are summed out of A
 // (B) into dense temporaries, the remaining pure contraction runs through the Strassen path with the
@@ -389,13 +385,22 @@ Status general_mult(const tblis::tblis_tensor* A, const label_type* idx_A, const
 
 Status tensor_mult(const tblis::tblis_tensor* A, const label_type* idx_A, const tblis::tblis_tensor* B,
                    const label_type* idx_B, tblis::tblis_tensor* C, const label_type* idx_C, int level, int impl,
-                   int precision, unsigned flags, hipStream_t stream, bool synchronous)
+                   int precision, unsigned flags, hipStream_t stream, bool synchronous, int tile)
 {
     Status st = validate(A, idx_A, B, idx_B, C, idx_C);
     if (!st.ok()) return st;
     if (precision != MISTRA_PREC_NATIVE)
         return Status::error(MISTRA_ERR_UNSUPPORTED, "precision mode not available in this build");
-    if (impl == MISTRA_IMPL_AUTO) impl = IMPL_ABC;
+    if (level < 0 || impl == MISTRA_IMPL_AUTO)
+    {
+        TuneChoice ch;
+        st = autotune(A, idx_A, B, idx_B, C, idx_C, false, stream, ch);
+        if (!st.ok()) return st;
+        if (level < 0) level = ch.level;
+        if (impl == MISTRA_IMPL_AUTO) impl = ch.impl;
+        if (tile < 0) tile = ch.tile;
+    }
+    if (level > 2) return Status::error(MISTRA_ERR_ARG, "level must be 0, 1 or 2");
     if (impl < 0 || impl > 4) return Status::error(MISTRA_ERR_ARG, "impl must be in 0..4");
 
     const double alpha = scalar_as_double(A) * scalar_as_double(B);
@@ -404,7 +409,7 @@ Status tensor_mult(const tblis::tblis_tensor* A, const label_type* idx_A, const 
     std::shared_ptr<Plan> plan;
     if (!empty_tensor(C) && alpha != 0.0 && !empty_tensor(A) && !empty_tensor(B))
     {
-        st = get_plan(A, idx_A, B, idx_B, C, idx_C, level, impl, -1, flags, plan);
+        st = get_plan(A, idx_A, B, idx_B, C, idx_C, level, impl, tile, flags, plan);
         if (!st.ok()) return st;
     }
     else if (A->type != C->type || B->type != C->type)
@@ -745,10 +750,18 @@ int mistra_tensor_mult_backward(const tblis_tensor* A, const label_type* idx_A, 
     return 0;
 }
 
-int mistra_autotune(const tblis_tensor*, const label_type*, const tblis_tensor*, const label_type*,
-                    const tblis_tensor*, const label_type*, int, int*, int*, int*, void*)
+int mistra_autotune(const tblis_tensor* A, const label_type* idx_A, const tblis_tensor* B, const label_type* idx_B,
+                    const tblis_tensor* C, const label_type* idx_C, int measure, int* level_out, int* impl_out,
+                    int* tile_out, void* stream)
 {
-    return fail(Status::error(MISTRA_ERR_UNSUPPORTED, "autotuner not available in this build"));
+    mistra::g_last_error.clear();
+    mistra::TuneChoice ch;
+    Status st = mistra::autotune(A, idx_A, B, idx_B, C, idx_C, measure != 0, static_cast<hipStream_t>(stream), ch);
+    if (!st.ok()) return fail(st);
+    if (level_out) *level_out = ch.level;
+    if (impl_out) *impl_out = ch.impl;
+    if (tile_out) *tile_out = ch.tile;
+    return 0;
 }
 
 }  // extern "C"

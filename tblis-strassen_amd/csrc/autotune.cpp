@@ -1,0 +1,223 @@
+// Autotuner: level, impl and register tile per contraction shape -- the GPU counterpart of the
+// reference's static per-ISA blocksize table and runtime kernel selection (src/configs/configs.hpp:89-268,
+// src/internal/3t/mult.cxx:26 for the impl global).  Two modes:
+//   * analytic: a time model of the launches a plan makes, from leaf-kernel rates measured on MI355X;
+//   * measured: every candidate timed on the device against a scratch copy of C, best kept per shape.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <sstream>
+
+#include "driver.hpp"
+
+namespace mistra {
+
+namespace {
+
+std::mutex g_tune_mutex;
+std::map<std::string, TuneChoice> g_measured;
+
+std::string shape_key(const tblis::tblis_tensor* A, const label_type* idx_A, const tblis::tblis_tensor* B,
+                      const label_type* idx_B, const tblis::tblis_tensor* C, const label_type* idx_C)
+{
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    std::ostringstream k;
+    k << dev << '|' << int(C->type);
+    for (auto t : {std::make_pair(A, idx_A), std::make_pair(B, idx_B), std::make_pair(static_cast<const tblis::tblis_tensor*>(C), idx_C)})
+    {
+        k << '|';
+        for (unsigned i = 0; i < t.first->ndim; i++)
+            k << t.second[i] << ':' << t.first->len[i] << ':' << t.first->stride[i] << ',';
+    }
+    return k.str();
+}
+
+// Leaf-kernel rates (TFLOP/s) measured on MI355X with the 8192^3 FP64 workload, by register tile and
+// number of sum terms (1, 2, 4); FP32 runs the same kernels at about twice the rate.
+double leaf_rate(int elem, int tile, int terms)
+{
+    static const double r128[3] = {62.0, 53.5, 45.0}, r64[3] = {57.0, 46.0, 38.0};
+    const int ti = terms <= 1 ? 0 : (terms == 2 ? 1 : 2);
+    const double r = tile == TILE_64 ? r64[ti] : r128[ti];
+    return elem == 8 ? r : 1.8 * r;
+}
+
+constexpr double kHbm = 2.7e12;       // bytes/s of the sum / combine kernels (measured: strided gathers)
+constexpr double kLaunch = 4.0e-6;    // seconds per launch (dispatch and tail)
+
+double launch_s(int elem, int tile, int terms, int64_t tiles, double flops)
+{
+    if (tiles <= 0 || flops <= 0) return 0.0;
+    const int64_t slots = tile == TILE_64 ? 1024 : 512;  // resident workgroups on 256 CUs
+    const double waves = std::ceil(double(tiles) / double(slots)) * double(slots) / double(tiles);
+    return flops / (leaf_rate(elem, tile, terms) * 1e12) * waves + kLaunch;
+}
+
+int64_t ntiles(int tile, int64_t ms, int64_t ns)
+{
+    const int64_t t = tile_extent(tile);
+    return ((ms + t - 1) / t) * ((ns + t - 1) / t);
+}
+
+}  // namespace
+
+double model_ms(const Plan& p0)
+{
+    Plan& p = const_cast<Plan&>(p0);
+    const int tile = p.tile;
+    const int e = p.elem;
+    double s = 0.0;
+    const int64_t nb = p.nbatch;
+    auto gemm = [&](int64_t ms, int64_t ns, int64_t ks, int nprod, int terms, int t) {
+        // work of whole register tiles (a partly filled tile costs as much as a full one)
+        const int64_t nt = ntiles(t, ms, ns) * nprod * nb;
+        const double te = double(tile_extent(t));
+        return launch_s(e, t, terms, nt, 2.0 * te * te * double(ks) * double(nt));
+    };
+    for (const Pass& f : p.fringe) s += gemm(f.ms, f.ns, f.ks, 1, 1, tile);
+    if (p.nprod == 0) return s * 1e3;
+    if (!p.core)
+        return (s + gemm(p.ms, p.ns, p.ks, 1, 1, tile)) * 1e3;
+    const double cbytes = double(p.ms) * p.ns * e * nb;
+    if (p.impl == IMPL_AB)
+    {
+        s += gemm(p.ms, p.ns, p.ks, p.nprod, max_terms(p.h_prods.data(), p.nprod), tile);
+        s += (double(p.nprod) + 2.0 * p.nsub) * cbytes / kHbm + kLaunch;  // combine: read M, update C
+    }
+    else if (p.impl == IMPL_NAIVE)
+    {
+        double terms = 0;
+        for (int q = 0; q < p.nprod; q++) terms += p.h_prods[q].nA * double(p.ms) * p.ks + p.h_prods[q].nB * double(p.ks) * p.ns;
+        const double sums = (terms + p.nprod * (double(p.ms) * p.ks + double(p.ks) * p.ns)) * e * nb;
+        s += sums / kHbm + 2 * kLaunch;
+        s += gemm(p.ms, p.ns, p.ks, p.nprod, 1, p.ms * p.ns >= int64_t(1024) * 1024 ? TILE_128 : TILE_64);
+        s += (double(p.nprod) + 2.0 * p.nsub) * cbytes / kHbm + kLaunch;
+    }
+    else
+    {
+        std::vector<int> order;
+        make_stages(p, order);
+        for (size_t st = 0; st + 1 < p.h_stage_begin.size(); st++)
+            s += gemm(p.ms, p.ns, p.ks, p.h_stage_begin[st + 1] - p.h_stage_begin[st], p.h_stage_terms[st], tile);
+    }
+    return s * 1e3;
+}
+
+Status autotune(const tblis::tblis_tensor* A, const label_type* idx_A, const tblis::tblis_tensor* B,
+                const label_type* idx_B, const tblis::tblis_tensor* C, const label_type* idx_C, bool measure,
+                hipStream_t stream, TuneChoice& out)
+{
+    const std::string key = shape_key(A, idx_A, B, idx_B, C, idx_C);
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mutex);
+        auto it = g_measured.find(key);
+        if (it != g_measured.end() && (it->second.measured || !measure))
+        {
+            out = it->second;
+            return Status();
+        }
+    }
+    struct Cand
+    {
+        int level, impl, tile;
+        double ms;
+    };
+    std::vector<Cand> cands;
+    for (int level = 0; level <= 2; level++)
+        for (int impl : {int(IMPL_ABC), int(IMPL_AB), int(IMPL_NAIVE)})
+        {
+            if (level == 0 && impl != IMPL_ABC) continue;
+            for (int tile : {int(TILE_128), int(TILE_64)})
+            {
+                Plan p;
+                Status st = build_plan_host(A, idx_A, B, idx_B, C, idx_C, level, impl, tile, 0, p);
+                if (!st.ok()) return st;
+                if (p.general)
+                {
+                    // the general path reduces operands first; its inner contraction takes the default
+                    out = TuneChoice{1, IMPL_ABC, -1, 0.0, false};
+                    return Status();
+                }
+                if (level > 0 && !p.core) continue;  // would only repeat the classical launch
+                cands.push_back(Cand{level, impl, tile, model_ms(p)});
+            }
+        }
+    std::sort(cands.begin(), cands.end(), [](const Cand& a, const Cand& b) { return a.ms < b.ms; });
+    TuneChoice best{cands[0].level, cands[0].impl, cands[0].tile, cands[0].ms, false};
+
+    if (measure)
+    {
+        for (const void* ptr : {static_cast<const void*>(A->data), static_cast<const void*>(B->data),
+                                static_cast<const void*>(C->data)})
+        {
+            hipPointerAttribute_t attr;
+            if (!ptr || hipPointerGetAttributes(&attr, ptr) != hipSuccess || attr.type != hipMemoryTypeDevice)
+            {
+                (void)hipGetLastError();
+                return Status::error(MISTRA_ERR_ARG, "measured tuning needs device tensors");
+            }
+        }
+        // time every candidate on a scratch copy of C (same strides), 1 warm-up + 3 timed calls each
+        const int elem = C->type == tblis::TYPE_DOUBLE ? 8 : 4;
+        int64_t lo = 0, hi = 0;
+        for (unsigned i = 0; i < C->ndim; i++)
+        {
+            if (C->len[i] <= 0) return Status::error(MISTRA_ERR_ARG, "empty C");
+            const int64_t ext = (C->len[i] - 1) * C->stride[i];
+            (ext < 0 ? lo : hi) += ext;
+        }
+        const size_t bytes = size_t(hi - lo + 1) * elem;
+        void* scratch = nullptr;
+        hipError_t e = hipMallocAsync(&scratch, bytes, stream);
+        if (e != hipSuccess) return Status::error(MISTRA_ERR_NOMEM, hipGetErrorString(e));
+        e = hipMemcpyAsync(scratch, static_cast<const char*>(C->data) + lo * elem, bytes, hipMemcpyDeviceToDevice,
+                           stream);
+        hipEvent_t t0, t1;
+        (void)hipEventCreate(&t0);
+        (void)hipEventCreate(&t1);
+        tblis::tblis_tensor Cs = *C;
+        Cs.data = static_cast<char*>(scratch) - lo * elem;
+        double best_ms = 1e30;
+        Status st;
+        for (const Cand& c : cands)
+        {
+            double ms_min = 1e30;
+            for (int r = 0; r < 4 && st.ok(); r++)
+            {
+                tblis::tblis_tensor Ct = Cs;  // tensor_mult resets the scalar of C after each call
+                (void)hipEventRecord(t0, stream);
+                st = tensor_mult(A, idx_A, B, idx_B, &Ct, idx_C, c.level, c.impl, MISTRA_PREC_NATIVE, 0, stream,
+                                 false, c.tile);
+                (void)hipEventRecord(t1, stream);
+                if (!st.ok()) break;
+                (void)hipEventSynchronize(t1);
+                float ms = 0;
+                (void)hipEventElapsedTime(&ms, t0, t1);
+                if (r > 0) ms_min = std::min(ms_min, double(ms));
+            }
+            if (!st.ok()) break;
+            if (ms_min < best_ms)
+            {
+                best_ms = ms_min;
+                best = TuneChoice{c.level, c.impl, c.tile, ms_min, true};
+            }
+        }
+        (void)hipEventDestroy(t0);
+        (void)hipEventDestroy(t1);
+        (void)hipFreeAsync(scratch, stream);
+        (void)hipStreamSynchronize(stream);
+        if (!st.ok()) return st;
+        if (e != hipSuccess) return Status::error(MISTRA_ERR_HIP, hipGetErrorString(e));
+    }
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mutex);
+        g_measured[key] = best;
+    }
+    out = best;
+    return Status();
+}
+
+}  // namespace mistra

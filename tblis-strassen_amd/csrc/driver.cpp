@@ -339,8 +339,6 @@ Status build_plan_host(const tblis::tblis_tensor* A, const label_type* idx_A, co
     return Status();
 }
 
-namespace {
-
 // Greedy launch stages for ABC: product q goes to the first stage after the last stage of every earlier
 // product that shares a C sub-block with it.  Products of one stage write disjoint C sub-blocks, and
 // each C sub-block receives its updates in product-table order, so the result is bitwise the serial one.
@@ -388,6 +386,8 @@ void make_stages(Plan& p, std::vector<int>& order)
         p.h_stage_terms.push_back(terms);
     }
 }
+
+namespace {
 
 Status upload(Plan& p)
 {

@@ -125,13 +125,32 @@ Status build_plan_host(const tblis::tblis_tensor* A, const label_type* idx_A, co
                        const label_type* idx_B, const tblis::tblis_tensor* C, const label_type* idx_C,
                        int level, int impl, int tile, unsigned flags, Plan& p);
 
+// ABC launch stages of a plan's core products (fills h_stage_begin / h_stage_terms; `order` = the
+// product indices in stage order).
+void make_stages(Plan& p, std::vector<int>& order);
+
 // Execute a plan on device pointers.
 Status run_plan(const Plan& p, const Call& c, hipStream_t stream);
 
-// Whole-call entry used by every C-ABI symbol.
+// Whole-call entry used by every C-ABI symbol.  level < 0 or impl == MISTRA_IMPL_AUTO: the autotuner's
+// choice (a measured one when mistra_autotune has timed this shape, else the analytic model); tile < 0:
+// the plan's default register tile.
 Status tensor_mult(const tblis::tblis_tensor* A, const label_type* idx_A, const tblis::tblis_tensor* B,
                    const label_type* idx_B, tblis::tblis_tensor* C, const label_type* idx_C, int level, int impl,
-                   int precision, unsigned flags, hipStream_t stream, bool synchronous);
+                   int precision, unsigned flags, hipStream_t stream, bool synchronous, int tile = -1);
+
+// Autotuner (autotune.cpp): level, impl and register tile for a contraction shape.
+struct TuneChoice
+{
+    int level = 1, impl = IMPL_ABC, tile = -1;
+    double ms = 0.0;        // measured (or modelled) milliseconds per call
+    bool measured = false;
+};
+Status autotune(const tblis::tblis_tensor* A, const label_type* idx_A, const tblis::tblis_tensor* B,
+                const label_type* idx_B, const tblis::tblis_tensor* C, const label_type* idx_C, bool measure,
+                hipStream_t stream, TuneChoice& out);
+// Modelled milliseconds of one candidate (host only).
+double model_ms(const Plan& p);
 
 // Kernel timing hook of the dominant leaf kernel.
 void timing_begin(hipStream_t s);

@@ -187,6 +187,19 @@ def tensor_mult_backward(alpha, A, idx_A, B, idx_B, dC, idx_C, dA=None, dB=None,
                                            _stream_ptr(stream)))
 
 
+def autotune(A, idx_A, B, idx_B, C, idx_C, measure=False, stream=None):
+    """(level, impl, tile) chosen for this contraction shape: the analytic model, or with measure=True
+    every candidate timed on the device (device tensors; C is not modified).  Cached per shape; calls
+    with level=-1 / impl="auto" use the cached choice."""
+    lib = load()
+    tA, tB, tC = make_tensor(A), make_tensor(B), make_tensor(C)
+    lv, im, ti = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    _check(lib.mistra_autotune(ctypes.byref(tA), idx_A.encode(), ctypes.byref(tB), idx_B.encode(), ctypes.byref(tC),
+                               idx_C.encode(), 1 if measure else 0, ctypes.byref(lv), ctypes.byref(im),
+                               ctypes.byref(ti), _stream_ptr(stream)))
+    return lv.value, im.value, ti.value
+
+
 def make_matrix(x, scalar=1.0):
     """tblis_matrix over a 2-d torch tensor or numpy array (strides in elements)."""
     shape, strides, ptr = _shape_strides_ptr(x)

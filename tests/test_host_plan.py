@@ -220,3 +220,20 @@ def test_struct_layouts_match_the_header(tmp_path):
     assert [int(x) for x in out[1].split()] == [ctypes.sizeof(M), M.scalar.offset, M.data.offset, M.m.offset,
                                                 M.cs.offset]
     assert ctypes.sizeof(T) == 56 and ctypes.sizeof(M) == 64
+
+
+def test_autotuner_analytic_model():
+    # host-only choice (no device data touched): big cubic FP64 contractions get 128x128 tiles and a
+    # Strassen level only where the model predicts a gain; tiny ones stay classical
+    import bench
+
+    case = bench.make_workload("c2")
+    A, B, C = (cases.host_tensor(case[f"shape_{x}"], np.zeros(1)) if False else
+               np.lib.stride_tricks.as_strided(np.zeros(1), shape=case[f"shape_{x}"],
+                                               strides=[0] * len(case[f"shape_{x}"])) for x in "ABC")
+    level, impl, tile = ts.autotune(A, case["idx_A"], B, case["idx_B"], C, case["idx_C"])
+    assert level in (0, 1, 2) and impl in (0, 3, 4) and tile == 0  # TILE_128
+    small = cases.make_case(3, m=(8,), n=(8,), k=(8,))
+    Z = [np.zeros(int(np.prod(small[f"shape_{x}"]))).reshape(small[f"shape_{x}"], order="F") for x in "ABC"]
+    level, impl, tile = ts.autotune(Z[0], small["idx_A"], Z[1], small["idx_B"], Z[2], small["idx_C"])
+    assert level == 0 and tile == 1  # TILE_64
