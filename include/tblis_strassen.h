@@ -123,7 +123,10 @@ enum
 {
     MISTRA_FLAG_REF_FRINGE = 1u << 0, /* drop the rows/cols/k that the 2^L split truncates, exactly as
                                          src/matrix/stra_tensor_view.hpp:246-247 does (default: computed) */
-    MISTRA_FLAG_SYNC       = 1u << 1  /* synchronize the stream before returning                          */
+    MISTRA_FLAG_SYNC       = 1u << 1, /* synchronize the stream before returning                          */
+    MISTRA_FLAG_GRAPH      = 1u << 2  /* mistra_batch_mult: capture the batch's launches in a HIP graph on
+                                         the first call and replay it on later calls with the same entries
+                                         (same pointers, shapes, scalars); needs a non-null stream        */
 };
 
 /* Error codes (0 = success). */
@@ -192,8 +195,12 @@ typedef struct mistra_contraction
     int impl;    /* -1 = autotuner's choice (only ABC is available inside a batch) */
 } mistra_contraction;
 
-/* Run count contractions (same element type; device data only) as one persistent launch.
- * Entries must not write overlapping C memory. */
+/* Run count independent contractions with few launches: the ABC launches of all entries are grouped
+ * by Strassen stage (round) and kernel variant, and each group is one grouped launch whose workgroups
+ * find their contraction in a device job table.  Entries must not write overlapping C memory.
+ * level/impl -1 take the autotuner's choice; entries that are not device-resident ABC contractions
+ * (host data, general index classes, AB/Naive) run as individual calls. Asynchronous on stream unless
+ * MISTRA_FLAG_SYNC. No counterpart in the reference (its harness loops over contractions). */
 int mistra_batch_mult(int count, const mistra_contraction* entries, unsigned flags, void* stream);
 
 /* ---- Adjoints of C = alpha*A.B (+beta*C) ------------------------------------------------------ */

@@ -21,6 +21,7 @@ SOURCES = [
     "api.cpp",
     "debug.cpp",
     "autotune.cpp",
+    "batch.cpp",
     "leaf_gemm.cpp",
     "leaf_f64_multi.hip",
     "leaf_f64_store.hip",

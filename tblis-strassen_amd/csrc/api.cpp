@@ -644,6 +644,7 @@ int mistra_get_impl(void) { return int(internal::impl); }
 
 int mistra_release_workspace(void)
 {
+    mistra::release_batch_graphs();
     mistra::release_all();
     return 0;
 }
@@ -716,9 +717,11 @@ int mistra_rel_error(int type, const void* X, const void* Y, long long n, double
     return 0;
 }
 
-int mistra_batch_mult(int, const mistra_contraction*, unsigned, void*)
+int mistra_batch_mult(int count, const mistra_contraction* entries, unsigned flags, void* stream)
 {
-    return fail(Status::error(MISTRA_ERR_UNSUPPORTED, "batching not available in this build"));
+    mistra::g_last_error.clear();
+    Status st = mistra::batch_mult(count, entries, flags, static_cast<hipStream_t>(stream));
+    return st.ok() ? 0 : fail(st);
 }
 
 int mistra_tensor_mult_backward(const tblis_tensor* A, const label_type* idx_A, const tblis_tensor* B,

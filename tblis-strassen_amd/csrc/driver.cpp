@@ -589,6 +589,49 @@ double leaf_flops(const Plan& p, int nprod, int64_t ms, int64_t ns, int64_t ks)
 
 }  // namespace
 
+Status plan_abc_jobs(const Plan& p, const Call& c, std::vector<LaunchJob>& out)
+{
+    if (p.general) return Status::error(MISTRA_ERR_UNSUPPORTED, "general index classes run outside a batch");
+    auto push = [&](GemmArgs g, int tile, int terms, bool kfast, int round, double flops) {
+        if (!kfast) tile = TILE_64;
+        const int te = tile_extent(tile);
+        g.tiles_m = int((g.ms + te - 1) / te);
+        g.tiles_n = int((g.ns + te - 1) / te);
+        if (g.tiles_m == 0 || g.tiles_n == 0 || g.nbatch == 0 || g.nprod == 0 || g.ks <= 0) return;
+        out.push_back(LaunchJob{g, p.elem, tile, terms, round, kfast, p.a_mfast, p.b_nfast, flops});
+    };
+    int round = 0;
+    if (p.nprod > 0)
+    {
+        GemmArgs g = base_args(p, c);
+        g.ms = p.ms;
+        g.ns = p.ns;
+        g.ks = p.ks;
+        g.ka_stride = p.kf_core.sa;
+        g.kb_stride = p.kf_core.sb;
+        for (size_t st = 0; st + 1 < p.h_stage_begin.size(); st++, round++)
+        {
+            const int b = p.h_stage_begin[st], n = p.h_stage_begin[st + 1] - b;
+            g.prods = (p.kf_core.on ? p.prods_fast_staged : p.prods_staged) + b;
+            g.nprod = n;
+            push(g, p.tile, p.h_stage_terms[st], p.kf_core.on, round, leaf_flops(p, n, p.ms, p.ns, p.ks));
+        }
+    }
+    for (const Pass& ps : p.fringe)
+    {
+        GemmArgs f = base_args(p, c);
+        f.ms = ps.ms;
+        f.ns = ps.ns;
+        f.ks = ps.ks;
+        f.prods = (ps.kf.on ? p.prods_fast : p.prods) + ps.prod_index;
+        f.nprod = 1;
+        f.ka_stride = ps.kf.sa;
+        f.kb_stride = ps.kf.sb;
+        push(f, TILE_64, 1, ps.kf.on, round++, leaf_flops(p, 1, ps.ms, ps.ns, ps.ks));
+    }
+    return Status();
+}
+
 Status run_plan(const Plan& p, const Call& c, hipStream_t stream)
 {
     if (p.general) return Status::error(MISTRA_ERR_UNSUPPORTED, "contractions with A-only, B-only or C-only indices are not supported yet");

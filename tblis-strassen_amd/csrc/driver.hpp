@@ -129,6 +129,17 @@ Status build_plan_host(const tblis::tblis_tensor* A, const label_type* idx_A, co
 // product indices in stage order).
 void make_stages(Plan& p, std::vector<int>& order);
 
+// One multi-C launch of an ABC plan, as the batch path groups them: launches of equal `round` may run
+// together (they write disjoint C); rounds run in order.
+struct LaunchJob
+{
+    GemmArgs g;  // tiles_m / tiles_n filled for the effective tile
+    int elem, tile, terms, round;
+    bool kfast, a_mfast, b_nfast;
+    double flops;
+};
+Status plan_abc_jobs(const Plan& p, const Call& c, std::vector<LaunchJob>& out);
+
 // Execute a plan on device pointers.
 Status run_plan(const Plan& p, const Call& c, hipStream_t stream);
 
@@ -151,6 +162,11 @@ Status autotune(const tblis::tblis_tensor* A, const label_type* idx_A, const tbl
                 hipStream_t stream, TuneChoice& out);
 // Modelled milliseconds of one candidate (host only).
 double model_ms(const Plan& p);
+
+// Batched contractions (batch.cpp).
+Status batch_mult(int count, const mistra_contraction* entries, unsigned flags, hipStream_t stream);
+void release_batch_graphs();
+double scalar_value(const tblis::tblis_tensor* t);
 
 // Kernel timing hook of the dominant leaf kernel.
 void timing_begin(hipStream_t s);

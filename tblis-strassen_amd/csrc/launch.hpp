@@ -26,6 +26,17 @@ hipError_t launch_leaf_gemm(int elem_size, int epi, int tile, int terms, bool kf
                             bool b_nfast, hipStream_t stream);
 int max_terms(const ProdDesc* prods, int nprod);
 
+// Grouped multi-C launch (batched contractions): device tables of jobs and of their first workgroups.
+struct Grouped
+{
+    const GemmArgs* jobs;
+    const int64_t* first;
+    int njobs;
+    int64_t blocks;
+};
+hipError_t launch_leaf_grouped(int elem_size, int tile, int terms, bool kfast, const Grouped& q, bool a_mfast,
+                               bool b_nfast, hipStream_t stream);
+
 // Explicit operand sums (Naive impl): for product p and its terms t,
 //   out[p*rows*cols + r + c*rows] = sum_t coeff_t * X[base_t + rs[roff_t + r] + cs[coff_t + c]]
 // `which` selects the A (0) or B (1) terms of each ProdDesc.

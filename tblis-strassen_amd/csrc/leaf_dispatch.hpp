@@ -19,6 +19,45 @@ hipError_t launch_variant(const GemmArgs& g, bool a_mfast, bool b_nfast, hipStre
     return hipGetLastError();
 }
 
+// Grouped launches (batched contractions), multi-C epilogue only.
+
+template <typename T, int TM, int TK, int TERMS, bool KF>
+hipError_t grouped_variant(const Grouped& q, bool a_mfast, bool b_nfast, hipStream_t s)
+{
+    const dim3 grid(unsigned(q.blocks));
+    if (a_mfast && b_nfast)
+        hipLaunchKernelGGL((leaf_gemm_grouped<T, TM, TM, TK, TERMS, true, true, EPI_MULTI_C, KF>), grid, dim3(256), 0,
+                           s, q.jobs, q.first, q.njobs);
+    else if (a_mfast)
+        hipLaunchKernelGGL((leaf_gemm_grouped<T, TM, TM, TK, TERMS, true, false, EPI_MULTI_C, KF>), grid, dim3(256),
+                           0, s, q.jobs, q.first, q.njobs);
+    else if (b_nfast)
+        hipLaunchKernelGGL((leaf_gemm_grouped<T, TM, TM, TK, TERMS, false, true, EPI_MULTI_C, KF>), grid, dim3(256),
+                           0, s, q.jobs, q.first, q.njobs);
+    else
+        hipLaunchKernelGGL((leaf_gemm_grouped<T, TM, TM, TK, TERMS, false, false, EPI_MULTI_C, KF>), grid, dim3(256),
+                           0, s, q.jobs, q.first, q.njobs);
+    return hipGetLastError();
+}
+
+template <typename T, int TM, int TK, bool KF>
+hipError_t grouped_terms(const Grouped& q, int terms, bool a_mfast, bool b_nfast, hipStream_t s)
+{
+    if (terms <= 1) return grouped_variant<T, TM, TK, 1, KF>(q, a_mfast, b_nfast, s);
+    if (terms == 2) return grouped_variant<T, TM, TK, 2, KF>(q, a_mfast, b_nfast, s);
+    return grouped_variant<T, TM, TK, 4, KF>(q, a_mfast, b_nfast, s);
+}
+
+template <typename T>
+hipError_t launch_grouped_typed(int tile, int terms, bool kfast, const Grouped& q, bool a_mfast, bool b_nfast,
+                                hipStream_t s)
+{
+    constexpr int TK = sizeof(T) == 8 ? 16 : 32;
+    if (!kfast) return grouped_terms<T, 64, TK, false>(q, terms, a_mfast, b_nfast, s);
+    if (tile == TILE_64) return grouped_terms<T, 64, TK, true>(q, terms, a_mfast, b_nfast, s);
+    return grouped_terms<T, 128, TK, true>(q, terms, a_mfast, b_nfast, s);
+}
+
 template <typename T, int TM, int TK, int EPI, bool KF>
 hipError_t launch_terms(const GemmArgs& g, int terms, bool a_mfast, bool b_nfast, hipStream_t s, dim3 grid)
 {

@@ -11,6 +11,19 @@ MISTRA_DECLARE_LEAF(launch_leaf_f64_store)
 MISTRA_DECLARE_LEAF(launch_leaf_f32_multi)
 MISTRA_DECLARE_LEAF(launch_leaf_f32_store)
 #undef MISTRA_DECLARE_LEAF
+hipError_t launch_leaf_f64_grouped(int tile, int terms, bool kfast, const Grouped& q, bool a_mfast, bool b_nfast,
+                                   hipStream_t s);
+hipError_t launch_leaf_f32_grouped(int tile, int terms, bool kfast, const Grouped& q, bool a_mfast, bool b_nfast,
+                                   hipStream_t s);
+
+hipError_t launch_leaf_grouped(int elem_size, int tile, int terms, bool kfast, const Grouped& q, bool a_mfast,
+                               bool b_nfast, hipStream_t stream)
+{
+    if (q.njobs <= 0 || q.blocks <= 0) return hipSuccess;
+    if (!kfast) tile = TILE_64;
+    return elem_size == 8 ? launch_leaf_f64_grouped(tile, terms, kfast, q, a_mfast, b_nfast, stream)
+                          : launch_leaf_f32_grouped(tile, terms, kfast, q, a_mfast, b_nfast, stream);
+}
 
 int tile_extent(int tile) { return tile == TILE_64 ? 64 : 128; }
 

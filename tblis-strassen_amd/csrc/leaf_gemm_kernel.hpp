@@ -127,8 +127,9 @@ __device__ __forceinline__ int64_t readlane64(int64_t x, int l)
     return int64_t((uint64_t(uint32_t(hi)) << 32) | uint64_t(uint32_t(lo)));
 }
 
+// The body of one workgroup: register tile bx of the launch's tile grid, product by, batch entry bz.
 template <typename T, int TM, int TN, int TK, int TERMS, bool A_MFAST, bool B_NFAST, int EPI, bool KFAST>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void leaf_gemm_kernel(GemmArgs g)
+__device__ __forceinline__ void leaf_gemm_body(const GemmArgs& g, int bx, int by, int64_t bz)
 {
     constexpr int NT = 256;
     constexpr int WM = TM / 2, WN = TN / 2;  // 2x2 waves
@@ -151,12 +152,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void l
     const int wn0 = (wave >> 1) * WN;
 
     int tm, tn;
-    tile_coords(blockIdx.x, g.tiles_m, g.tiles_n, tm, tn);
+    tile_coords(bx, g.tiles_m, g.tiles_n, tm, tn);
     const int64_t row0 = int64_t(tm) * TM;
     const int64_t col0 = int64_t(tn) * TN;
     const int64_t ms = g.ms, ns = g.ns, ks = g.ks;
 
-    const int64_t z = blockIdx.z;
+    const int64_t z = bz;
     const T* __restrict__ Ag = static_cast<const T*>(g.A) + (g.batch_a ? cptr(g.batch_a)[z] : 0);
     const T* __restrict__ Bg = static_cast<const T*>(g.B) + (g.batch_b ? cptr(g.batch_b)[z] : 0);
     T* __restrict__ Cg = static_cast<T*>(g.C) + (g.batch_c ? cptr(g.batch_c)[z] : 0);
@@ -272,7 +273,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void l
         // product p sits at a uniform shift (ProdDesc base, precomputed on the host) from the first
         // sub-block, every element's byte offset within a K-tile is the same for all terms and all
         // K-tiles, and a K-tile load is: uniform base (shift + the tile's first K offset) + that offset.
-        const int p = blockIdx.y;
+        const int p = by;
         const ProdDesc& P = g.prods[p];
         const int nA = __builtin_amdgcn_readfirstlane(P.nA), nB = __builtin_amdgcn_readfirstlane(P.nB);
 
@@ -487,11 +488,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void l
     }
     else
     {
-    // One product per workgroup: blockIdx.y selects it among the launch's products.  (ABC launches
+    // One product per workgroup: `by` selects it among the launch's products.  (ABC launches
         // one product at a time, in stream order, so that products updating the same C sub-block are
         // serialized; a loop over products inside the kernel costs registers in the main loop.)
         {
-            const int p = blockIdx.y;
+            const int p = by;
             const ProdDesc& P = g.prods[p];
             const int nA = __builtin_amdgcn_readfirstlane(P.nA), nB = __builtin_amdgcn_readfirstlane(P.nB);
     
@@ -742,6 +743,37 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void l
             epilogue(P, p, acc, false);
         }
     }
+}
+
+// One launch: grid (tiles, products, batch entries).
+template <typename T, int TM, int TN, int TK, int TERMS, bool A_MFAST, bool B_NFAST, int EPI, bool KFAST>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void leaf_gemm_kernel(GemmArgs g)
+{
+    leaf_gemm_body<T, TM, TN, TK, TERMS, A_MFAST, B_NFAST, EPI, KFAST>(g, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+// Grouped launch (batched contractions): job j owns workgroups [first[j], first[j + 1]) of a 1-D grid,
+// laid out as (tile, product, batch entry) of its own GemmArgs; jobs of one launch write disjoint C.
+template <typename T, int TM, int TN, int TK, int TERMS, bool A_MFAST, bool B_NFAST, int EPI, bool KFAST>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void leaf_gemm_grouped(
+    const GemmArgs* __restrict__ jobs, const int64_t* __restrict__ first, int njobs)
+{
+    const int64_t b = blockIdx.x;
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi)  // last job with first[j] <= b
+    {
+        const int mid = (lo + hi + 1) >> 1;
+        if (first[mid] <= b)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const GemmArgs& g = jobs[lo];
+    const int64_t local = b - first[lo];
+    const int64_t tiles = int64_t(g.tiles_m) * g.tiles_n;
+    const int64_t rest = local / tiles;
+    leaf_gemm_body<T, TM, TN, TK, TERMS, A_MFAST, B_NFAST, EPI, KFAST>(g, int(local % tiles), int(rest % g.nprod),
+                                                                      rest / g.nprod);
 }
 
 }  // namespace mistra

@@ -20,6 +20,7 @@ IMPL = {"abc": 0, "blis": 0, "blas": 1, "ref": 2, "naive": 3, "ab": 4, "auto": -
 PREC = {"native": 0, "bf16x3": 1, "ozaki_bf16": 2, "fp8_block": 3}
 FLAG_REF_FRINGE = 1
 FLAG_SYNC = 2
+FLAG_GRAPH = 4
 
 
 class MistraError(RuntimeError):
@@ -198,6 +199,44 @@ def autotune(A, idx_A, B, idx_B, C, idx_C, measure=False, stream=None):
                                idx_C.encode(), 1 if measure else 0, ctypes.byref(lv), ctypes.byref(im),
                                ctypes.byref(ti), _stream_ptr(stream)))
     return lv.value, im.value, ti.value
+
+
+class Batch:
+    """Keeps the tblis_tensor images of a batch alive for as long as the batch is reused (graph replays
+    key on the tensors' pointers, shapes, strides and scalars)."""
+
+    def __init__(self, entries):
+        self.tensors = []
+        arr = (Contraction * max(1, len(entries)))()
+        for i, e in enumerate(entries):
+            tA = make_tensor(e["A"], e.get("alpha", 1.0))
+            tB = make_tensor(e["B"], 1.0)
+            tC = make_tensor(e["C"], e.get("beta", 0.0))
+            self.tensors += [tA, tB, tC]
+            arr[i].A, arr[i].B, arr[i].C = ctypes.pointer(tA), ctypes.pointer(tB), ctypes.pointer(tC)
+            arr[i].idx_A, arr[i].idx_B, arr[i].idx_C = (e[k].encode() for k in ("idx_A", "idx_B", "idx_C"))
+            impl = e.get("impl", "abc")
+            arr[i].level = e.get("level", 1)
+            arr[i].impl = IMPL[impl] if isinstance(impl, str) else int(impl)
+        self.entries = entries
+        self.arr = arr
+        self.count = len(entries)
+
+    def run(self, flags=0, stream=None):
+        # C's scalar reads 1 after every call (as in tblis); restore each entry's beta first
+        for i, e in enumerate(self.entries):
+            t = self.tensors[3 * i + 2]
+            beta = e.get("beta", 0.0)
+            buf = (ctypes.c_double(beta) if t.type == TYPE_DOUBLE else ctypes.c_float(beta))
+            ctypes.memmove(ctypes.addressof(t) + TblisTensor.scalar.offset, ctypes.addressof(buf), ctypes.sizeof(buf))
+        _check(load().mistra_batch_mult(self.count, self.arr, flags, _stream_ptr(stream)))
+
+
+def batch_mult(entries, flags=0, stream=None):
+    """mistra_batch_mult over a list of dicts (A, idx_A, B, idx_B, C, idx_C[, alpha, beta, level, impl])."""
+    b = Batch(entries)
+    b.run(flags, stream)
+    return b
 
 
 def make_matrix(x, scalar=1.0):

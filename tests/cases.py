@@ -108,3 +108,32 @@ def rel_err(x, y):
     y = np.asarray(y, dtype=np.float64).ravel()
     den = np.linalg.norm(y)
     return float(np.linalg.norm(x - y) / (den if den > 0 else 1.0))
+
+
+def c4_batch(seed, count=256, lo=64, hi=4096, lmin=8, lmax=72):
+    """Random batch of contractions (the batched configuration): per entry 3..6 indices split among
+    m/n/k (each >= 1), index lengths in [lmin, lmax], group sizes log-uniform in [lo, hi], seeded
+    per-tensor index permutations.
+
+    The size and length constraints conflict (a group of c indices spans only [lmin^c, lmax^c]), so a
+    group's log-uniform target is clamped to that range and factored into c lengths: the first c-1 are
+    the rounded c-th root of the target, the last absorbs the remainder; every length is clamped to
+    [lmin, lmax]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        total = int(rng.integers(3, 7))
+        counts = [1, 1, 1]
+        for _ in range(total - 3):
+            counts[int(rng.integers(0, 3))] += 1
+        groups = []
+        for c in counts:
+            target = float(np.exp(rng.uniform(np.log(lo), np.log(hi))))
+            target = min(max(target, float(lmin) ** c), float(lmax) ** c)
+            root = int(round(target ** (1.0 / c)))
+            lens = [min(max(root, lmin), lmax) for _ in range(c - 1)]
+            rest = target / float(np.prod(lens)) if lens else target
+            lens.append(min(max(int(round(rest)), lmin), lmax))
+            groups.append(tuple(lens))
+        out.append(make_case(int(rng.integers(0, 2 ** 31)), m=groups[0], n=groups[1], k=groups[2]))
+    return out
