@@ -113,9 +113,12 @@ enum
 enum
 {
     MISTRA_PREC_NATIVE     = 0, /* FP64 MFMA for double, FP32 MFMA for float                          */
-    MISTRA_PREC_BF16X3     = 1, /* double only: hi/lo BF16 split, 3 products, FP32 accumulate          */
-    MISTRA_PREC_OZAKI_BF16 = 2, /* double only: Ozaki error-free BF16 slicing, exact FP32 chunks, FP64 sum */
-    MISTRA_PREC_FP8_BLOCK  = 3  /* double/float: FP8 E4M3 with one scale per 128-long K block          */
+    MISTRA_PREC_BF16X3     = 1, /* double only: hi/lo BF16 split (2 digits per row / column scale),
+                                   3 products, FP32 accumulate, FP64 combine                          */
+    MISTRA_PREC_OZAKI_BF16 = 2, /* double only: Ozaki BF16 digit slicing (5 digits, 15 products; env
+                                   MISTRA_OZAKI_SLICES = 2..7), exact FP32 chunks, FP64 sum           */
+    MISTRA_PREC_FP8_BLOCK  = 3  /* double only: FP8 E4M3 with one scale per row (column) and 128-long
+                                   K block, FP32 accumulate within a block, FP64 across blocks        */
 };
 
 /* Flags for mistra_tensor_mult. */

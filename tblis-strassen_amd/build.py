@@ -28,6 +28,7 @@ SOURCES = [
     "leaf_f32_multi.hip",
     "leaf_f32_store.hip",
     "aux_kernels.hip",
+    "lowp_kernels.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

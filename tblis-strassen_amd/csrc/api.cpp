@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -161,6 +162,17 @@ Status validate(const tblis::tblis_tensor* A, const label_type* idx_A, const tbl
     if ((A->ndim && !idx_A) || (B->ndim && !idx_B) || (C->ndim && !idx_C))
         return Status::error(MISTRA_ERR_ARG, "null index string");
     return Status();
+}
+
+// Digit slices of the Ozaki mode: 5 (15 products, about 40 bits) unless MISTRA_OZAKI_SLICES says 2..7.
+int ozaki_slices()
+{
+    static const int n = [] {
+        const char* v = std::getenv("MISTRA_OZAKI_SLICES");
+        const int x = v ? std::atoi(v) : 5;
+        return x >= 2 && x <= 7 ? x : 5;
+    }();
+    return n;
 }
 
 }  // namespace
@@ -389,8 +401,10 @@ Status tensor_mult(const tblis::tblis_tensor* A, const label_type* idx_A, const 
 {
     Status st = validate(A, idx_A, B, idx_B, C, idx_C);
     if (!st.ok()) return st;
-    if (precision != MISTRA_PREC_NATIVE)
-        return Status::error(MISTRA_ERR_UNSUPPORTED, "precision mode not available in this build");
+    if (precision < MISTRA_PREC_NATIVE || precision > MISTRA_PREC_FP8_BLOCK)
+        return Status::error(MISTRA_ERR_ARG, "unknown precision mode");
+    if (precision != MISTRA_PREC_NATIVE && C->type != tblis::TYPE_DOUBLE)
+        return Status::error(MISTRA_ERR_UNSUPPORTED, "low-precision modes take FP64 tensors");
     if (level < 0 || impl == MISTRA_IMPL_AUTO)
     {
         TuneChoice ch;
@@ -402,6 +416,8 @@ Status tensor_mult(const tblis::tblis_tensor* A, const label_type* idx_A, const 
     }
     if (level > 2) return Status::error(MISTRA_ERR_ARG, "level must be 0, 1 or 2");
     if (impl < 0 || impl > 4) return Status::error(MISTRA_ERR_ARG, "impl must be in 0..4");
+    // low-precision leaves always stage the operand sums (the Naive tables of the plan)
+    if (precision != MISTRA_PREC_NATIVE) impl = IMPL_NAIVE;
 
     const double alpha = scalar_as_double(A) * scalar_as_double(B);
     const double beta = scalar_as_double(C);
@@ -480,7 +496,11 @@ Status tensor_mult(const tblis::tblis_tensor* A, const label_type* idx_A, const 
             {
                 Call call{dA, dB, dC, alpha, beta};
                 if (plan->transposed) std::swap(call.A, call.B);
-                st = run_plan(*plan, call, stream);
+                if (precision == MISTRA_PREC_NATIVE)
+                    st = run_plan(*plan, call, stream);
+                else
+                    st = run_plan_lowp(*plan, call, precision,
+                                       precision == MISTRA_PREC_BF16X3 ? 2 : ozaki_slices(), stream);
             }
         }
         if (!st.ok())

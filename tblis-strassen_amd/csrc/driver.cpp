@@ -11,6 +11,7 @@
 // (dropped by the reference, src/matrix/stra_tensor_view.hpp:246-247) are computed by classical
 // passes unless MISTRA_FLAG_REF_FRINGE asks for the reference's truncation.
 #include "driver.hpp"
+#include "lowp.hpp"
 
 #include <algorithm>
 #include <cstdlib>
@@ -280,6 +281,12 @@ Status build_plan_host(const tblis::tblis_tensor* A, const label_type* idx_A, co
         p.ks = p.k;
         p.h_prods = {single_desc(0, 0, 0, 0, 0, 0, true)};
         p.nprod = 1;
+        // one C block fed by the one product (used when the product is staged, as in the low-precision path)
+        p.nsub = 1;
+        p.h_term_begin = {0, 1};
+        p.h_sub_roff = {0};
+        p.h_sub_coff = {0};
+        p.h_terms = {CombineTerm{0, 1.0}};
     }
     else
     {
@@ -790,7 +797,12 @@ Status run_plan(const Plan& p, const Call& c, hipStream_t stream)
         HIP_TRY(f3);
     }
 
-    // classical passes over the remainder of the 2^L split
+    return run_fringe(p, c, stream);
+}
+
+// Classical FP64 / FP32 passes over the remainder of the 2^L split.
+Status run_fringe(const Plan& p, const Call& c, hipStream_t stream)
+{
     for (const Pass& ps : p.fringe)
     {
         GemmArgs f = base_args(p, c);
@@ -804,6 +816,126 @@ Status run_plan(const Plan& p, const Call& c, hipStream_t stream)
         HIP_TRY(launch_leaf_gemm(p.elem, EPI_MULTI_C, TILE_64, 1, ps.kf.on, f, p.a_mfast, p.b_nfast, stream));
     }
     return Status();
+}
+
+// Low-precision leaf products of an FP64 plan: the Strassen operand sums are formed in FP64 (as for
+// Naive), sliced into BF16 digits (Ozaki) or FP8 E4M3 codes with per-128 K-block scales, multiplied on
+// BF16 / FP8 MFMA with FP32 accumulation (FP64 across exact chunks / blocks), and the products are
+// combined into C in FP64.  The fringe of the 2^L split runs in FP64.
+Status run_plan_lowp(const Plan& p, const Call& c, int precision, int nslices, hipStream_t stream)
+{
+    if (p.elem != 8) return Status::error(MISTRA_ERR_UNSUPPORTED, "low-precision modes take FP64 tensors");
+    if (p.general) return Status::error(MISTRA_ERR_UNSUPPORTED, "low-precision modes need a pure contraction");
+    if (precision != MISTRA_PREC_FP8_BLOCK && (nslices < 2 || nslices > 7))
+        return Status::error(MISTRA_ERR_ARG, "Ozaki slice count must be in 2..7");
+    if (p.nprod > 0)
+    {
+        const bool fp8 = precision == MISTRA_PREC_FP8_BLOCK;
+        const int64_t ms = p.ms, ns = p.ns, ks = p.ks, nq = p.nprod;
+        auto up = [](int64_t x, int64_t m) { return (x + m - 1) / m * m; };
+        const int64_t mp = up(ms, 64), np = up(ns, 64), kp = up(ks, fp8 ? 128 : 64);
+        const int nsl = fp8 ? 1 : nslices;
+        const size_t esz = fp8 ? 1 : 2;
+        struct Buf
+        {
+            void* p = nullptr;
+        } S, T, M, EA, EB, XA, XB;
+        const size_t nS = size_t(nq * ms * ks) * 8, nT = size_t(nq * ks * ns) * 8, nM = size_t(nq * ms * ns) * 8;
+        const size_t nEA = fp8 ? size_t(nq * mp * (kp / 128)) * 4 : size_t(nq * ms) * 4;
+        const size_t nEB = fp8 ? size_t(nq * np * (kp / 128)) * 4 : size_t(nq * ns) * 4;
+        const size_t nXA = size_t(nsl) * size_t(nq * mp * kp) * esz, nXB = size_t(nsl) * size_t(nq * np * kp) * esz;
+        hipError_t e = hipSuccess;
+        for (auto pr : {std::make_pair(&S, nS), std::make_pair(&T, nT), std::make_pair(&M, nM), std::make_pair(&EA, nEA),
+                        std::make_pair(&EB, nEB), std::make_pair(&XA, nXA), std::make_pair(&XB, nXB)})
+            if (e == hipSuccess) e = hipMallocAsync(&pr.first->p, pr.second ? pr.second : 1, stream);
+        for (int64_t z = 0; z < p.nbatch && e == hipSuccess; z++)
+        {
+            SumArgs sa;
+            std::memset(&sa, 0, sizeof(sa));
+            sa.X = c.A;
+            sa.rs = p.a_m;
+            sa.cs = p.a_k;
+            sa.rows = ms;
+            sa.cols = ks;
+            sa.prods = p.prods;
+            sa.nprod = p.nprod;
+            sa.which = 0;
+            sa.rows_fast = p.a_mfast;
+            sa.out = S.p;
+            sa.zoff = p.h_batch_a[size_t(z)];
+            e = launch_operand_sums(8, sa, stream);
+            if (e != hipSuccess) break;
+            sa.X = c.B;
+            sa.rs = p.b_k;
+            sa.cs = p.b_n;
+            sa.rows = ks;
+            sa.cols = ns;
+            sa.which = 1;
+            sa.rows_fast = !p.b_nfast;
+            sa.out = T.p;
+            sa.zoff = p.h_batch_b[size_t(z)];
+            e = launch_operand_sums(8, sa, stream);
+            if (e != hipSuccess) break;
+
+            LowpSliceArgs la;
+            la.S = static_cast<const double*>(S.p);
+            la.T = static_cast<const double*>(T.p);
+            la.ms = ms;
+            la.ns = ns;
+            la.ks = ks;
+            la.mp = mp;
+            la.np = np;
+            la.kp = kp;
+            la.nprod = p.nprod;
+            la.ea = static_cast<int*>(EA.p);
+            la.eb = static_cast<int*>(EB.p);
+            la.outA = XA.p;
+            la.outB = XB.p;
+            LowpGemmArgs lg{ms, ns, mp, np, kp, p.nprod};
+            if (fp8)
+            {
+                e = launch_fp8_slices(la, static_cast<float*>(EA.p), static_cast<float*>(EB.p), stream);
+                if (e != hipSuccess) break;
+                timing_begin(stream);
+                e = launch_fp8_gemm(XA.p, XB.p, static_cast<const float*>(EA.p), static_cast<const float*>(EB.p),
+                                    static_cast<double*>(M.p), lg, stream);
+                timing_end(stream, 2.0 * double(nq) * double(mp) * double(np) * double(kp));
+            }
+            else
+            {
+                e = launch_lowp_exponents(la, stream);
+                if (e == hipSuccess) e = launch_ozaki_slices(nsl, la, stream);
+                if (e != hipSuccess) break;
+                timing_begin(stream);
+                e = launch_ozaki_gemm(nsl, XA.p, XB.p, la.ea, la.eb, static_cast<double*>(M.p), lg, stream);
+                timing_end(stream, 2.0 * double(nq) * double(mp) * double(np) * double(kp) * (nsl * (nsl + 1) / 2));
+            }
+            if (e != hipSuccess) break;
+
+            CombineArgs ca;
+            std::memset(&ca, 0, sizeof(ca));
+            ca.C = static_cast<char*>(c.C) + p.h_batch_c[size_t(z)] * p.elem;
+            ca.M = M.p;
+            ca.c_m = p.c_m;
+            ca.c_n = p.c_n;
+            ca.ms = ms;
+            ca.ns = ns;
+            ca.nsub = p.nsub;
+            ca.sub_roff = p.sub_roff;
+            ca.sub_coff = p.sub_coff;
+            ca.term_begin = p.term_begin;
+            ca.terms = p.terms;
+            ca.nprod = p.nprod;
+            ca.alpha = c.alpha;
+            ca.beta = c.beta;
+            ca.nbatch = 1;
+            e = launch_combine(8, ca, stream);
+        }
+        for (Buf* b : {&S, &T, &M, &EA, &EB, &XA, &XB})
+            if (b->p) (void)hipFreeAsync(b->p, stream);
+        HIP_TRY(e);
+    }
+    return run_fringe(p, c, stream);
 }
 
 }  // namespace mistra

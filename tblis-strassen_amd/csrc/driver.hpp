@@ -142,6 +142,9 @@ Status plan_abc_jobs(const Plan& p, const Call& c, std::vector<LaunchJob>& out);
 
 // Execute a plan on device pointers.
 Status run_plan(const Plan& p, const Call& c, hipStream_t stream);
+Status run_fringe(const Plan& p, const Call& c, hipStream_t stream);
+// Low-precision leaf products (MISTRA_PREC_*) of an FP64 plan built with IMPL_NAIVE tables.
+Status run_plan_lowp(const Plan& p, const Call& c, int precision, int nslices, hipStream_t stream);
 
 // Whole-call entry used by every C-ABI symbol.  level < 0 or impl == MISTRA_IMPL_AUTO: the autotuner's
 // choice (a measured one when mistra_autotune has timed this shape, else the analytic model); tile < 0:

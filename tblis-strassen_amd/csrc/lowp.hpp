@@ -1,0 +1,39 @@
+// Low-precision leaf products (lowp_kernels.hip): Ozaki BF16 digit slicing and FP8 E4M3 block scaling.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace mistra {
+
+// Dense FP64 operands of nprod products: S[q] (ms x ks, rows fastest) and T[q] (ks x ns, k fastest);
+// sliced outputs in [slice][q][row][k] (k fastest) with rows padded to mp / np (multiples of 64) and k to
+// kp (a multiple of 64 for digits, 128 for FP8).
+struct LowpSliceArgs
+{
+    const double* S;
+    const double* T;
+    int64_t ms, ns, ks, mp, np, kp;
+    int nprod;
+    int* ea;  // [q][ms] row exponents of S
+    int* eb;  // [q][ns] column exponents of T
+    void* outA;
+    void* outB;
+};
+
+struct LowpGemmArgs
+{
+    int64_t ms, ns, mp, np, kp;
+    int nprod;
+};
+
+hipError_t launch_lowp_exponents(const LowpSliceArgs& a, hipStream_t s);
+hipError_t launch_ozaki_slices(int nslices, const LowpSliceArgs& a, hipStream_t s);
+hipError_t launch_ozaki_gemm(int nslices, const void* As, const void* Bs, const int* ea, const int* eb, double* M,
+                             const LowpGemmArgs& g, hipStream_t s);
+hipError_t launch_fp8_slices(const LowpSliceArgs& a, float* sa, float* sb, hipStream_t s);
+hipError_t launch_fp8_gemm(const void* As, const void* Bs, const float* sa, const float* sb, double* M,
+                           const LowpGemmArgs& g, hipStream_t s);
+
+}  // namespace mistra

@@ -1,0 +1,457 @@
+// Low-precision leaf products with FP64 accuracy won back (gfx950 MFMA):
+//
+//  * Ozaki-style BF16 slicing.  Each row i of an FP64 operand S (each column j of T) is scaled by a power
+//    of two 2^e so that |x| < 1, then split into `nsl` signed integer digits d_s (|d_0| <= 256,
+//    |d_s| <= 128), x = sum_s d_s 2^(-8(s+1)) up to the last digit's rounding.  A digit is an integer of
+//    at most 8 significant bits, exact in BF16.  The product of two digits is an integer below 2^16,
+//    so an FP32 MFMA accumulation of up to 2^24 / ((d+1) 2^16) of them (d+1 digit pairs with the same
+//    weight 2^(-8(d+2)) summed together) is exact; each such chunk is then added to an FP64 accumulator.
+//    Pairs with s + t >= nsl are dropped (Ozaki truncation): nsl = 2 is the hi/lo split with 3 products,
+//    nsl = 5 keeps 15 products and about 40 bits.
+//    (Ozaki, Ogita, Oishi, Rump, "Error-free transformations of matrix multiplication by using fast
+//    routines of matrix multiplication and its applications", Numer. Algorithms 59, 2012.)
+//  * FP8 E4M3 with one FP32 scale per row (column) and 128-long K block: FP32 MFMA accumulation inside a
+//    block, FP64 across blocks.
+//
+// Operand layouts (written by the slice kernels): digits / FP8 codes in [slice][product][row][k] with k
+// fastest, rows padded to a multiple of 64 and k to a multiple of 32 (FP8: 128), padding zero.  The
+// MFMA operand of lane l is row (l % 16), k = 8 (l / 16) + [0, 8) -- 16 (BF16) or 8 (FP8) contiguous bytes.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "lowp.hpp"
+
+namespace mistra {
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ int exp_of(double maxabs)
+{
+    int e = 0;
+    if (maxabs > 0.0) (void)frexp(maxabs, &e);  // maxabs = f 2^e, f in [0.5, 1): |x| 2^-e < 1
+    return e;
+}
+
+// ---- scale exponents ---------------------------------------------------------------------------------
+
+// S: [q][i + k * rows] (rows fastest): one thread per (q, i)
+__global__ __launch_bounds__(256) void row_exp_kernel(const double* __restrict__ S, int64_t rows, int64_t cols,
+                                                      int nmat, int* __restrict__ e)
+{
+    const int64_t n = int64_t(nmat) * rows;
+    for (int64_t x = int64_t(blockIdx.x) * 256 + threadIdx.x; x < n; x += int64_t(gridDim.x) * 256)
+    {
+        const int64_t q = x / rows, i = x % rows;
+        const double* s = S + q * rows * cols + i;
+        double m = 0.0;
+        for (int64_t k = 0; k < cols; k++) m = fmax(m, fabs(s[k * rows]));
+        e[x] = exp_of(m);
+    }
+}
+
+// T: [q][k + j * kk] (k fastest): one wave per (q, j)
+__global__ __launch_bounds__(256) void col_exp_kernel(const double* __restrict__ T, int64_t kk, int64_t cols,
+                                                      int nmat, int* __restrict__ e)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t n = int64_t(nmat) * cols;
+    for (int64_t x = (int64_t(blockIdx.x) * 256 + threadIdx.x) / 64; x < n; x += int64_t(gridDim.x) * 4)
+    {
+        const double* t = T + x * kk;
+        double m = 0.0;
+        for (int64_t k = lane; k < kk; k += 64) m = fmax(m, fabs(t[k]));
+        for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+        if (lane == 0) e[x] = exp_of(m);
+    }
+}
+
+// ---- Ozaki digits -----------------------------------------------------------------------------------
+
+template <int NSL>
+__device__ __forceinline__ void digits(double x, __bf16 (&d)[NSL])
+{
+    // x in (-1, 1): d_0 = round(x 2^8), r = x 2^8 - d_0 (exact), d_1 = round(r 2^8), ...
+    double r = x;
+#pragma unroll
+    for (int s = 0; s < NSL; s++)
+    {
+        r *= 256.0;
+        const double q = rint(r);
+        r -= q;
+        d[s] = __bf16(float(q));
+    }
+}
+
+// S (rows fastest) -> out[s][q][i][k] (k fastest), through a 64 x 64 LDS transpose
+template <int NSL>
+__global__ __launch_bounds__(256) void slice_rows_kernel(const double* __restrict__ S, int64_t rows, int64_t cols,
+                                                         const int* __restrict__ e, __bf16* __restrict__ out,
+                                                         int64_t rp, int64_t kp, int nmat)
+{
+    __shared__ double tile[64][65];
+    const int64_t tr = blockIdx.x, tk = blockIdx.y, q = blockIdx.z;
+    const int tid = threadIdx.x;
+    const double* s = S + q * rows * cols;
+    for (int x = tid; x < 64 * 64; x += 256)
+    {
+        const int r = x % 64, c = x / 64;  // consecutive threads on consecutive rows: coalesced reads
+        const int64_t i = tr * 64 + r, k = tk * 64 + c;
+        double v = 0.0;
+        if (i < rows && k < cols) v = ldexp(s[i + k * rows], -e[q * rows + i]);
+        tile[r][c] = v;
+    }
+    __syncthreads();
+    const int64_t slab = int64_t(nmat) * rp * kp;
+    for (int x = tid; x < 64 * 64; x += 256)
+    {
+        const int c = x % 64, r = x / 64;  // consecutive threads on consecutive k: coalesced writes
+        const int64_t i = tr * 64 + r, k = tk * 64 + c;
+        if (i >= rp || k >= kp) continue;
+        __bf16 d[NSL];
+        digits<NSL>(tile[r][c], d);
+        __bf16* o = out + (q * rp + i) * kp + k;
+#pragma unroll
+        for (int sl = 0; sl < NSL; sl++) o[sl * slab] = d[sl];
+    }
+}
+
+// T (k fastest, [q][j][k]) -> out[s][q][j][k]
+template <int NSL>
+__global__ __launch_bounds__(256) void slice_cols_kernel(const double* __restrict__ T, int64_t kk, int64_t cols,
+                                                         const int* __restrict__ e, __bf16* __restrict__ out,
+                                                         int64_t cp, int64_t kp, int nmat)
+{
+    const int64_t n = int64_t(nmat) * cp * kp, slab = n;
+    for (int64_t x = int64_t(blockIdx.x) * 256 + threadIdx.x; x < n; x += int64_t(gridDim.x) * 256)
+    {
+        const int64_t k = x % kp, j = (x / kp) % cp, q = x / (kp * cp);
+        double v = 0.0;
+        if (j < cols && k < kk) v = ldexp(T[(q * cols + j) * kk + k], -e[q * cols + j]);
+        __bf16 d[NSL];
+        digits<NSL>(v, d);
+#pragma unroll
+        for (int sl = 0; sl < NSL; sl++) out[sl * slab + x] = d[sl];
+    }
+}
+
+// ---- Ozaki BF16 GEMM --------------------------------------------------------------------------------
+
+// Digit pairs (s, t) with s + t = d share the weight 2^(-8(d+2)); group d (d + 1 pairs) is exact in FP32
+// for 2^24 / ((d + 1) 2^16) = 256 / (d + 1) terms, i.e. floor(8 / (d + 1)) K-steps of 32 (at least 1:
+// d + 1 <= 8 pairs of 32 products below 2^16 stay below 2^24).
+__host__ __device__ constexpr int flush_period(int d)
+{
+    return (8 / (d + 1)) >= 4 ? (8 / (d + 1)) : ((8 / (d + 1)) >= 2 ? 2 : 1);
+}
+
+// One 64 x 64 tile of M_q per workgroup (4 waves, 32 x 32 each as 2 x 2 MFMA blocks of 16 x 16).
+template <int NSL>
+__global__ __launch_bounds__(256) void ozaki_gemm_kernel(const __bf16* __restrict__ As, const __bf16* __restrict__ Bs,
+                                                         const int* __restrict__ ea, const int* __restrict__ eb,
+                                                         double* __restrict__ M, LowpGemmArgs g)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int tiles_m = int(g.mp / 64);
+    const int tm = blockIdx.x % tiles_m, tn = blockIdx.x / tiles_m;
+    const int q = blockIdx.y;
+    const int64_t row0 = int64_t(tm) * 64 + (wave & 1) * 32, col0 = int64_t(tn) * 64 + (wave >> 1) * 32;
+    const int64_t slab_a = int64_t(g.nprod) * g.mp * g.kp, slab_b = int64_t(g.nprod) * g.np * g.kp;
+    // lane's operand rows / columns and k offset
+    const __bf16* pa = As + (int64_t(q) * g.mp + row0 + (lane & 15)) * g.kp + 8 * (lane >> 4);
+    const __bf16* pb = Bs + (int64_t(q) * g.np + col0 + (lane & 15)) * g.kp + 8 * (lane >> 4);
+
+    double acc[2][2][4];
+    f32x4 a32[NSL][2][2];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+        {
+#pragma unroll
+            for (int v = 0; v < 4; v++) acc[i][j][v] = 0.0;
+#pragma unroll
+            for (int d = 0; d < NSL; d++) a32[d][i][j] = f32x4{0, 0, 0, 0};
+        }
+    auto flush = [&](int d) {
+        const double w = ldexp(1.0, -8 * (d + 2));
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++)
+            {
+#pragma unroll
+                for (int v = 0; v < 4; v++) acc[i][j][v] += w * double(a32[d][i][j][v]);
+                a32[d][i][j] = f32x4{0, 0, 0, 0};
+            }
+    };
+
+    const int64_t nsteps = g.kp / 32;
+    for (int64_t st = 0; st < nsteps; st++)
+    {
+        bf16x8 fa[NSL][2], fb[NSL][2];
+#pragma unroll
+        for (int s = 0; s < NSL; s++)
+#pragma unroll
+            for (int i = 0; i < 2; i++)
+            {
+                fa[s][i] = *reinterpret_cast<const bf16x8*>(pa + s * slab_a + int64_t(i) * 16 * g.kp + st * 32);
+                fb[s][i] = *reinterpret_cast<const bf16x8*>(pb + s * slab_b + int64_t(i) * 16 * g.kp + st * 32);
+            }
+#pragma unroll
+        for (int s = 0; s < NSL; s++)
+#pragma unroll
+            for (int t = 0; s + t < NSL; t++)
+#pragma unroll
+                for (int i = 0; i < 2; i++)
+#pragma unroll
+                    for (int j = 0; j < 2; j++)
+                        a32[s + t][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[s][i], fb[t][j], a32[s + t][i][j], 0, 0, 0);
+#pragma unroll
+        for (int d = 0; d < NSL; d++)
+            if ((st + 1) % flush_period(d) == 0) flush(d);
+    }
+#pragma unroll
+    for (int d = 0; d < NSL; d++) flush(d);
+
+    // M_q[row + col * ms] = acc * 2^(ea[row] + eb[col]); element v of a lane: row 4 (l / 16) + v, col l % 16
+    double* Mq = M + int64_t(q) * g.ms * g.ns;
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+        {
+            const int64_t col = col0 + j * 16 + (lane & 15);
+            if (col >= g.ns) continue;
+            const int ec = eb[int64_t(q) * g.ns + col];
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+            {
+                const int64_t row = row0 + i * 16 + 4 * (lane >> 4) + v;
+                if (row < g.ms) Mq[row + col * g.ms] = ldexp(acc[i][j][v], ea[int64_t(q) * g.ms + row] + ec);
+            }
+        }
+}
+
+// ---- FP8 E4M3 with per-128 K-block scales -------------------------------------------------------------
+
+constexpr float kE4M3Max = 240.0f;  // largest finite E4M3 value common to the OCP and FNUZ encodings
+
+__device__ __forceinline__ uint32_t fp8x4(float a, float b, float c, float d)
+{
+    int lo = __builtin_amdgcn_cvt_pk_fp8_f32(a, b, 0, false);
+    return uint32_t(__builtin_amdgcn_cvt_pk_fp8_f32(c, d, lo, true));
+}
+
+// S (rows fastest) -> codes[q][i][k] (k fastest, kp multiple of 128), scales sa[q][i][kb]: one thread
+// per (q, i, kb) (reads strided by `rows`, writes 128 contiguous bytes)
+__global__ __launch_bounds__(256) void fp8_rows_kernel(const double* __restrict__ S, int64_t rows, int64_t cols,
+                                                       uint8_t* __restrict__ out, float* __restrict__ sc,
+                                                       int64_t rp, int64_t kp, int nmat)
+{
+    const int64_t nkb = kp / 128, n = int64_t(nmat) * rp * nkb;
+    for (int64_t x = int64_t(blockIdx.x) * 256 + threadIdx.x; x < n; x += int64_t(gridDim.x) * 256)
+    {
+        const int64_t i = x % rp, kb = (x / rp) % nkb, q = x / (rp * nkb);
+        const double* s = S + q * rows * cols + i;
+        double m = 0.0;
+        if (i < rows)
+            for (int64_t k = kb * 128; k < kb * 128 + 128 && k < cols; k++) m = fmax(m, fabs(s[k * rows]));
+        const float scale = m > 0.0 ? float(m) / kE4M3Max : 1.0f;
+        sc[(q * rp + i) * nkb + kb] = scale;
+        uint32_t* o = reinterpret_cast<uint32_t*>(out + (q * rp + i) * kp + kb * 128);
+        for (int c = 0; c < 128; c += 4)
+        {
+            float v[4];
+            for (int u = 0; u < 4; u++)
+            {
+                const int64_t k = kb * 128 + c + u;
+                v[u] = (i < rows && k < cols) ? float(s[k * rows] / double(scale)) : 0.0f;
+            }
+            o[c / 4] = fp8x4(v[0], v[1], v[2], v[3]);
+        }
+    }
+}
+
+// T (k fastest, [q][j][k]) -> codes[q][j][k], scales sb[q][j][kb]: one wave per (q, j, kb)
+__global__ __launch_bounds__(256) void fp8_cols_kernel(const double* __restrict__ T, int64_t kk, int64_t cols,
+                                                       uint8_t* __restrict__ out, float* __restrict__ sc,
+                                                       int64_t cp, int64_t kp, int nmat)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t nkb = kp / 128, n = int64_t(nmat) * cp * nkb;
+    for (int64_t x = (int64_t(blockIdx.x) * 256 + threadIdx.x) / 64; x < n; x += int64_t(gridDim.x) * 4)
+    {
+        const int64_t kb = x % nkb, j = (x / nkb) % cp, q = x / (nkb * cp);
+        const int64_t k0 = kb * 128 + 2 * lane;
+        const double* t = T + (q * cols + j) * kk;
+        const double v0 = (j < cols && k0 < kk) ? t[k0] : 0.0, v1 = (j < cols && k0 + 1 < kk) ? t[k0 + 1] : 0.0;
+        double m = fmax(fabs(v0), fabs(v1));
+        for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+        const float scale = m > 0.0 ? float(m) / kE4M3Max : 1.0f;
+        if (lane == 0) sc[(q * cp + j) * nkb + kb] = scale;
+        const int pk = __builtin_amdgcn_cvt_pk_fp8_f32(float(v0 / double(scale)), float(v1 / double(scale)), 0, false);
+        reinterpret_cast<uint16_t*>(out + (q * cp + j) * kp + kb * 128)[lane] = uint16_t(pk & 0xffff);
+    }
+}
+
+__global__ __launch_bounds__(256) void fp8_gemm_kernel(const uint8_t* __restrict__ As, const uint8_t* __restrict__ Bs,
+                                                       const float* __restrict__ sa, const float* __restrict__ sb,
+                                                       double* __restrict__ M, LowpGemmArgs g)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int tiles_m = int(g.mp / 64);
+    const int tm = blockIdx.x % tiles_m, tn = blockIdx.x / tiles_m;
+    const int q = blockIdx.y;
+    const int64_t row0 = int64_t(tm) * 64 + (wave & 1) * 32, col0 = int64_t(tn) * 64 + (wave >> 1) * 32;
+    const int64_t nkb = g.kp / 128;
+    const uint8_t* pa = As + (int64_t(q) * g.mp + row0 + (lane & 15)) * g.kp + 8 * (lane >> 4);
+    const uint8_t* pb = Bs + (int64_t(q) * g.np + col0 + (lane & 15)) * g.kp + 8 * (lane >> 4);
+    double acc[2][2][4];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) acc[i][j][v] = 0.0;
+    for (int64_t kb = 0; kb < nkb; kb++)
+    {
+        f32x4 a32[2][2];
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++) a32[i][j] = f32x4{0, 0, 0, 0};
+#pragma unroll
+        for (int st = 0; st < 4; st++)
+        {
+            long fa[2], fb[2];
+#pragma unroll
+            for (int i = 0; i < 2; i++)
+            {
+                fa[i] = *reinterpret_cast<const long*>(pa + int64_t(i) * 16 * g.kp + kb * 128 + st * 32);
+                fb[i] = *reinterpret_cast<const long*>(pb + int64_t(i) * 16 * g.kp + kb * 128 + st * 32);
+            }
+#pragma unroll
+            for (int i = 0; i < 2; i++)
+#pragma unroll
+                for (int j = 0; j < 2; j++)
+                    a32[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(fa[i], fb[j], a32[i][j], 0, 0, 0);
+        }
+        // FP64 accumulation across blocks with the row and column scales of this block
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++)
+            {
+                const double cs = sb[(int64_t(q) * g.np + col0 + j * 16 + (lane & 15)) * nkb + kb];
+#pragma unroll
+                for (int v = 0; v < 4; v++)
+                {
+                    const double rs = sa[(int64_t(q) * g.mp + row0 + i * 16 + 4 * (lane >> 4) + v) * nkb + kb];
+                    acc[i][j][v] += double(a32[i][j][v]) * rs * cs;
+                }
+            }
+    }
+    double* Mq = M + int64_t(q) * g.ms * g.ns;
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+        {
+            const int64_t col = col0 + j * 16 + (lane & 15);
+            if (col >= g.ns) continue;
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+            {
+                const int64_t row = row0 + i * 16 + 4 * (lane >> 4) + v;
+                if (row < g.ms) Mq[row + col * g.ms] = acc[i][j][v];
+            }
+        }
+}
+
+unsigned grid_of(int64_t n, int per)
+{
+    int64_t b = (n + per - 1) / per;
+    return unsigned(b < 1 ? 1 : (b > 65536 ? 65536 : b));
+}
+
+template <int NSL>
+hipError_t slices_t(const LowpSliceArgs& a, hipStream_t s)
+{
+    hipLaunchKernelGGL(slice_rows_kernel<NSL>, dim3(unsigned(a.mp / 64), unsigned(a.kp / 64), unsigned(a.nprod)),
+                       dim3(256), 0, s, a.S, a.ms, a.ks, a.ea, static_cast<__bf16*>(a.outA), a.mp, a.kp, a.nprod);
+    hipLaunchKernelGGL(slice_cols_kernel<NSL>, dim3(grid_of(int64_t(a.nprod) * a.np * a.kp, 256)), dim3(256), 0, s,
+                       a.T, a.ks, a.ns, a.eb, static_cast<__bf16*>(a.outB), a.np, a.kp, a.nprod);
+    return hipGetLastError();
+}
+
+template <int NSL>
+hipError_t gemm_t(const void* As, const void* Bs, const int* ea, const int* eb, double* M, const LowpGemmArgs& g,
+                  hipStream_t s)
+{
+    hipLaunchKernelGGL(ozaki_gemm_kernel<NSL>, dim3(unsigned((g.mp / 64) * (g.np / 64)), unsigned(g.nprod)), dim3(256),
+                       0, s, static_cast<const __bf16*>(As), static_cast<const __bf16*>(Bs), ea, eb, M, g);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_lowp_exponents(const LowpSliceArgs& a, hipStream_t s)
+{
+    hipLaunchKernelGGL(row_exp_kernel, dim3(grid_of(int64_t(a.nprod) * a.ms, 256)), dim3(256), 0, s, a.S, a.ms, a.ks,
+                       a.nprod, a.ea);
+    hipLaunchKernelGGL(col_exp_kernel, dim3(grid_of(int64_t(a.nprod) * a.ns, 4)), dim3(256), 0, s, a.T, a.ks, a.ns,
+                       a.nprod, a.eb);
+    return hipGetLastError();
+}
+
+hipError_t launch_ozaki_slices(int nsl, const LowpSliceArgs& a, hipStream_t s)
+{
+    switch (nsl)
+    {
+    case 2: return slices_t<2>(a, s);
+    case 3: return slices_t<3>(a, s);
+    case 4: return slices_t<4>(a, s);
+    case 5: return slices_t<5>(a, s);
+    case 6: return slices_t<6>(a, s);
+    case 7: return slices_t<7>(a, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_ozaki_gemm(int nsl, const void* As, const void* Bs, const int* ea, const int* eb, double* M,
+                             const LowpGemmArgs& g, hipStream_t s)
+{
+    switch (nsl)
+    {
+    case 2: return gemm_t<2>(As, Bs, ea, eb, M, g, s);
+    case 3: return gemm_t<3>(As, Bs, ea, eb, M, g, s);
+    case 4: return gemm_t<4>(As, Bs, ea, eb, M, g, s);
+    case 5: return gemm_t<5>(As, Bs, ea, eb, M, g, s);
+    case 6: return gemm_t<6>(As, Bs, ea, eb, M, g, s);
+    case 7: return gemm_t<7>(As, Bs, ea, eb, M, g, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_fp8_slices(const LowpSliceArgs& a, float* sa, float* sb, hipStream_t s)
+{
+    hipLaunchKernelGGL(fp8_rows_kernel, dim3(grid_of(int64_t(a.nprod) * a.mp * (a.kp / 128), 256)), dim3(256), 0, s,
+                       a.S, a.ms, a.ks, static_cast<uint8_t*>(a.outA), sa, a.mp, a.kp, a.nprod);
+    hipLaunchKernelGGL(fp8_cols_kernel, dim3(grid_of(int64_t(a.nprod) * a.np * (a.kp / 128), 4)), dim3(256), 0, s,
+                       a.T, a.ks, a.ns, static_cast<uint8_t*>(a.outB), sb, a.np, a.kp, a.nprod);
+    return hipGetLastError();
+}
+
+hipError_t launch_fp8_gemm(const void* As, const void* Bs, const float* sa, const float* sb, double* M,
+                           const LowpGemmArgs& g, hipStream_t s)
+{
+    hipLaunchKernelGGL(fp8_gemm_kernel, dim3(unsigned((g.mp / 64) * (g.np / 64)), unsigned(g.nprod)), dim3(256), 0, s,
+                       static_cast<const uint8_t*>(As), static_cast<const uint8_t*>(Bs), sa, sb, M, g);
+    return hipGetLastError();
+}
+
+}  // namespace mistra
