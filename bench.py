@@ -15,6 +15,17 @@ roofline: FP64 MFMA bound of the leaf kernel of the reported variant: achieved =
 average launch duration (HIP events recorded by the library on the launch stream); traffic comes from the
 committed counter summary profiles/traffic_<workload>.json when present.
 cpu_baseline: the CPU oracle (a port of the reference algorithm) on a bounded sample of the workload.
+
+Other workloads (one JSON line each, same fields; --workload NAME):
+  c1  the same measurement at m=n=k=1024 (2 indices of 32 per group);
+  c3  precision modes at m=n=16384, k=1024 (groups 128x128, 128x128, 32x32; A, B uniform times 2^u,
+      u in [-8, 8]): FP64 classical and Strassen, BF16 hi/lo split, Ozaki BF16 slicing and FP8 block
+      scaling, each with its error against the FP64 classical result; value = the fastest mode within
+      1e-10;
+  c4  256 seeded FP32 contractions of uneven shape (tests/cases.py:c4_batch): a loop of individual
+      calls, the grouped batch, and its HIP-graph replay; value = the batch throughput;
+  c5  C[abce] = A[dcaf] B[dfbe], all lengths 64, FP32: forward + dA + dB with 1-level ABC (and the
+      BF16 split on FP64 copies); value = forward+backward TFLOP/s.
 """
 import argparse
 import json
@@ -112,12 +123,198 @@ def cpu_baseline(threads):
     }
 
 
+def timed(torch, fn, steps, warmup, stream):
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for _ in range(steps):
+        fn()
+    stop.record(stream)
+    torch.cuda.synchronize()
+    return start.elapsed_time(stop) / steps
+
+
+def common(args, metric, value, unit, ms, dtype, config, roofline, extra=None):
+    out = {
+        "metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": dtype, "data": "synthetic (splitmix64 uniform[-1,1), seeded index permutations)", "config": config,
+        "roofline": roofline, "cpu_baseline": None,
+    }
+    if extra:
+        out.update(extra)
+    return out
+
+
+PEAK = {"f64": 78.6, "f32": 157.3, "bf16": 2500.0, "fp8": 5000.0}
+
+
+def bench_c3(ts, torch, args, stream):
+    """Precision modes at m=n=16384, k=1024 with 2^u-scaled inputs; errors vs FP64 classical."""
+    import cases
+
+    case = cases.make_case(4242, m=(128, 128), n=(128, 128), k=(32, 32))
+    m, n, k = cases.mnk(case)
+    eff = 2.0 * m * n * k
+    A, abuf = device_tensor(case["shape_A"], torch.float64, torch)
+    B, bbuf = device_tensor(case["shape_B"], torch.float64, torch)
+    ts.fill_uniform(abuf, 1, emax=8, stream=stream)
+    ts.fill_uniform(bbuf, 2, emax=8, stream=stream)
+    Cref, crefbuf = device_tensor(case["shape_C"], torch.float64, torch)
+    crefbuf.zero_()
+    ts.tensor_mult(1.0, A, case["idx_A"], B, case["idx_B"], 0.0, Cref, case["idx_C"], level=0, stream=stream)
+    C, cbuf = device_tensor(case["shape_C"], torch.float64, torch)
+    log(f"workload c3: m={m} n={n} k={k}")
+    modes = [("fp64", 0), ("fp64", 1), ("fp64", 2), ("bf16x3", 1), ("bf16x3", 2), ("ozaki_bf16", 0),
+             ("ozaki_bf16", 1), ("ozaki_bf16", 2), ("fp8_block", 1)]
+    res = {}
+    for prec, level in modes:
+        pr = "native" if prec == "fp64" else prec
+
+        def call():
+            ts.tensor_mult(1.0, A, case["idx_A"], B, case["idx_B"], 0.0, C, case["idx_C"], level=level, impl="abc",
+                           precision=pr, stream=stream)
+
+        call()
+        err = ts.rel_error(cbuf, crefbuf, stream=stream)
+        ts.kernel_timing(True)
+        ts.kernel_timing_read(reset=True)
+        ms = timed(torch, call, args.steps, args.warmup, stream)
+        kms, launches, kflops = ts.kernel_timing_read(reset=True)
+        ts.kernel_timing(False)
+        name = f"{prec}_L{level}"
+        peak = PEAK["f64"] if prec == "fp64" else (PEAK["fp8"] if prec == "fp8_block" else PEAK["bf16"])
+        leaf = kflops / kms / 1e9 if kms else None
+        res[name] = {"precision": prec, "level": level, "ms_per_step": round(ms, 4),
+                     "eff_tflops": round(eff / ms / 1e9, 3), "rel_err_vs_fp64": err,
+                     "leaf_kernel_tflops": round(leaf, 2) if leaf else None,
+                     "leaf_peak_tflops": peak, "leaf_frac": round(leaf / peak, 4) if leaf else None}
+        log(f"{name:14s} {ms:8.3f} ms {eff / ms / 1e9:8.2f} eff TFLOP/s  err {err:.2e}  leaf {leaf or 0:.1f} TFLOP/s")
+    ok = [v for v in res.values() if v["rel_err_vs_fp64"] <= 1e-10 and v["precision"] != "fp64"]
+    head = max(ok, key=lambda v: v["eff_tflops"]) if ok else res["fp64_L0"]
+    hname = f"{head['precision']}_L{head['level']}"
+    return common(args, "effective TFLOP/s (2mnk/t) at <=1e-10 rel-err vs FP64, low-precision leaves",
+                  head["eff_tflops"], "TFLOP/s", head["ms_per_step"], "bf16" if "bf16" in hname else "f64",
+                  {"workload": f"c3: m=n=16384, k=1024, A,B uniform*2^u (u in [-8,8]); reported mode {hname}",
+                   "m": m, "n": n, "k": k, "reported_mode": hname,
+                   "fp64_classical_eff_tflops": res["fp64_L0"]["eff_tflops"]},
+                  {"bound": "mfma", "achieved": head["leaf_kernel_tflops"], "peak": head["leaf_peak_tflops"],
+                   "unit": "TFLOP/s", "frac": head["leaf_frac"], "traffic": None},
+                  {"modes": res})
+
+
+def bench_c4(ts, torch, args, stream):
+    """256 FP32 contractions: individual-call loop vs grouped batch vs HIP-graph replay."""
+    import numpy as np
+
+    import cases
+
+    batch = cases.c4_batch(2024)
+    flops = sum(2.0 * float(np.prod([c["lengths"][x] for x in set(c["idx_A"]) | set(c["idx_B"])])) for c in batch)
+    ents, refs = [], []
+    for i, c in enumerate(batch):
+        A, ab = device_tensor(c["shape_A"], torch.float32, torch)
+        B, bb = device_tensor(c["shape_B"], torch.float32, torch)
+        C, cb = device_tensor(c["shape_C"], torch.float32, torch)
+        ts.fill_uniform(ab, 3 * i + 1, stream=stream)
+        ts.fill_uniform(bb, 3 * i + 2, stream=stream)
+        cb.zero_()
+        ents.append(dict(A=A, idx_A=c["idx_A"], B=B, idx_B=c["idx_B"], C=C, idx_C=c["idx_C"], beta=0.0, level=-1,
+                         impl="abc"))
+        refs.append((ab, bb, cb, c))
+    log(f"workload c4: {len(batch)} FP32 contractions, {flops / 1e9:.1f} GFLOP")
+    cstream = torch.cuda.Stream()
+    torch.cuda.set_stream(cstream)
+    stream = cstream
+
+    def loop():
+        for e in ents:
+            ts.tensor_mult(1.0, e["A"], e["idx_A"], e["B"], e["idx_B"], 0.0, e["C"], e["idx_C"], level=-1,
+                           impl="abc", stream=stream)
+
+    b = ts.Batch(ents)
+    ms_loop = timed(torch, loop, args.steps, args.warmup, stream)
+    ms_batch = timed(torch, lambda: b.run(0, stream), args.steps, args.warmup, stream)
+    ms_graph = timed(torch, lambda: b.run(ts.FLAG_GRAPH, stream), args.steps, args.warmup, stream)
+    # accuracy: every entry against its FP64 classical result
+    worst = 0.0
+    for ab, bb, cb, c in refs:
+        A64, a64 = device_tensor(c["shape_A"], torch.float64, torch)
+        B64, b64 = device_tensor(c["shape_B"], torch.float64, torch)
+        C64, c64 = device_tensor(c["shape_C"], torch.float64, torch)
+        a64.copy_(ab)
+        b64.copy_(bb)
+        c64.zero_()
+        ts.tensor_mult(1.0, A64, c["idx_A"], B64, c["idx_B"], 0.0, C64, c["idx_C"], level=0, stream=stream)
+        torch.cuda.synchronize()
+        worst = max(worst, float(torch.linalg.norm(cb.double() - c64) / torch.linalg.norm(c64)))
+    rates = {"loop": flops / ms_loop / 1e9, "batch": flops / ms_batch / 1e9, "graph": flops / ms_graph / 1e9}
+    log(f"loop {ms_loop:.3f} ms  batch {ms_batch:.3f} ms  graph {ms_graph:.3f} ms  worst err {worst:.2e}")
+    best = min(ms_batch, ms_graph)
+    torch.cuda.set_stream(torch.cuda.default_stream())
+    return common(args, "batch throughput TFLOP/s (sum 2mnk / t), 256 FP32 contractions, one grouped launch per "
+                        "stage round", round(flops / best / 1e9, 3), "TFLOP/s", best, "f32",
+                  {"workload": "c4: 256 seeded FP32 contractions, 3-6 indices, lengths 8-72, group sizes "
+                               "log-uniform [64,4096] clamped to the achievable range", "gflop": round(flops / 1e9, 3),
+                   "loop_ms": round(ms_loop, 4), "batch_ms": round(ms_batch, 4), "graph_ms": round(ms_graph, 4),
+                   "speedup_vs_loop": round(ms_loop / best, 3), "worst_rel_err_vs_fp64": worst,
+                   "tflops": {k: round(v, 3) for k, v in rates.items()}},
+                  {"bound": "mfma", "achieved": round(flops / best / 1e9, 3), "peak": PEAK["f32"], "unit": "TFLOP/s",
+                   "frac": round(flops / best / 1e9 / PEAK["f32"], 4), "traffic": None})
+
+
+def bench_c5(ts, torch, args, stream):
+    """Forward + adjoints of C[abce] = A[dcaf] B[dfbe] (all lengths 64), FP32 L1 ABC and BF16 split."""
+    L = 64
+    shape = [L] * 4
+    flops = 3 * 2.0 * float(L) ** 6
+    res = {}
+    ref = None
+    for name, dt, level, prec in (("fp64_L0", torch.float64, 0, "native"), ("fp32_L0", torch.float32, 0, "native"),
+                                  ("fp32_L1", torch.float32, 1, "native"), ("bf16x3_L1", torch.float64, 1, "bf16x3")):
+        A, ab = device_tensor(shape, dt, torch)
+        B, bb = device_tensor(shape, dt, torch)
+        dC, db = device_tensor(shape, dt, torch)
+        C, cb = device_tensor(shape, dt, torch)
+        dA, dab = device_tensor(shape, dt, torch)
+        dB, dbb = device_tensor(shape, dt, torch)
+        ts.fill_uniform(ab, 1, stream=stream)
+        ts.fill_uniform(bb, 2, stream=stream)
+        ts.fill_uniform(db, 3, stream=stream)
+
+        def step():
+            ts.tensor_mult(1.0, A, "dcaf", B, "dfbe", 0.0, C, "abce", level=level, impl="abc", precision=prec,
+                           stream=stream)
+            ts.tensor_mult_backward(1.0, A, "dcaf", B, "dfbe", dC, "abce", dA=dA, dB=dB, beta=0.0, level=level,
+                                    impl="abc", precision=prec, stream=stream)
+
+        ms = timed(torch, step, args.steps, args.warmup, stream)
+        outs = (cb.double(), dab.double(), dbb.double())
+        if ref is None:
+            ref = outs
+            errs = {"C": 0.0, "dA": 0.0, "dB": 0.0}
+        else:
+            errs = {nm: float(torch.linalg.norm(x - y) / torch.linalg.norm(y)) for nm, x, y in zip(("C", "dA", "dB"), outs, ref)}
+        res[name] = {"ms_per_step": round(ms, 4), "tflops": round(flops / ms / 1e9, 3), "rel_err_vs_fp64": errs}
+        log(f"{name:10s} {ms:8.3f} ms  {flops / ms / 1e9:7.2f} TFLOP/s  errs {errs}")
+        del A, B, dC, C, dA, dB, ab, bb, db, cb, dab, dbb
+    head = res["fp32_L1"]
+    return common(args, "forward+backward TFLOP/s (3 x 2mnk / t), C[abce]=A[dcaf]B[dfbe], lengths 64",
+                  head["tflops"], "TFLOP/s", head["ms_per_step"], "f32",
+                  {"workload": "c5: FP32 64^6 forward + dA + dB, 1-level ABC", "m": 4096, "n": 4096, "k": 4096,
+                   "modes": res},
+                  {"bound": "mfma", "achieved": head["tflops"], "peak": PEAK["f32"], "unit": "TFLOP/s",
+                   "frac": round(head["tflops"] / PEAK["f32"], 4), "traffic": None})
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS) + ["c3", "c4", "c5"])
     ap.add_argument("--variants", default="0:abc,1:abc,2:abc,1:ab,2:ab,1:naive,2:naive")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -143,6 +340,10 @@ def main():
         ts.copy_bytes(dst, src, nbytes, stream=stream)
         torch.cuda.synchronize()
         del src, dst
+    if args.workload in ("c3", "c4", "c5"):
+        out = {"c3": bench_c3, "c4": bench_c4, "c5": bench_c5}[args.workload](ts, torch, args, stream)
+        print(json.dumps(out))
+        return
     case = make_workload(args.workload)
     import cases
 

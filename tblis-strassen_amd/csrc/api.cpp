@@ -405,7 +405,9 @@ Status tensor_mult(const tblis::tblis_tensor* A, const label_type* idx_A, const 
         return Status::error(MISTRA_ERR_ARG, "unknown precision mode");
     if (precision != MISTRA_PREC_NATIVE && C->type != tblis::TYPE_DOUBLE)
         return Status::error(MISTRA_ERR_UNSUPPORTED, "low-precision modes take FP64 tensors");
-    if (level < 0 || impl == MISTRA_IMPL_AUTO)
+    if (level < 0 && impl != MISTRA_IMPL_AUTO)
+        level = best_level(A, idx_A, B, idx_B, C, idx_C, impl);
+    else if (level < 0 || impl == MISTRA_IMPL_AUTO)
     {
         TuneChoice ch;
         st = autotune(A, idx_A, B, idx_B, C, idx_C, false, stream, ch);

@@ -220,4 +220,24 @@ Status autotune(const tblis::tblis_tensor* A, const label_type* idx_A, const tbl
     return Status();
 }
 
+int best_level(const tblis::tblis_tensor* A, const label_type* idx_A, const tblis::tblis_tensor* B,
+               const label_type* idx_B, const tblis::tblis_tensor* C, const label_type* idx_C, int impl)
+{
+    int best = 0;
+    double best_ms = 1e300;
+    for (int level = 0; level <= 2; level++)
+    {
+        Plan p;
+        if (!build_plan_host(A, idx_A, B, idx_B, C, idx_C, level, impl, -1, 0, p).ok() || p.general) return 1;
+        if (level > 0 && !p.core) continue;
+        const double ms = model_ms(p);
+        if (ms < best_ms)
+        {
+            best_ms = ms;
+            best = level;
+        }
+    }
+    return best;
+}
+
 }  // namespace mistra

@@ -90,7 +90,9 @@ Status prepare(int count, const mistra_contraction* e, std::vector<std::shared_p
         const mistra_contraction& x = e[i];
         if (!x.A || !x.B || !x.C) return Status::error(MISTRA_ERR_ARG, "null tensor in batch entry");
         int level = x.level, impl = x.impl;
-        if (level < 0 || impl < 0)
+        if (level < 0 && impl >= 0)
+            level = best_level(x.A, x.idx_A, x.B, x.idx_B, x.C, x.idx_C, impl);  // the model's level for that impl
+        else if (level < 0 || impl < 0)
         {
             TuneChoice ch;
             Status st = autotune(x.A, x.idx_A, x.B, x.idx_B, x.C, x.idx_C, false, stream, ch);

@@ -235,7 +235,22 @@ Status build_plan_host(const tblis::tblis_tensor* A, const label_type* idx_A, co
     // src/stra_internal/3t/stra_mult.cxx:336-356: AC by (C, A) strides, BC by (C, B), AB by (A, B).
     permute_group<int>(g.AC, sort_by_stride(g.AC.sC, g.AC.sA));
     permute_group<int>(g.BC, sort_by_stride(g.BC.sC, g.BC.sB));
-    permute_group<int>(g.AB, sort_by_stride(g.AB.sA, g.AB.sB));
+    // K order: the reference sorts by (A, B) strides; a different order only changes the summation
+    // order.  When no K index of A has unit stride but one of B does, sort by (B, A) instead, so that
+    // the K runs of B (which become the rows of A when C is transposed) are unit-stride and load
+    // coalesced.
+    {
+        bool a_unit = false, b_unit = false;
+        for (size_t i = 0; i < g.AB.len.size(); i++)
+        {
+            a_unit = a_unit || (g.AB.sA[i] == 1 && g.AB.len[i] > 1);
+            b_unit = b_unit || (g.AB.sB[i] == 1 && g.AB.len[i] > 1);
+        }
+        if (!a_unit && b_unit)
+            permute_group<int>(g.AB, sort_by_stride(g.AB.sB, g.AB.sA));
+        else
+            permute_group<int>(g.AB, sort_by_stride(g.AB.sA, g.AB.sB));
+    }
 
     std::vector<int64_t> a_m = scatter(g.AC.len, g.AC.sA), c_m = scatter(g.AC.len, g.AC.sC);
     std::vector<int64_t> a_k = scatter(g.AB.len, g.AB.sA), b_k = scatter(g.AB.len, g.AB.sB);

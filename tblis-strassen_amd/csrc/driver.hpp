@@ -163,6 +163,9 @@ struct TuneChoice
 Status autotune(const tblis::tblis_tensor* A, const label_type* idx_A, const tblis::tblis_tensor* B,
                 const label_type* idx_B, const tblis::tblis_tensor* C, const label_type* idx_C, bool measure,
                 hipStream_t stream, TuneChoice& out);
+// The modelled best level for a given impl (host only).
+int best_level(const tblis::tblis_tensor* A, const label_type* idx_A, const tblis::tblis_tensor* B,
+               const label_type* idx_B, const tblis::tblis_tensor* C, const label_type* idx_C, int impl);
 // Modelled milliseconds of one candidate (host only).
 double model_ms(const Plan& p);
 
