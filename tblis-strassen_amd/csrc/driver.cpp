@@ -233,8 +233,21 @@ Status build_plan_host(const tblis::tblis_tensor* A, const label_type* idx_A, co
     }
 
     // src/stra_internal/3t/stra_mult.cxx:336-356: AC by (C, A) strides, BC by (C, B), AB by (A, B).
-    permute_group<int>(g.AC, sort_by_stride(g.AC.sC, g.AC.sA));
-    permute_group<int>(g.BC, sort_by_stride(g.BC.sC, g.BC.sB));
+    // M and N orders: the reference sorts by (C, operand) strides.  C's unit-stride index lies in one
+    // of the two groups; the other group is sorted by its operand's strides instead, so that the operand
+    // (not C, which is written along the first group) gets unit-stride rows.
+    {
+        auto has_unit = [](const std::vector<stride_type>& st, const std::vector<len_type>& len) {
+            for (size_t i = 0; i < st.size(); i++)
+                if (st[i] == 1 && len[i] > 1) return true;
+            return false;
+        };
+        const bool c_in_ac = has_unit(g.AC.sC, g.AC.len), c_in_bc = has_unit(g.BC.sC, g.BC.len);
+        permute_group<int>(g.AC, !c_in_ac && c_in_bc ? sort_by_stride(g.AC.sA, g.AC.sC)
+                                                     : sort_by_stride(g.AC.sC, g.AC.sA));
+        permute_group<int>(g.BC, !c_in_bc && c_in_ac ? sort_by_stride(g.BC.sB, g.BC.sC)
+                                                     : sort_by_stride(g.BC.sC, g.BC.sB));
+    }
     // K order: the reference sorts by (A, B) strides; a different order only changes the summation
     // order.  When no K index of A has unit stride but one of B does, sort by (B, A) instead, so that
     // the K runs of B (which become the rows of A when C is transposed) are unit-stride and load
