@@ -45,8 +45,8 @@ bool on_device(const void* p)
 
 struct Group
 {
-    int elem, tile, terms;
-    bool kfast, a_mfast, b_nfast;
+    int elem, tile, terms, kmode;
+    bool a_mfast, b_nfast;
     size_t job_off, first_off;  // offsets of the job and first tables in the device buffer
     int njobs;
     int64_t blocks;
@@ -126,13 +126,13 @@ Status prepare(int count, const mistra_contraction* e, std::vector<std::shared_p
         plans.push_back(plan);
     }
     // group by (round, variant)
-    std::map<std::tuple<int, int, int, int, bool, bool, bool>, std::vector<const LaunchJob*>> by;
+    std::map<std::tuple<int, int, int, int, int, bool, bool>, std::vector<const LaunchJob*>> by;
     for (const LaunchJob& j : all)
-        by[std::make_tuple(j.round, j.elem, j.tile, j.terms, j.kfast, j.a_mfast, j.b_nfast)].push_back(&j);
+        by[std::make_tuple(j.round, j.elem, j.tile, j.terms, j.kmode, j.a_mfast, j.b_nfast)].push_back(&j);
     for (auto& kv : by)
     {
         Group gr;
-        std::tie(std::ignore, gr.elem, gr.tile, gr.terms, gr.kfast, gr.a_mfast, gr.b_nfast) = kv.first;
+        std::tie(std::ignore, gr.elem, gr.tile, gr.terms, gr.kmode, gr.a_mfast, gr.b_nfast) = kv.first;
         gr.job_off = jobs_host.size();
         gr.first_off = first_host.size();
         gr.njobs = int(kv.second.size());
@@ -181,7 +181,7 @@ Status launch_groups(const std::vector<Group>& groups, void* tables, size_t firs
         q.njobs = gr.njobs;
         q.blocks = gr.blocks;
         if (timed) timing_begin(stream);
-        hipError_t e = launch_leaf_grouped(gr.elem, gr.tile, gr.terms, gr.kfast, q, gr.a_mfast, gr.b_nfast, stream);
+        hipError_t e = launch_leaf_grouped(gr.elem, gr.tile, gr.terms, gr.kmode, q, gr.a_mfast, gr.b_nfast, stream);
         if (timed) timing_end(stream, gr.flops);
         if (e != hipSuccess) return Status::error(MISTRA_ERR_HIP, std::string("grouped launch: ") + hipGetErrorString(e));
     }

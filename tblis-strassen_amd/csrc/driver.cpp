@@ -143,11 +143,12 @@ bool is_translate(const std::vector<int64_t>& v, int64_t off, int64_t len)
     return true;
 }
 
-// A launch may use the fast-K kernel body when (1) every sum term's sub-block is a translate of the
-// first sub-block in its operand's row and column scatter vectors (so a term is the first sub-block
-// shifted by a uniform offset, which the fast product table stores as its base), (2) the first
-// sub-block's K runs are affine at every K-tile start, with one stride for A and one for B, and (3) the
-// per-element offsets inside a K-tile fit 31 bits.  All C targets must be translates as well.
+// A launch may use the fast kernel body when every sum term's sub-block (A, B and the C targets) is a
+// translate of the first sub-block in its operand's scatter vectors, so that a term is the first
+// sub-block shifted by a uniform offset (the fast product table stores it as the term's base).  Then:
+// KMODE_AFFINE if the first sub-block's K runs are affine at every K-tile start with one stride per
+// operand and the offsets inside a K-tile fit 31 bits; else KMODE_GATHER if row + K offsets fit 31 bits;
+// else (and without the translate property) the general body.
 KFast kfast_check(const Plan& p, const std::vector<ProdDesc>& prods, int first, int count, int64_t ms, int64_t ns,
                   int64_t ks, int TK, const std::vector<int64_t>& a_krun, const std::vector<int64_t>& b_krun)
 {
@@ -163,10 +164,6 @@ KFast kfast_check(const Plan& p, const std::vector<ProdDesc>& prods, int first, 
         for (int t = 0; t < pd.nC; t++)
             if (!is_translate(p.h_c_m, pd.c[t].roff, ms) || !is_translate(p.h_c_n, pd.c[t].coff, ns)) return kf;
     }
-    const int64_t sa = a_krun[0], sb = b_krun[0];
-    if (sa <= 0 || sb <= 0) return kf;
-    for (int64_t k0 = 0; k0 < ks; k0 += TK)
-        if (a_krun[size_t(k0)] != sa || b_krun[size_t(k0)] != sb) return kf;
     const int64_t limit = (int64_t(1) << 31) / p.elem;
     int64_t am = 0, bn = 0;
     for (int64_t i = 0; i < ms; i++)
@@ -179,10 +176,32 @@ KFast kfast_check(const Plan& p, const std::vector<ProdDesc>& prods, int first, 
         if (p.h_b_n[size_t(i)] < 0) return kf;
         bn = std::max(bn, p.h_b_n[size_t(i)]);
     }
-    if (am + (TK - 1) * sa >= limit || bn + (TK - 1) * sb >= limit) return kf;
-    kf.on = true;
-    kf.sa = sa;
-    kf.sb = sb;
+    // affine K runs with one stride per operand at every K-tile start
+    const int64_t sa = a_krun[0], sb = b_krun[0];
+    bool affine = sa > 0 && sb > 0;
+    for (int64_t k0 = 0; affine && k0 < ks; k0 += TK)
+        affine = a_krun[size_t(k0)] == sa && b_krun[size_t(k0)] == sb;
+    if (affine && am + (TK - 1) * sa < limit && bn + (TK - 1) * sb < limit)
+    {
+        kf.on = true;
+        kf.mode = KMODE_AFFINE;
+        kf.sa = sa;
+        kf.sb = sb;
+        return kf;
+    }
+    // gathered K offsets: row offset + K offset must fit the 31-bit buffer range
+    int64_t ak = 0, bk = 0;
+    for (int64_t k = 0; k < ks; k++)
+    {
+        if (p.h_a_k[size_t(k)] < 0 || p.h_b_k[size_t(k)] < 0) return kf;
+        ak = std::max(ak, p.h_a_k[size_t(k)]);
+        bk = std::max(bk, p.h_b_k[size_t(k)]);
+    }
+    if (am + ak < limit && bn + bk < limit)
+    {
+        kf.on = true;
+        kf.mode = KMODE_GATHER;
+    }
     return kf;
 }
 
@@ -446,6 +465,7 @@ Status upload(Plan& p)
         // dense S_p (ms x ks, column-major) and T_p (ks x ns): K strides ms and 1, terms at base offsets
         const int64_t limit = (int64_t(1) << 31) / p.elem;
         p.kf_dense.on = (p.ms - 1) + (TK - 1) * p.ms < limit && (p.ns - 1) * p.ks + (TK - 1) < limit;
+        p.kf_dense.mode = p.kf_dense.on ? KMODE_AFFINE : KMODE_GENERAL;
         p.kf_dense.sa = p.ms;
         p.kf_dense.sb = 1;
     }
@@ -627,13 +647,13 @@ double leaf_flops(const Plan& p, int nprod, int64_t ms, int64_t ns, int64_t ks)
 Status plan_abc_jobs(const Plan& p, const Call& c, std::vector<LaunchJob>& out)
 {
     if (p.general) return Status::error(MISTRA_ERR_UNSUPPORTED, "general index classes run outside a batch");
-    auto push = [&](GemmArgs g, int tile, int terms, bool kfast, int round, double flops) {
-        if (!kfast) tile = TILE_64;
+    auto push = [&](GemmArgs g, int tile, int terms, int kmode, int round, double flops) {
+        tile = effective_tile(tile, kmode, terms, p.a_mfast, p.b_nfast);
         const int te = tile_extent(tile);
         g.tiles_m = int((g.ms + te - 1) / te);
         g.tiles_n = int((g.ns + te - 1) / te);
         if (g.tiles_m == 0 || g.tiles_n == 0 || g.nbatch == 0 || g.nprod == 0 || g.ks <= 0) return;
-        out.push_back(LaunchJob{g, p.elem, tile, terms, round, kfast, p.a_mfast, p.b_nfast, flops});
+        out.push_back(LaunchJob{g, p.elem, tile, terms, round, kmode, p.a_mfast, p.b_nfast, flops});
     };
     int round = 0;
     if (p.nprod > 0)
@@ -649,7 +669,7 @@ Status plan_abc_jobs(const Plan& p, const Call& c, std::vector<LaunchJob>& out)
             const int b = p.h_stage_begin[st], n = p.h_stage_begin[st + 1] - b;
             g.prods = (p.kf_core.on ? p.prods_fast_staged : p.prods_staged) + b;
             g.nprod = n;
-            push(g, p.tile, p.h_stage_terms[st], p.kf_core.on, round, leaf_flops(p, n, p.ms, p.ns, p.ks));
+            push(g, p.tile, p.h_stage_terms[st], p.kf_core.mode, round, leaf_flops(p, n, p.ms, p.ns, p.ks));
         }
     }
     for (const Pass& ps : p.fringe)
@@ -662,7 +682,7 @@ Status plan_abc_jobs(const Plan& p, const Call& c, std::vector<LaunchJob>& out)
         f.nprod = 1;
         f.ka_stride = ps.kf.sa;
         f.kb_stride = ps.kf.sb;
-        push(f, TILE_64, 1, ps.kf.on, round++, leaf_flops(p, 1, ps.ms, ps.ns, ps.ks));
+        push(f, TILE_64, 1, ps.kf.mode, round++, leaf_flops(p, 1, ps.ms, ps.ns, ps.ks));
     }
     return Status();
 }
@@ -687,7 +707,7 @@ Status run_plan(const Plan& p, const Call& c, hipStream_t stream)
             g.prods = (p.kf_core.on ? p.prods_fast_staged : p.prods_staged) + b;
             g.nprod = n;
             timing_begin(stream);
-            HIP_TRY(launch_leaf_gemm(p.elem, EPI_MULTI_C, p.tile, p.h_stage_terms[st], p.kf_core.on, g, p.a_mfast,
+            HIP_TRY(launch_leaf_gemm(p.elem, EPI_MULTI_C, p.tile, p.h_stage_terms[st], p.kf_core.mode, g, p.a_mfast,
                                      p.b_nfast, stream));
             timing_end(stream, leaf_flops(p, n, p.ms, p.ns, p.ks));
         }
@@ -707,7 +727,7 @@ Status run_plan(const Plan& p, const Call& c, hipStream_t stream)
         g.kb_stride = p.kf_core.sb;
         timing_begin(stream);
         hipError_t e = launch_leaf_gemm(p.elem, EPI_STORE, p.tile, max_terms(p.h_prods.data(), p.nprod),
-                                        p.kf_core.on, g, p.a_mfast, p.b_nfast, stream);
+                                        p.kf_core.mode, g, p.a_mfast, p.b_nfast, stream);
         timing_end(stream, leaf_flops(p, p.nprod, p.ms, p.ns, p.ks));
         if (e == hipSuccess)
         {
@@ -795,7 +815,7 @@ Status run_plan(const Plan& p, const Call& c, hipStream_t stream)
             gd.ka_stride = p.kf_dense.sa;
             gd.kb_stride = p.kf_dense.sb;
             timing_begin(stream);
-            e = launch_leaf_gemm(p.elem, EPI_STORE, dense_tile(p), 1, p.kf_dense.on, gd, true, false, stream);
+            e = launch_leaf_gemm(p.elem, EPI_STORE, dense_tile(p), 1, p.kf_dense.mode, gd, true, false, stream);
             timing_end(stream, 2.0 * double(p.nprod) * double(p.ms) * double(p.ns) * double(p.ks));
             if (e != hipSuccess) break;
 
@@ -841,7 +861,7 @@ Status run_fringe(const Plan& p, const Call& c, hipStream_t stream)
         f.nprod = 1;
         f.ka_stride = ps.kf.sa;
         f.kb_stride = ps.kf.sb;
-        HIP_TRY(launch_leaf_gemm(p.elem, EPI_MULTI_C, TILE_64, 1, ps.kf.on, f, p.a_mfast, p.b_nfast, stream));
+        HIP_TRY(launch_leaf_gemm(p.elem, EPI_MULTI_C, TILE_64, 1, ps.kf.mode, f, p.a_mfast, p.b_nfast, stream));
     }
     return Status();
 }

@@ -37,8 +37,9 @@ enum Impl
 // Fast-K eligibility of one launch (see kfast_check in driver.cpp).
 struct KFast
 {
-    bool on = false;
-    int64_t sa = 0, sb = 0;  // common K-run strides of A and B
+    bool on = false;          // a fast-body mode (the launch uses the shifted product table)
+    int mode = KMODE_GENERAL;  // KMode of the launch
+    int64_t sa = 0, sb = 0;   // common K-run strides of A and B (KMODE_AFFINE)
 };
 
 struct Pass  // one classical launch over a sub-range (the fringe the 2^L split leaves over)
@@ -135,7 +136,8 @@ struct LaunchJob
 {
     GemmArgs g;  // tiles_m / tiles_n filled for the effective tile
     int elem, tile, terms, round;
-    bool kfast, a_mfast, b_nfast;
+    int kmode;
+    bool a_mfast, b_nfast;
     double flops;
 };
 Status plan_abc_jobs(const Plan& p, const Call& c, std::vector<LaunchJob>& out);

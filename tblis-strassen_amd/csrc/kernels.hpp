@@ -36,6 +36,14 @@ enum EpiKind
     EPI_STORE = 1     // M_p stored dense, column-major, ld = ms, product p at M + p*ms*ns (AB / Naive)
 };
 
+// How the leaf kernel addresses K (chosen per launch by the host).
+enum KMode
+{
+    KMODE_GENERAL = 0,  // per-term gathered K positions (any sub-block layout)
+    KMODE_AFFINE = 1,   // translate sub-blocks, K runs affine per K-tile
+    KMODE_GATHER = 2    // translate sub-blocks, K offsets gathered per lane
+};
+
 struct GemmArgs
 {
     const void* A;

@@ -20,11 +20,14 @@ int tile_kdepth(int tile, int elem_size);
 
 // Leaf GEMM over scatter vectors with fused operand sums (leaf_gemm_kernel.hpp); `terms` is the
 // largest number of sub-blocks in any operand sum of the launch (1, 2 or 4).
-// `kfast`: every term's K-tiles are affine with g.ka_stride / g.kb_stride (host-checked, see
-// kfast_check in the driver); otherwise the gathered-K variant runs.
-hipError_t launch_leaf_gemm(int elem_size, int epi, int tile, int terms, bool kfast, GemmArgs g, bool a_mfast,
+// `kmode` (KMode): the K addressing the host checked for this launch (kfast_check in the driver).
+hipError_t launch_leaf_gemm(int elem_size, int epi, int tile, int terms, int kmode, GemmArgs g, bool a_mfast,
                             bool b_nfast, hipStream_t stream);
 int max_terms(const ProdDesc* prods, int nprod);
+
+// The register tile a launch really uses (the general and some gathered-K variants only exist, or
+// only run without spills, with the 64x64 tile).
+int effective_tile(int tile, int kmode, int terms, bool a_mfast, bool b_nfast);
 
 // Grouped multi-C launch (batched contractions): device tables of jobs and of their first workgroups.
 struct Grouped
@@ -34,7 +37,7 @@ struct Grouped
     int njobs;
     int64_t blocks;
 };
-hipError_t launch_leaf_grouped(int elem_size, int tile, int terms, bool kfast, const Grouped& q, bool a_mfast,
+hipError_t launch_leaf_grouped(int elem_size, int tile, int terms, int kmode, const Grouped& q, bool a_mfast,
                                bool b_nfast, hipStream_t stream);
 
 // Explicit operand sums (Naive impl): for product p and its terms t,

@@ -5,7 +5,7 @@
 
 namespace mistra {
 
-template <typename T, int TM, int TK, int TERMS, int EPI, bool KF>
+template <typename T, int TM, int TK, int TERMS, int EPI, int KF>
 hipError_t launch_variant(const GemmArgs& g, bool a_mfast, bool b_nfast, hipStream_t s, dim3 grid)
 {
     if (a_mfast && b_nfast)
@@ -21,7 +21,7 @@ hipError_t launch_variant(const GemmArgs& g, bool a_mfast, bool b_nfast, hipStre
 
 // Grouped launches (batched contractions), multi-C epilogue only.
 
-template <typename T, int TM, int TK, int TERMS, bool KF>
+template <typename T, int TM, int TK, int TERMS, int KF>
 hipError_t grouped_variant(const Grouped& q, bool a_mfast, bool b_nfast, hipStream_t s)
 {
     const dim3 grid(unsigned(q.blocks));
@@ -40,7 +40,7 @@ hipError_t grouped_variant(const Grouped& q, bool a_mfast, bool b_nfast, hipStre
     return hipGetLastError();
 }
 
-template <typename T, int TM, int TK, bool KF>
+template <typename T, int TM, int TK, int KF>
 hipError_t grouped_terms(const Grouped& q, int terms, bool a_mfast, bool b_nfast, hipStream_t s)
 {
     if (terms <= 1) return grouped_variant<T, TM, TK, 1, KF>(q, a_mfast, b_nfast, s);
@@ -49,16 +49,19 @@ hipError_t grouped_terms(const Grouped& q, int terms, bool a_mfast, bool b_nfast
 }
 
 template <typename T>
-hipError_t launch_grouped_typed(int tile, int terms, bool kfast, const Grouped& q, bool a_mfast, bool b_nfast,
+hipError_t launch_grouped_typed(int tile, int terms, int kmode, const Grouped& q, bool a_mfast, bool b_nfast,
                                 hipStream_t s)
 {
     constexpr int TK = sizeof(T) == 8 ? 16 : 32;
-    if (!kfast) return grouped_terms<T, 64, TK, false>(q, terms, a_mfast, b_nfast, s);
-    if (tile == TILE_64) return grouped_terms<T, 64, TK, true>(q, terms, a_mfast, b_nfast, s);
-    return grouped_terms<T, 128, TK, true>(q, terms, a_mfast, b_nfast, s);
+    if (kmode == KMODE_GENERAL) return grouped_terms<T, 64, TK, KMODE_GENERAL>(q, terms, a_mfast, b_nfast, s);
+    if (kmode == KMODE_GATHER)
+        return tile == TILE_64 ? grouped_terms<T, 64, TK, KMODE_GATHER>(q, terms, a_mfast, b_nfast, s)
+                               : grouped_terms<T, 128, TK, KMODE_GATHER>(q, terms, a_mfast, b_nfast, s);
+    if (tile == TILE_64) return grouped_terms<T, 64, TK, KMODE_AFFINE>(q, terms, a_mfast, b_nfast, s);
+    return grouped_terms<T, 128, TK, KMODE_AFFINE>(q, terms, a_mfast, b_nfast, s);
 }
 
-template <typename T, int TM, int TK, int EPI, bool KF>
+template <typename T, int TM, int TK, int EPI, int KF>
 hipError_t launch_terms(const GemmArgs& g, int terms, bool a_mfast, bool b_nfast, hipStream_t s, dim3 grid)
 {
     if (terms <= 1) return launch_variant<T, TM, TK, 1, EPI, KF>(g, a_mfast, b_nfast, s, grid);
@@ -66,16 +69,19 @@ hipError_t launch_terms(const GemmArgs& g, int terms, bool a_mfast, bool b_nfast
     return launch_variant<T, TM, TK, 4, EPI, KF>(g, a_mfast, b_nfast, s, grid);
 }
 
-// One element type and epilogue; `kfast` selects the affine-K variant.  The general (gathered K)
-// variant exists only with the 64x64 tile: it serves layouts whose K runs are not affine per tile.
+// One element type and epilogue; `kmode` selects the K addressing (KMode).  The general (per-term
+// gathered K) variant exists only with the 64x64 tile.
 template <typename T, int EPI>
-hipError_t launch_leaf_typed(int tile, int terms, bool kfast, const GemmArgs& g, bool a_mfast, bool b_nfast,
+hipError_t launch_leaf_typed(int tile, int terms, int kmode, const GemmArgs& g, bool a_mfast, bool b_nfast,
                              hipStream_t s, dim3 grid)
 {
     constexpr int TK = sizeof(T) == 8 ? 16 : 32;
-    if (!kfast) return launch_terms<T, 64, TK, EPI, false>(g, terms, a_mfast, b_nfast, s, grid);
-    if (tile == TILE_64) return launch_terms<T, 64, TK, EPI, true>(g, terms, a_mfast, b_nfast, s, grid);
-    return launch_terms<T, 128, TK, EPI, true>(g, terms, a_mfast, b_nfast, s, grid);
+    if (kmode == KMODE_GENERAL) return launch_terms<T, 64, TK, EPI, KMODE_GENERAL>(g, terms, a_mfast, b_nfast, s, grid);
+    if (kmode == KMODE_GATHER)
+        return tile == TILE_64 ? launch_terms<T, 64, TK, EPI, KMODE_GATHER>(g, terms, a_mfast, b_nfast, s, grid)
+                               : launch_terms<T, 128, TK, EPI, KMODE_GATHER>(g, terms, a_mfast, b_nfast, s, grid);
+    if (tile == TILE_64) return launch_terms<T, 64, TK, EPI, KMODE_AFFINE>(g, terms, a_mfast, b_nfast, s, grid);
+    return launch_terms<T, 128, TK, EPI, KMODE_AFFINE>(g, terms, a_mfast, b_nfast, s, grid);
 }
 
 }  // namespace mistra

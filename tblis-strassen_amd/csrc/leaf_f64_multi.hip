@@ -3,16 +3,16 @@
 
 namespace mistra {
 
-hipError_t launch_leaf_f64_multi(int tile, int terms, bool kfast, const GemmArgs& g, bool a_mfast, bool b_nfast,
+hipError_t launch_leaf_f64_multi(int tile, int terms, int kmode, const GemmArgs& g, bool a_mfast, bool b_nfast,
                                hipStream_t s, dim3 grid)
 {
-    return launch_leaf_typed<double, EPI_MULTI_C>(tile, terms, kfast, g, a_mfast, b_nfast, s, grid);
+    return launch_leaf_typed<double, EPI_MULTI_C>(tile, terms, kmode, g, a_mfast, b_nfast, s, grid);
 }
 
-hipError_t launch_leaf_f64_grouped(int tile, int terms, bool kfast, const Grouped& q, bool a_mfast, bool b_nfast,
+hipError_t launch_leaf_f64_grouped(int tile, int terms, int kmode, const Grouped& q, bool a_mfast, bool b_nfast,
                                  hipStream_t s)
 {
-    return launch_grouped_typed<double>(tile, terms, kfast, q, a_mfast, b_nfast, s);
+    return launch_grouped_typed<double>(tile, terms, kmode, q, a_mfast, b_nfast, s);
 }
 
 }  // namespace mistra

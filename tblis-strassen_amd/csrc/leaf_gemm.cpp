@@ -4,25 +4,33 @@
 namespace mistra {
 
 #define MISTRA_DECLARE_LEAF(name)                                                                     \
-    hipError_t name(int tile, int terms, bool kfast, const GemmArgs& g, bool a_mfast, bool b_nfast, hipStream_t s, \
+    hipError_t name(int tile, int terms, int kmode, const GemmArgs& g, bool a_mfast, bool b_nfast, hipStream_t s, \
                     dim3 grid);
 MISTRA_DECLARE_LEAF(launch_leaf_f64_multi)
 MISTRA_DECLARE_LEAF(launch_leaf_f64_store)
 MISTRA_DECLARE_LEAF(launch_leaf_f32_multi)
 MISTRA_DECLARE_LEAF(launch_leaf_f32_store)
 #undef MISTRA_DECLARE_LEAF
-hipError_t launch_leaf_f64_grouped(int tile, int terms, bool kfast, const Grouped& q, bool a_mfast, bool b_nfast,
+hipError_t launch_leaf_f64_grouped(int tile, int terms, int kmode, const Grouped& q, bool a_mfast, bool b_nfast,
                                    hipStream_t s);
-hipError_t launch_leaf_f32_grouped(int tile, int terms, bool kfast, const Grouped& q, bool a_mfast, bool b_nfast,
+hipError_t launch_leaf_f32_grouped(int tile, int terms, int kmode, const Grouped& q, bool a_mfast, bool b_nfast,
                                    hipStream_t s);
 
-hipError_t launch_leaf_grouped(int elem_size, int tile, int terms, bool kfast, const Grouped& q, bool a_mfast,
+int effective_tile(int tile, int kmode, int terms, bool a_mfast, bool b_nfast)
+{
+    if (kmode == KMODE_GENERAL) return TILE_64;  // the general variant exists with the 64x64 tile only
+    // gathered K offsets cost registers per lane: these 128x128 variants would spill
+    if (kmode == KMODE_GATHER && (terms >= 4 || (terms >= 2 && a_mfast && b_nfast))) return TILE_64;
+    return tile;
+}
+
+hipError_t launch_leaf_grouped(int elem_size, int tile, int terms, int kmode, const Grouped& q, bool a_mfast,
                                bool b_nfast, hipStream_t stream)
 {
     if (q.njobs <= 0 || q.blocks <= 0) return hipSuccess;
-    if (!kfast) tile = TILE_64;
-    return elem_size == 8 ? launch_leaf_f64_grouped(tile, terms, kfast, q, a_mfast, b_nfast, stream)
-                          : launch_leaf_f32_grouped(tile, terms, kfast, q, a_mfast, b_nfast, stream);
+    if (tile != effective_tile(tile, kmode, terms, a_mfast, b_nfast)) return hipErrorInvalidValue;  // jobs hold tile counts
+    return elem_size == 8 ? launch_leaf_f64_grouped(tile, terms, kmode, q, a_mfast, b_nfast, stream)
+                          : launch_leaf_f32_grouped(tile, terms, kmode, q, a_mfast, b_nfast, stream);
 }
 
 int tile_extent(int tile) { return tile == TILE_64 ? 64 : 128; }
@@ -44,10 +52,10 @@ int max_terms(const ProdDesc* prods, int nprod)
     return t;
 }
 
-hipError_t launch_leaf_gemm(int elem_size, int epi, int tile, int terms, bool kfast, GemmArgs g, bool a_mfast,
+hipError_t launch_leaf_gemm(int elem_size, int epi, int tile, int terms, int kmode, GemmArgs g, bool a_mfast,
                             bool b_nfast, hipStream_t stream)
 {
-    if (!kfast) tile = TILE_64;  // the gathered-K variant exists with the 64x64 tile only
+    tile = effective_tile(tile, kmode, terms, a_mfast, b_nfast);
     const int te = tile_extent(tile);
     g.tiles_m = int((g.ms + te - 1) / te);
     g.tiles_n = int((g.ns + te - 1) / te);
@@ -57,10 +65,10 @@ hipError_t launch_leaf_gemm(int elem_size, int epi, int tile, int terms, bool kf
     // sub-blocks (the caller groups them so), since their read-modify-write updates are not ordered
     dim3 grid(unsigned(g.tiles_m) * unsigned(g.tiles_n), unsigned(g.nprod), unsigned(g.nbatch));
     if (elem_size == 8)
-        return epi == EPI_STORE ? launch_leaf_f64_store(tile, terms, kfast, g, a_mfast, b_nfast, stream, grid)
-                                : launch_leaf_f64_multi(tile, terms, kfast, g, a_mfast, b_nfast, stream, grid);
-    return epi == EPI_STORE ? launch_leaf_f32_store(tile, terms, kfast, g, a_mfast, b_nfast, stream, grid)
-                            : launch_leaf_f32_multi(tile, terms, kfast, g, a_mfast, b_nfast, stream, grid);
+        return epi == EPI_STORE ? launch_leaf_f64_store(tile, terms, kmode, g, a_mfast, b_nfast, stream, grid)
+                                : launch_leaf_f64_multi(tile, terms, kmode, g, a_mfast, b_nfast, stream, grid);
+    return epi == EPI_STORE ? launch_leaf_f32_store(tile, terms, kmode, g, a_mfast, b_nfast, stream, grid)
+                            : launch_leaf_f32_multi(tile, terms, kmode, g, a_mfast, b_nfast, stream, grid);
 }
 
 }  // namespace mistra

@@ -128,7 +128,7 @@ __device__ __forceinline__ int64_t readlane64(int64_t x, int l)
 }
 
 // The body of one workgroup: register tile bx of the launch's tile grid, product by, batch entry bz.
-template <typename T, int TM, int TN, int TK, int TERMS, bool A_MFAST, bool B_NFAST, int EPI, bool KFAST>
+template <typename T, int TM, int TN, int TK, int TERMS, bool A_MFAST, bool B_NFAST, int EPI, int KMODE>
 __device__ __forceinline__ void leaf_gemm_body(const GemmArgs& g, int bx, int by, int64_t bz)
 {
     constexpr int NT = 256;
@@ -264,15 +264,20 @@ __device__ __forceinline__ void leaf_gemm_body(const GemmArgs& g, int bx, int by
         }
     };
 
-    if constexpr (KFAST)
+    if constexpr (KMODE != KMODE_GENERAL)
     {
         // ---- fast body -------------------------------------------------------------------------------
         // Chosen by the host (kfast_check) when every sum term's sub-block is a translate of the first
-        // sub-block in all three scatter vectors of its operand, and the K runs of that first sub-block
-        // are affine at every K-tile start with stride g.ka_stride (A) / g.kb_stride (B).  Then term t of
-        // product p sits at a uniform shift (ProdDesc base, precomputed on the host) from the first
-        // sub-block, every element's byte offset within a K-tile is the same for all terms and all
-        // K-tiles, and a K-tile load is: uniform base (shift + the tile's first K offset) + that offset.
+        // sub-block in all three scatter vectors of its operand: term t of product p then sits at a
+        // uniform shift (ProdDesc base, precomputed on the host) from the first sub-block.
+        //   KMODE_AFFINE: the K runs of the first sub-block are affine at every K-tile start with stride
+        //     g.ka_stride (A) / g.kb_stride (B), so an element's byte offset within a K-tile is the same
+        //     for all terms and K-tiles: a load is uniform base (shift + the tile's first K offset) +
+        //     that offset.
+        //   KMODE_GATHER: any K layout: each lane reads the K offsets of its own K positions for the
+        //     next tile (a_k / b_k, one small load each), a load is uniform base (shift) + row offset +
+        //     K offset.
+        constexpr bool KG = KMODE == KMODE_GATHER;
         const int p = by;
         const ProdDesc& P = g.prods[p];
         const int nA = __builtin_amdgcn_readfirstlane(P.nA), nB = __builtin_amdgcn_readfirstlane(P.nB);
@@ -298,12 +303,14 @@ __device__ __forceinline__ void leaf_gemm_body(const GemmArgs& g, int bx, int by
 #pragma unroll
             for (int e = 0; e < EA; e++)
                 aoff[e] = aok[A_MFAST ? 0 : e]
-                              ? uint32_t((ar[A_MFAST ? 0 : e] + int64_t(a_k + e * A_KSTEP) * g.ka_stride) * int64_t(sizeof(T)))
+                              ? uint32_t((ar[A_MFAST ? 0 : e] + (KG ? 0 : int64_t(a_k + e * A_KSTEP) * g.ka_stride)) *
+                                         int64_t(sizeof(T)))
                               : kOob;
 #pragma unroll
             for (int e = 0; e < EB; e++)
                 boff[e] = bok[B_NFAST ? 0 : e]
-                              ? uint32_t((bc[B_NFAST ? 0 : e] + int64_t(b_k + e * B_KSTEP) * g.kb_stride) * int64_t(sizeof(T)))
+                              ? uint32_t((bc[B_NFAST ? 0 : e] + (KG ? 0 : int64_t(b_k + e * B_KSTEP) * g.kb_stride)) *
+                                         int64_t(sizeof(T)))
                               : kOob;
         }
         T acoef[TERMS], bcoef[TERMS];
@@ -318,15 +325,41 @@ __device__ __forceinline__ void leaf_gemm_body(const GemmArgs& g, int bx, int by
         }
 
         const int64_t nkt = (ks + TK - 1) / TK;
-        // First K offsets of a tile (lane 0: A, lane 1: B): one vector load, read into SGPRs with
-        // v_readlane two tiles before the tile is loaded.
-        int64_t kinfo = 0;
-        auto fetch_kinfo = [&](int64_t kt) { kinfo = (lane == 1 ? g.b_k : g.a_k)[lmin(kt, nkt - 1) * TK]; };
+        // K offsets of a tile.  Affine: the tile's first K offsets (lane 0: A, lane 1: B), one vector
+        // load, read into SGPRs with v_readlane.  Gathered: each lane's K offsets for its own K
+        // positions (positions past ks clamp to ks - 1; their elements read 0).  Fetched one tile
+        // before the tile's loads are issued.
+        constexpr int NKA = KG ? (A_MFAST ? EA : 1) : 1, NKB = KG ? (B_NFAST ? EB : 1) : 1;
+        typedef std::conditional_t<KG, uint32_t, int64_t> KT;  // gathered offsets fit 31 bits (host-checked)
         struct KOff
         {
-            int64_t a, b;
+            KT a[NKA], b[NKB];
         };
-        auto read_kinfo = [&]() { return KOff{readlane64(kinfo, 0), readlane64(kinfo, 1)}; };
+        int64_t kinfo = 0;
+        KOff kgat;
+        auto fetch_kinfo = [&](int64_t kt) {
+            const int64_t k0 = lmin(kt, nkt - 1) * TK;
+            if constexpr (KG)
+            {
+#pragma unroll
+                for (int i = 0; i < NKA; i++) kgat.a[i] = KT(g.a_k[lmin(k0 + a_k + i * A_KSTEP, ks - 1)]);
+#pragma unroll
+                for (int i = 0; i < NKB; i++) kgat.b[i] = KT(g.b_k[lmin(k0 + b_k + i * B_KSTEP, ks - 1)]);
+            }
+            else
+                kinfo = (lane == 1 ? g.b_k : g.a_k)[k0];
+        };
+        auto read_kinfo = [&]() {
+            if constexpr (KG)
+                return kgat;
+            else
+            {
+                KOff k;
+                k.a[0] = readlane64(kinfo, 0);
+                k.b[0] = readlane64(kinfo, 1);
+                return k;
+            }
+        };
 
         // Load jobs: job (tile kt, term t) reads term t of K-tile kt into one of two register sets;
         // the K loop runs TERMS phases per tile, and phase q issues job q + TERMS + 1 and adds job
@@ -340,16 +373,24 @@ __device__ __forceinline__ void leaf_gemm_body(const GemmArgs& g, int bx, int by
         // out of range, i.e. zeros with no memory traffic): the load count between any two points of
         // the loop is then the same on every path, so each wait covers exactly the loads it needs
         // (a skipped load on one path would make the waits count conservatively, down to vmcnt(0)).
-        auto issue = [&](auto set, int t, int64_t kt, KOff ko) {
+        auto issue = [&](auto set, int t, int64_t kt, const KOff& ko) {
             constexpr int S = decltype(set)::value;
             const int klim = kt < nkt ? int(lmin(ks - 1 - kt * TK, TK - 1)) : -1;  // k > klim reads 0
             const int kla = t < nA ? klim : -1, klb = t < nB ? klim : -1;
-            const __amdgpu_buffer_rsrc_t ra_ = make_rsrc(Ag + ash[t] + ko.a);
+            const __amdgpu_buffer_rsrc_t ra_ = make_rsrc(Ag + ash[t] + (KG ? 0 : ko.a[0]));
 #pragma unroll
-            for (int e = 0; e < EA; e++) ra[S][e] = bload<T>(ra_, aoff[e] | past(a_k + e * A_KSTEP, kla));
-            const __amdgpu_buffer_rsrc_t rb_ = make_rsrc(Bg + bsh[t] + ko.b);
+            for (int e = 0; e < EA; e++)
+            {
+                const uint32_t kof = KG ? uint32_t(ko.a[A_MFAST ? e : 0]) * uint32_t(sizeof(T)) : 0u;  // < 2^31
+                ra[S][e] = bload<T>(ra_, (aoff[e] + kof) | past(a_k + e * A_KSTEP, kla));
+            }
+            const __amdgpu_buffer_rsrc_t rb_ = make_rsrc(Bg + bsh[t] + (KG ? 0 : ko.b[0]));
 #pragma unroll
-            for (int e = 0; e < EB; e++) rb[S][e] = bload<T>(rb_, boff[e] | past(b_k + e * B_KSTEP, klb));
+            for (int e = 0; e < EB; e++)
+            {
+                const uint32_t kof = KG ? uint32_t(ko.b[B_NFAST ? e : 0]) * uint32_t(sizeof(T)) : 0u;
+                rb[S][e] = bload<T>(rb_, (boff[e] + kof) | past(b_k + e * B_KSTEP, klb));
+            }
         };
         // Term sums: term 0 stores its scaled values into the LDS buffer, later terms read-modify-write
         // their own elements (a running sum in registers measured slower: it costs the registers that
@@ -746,15 +787,15 @@ __device__ __forceinline__ void leaf_gemm_body(const GemmArgs& g, int bx, int by
 }
 
 // One launch: grid (tiles, products, batch entries).
-template <typename T, int TM, int TN, int TK, int TERMS, bool A_MFAST, bool B_NFAST, int EPI, bool KFAST>
+template <typename T, int TM, int TN, int TK, int TERMS, bool A_MFAST, bool B_NFAST, int EPI, int KMODE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void leaf_gemm_kernel(GemmArgs g)
 {
-    leaf_gemm_body<T, TM, TN, TK, TERMS, A_MFAST, B_NFAST, EPI, KFAST>(g, blockIdx.x, blockIdx.y, blockIdx.z);
+    leaf_gemm_body<T, TM, TN, TK, TERMS, A_MFAST, B_NFAST, EPI, KMODE>(g, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
 // Grouped launch (batched contractions): job j owns workgroups [first[j], first[j + 1]) of a 1-D grid,
 // laid out as (tile, product, batch entry) of its own GemmArgs; jobs of one launch write disjoint C.
-template <typename T, int TM, int TN, int TK, int TERMS, bool A_MFAST, bool B_NFAST, int EPI, bool KFAST>
+template <typename T, int TM, int TN, int TK, int TERMS, bool A_MFAST, bool B_NFAST, int EPI, int KMODE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void leaf_gemm_grouped(
     const GemmArgs* __restrict__ jobs, const int64_t* __restrict__ first, int njobs)
 {
@@ -772,7 +813,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void l
     const int64_t local = b - first[lo];
     const int64_t tiles = int64_t(g.tiles_m) * g.tiles_n;
     const int64_t rest = local / tiles;
-    leaf_gemm_body<T, TM, TN, TK, TERMS, A_MFAST, B_NFAST, EPI, KFAST>(g, int(local % tiles), int(rest % g.nprod),
+    leaf_gemm_body<T, TM, TN, TK, TERMS, A_MFAST, B_NFAST, EPI, KMODE>(g, int(local % tiles), int(rest % g.nprod),
                                                                       rest / g.nprod);
 }
 
