@@ -184,7 +184,8 @@ KFast kfast_check(const Plan& p, const std::vector<ProdDesc>& prods, int first, 
     if (affine && am + (TK - 1) * sa < limit && bn + (TK - 1) * sb < limit)
     {
         kf.on = true;
-        kf.mode = KMODE_AFFINE;
+        // whole K-tiles and a K-fast operand: 16-byte loads along its unit-stride K
+        kf.mode = ks % TK == 0 && (!p.a_mfast || !p.b_nfast) ? KMODE_AFFINE_VEC : KMODE_AFFINE;
         kf.sa = sa;
         kf.sb = sb;
         return kf;
@@ -465,7 +466,7 @@ Status upload(Plan& p)
         // dense S_p (ms x ks, column-major) and T_p (ks x ns): K strides ms and 1, terms at base offsets
         const int64_t limit = (int64_t(1) << 31) / p.elem;
         p.kf_dense.on = (p.ms - 1) + (TK - 1) * p.ms < limit && (p.ns - 1) * p.ks + (TK - 1) < limit;
-        p.kf_dense.mode = p.kf_dense.on ? KMODE_AFFINE : KMODE_GENERAL;
+        p.kf_dense.mode = !p.kf_dense.on ? KMODE_GENERAL : (p.ks % TK == 0 ? KMODE_AFFINE_VEC : KMODE_AFFINE);
         p.kf_dense.sa = p.ms;
         p.kf_dense.sb = 1;
     }

@@ -41,7 +41,9 @@ enum KMode
 {
     KMODE_GENERAL = 0,  // per-term gathered K positions (any sub-block layout)
     KMODE_AFFINE = 1,   // translate sub-blocks, K runs affine per K-tile
-    KMODE_GATHER = 2    // translate sub-blocks, K offsets gathered per lane
+    KMODE_GATHER = 2,   // translate sub-blocks, K offsets gathered per lane
+    KMODE_AFFINE_VEC = 3  // KMODE_AFFINE with ks a multiple of the K-tile: K-fast operands (unit-stride
+                          // K) load 16 bytes per lane
 };
 
 struct GemmArgs

@@ -57,6 +57,9 @@ hipError_t launch_grouped_typed(int tile, int terms, int kmode, const Grouped& q
     if (kmode == KMODE_GATHER)
         return tile == TILE_64 ? grouped_terms<T, 64, TK, KMODE_GATHER>(q, terms, a_mfast, b_nfast, s)
                                : grouped_terms<T, 128, TK, KMODE_GATHER>(q, terms, a_mfast, b_nfast, s);
+    if (kmode == KMODE_AFFINE_VEC)
+        return tile == TILE_64 ? grouped_terms<T, 64, TK, KMODE_AFFINE_VEC>(q, terms, a_mfast, b_nfast, s)
+                               : grouped_terms<T, 128, TK, KMODE_AFFINE_VEC>(q, terms, a_mfast, b_nfast, s);
     if (tile == TILE_64) return grouped_terms<T, 64, TK, KMODE_AFFINE>(q, terms, a_mfast, b_nfast, s);
     return grouped_terms<T, 128, TK, KMODE_AFFINE>(q, terms, a_mfast, b_nfast, s);
 }
@@ -80,6 +83,9 @@ hipError_t launch_leaf_typed(int tile, int terms, int kmode, const GemmArgs& g, 
     if (kmode == KMODE_GATHER)
         return tile == TILE_64 ? launch_terms<T, 64, TK, EPI, KMODE_GATHER>(g, terms, a_mfast, b_nfast, s, grid)
                                : launch_terms<T, 128, TK, EPI, KMODE_GATHER>(g, terms, a_mfast, b_nfast, s, grid);
+    if (kmode == KMODE_AFFINE_VEC)
+        return tile == TILE_64 ? launch_terms<T, 64, TK, EPI, KMODE_AFFINE_VEC>(g, terms, a_mfast, b_nfast, s, grid)
+                               : launch_terms<T, 128, TK, EPI, KMODE_AFFINE_VEC>(g, terms, a_mfast, b_nfast, s, grid);
     if (tile == TILE_64) return launch_terms<T, 64, TK, EPI, KMODE_AFFINE>(g, terms, a_mfast, b_nfast, s, grid);
     return launch_terms<T, 128, TK, EPI, KMODE_AFFINE>(g, terms, a_mfast, b_nfast, s, grid);
 }

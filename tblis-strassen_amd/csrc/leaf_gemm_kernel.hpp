@@ -102,6 +102,14 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* base)
 __device__ __forceinline__ uint32_t past(int k, int klim) { return uint32_t(klim - k) & kOob; }
 template <typename T>
 __device__ __forceinline__ T bload(__amdgpu_buffer_rsrc_t r, uint32_t off);
+// 16 bytes (W elements of T) in one load
+template <typename T, int W>
+__device__ __forceinline__ void bload16(__amdgpu_buffer_rsrc_t r, uint32_t off, T* out)
+{
+    static_assert(W * sizeof(T) == 16, "16-byte load");
+    auto x = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+    __builtin_memcpy(out, &x, 16);
+}
 template <>
 __device__ __forceinline__ double bload<double>(__amdgpu_buffer_rsrc_t r, uint32_t off)
 {
@@ -135,8 +143,12 @@ __device__ __forceinline__ void leaf_gemm_body(const GemmArgs& g, int bx, int by
     constexpr int WM = TM / 2, WN = TN / 2;  // 2x2 waves
     constexpr int FM = WM / 16, FN = WN / 16;
     constexpr int LDA = TM + 1, LDB = TN + 1;  // +1 element: conflict-free K-fast LDS stores
-    constexpr int EA = TM * TK / NT, EB = TN * TK / NT;
-    constexpr int RA = A_MFAST ? 1 : EA, RB = B_NFAST ? 1 : EB;  // row (col) positions per thread
+    constexpr int EA = TM * TK / NT, EB = TN * TK / NT;  // elements per thread and K-tile
+    // 16-byte loads along unit-stride K (KMODE_AFFINE_VEC, K-fast operands): WA (WB) consecutive k per load
+    constexpr bool VA = KMODE == KMODE_AFFINE_VEC && !A_MFAST, VB = KMODE == KMODE_AFFINE_VEC && !B_NFAST;
+    constexpr int WA = VA ? 16 / int(sizeof(T)) : 1, WB = VB ? 16 / int(sizeof(T)) : 1;
+    constexpr int LA = EA / WA, LB = EB / WB;                      // loads per thread and K-tile
+    constexpr int RA = A_MFAST ? 1 : LA, RB = B_NFAST ? 1 : LB;    // row (col) positions per thread
     static_assert(EA >= 1 && EB >= 1, "tile too small for 256 threads");
     static_assert(NT % TM == 0 && NT % TN == 0 && NT % TK == 0, "loader layout");
     typedef typename Mfma<T>::v4 v4;
@@ -165,13 +177,13 @@ __device__ __forceinline__ void leaf_gemm_body(const GemmArgs& g, int bx, int by
 
     // Loader positions inside the tile.  A: M-fast -> consecutive lanes walk m (coalesced when m has
     // unit stride); else lanes walk k.  B likewise with n.
-    const int a_m = A_MFAST ? tid % TM : tid / TK;
-    const int a_k = A_MFAST ? tid / TM : tid % TK;
-    constexpr int A_MSTEP = A_MFAST ? 0 : NT / TK;
+    const int a_m = A_MFAST ? tid % TM : tid / (TK / WA);
+    const int a_k = A_MFAST ? tid / TM : (tid % (TK / WA)) * WA;
+    constexpr int A_MSTEP = A_MFAST ? 0 : NT * WA / TK;
     constexpr int A_KSTEP = A_MFAST ? NT / TM : 0;
-    const int b_n = B_NFAST ? tid % TN : tid / TK;
-    const int b_k = B_NFAST ? tid / TN : tid % TK;
-    constexpr int B_NSTEP = B_NFAST ? 0 : NT / TK;
+    const int b_n = B_NFAST ? tid % TN : tid / (TK / WB);
+    const int b_k = B_NFAST ? tid / TN : (tid % (TK / WB)) * WB;
+    constexpr int B_NSTEP = B_NFAST ? 0 : NT * WB / TK;
     constexpr int B_KSTEP = B_NFAST ? NT / TN : 0;
 
     // Epilogue, shared by both bodies.  Element v of a lane's 16x16 accumulator block is
@@ -284,7 +296,7 @@ __device__ __forceinline__ void leaf_gemm_body(const GemmArgs& g, int bx, int by
 
         // Per-lane byte offsets of the K-tile elements, relative to the tile's first K offset; rows
         // (columns) past the tile edge get an out-of-range offset and read 0.
-        uint32_t aoff[EA], boff[EB];
+        uint32_t aoff[LA], boff[LB];
         {
             int64_t ar[RA], bc[RB];
             bool aok[RA], bok[RB];
@@ -301,13 +313,13 @@ __device__ __forceinline__ void leaf_gemm_body(const GemmArgs& g, int bx, int by
                 bc[e] = g.b_n[lmin(col0 + b_n + e * B_NSTEP, ns - 1)];
             }
 #pragma unroll
-            for (int e = 0; e < EA; e++)
+            for (int e = 0; e < LA; e++)
                 aoff[e] = aok[A_MFAST ? 0 : e]
                               ? uint32_t((ar[A_MFAST ? 0 : e] + (KG ? 0 : int64_t(a_k + e * A_KSTEP) * g.ka_stride)) *
                                          int64_t(sizeof(T)))
                               : kOob;
 #pragma unroll
-            for (int e = 0; e < EB; e++)
+            for (int e = 0; e < LB; e++)
                 boff[e] = bok[B_NFAST ? 0 : e]
                               ? uint32_t((bc[B_NFAST ? 0 : e] + (KG ? 0 : int64_t(b_k + e * B_KSTEP) * g.kb_stride)) *
                                          int64_t(sizeof(T)))
@@ -379,17 +391,25 @@ __device__ __forceinline__ void leaf_gemm_body(const GemmArgs& g, int bx, int by
             const int kla = t < nA ? klim : -1, klb = t < nB ? klim : -1;
             const __amdgpu_buffer_rsrc_t ra_ = make_rsrc(Ag + ash[t] + (KG ? 0 : ko.a[0]));
 #pragma unroll
-            for (int e = 0; e < EA; e++)
+            for (int e = 0; e < LA; e++)
             {
                 const uint32_t kof = KG ? uint32_t(ko.a[A_MFAST ? e : 0]) * uint32_t(sizeof(T)) : 0u;  // < 2^31
-                ra[S][e] = bload<T>(ra_, (aoff[e] + kof) | past(a_k + e * A_KSTEP, kla));
+                const uint32_t off = (aoff[e] + kof) | past(a_k + e * A_KSTEP, kla);
+                if constexpr (VA)
+                    bload16<T, WA>(ra_, off, &ra[S][e * WA]);  // ks is a multiple of TK: whole vectors
+                else
+                    ra[S][e] = bload<T>(ra_, off);
             }
             const __amdgpu_buffer_rsrc_t rb_ = make_rsrc(Bg + bsh[t] + (KG ? 0 : ko.b[0]));
 #pragma unroll
-            for (int e = 0; e < EB; e++)
+            for (int e = 0; e < LB; e++)
             {
                 const uint32_t kof = KG ? uint32_t(ko.b[B_NFAST ? e : 0]) * uint32_t(sizeof(T)) : 0u;
-                rb[S][e] = bload<T>(rb_, (boff[e] + kof) | past(b_k + e * B_KSTEP, klb));
+                const uint32_t off = (boff[e] + kof) | past(b_k + e * B_KSTEP, klb);
+                if constexpr (VB)
+                    bload16<T, WB>(rb_, off, &rb[S][e * WB]);
+                else
+                    rb[S][e] = bload<T>(rb_, off);
             }
         };
         // Term sums: term 0 stores its scaled values into the LDS buffer, later terms read-modify-write
@@ -400,22 +420,26 @@ __device__ __forceinline__ void leaf_gemm_body(const GemmArgs& g, int bx, int by
             if (t < nA)
             {
 #pragma unroll
-                for (int e = 0; e < EA; e++)
-                {
-                    T& dst = sA[buf][(a_k + e * A_KSTEP) * LDA + a_m + e * A_MSTEP];
-                    const T v = acoef[t] * ra[S][e];
-                    dst = t == 0 ? v : dst + v;
-                }
+                for (int e = 0; e < LA; e++)
+#pragma unroll
+                    for (int j = 0; j < WA; j++)
+                    {
+                        T& dst = sA[buf][(a_k + j + e * A_KSTEP) * LDA + a_m + e * A_MSTEP];
+                        const T v = acoef[t] * ra[S][e * WA + j];
+                        dst = t == 0 ? v : dst + v;
+                    }
             }
             if (t < nB)
             {
 #pragma unroll
-                for (int e = 0; e < EB; e++)
-                {
-                    T& dst = sB[buf][(b_k + e * B_KSTEP) * LDB + b_n + e * B_NSTEP];
-                    const T v = bcoef[t] * rb[S][e];
-                    dst = t == 0 ? v : dst + v;
-                }
+                for (int e = 0; e < LB; e++)
+#pragma unroll
+                    for (int j = 0; j < WB; j++)
+                    {
+                        T& dst = sB[buf][(b_k + j + e * B_KSTEP) * LDB + b_n + e * B_NSTEP];
+                        const T v = bcoef[t] * rb[S][e * WB + j];
+                        dst = t == 0 ? v : dst + v;
+                    }
             }
         };
         typedef std::integral_constant<int, 0> S0;
