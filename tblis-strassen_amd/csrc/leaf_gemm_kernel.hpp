@@ -83,11 +83,13 @@ __device__ __forceinline__ int64_t lmin(int64_t a, int64_t b) { return a < b ? a
 
 // Multi-term products in the fast body: with one register set the loads of each term land during
 // 1/TERMS of a tile's MFMA work; with two sets each load gets two phases.  Measured on gfx950 (128 x 128
-// tile, FP64): one set is faster for 2 terms (the register file is nearly full with two), two sets for 4.
-template <int TERMS>
+// tile, FP64, 16-byte K loads): two sets are faster for 2 and 4 terms (at 2 terms 57.3 vs 54.5 TFLOP/s
+// per leaf; before the 16-byte loads freed address registers, one set was faster).
+template <int TERMS, int KMODE>
 constexpr bool two_sets()
 {
-    return TERMS >= 4;
+    // 2 terms: only where the 16-byte loads leave the registers for it (other modes would spill)
+    return TERMS >= 4 || (TERMS == 2 && KMODE == KMODE_AFFINE_VEC);
 }
 
 // Raw buffer loads for the fast body: a uniform base (the SGPR resource) plus a 32-bit lane offset,
@@ -379,7 +381,7 @@ __device__ __forceinline__ void leaf_gemm_body(const GemmArgs& g, int bx, int by
         // terms read-modify-write their own elements, so phases need no barrier.  Every load has two
         // phases of MFMA work to land in, and the register cost is two single-term sets whatever
         // the number of terms.
-        constexpr bool kTwoSets = two_sets<TERMS>();
+        constexpr bool kTwoSets = two_sets<TERMS, KMODE>();
         T ra[kTwoSets ? 2 : 1][EA], rb[kTwoSets ? 2 : 1][EB];
         // Every issue executes its loads (a job past the last tile or a term the product lacks reads
         // out of range, i.e. zeros with no memory traffic): the load count between any two points of
