@@ -882,14 +882,16 @@ Status run_plan_lowp(const Plan& p, const Call& c, int precision, int nslices, h
         const bool fp8 = precision == MISTRA_PREC_FP8_BLOCK;
         const int64_t ms = p.ms, ns = p.ns, ks = p.ks, nq = p.nprod;
         auto up = [](int64_t x, int64_t m) { return (x + m - 1) / m * m; };
-        const int64_t mp = up(ms, 64), np = up(ns, 64), kp = up(ks, fp8 ? 128 : 64);
+        const int64_t mp = up(ms, 128), np = up(ns, 64), kp = up(ks, fp8 ? 128 : 64);
         const int nsl = fp8 ? 1 : nslices;
         const size_t esz = fp8 ? 1 : 2;
         struct Buf
         {
             void* p = nullptr;
         } S, T, M, EA, EB, XA, XB;
-        const size_t nS = size_t(nq * ms * ks) * 8, nT = size_t(nq * ks * ns) * 8, nM = size_t(nq * ms * ns) * 8;
+        const bool direct = !p.core && p.nprod == 1;  // one product = all of C: no staging, no combine
+        const size_t nS = size_t(nq * ms * ks) * 8, nT = size_t(nq * ks * ns) * 8,
+                     nM = direct ? 8 : size_t(nq * ms * ns) * 8;
         const size_t nEA = fp8 ? size_t(nq * mp * (kp / 128)) * 4 : size_t(nq * ms) * 4;
         const size_t nEB = fp8 ? size_t(nq * np * (kp / 128)) * 4 : size_t(nq * ns) * 4;
         const size_t nXA = size_t(nsl) * size_t(nq * mp * kp) * esz, nXB = size_t(nsl) * size_t(nq * np * kp) * esz;
@@ -940,7 +942,13 @@ Status run_plan_lowp(const Plan& p, const Call& c, int precision, int nslices, h
             la.eb = static_cast<int*>(EB.p);
             la.outA = XA.p;
             la.outB = XB.p;
-            LowpGemmArgs lg{ms, ns, mp, np, kp, p.nprod};
+            LowpGemmArgs lg{ms, ns, mp, np, kp, p.nprod, nullptr, nullptr, nullptr, c.alpha, c.beta};
+            if (direct)
+            {
+                lg.C = reinterpret_cast<double*>(static_cast<char*>(c.C) + p.h_batch_c[size_t(z)] * p.elem);
+                lg.c_m = p.c_m;
+                lg.c_n = p.c_n;
+            }
             if (fp8)
             {
                 e = launch_fp8_slices(la, static_cast<float*>(EA.p), static_cast<float*>(EB.p), stream);
@@ -960,6 +968,7 @@ Status run_plan_lowp(const Plan& p, const Call& c, int precision, int nslices, h
                 timing_end(stream, 2.0 * double(nq) * double(mp) * double(np) * double(kp) * (nsl * (nsl + 1) / 2));
             }
             if (e != hipSuccess) break;
+            if (direct) continue;
 
             CombineArgs ca;
             std::memset(&ca, 0, sizeof(ca));

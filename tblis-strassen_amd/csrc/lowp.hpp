@@ -26,6 +26,12 @@ struct LowpGemmArgs
 {
     int64_t ms, ns, mp, np, kp;
     int nprod;
+    // direct epilogue (a single product feeding all of C with coefficient 1): C[c_m[r] + c_n[c]] =
+    // alpha * M + beta * C instead of staging M
+    double* C;
+    const int64_t* c_m;
+    const int64_t* c_n;
+    double alpha, beta;
 };
 
 hipError_t launch_lowp_exponents(const LowpSliceArgs& a, hipStream_t s);

@@ -138,14 +138,31 @@ __global__ __launch_bounds__(256) void slice_cols_kernel(const double* __restric
     }
 }
 
+// ---- epilogue -----------------------------------------------------------------------------------------
+
+// M_q[row + col * ms] = v, or with the direct epilogue C = alpha * v + beta * C at the scattered element
+__device__ __forceinline__ void put_out(const LowpGemmArgs& g, double* Mq, int64_t row, int64_t col, double v)
+{
+    if (g.C)
+    {
+        double* c = g.C + g.c_m[row] + g.c_n[col];
+        *c = g.beta == 0.0 ? g.alpha * v : g.alpha * v + g.beta * *c;
+    }
+    else
+        Mq[row + col * g.ms] = v;
+}
+
 // ---- Ozaki BF16 GEMM --------------------------------------------------------------------------------
 
-// Digit pairs (s, t) with s + t = d share the weight 2^(-8(d+2)); group d (d + 1 pairs) is exact in FP32
-// for 2^24 / ((d + 1) 2^16) = 256 / (d + 1) terms, i.e. floor(8 / (d + 1)) K-steps of 32 (at least 1:
-// d + 1 <= 8 pairs of 32 products below 2^16 stay below 2^24).
+// Digit pairs (s, t) with s + t = d share the weight 2^(-8(d+2)) and are summed in one FP32 accumulator,
+// which stays exact while every partial sum is an integer of magnitude <= 2^24.  With |d_0| <= 256 and
+// |d_s| <= 128 (s >= 1), a K element adds at most 256^2 = 2^16 to group 0 and at most
+// 2 * 256 * 128 + (d - 1) * 128^2 = (d + 3) 2^14 to group d >= 1, so group 0 holds 256 K elements and
+// group d >= 1 holds 1024 / (d + 3): flushed to FP64 every 8 K-steps of 32 (d = 0, 1), 6 (d = 2),
+// 5 (d = 3), 4 (d = 4, 5), 3 (d = 6).
 __host__ __device__ constexpr int flush_period(int d)
 {
-    return (8 / (d + 1)) >= 4 ? (8 / (d + 1)) : ((8 / (d + 1)) >= 2 ? 2 : 1);
+    return d == 0 ? 8 : 32 / (d + 3);
 }
 
 // One 64 x 64 tile of M_q per workgroup (4 waves, 32 x 32 each as 2 x 2 MFMA blocks of 16 x 16).
@@ -231,7 +248,129 @@ __global__ __launch_bounds__(256) void ozaki_gemm_kernel(const __bf16* __restric
             for (int v = 0; v < 4; v++)
             {
                 const int64_t row = row0 + i * 16 + 4 * (lane >> 4) + v;
-                if (row < g.ms) Mq[row + col * g.ms] = ldexp(acc[i][j][v], ea[int64_t(q) * g.ms + row] + ec);
+                if (row < g.ms) put_out(g, Mq, row, col, ldexp(acc[i][j][v], ea[int64_t(q) * g.ms + row] + ec));
+            }
+        }
+}
+
+// LDS-staged form: 512 threads (8 waves as 4 x 2, 32 x 32 each) per 128 x 64 tile of M_q.  Every digit
+// element of a K-step (32) is read from HBM/L2 once per workgroup into a double-buffered LDS tile
+// (rows of 32 BF16 padded to 40, so the 16-byte fragment reads of 16 consecutive rows spread over the
+// banks); the waves read their MFMA fragments from LDS.  Needs mp % 128 == 0, np % 64 == 0, kp % 32 == 0.
+constexpr int kOzLd = 40;  // LDS row stride in BF16 elements
+template <int NSL>
+__global__ __launch_bounds__(512) void ozaki_gemm_lds_kernel(const __bf16* __restrict__ As, const __bf16* __restrict__ Bs,
+                                                             const int* __restrict__ ea, const int* __restrict__ eb,
+                                                             double* __restrict__ M, LowpGemmArgs g)
+{
+    __shared__ __attribute__((aligned(16))) __bf16 sa[2][NSL][128 * kOzLd];
+    __shared__ __attribute__((aligned(16))) __bf16 sb[2][NSL][64 * kOzLd];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tiles_m = int(g.mp / 128);
+    const int tm = blockIdx.x % tiles_m, tn = blockIdx.x / tiles_m;
+    const int q = blockIdx.y;
+    const int64_t row0 = int64_t(tm) * 128, col0 = int64_t(tn) * 64;
+    const int wm = (wave & 3) * 32, wn = (wave >> 2) * 32;
+    const int64_t slab_a = int64_t(g.nprod) * g.mp * g.kp, slab_b = int64_t(g.nprod) * g.np * g.kp;
+    // loader: A tile = 128 rows x 32 k = 512 x 16 B per slice (one per thread); B tile = 64 x 32 k = 256 x 16 B
+    const int lr = tid >> 2, lc = (tid & 3) * 8;  // row, k offset of this thread's 16-byte chunk
+    const __bf16* ga = As + (int64_t(q) * g.mp + row0 + lr) * g.kp + lc;
+    const __bf16* gb = Bs + (int64_t(q) * g.np + col0 + (lr & 63)) * g.kp + lc;
+    const bool bload = tid < 256;
+
+    double acc[2][2][4];
+    f32x4 a32[NSL][2][2];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+        {
+#pragma unroll
+            for (int v = 0; v < 4; v++) acc[i][j][v] = 0.0;
+#pragma unroll
+            for (int d = 0; d < NSL; d++) a32[d][i][j] = f32x4{0, 0, 0, 0};
+        }
+    auto flush = [&](int d) {
+        const double w = ldexp(1.0, -8 * (d + 2));
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++)
+            {
+#pragma unroll
+                for (int v = 0; v < 4; v++) acc[i][j][v] += w * double(a32[d][i][j][v]);
+                a32[d][i][j] = f32x4{0, 0, 0, 0};
+            }
+    };
+
+    bf16x8 ra[NSL], rb[NSL];
+    auto gload = [&](int64_t st) {
+#pragma unroll
+        for (int s = 0; s < NSL; s++)
+        {
+            ra[s] = *reinterpret_cast<const bf16x8*>(ga + s * slab_a + st * 32);
+            rb[s] = bload ? *reinterpret_cast<const bf16x8*>(gb + s * slab_b + st * 32) : bf16x8{};
+        }
+    };
+    auto lstore = [&](int buf) {
+#pragma unroll
+        for (int s = 0; s < NSL; s++)
+        {
+            *reinterpret_cast<bf16x8*>(&sa[buf][s][lr * kOzLd + lc]) = ra[s];
+            if (bload) *reinterpret_cast<bf16x8*>(&sb[buf][s][lr * kOzLd + lc]) = rb[s];
+        }
+    };
+
+    const int64_t nsteps = g.kp / 32;
+    gload(0);
+    lstore(0);
+    __syncthreads();
+    for (int64_t st = 0; st < nsteps; st++)
+    {
+        const int cur = int(st & 1);
+        if (st + 1 < nsteps) gload(st + 1);
+        bf16x8 fa[NSL][2], fb[NSL][2];
+#pragma unroll
+        for (int s = 0; s < NSL; s++)
+#pragma unroll
+            for (int i = 0; i < 2; i++)
+            {
+                fa[s][i] = *reinterpret_cast<const bf16x8*>(&sa[cur][s][(wm + i * 16 + (lane & 15)) * kOzLd + 8 * (lane >> 4)]);
+                fb[s][i] = *reinterpret_cast<const bf16x8*>(&sb[cur][s][(wn + i * 16 + (lane & 15)) * kOzLd + 8 * (lane >> 4)]);
+            }
+#pragma unroll
+        for (int s = 0; s < NSL; s++)
+#pragma unroll
+            for (int t = 0; s + t < NSL; t++)
+#pragma unroll
+                for (int i = 0; i < 2; i++)
+#pragma unroll
+                    for (int j = 0; j < 2; j++)
+                        a32[s + t][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[s][i], fb[t][j], a32[s + t][i][j], 0, 0, 0);
+#pragma unroll
+        for (int d = 0; d < NSL; d++)
+            if ((st + 1) % flush_period(d) == 0) flush(d);
+        if (st + 1 < nsteps) lstore(cur ^ 1);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int d = 0; d < NSL; d++) flush(d);
+
+    double* Mq = M + int64_t(q) * g.ms * g.ns;
+    const int64_t r0 = row0 + wm, c0 = col0 + wn;
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+        {
+            const int64_t col = c0 + j * 16 + (lane & 15);
+            if (col >= g.ns) continue;
+            const int ec = eb[int64_t(q) * g.ns + col];
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+            {
+                const int64_t row = r0 + i * 16 + 4 * (lane >> 4) + v;
+                if (row < g.ms) put_out(g, Mq, row, col, ldexp(acc[i][j][v], ea[int64_t(q) * g.ms + row] + ec));
             }
         }
 }
@@ -367,7 +506,7 @@ __global__ __launch_bounds__(256) void fp8_gemm_kernel(const uint8_t* __restrict
             for (int v = 0; v < 4; v++)
             {
                 const int64_t row = row0 + i * 16 + 4 * (lane >> 4) + v;
-                if (row < g.ms) Mq[row + col * g.ms] = acc[i][j][v];
+                if (row < g.ms) put_out(g, Mq, row, col, acc[i][j][v]);
             }
         }
 }
@@ -392,8 +531,18 @@ template <int NSL>
 hipError_t gemm_t(const void* As, const void* Bs, const int* ea, const int* eb, double* M, const LowpGemmArgs& g,
                   hipStream_t s)
 {
-    hipLaunchKernelGGL(ozaki_gemm_kernel<NSL>, dim3(unsigned((g.mp / 64) * (g.np / 64)), unsigned(g.nprod)), dim3(256),
-                       0, s, static_cast<const __bf16*>(As), static_cast<const __bf16*>(Bs), ea, eb, M, g);
+    if constexpr (NSL <= 5)  // the double-buffered digit tiles of 6 or 7 slices exceed the 160 KiB of LDS
+    {
+        if (g.mp % 128 == 0 && g.np % 64 == 0 && g.kp % 32 == 0)
+        {
+            hipLaunchKernelGGL(ozaki_gemm_lds_kernel<NSL>, dim3(unsigned((g.mp / 128) * (g.np / 64)), unsigned(g.nprod)),
+                               dim3(512), 0, s, static_cast<const __bf16*>(As), static_cast<const __bf16*>(Bs), ea, eb,
+                               M, g);
+            return hipGetLastError();
+        }
+    }
+        hipLaunchKernelGGL(ozaki_gemm_kernel<NSL>, dim3(unsigned((g.mp / 64) * (g.np / 64)), unsigned(g.nprod)),
+                           dim3(256), 0, s, static_cast<const __bf16*>(As), static_cast<const __bf16*>(Bs), ea, eb, M, g);
     return hipGetLastError();
 }
 
