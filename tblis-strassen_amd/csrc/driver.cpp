@@ -399,46 +399,76 @@ Status build_plan_host(const tblis::tblis_tensor* A, const label_type* idx_A, co
 // each C sub-block receives its updates in product-table order, so the result is bitwise the serial one.
 void make_stages(Plan& p, std::vector<int>& order)
 {
+    // C target of a term: (staging base, row offset, column offset)
     std::vector<std::pair<int64_t, std::pair<int64_t, int64_t>>> keys;
-    std::vector<int> last;
-    std::vector<int> stage(size_t(p.nprod), 0);
-    int nst = 0;
+    std::vector<std::vector<size_t>> targets(size_t(p.nprod));
     for (int q = 0; q < p.nprod; q++)
     {
         const ProdDesc& pd = p.h_prods[size_t(q)];
-        int st = 0;
-        std::vector<size_t> ids;
         for (int t = 0; t < pd.nC; t++)
         {
             const auto key = std::make_pair(pd.c[t].base, std::make_pair(pd.c[t].roff, pd.c[t].coff));
             size_t i = 0;
             while (i < keys.size() && keys[i] != key) i++;
-            if (i == keys.size())
-            {
-                keys.push_back(key);
-                last.push_back(-1);
-            }
-            ids.push_back(i);
-            st = std::max(st, last[i] + 1);
+            if (i == keys.size()) keys.push_back(key);
+            targets[size_t(q)].push_back(i);
         }
-        for (size_t i : ids) last[i] = st;
-        stage[size_t(q)] = st;
-        nst = std::max(nst, st + 1);
+    }
+    // First-fit decreasing: products with more C targets first, each into the first stage whose
+    // targets it does not share.  For the 1- and 2-level tables this gives 4 stages of {1,2,2,2}
+    // products and 16 stages of {1, 2 x6, 4 x9} products (each 2-level stage is the Kronecker product
+    // of two 1-level stages), the fewest launches whose workgroup count is an even number of products.
+    std::vector<int> by_targets(size_t(p.nprod));
+    for (int q = 0; q < p.nprod; q++) by_targets[size_t(q)] = q;
+    std::stable_sort(by_targets.begin(), by_targets.end(),
+                     [&](int x, int y) { return targets[size_t(x)].size() > targets[size_t(y)].size(); });
+    std::vector<std::vector<int>> stages;
+    std::vector<std::vector<char>> used;
+    for (int q : by_targets)
+    {
+        size_t st = 0;
+        for (; st < stages.size(); st++)
+        {
+            bool free = true;
+            for (size_t i : targets[size_t(q)]) free = free && !used[st][i];
+            if (free) break;
+        }
+        if (st == stages.size())
+        {
+            stages.emplace_back();
+            used.emplace_back(keys.size(), 0);
+        }
+        stages[st].push_back(q);
+        for (size_t i : targets[size_t(q)]) used[st][i] = 1;
     }
     order.clear();
     p.h_stage_begin.assign(1, 0);
     p.h_stage_terms.clear();
-    for (int st = 0; st < nst; st++)
+    for (const std::vector<int>& s : stages)
     {
         int terms = 1;
-        for (int q = 0; q < p.nprod; q++)
-            if (stage[size_t(q)] == st)
-            {
-                order.push_back(q);
-                terms = std::max(terms, max_terms(&p.h_prods[size_t(q)], 1));
-            }
+        for (int q : s)
+        {
+            order.push_back(q);
+            terms = std::max(terms, max_terms(&p.h_prods[size_t(q)], 1));
+        }
         p.h_stage_begin.push_back(int(order.size()));
         p.h_stage_terms.push_back(terms);
+    }
+    // beta goes with the first writer of each C target in stage order
+    p.h_stage_first.assign(size_t(p.nprod), 0);
+    std::vector<char> seen(keys.size(), 0);
+    for (size_t j = 0; j < order.size(); j++)
+    {
+        const int q = order[j];
+        int mask = 0;
+        for (size_t t = 0; t < targets[size_t(q)].size(); t++)
+            if (!seen[targets[size_t(q)][t]])
+            {
+                mask |= 1 << int(t);
+                seen[targets[size_t(q)][t]] = 1;
+            }
+        p.h_stage_first[j] = mask;
     }
 }
 
@@ -456,10 +486,11 @@ Status upload(Plan& p)
     std::vector<int> order;
     make_stages(p, order);
     std::vector<ProdDesc> staged, fast_staged;
-    for (int q : order)
+    for (size_t j = 0; j < order.size(); j++)
     {
-        staged.push_back(p.h_prods[size_t(q)]);
-        fast_staged.push_back(fast_prods[size_t(q)]);
+        staged.push_back(p.h_prods[size_t(order[j])]);
+        fast_staged.push_back(fast_prods[size_t(order[j])]);
+        staged.back().first_mask = fast_staged.back().first_mask = p.h_stage_first[j];
     }
     if (p.core)
     {

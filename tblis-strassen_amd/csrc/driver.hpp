@@ -79,8 +79,9 @@ struct Plan
     std::vector<ProdDesc> h_dense_prods;
 
     // ABC launch stages: products [stage_begin[s], stage_begin[s + 1]) of the staged tables run in one
-    // launch; their C targets are disjoint, and every C sub-block sees its products in table order
-    std::vector<int> h_stage_begin, h_stage_terms;
+    // launch; their C targets are disjoint.  Every C sub-block sees its products in stage order, and
+    // h_stage_first holds the first-writer masks (beta) for that order, per staged product.
+    std::vector<int> h_stage_begin, h_stage_terms, h_stage_first;
     const ProdDesc* prods_staged = nullptr;
     const ProdDesc* prods_fast_staged = nullptr;
 
