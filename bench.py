@@ -7,7 +7,9 @@ seeded order; each tensor's index order is a seeded permutation; A, B ~ uniform[
 generated in HBM); C starts at zero and every step is C += A.B (beta = 1, as the reference's
 test_strassen does).  A step is one contraction call; inputs are resident in HBM before timing.
 
-Variants: L0 (classical MFMA TC), L1/L2 x {ABC, AB, Naive}.  Errors are ||C - C_L0||_F / ||C_L0||_F of a
+Variants: L0 (classical MFMA TC), L1/L2 x {ABC, AB, Naive}; 2-level ABC in the library's default form
+(at this size: the outer level's 7 operand sums of A and of B staged in HBM, 2-term inner sums in the
+leaf kernel) and in the fused form (every 4-term sum in the leaf kernel, no workspace).  Errors are ||C - C_L0||_F / ||C_L0||_F of a
 single call from C = 0.  The reported `value` is the fastest Strassen variant (L1 or L2, any impl) that
 meets its tolerance (1e-12 at L1, 5e-12 at L2); every variant, ABC included, is listed under `variants`.
 
@@ -73,11 +75,11 @@ def device_tensor(shape, dtype, torch):
     return buf.as_strided(tuple(shape), tuple(cases.colmajor_strides(shape))), buf
 
 
-def run_variant(ts, torch, case, A, B, C, level, impl, steps, warmup, stream):
+def run_variant(ts, torch, case, A, B, C, level, impl, steps, warmup, stream, flags=0):
     """Time `steps` calls after `warmup` calls; returns (ms/step, kernel ms/launch, launches, leaf flops/launch)."""
     for _ in range(warmup):
         ts.tensor_mult(1.0, A, case["idx_A"], B, case["idx_B"], 1.0, C, case["idx_C"], level=level, impl=impl,
-                       stream=stream)
+                       stream=stream, flags=flags)
     torch.cuda.synchronize()
     ts.kernel_timing(True)
     ts.kernel_timing_read(reset=True)
@@ -86,7 +88,7 @@ def run_variant(ts, torch, case, A, B, C, level, impl, steps, warmup, stream):
     start.record(stream)
     for _ in range(steps):
         ts.tensor_mult(1.0, A, case["idx_A"], B, case["idx_B"], 1.0, C, case["idx_C"], level=level, impl=impl,
-                       stream=stream)
+                       stream=stream, flags=flags)
     stop.record(stream)
     torch.cuda.synchronize()
     ms = start.elapsed_time(stop) / steps
@@ -315,7 +317,9 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS) + ["c3", "c4", "c5"])
-    ap.add_argument("--variants", default="0:abc,1:abc,2:abc,1:ab,2:ab,1:naive,2:naive")
+    ap.add_argument("--variants", default="0:abc,1:abc,2:abc,2:abc:fused,1:ab,2:ab,1:naive,2:naive",
+                    help="level:impl[:form], form = fused (every operand sum in the leaf kernel) or staged "
+                         "(2-level ABC with the outer sums staged in HBM); default: the library's choice")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run one device copy of MIB MiB (a kernel of known traffic for counter scaling)")
@@ -361,19 +365,22 @@ def main():
     C, cbuf = device_tensor(case["shape_C"], torch.float64, torch)
 
     variants = {}
+    forms = {"": 0, "fused": ts.FLAG_FUSED_SUMS, "staged": ts.FLAG_OUTER_STAGED}
     for spec in args.variants.split(","):
-        lv, impl = spec.split(":")
+        lv, impl, *form = spec.split(":")
+        form = form[0] if form else ""
+        flags = forms[form]
         level = int(lv)
-        name = f"L{level}_{impl}"
+        name = f"L{level}_{impl}" + (f"_{form}" if form else "")
         cbuf.zero_()
         ts.tensor_mult(1.0, A, case["idx_A"], B, case["idx_B"], 1.0, C, case["idx_C"], level=level, impl=impl,
-                       stream=stream)
+                       stream=stream, flags=flags)
         err = ts.rel_error(cbuf, crefbuf, stream=stream)
         cbuf.zero_()
         ms, kms, launches, kflops = run_variant(ts, torch, case, A, B, C, level, impl, args.steps, args.warmup,
-                                                stream)
+                                                stream, flags)
         variants[name] = {
-            "level": level, "impl": impl, "ms_per_step": round(ms, 4),
+            "level": level, "impl": impl, "form": form or "default", "ms_per_step": round(ms, 4),
             "eff_tflops": round(eff_flops / ms / 1e9, 3), "rel_err_vs_L0": err,
             "tol": TOL[level], "ok": err <= TOL[level] if level else True,
             "leaf_kernel_ms": round(kms, 4), "leaf_kernel_tflops": round(kflops / kms / 1e9, 3) if kms else None,
@@ -385,7 +392,7 @@ def main():
     cands = [v for v in variants.values() if v["level"] > 0 and v["ok"]]
     head = max(cands, key=lambda v: v["eff_tflops"]) if cands else variants.get("L0_abc")
     classical = variants.get("L0_abc")
-    head_name = f"L{head['level']}_{head['impl']}"
+    head_name = next(k for k, v in variants.items() if v is head)
 
     traffic = None
     tpath = os.path.join(ROOT, "profiles", f"traffic_{args.workload}_{head_name}.json")

@@ -127,9 +127,15 @@ enum
     MISTRA_FLAG_REF_FRINGE = 1u << 0, /* drop the rows/cols/k that the 2^L split truncates, exactly as
                                          src/matrix/stra_tensor_view.hpp:246-247 does (default: computed) */
     MISTRA_FLAG_SYNC       = 1u << 1, /* synchronize the stream before returning                          */
-    MISTRA_FLAG_GRAPH      = 1u << 2  /* mistra_batch_mult: capture the batch's launches in a HIP graph on
+    MISTRA_FLAG_GRAPH      = 1u << 2, /* mistra_batch_mult: capture the batch's launches in a HIP graph on
                                          the first call and replay it on later calls with the same entries
                                          (same pointers, shapes, scalars); needs a non-null stream        */
+    MISTRA_FLAG_FUSED_SUMS = 1u << 3, /* 2-level ABC: form every operand sum inside the leaf kernel (no
+                                         workspace), never staging the outer level's sums                */
+    MISTRA_FLAG_OUTER_STAGED = 1u << 4 /* 2-level ABC: stage the outer level's 7 operand sums of A and of
+                                         B in HBM workspace (7/4 of A's and of B's size) and form only
+                                         the inner 2-term sums in the leaf kernel, whenever the shape
+                                         allows it.  Without either flag the cost model decides.        */
 };
 
 /* Error codes (0 = success). */

@@ -13,11 +13,12 @@ namespace tblis {
 extern "C" {
 #endif
 
-/* dims[0..12] = m, n, k, nbatch, pure, transposed, ms, ns, ks, nprod (core products), nfringe,
- *               a_mfast, b_nfast.
+/* dims[0..14] = m, n, k, nbatch, pure, transposed, ms, ns, ks, nprod (core products), nfringe,
+ *               a_mfast, b_nfast, outer_staged (2-level ABC with the outer operand sums staged),
+ *               number of ABC launch stages.
  * scat[6*cap]  = a_m, a_k, b_k, b_n, c_m, c_n (in the plan's own orientation), -1 padded.
  * prods[cap_prods*40] per product (core then fringe): nA, nB, nC, first_mask, then 4 x (roff, coff,
- *               coeff, base) for A, B and C (-1 padded).  Returns the number of products or < 0. */
+ *               coeff) for A, B and C (-1, -1, 0 padded).  Returns the number of products or < 0. */
 int mistra_debug_plan(const tblis_tensor* A, const label_type* idx_A,
                       const tblis_tensor* B, const label_type* idx_B,
                       const tblis_tensor* C, const label_type* idx_C,

@@ -101,7 +101,27 @@ double model_ms(const Plan& p0)
         std::vector<int> order;
         make_stages(p, order);
         for (size_t st = 0; st + 1 < p.h_stage_begin.size(); st++)
-            s += gemm(p.ms, p.ns, p.ks, p.h_stage_begin[st + 1] - p.h_stage_begin[st], p.h_stage_terms[st], tile);
+        {
+            int terms = p.h_stage_terms[st];
+            if (p.outer_staged)
+            {
+                terms = 1;
+                for (int j = p.h_stage_begin[st]; j < p.h_stage_begin[st + 1]; j++)
+                    terms = std::max(terms, max_terms(&p.h_inner[size_t(order[size_t(j)])], 1));
+            }
+            s += gemm(p.ms, p.ns, p.ks, p.h_stage_begin[st + 1] - p.h_stage_begin[st], terms, tile);
+        }
+        if (p.outer_staged)
+        {
+            // outer sums: read every outer term's sub-block, write the 7 dense sums, for A and for B
+            double ra = 0, rb = 0;
+            for (const ProdDesc& sd : p.h_outer_sums)
+            {
+                ra += sd.nA + 1;
+                rb += sd.nB + 1;
+            }
+            s += (ra * double(p.os_m) * p.os_k + rb * double(p.os_k) * p.os_n) * e / kHbm + 2 * kLaunch;
+        }
     }
     return s * 1e3;
 }

@@ -59,8 +59,85 @@ __global__ __launch_bounds__(256) void operand_sums_kernel(SumArgs a)
     }
 }
 
+// Translate-form sums along unit-stride rows: each thread owns RPT consecutive rows (one row-vector
+// load for the whole tile) and TC / CG columns; a term costs one RPT-wide load per column.
+template <typename T, bool PAIR>
+__global__ __launch_bounds__(256) void operand_sums_rows_kernel(SumArgs a)
+{
+    constexpr int RPT = PAIR ? 2 : 1;          // rows per thread
+    constexpr int TR = 256;                    // rows per tile
+    constexpr int RT = TR / RPT;               // threads along the rows
+    constexpr int CG = 256 / RT;               // column groups
+    constexpr int TC = 16;                     // columns per tile
+    typedef T vec2 __attribute__((ext_vector_type(2)));
+    const int64_t tiles_r = (a.rows + TR - 1) / TR;
+    const int64_t tr = blockIdx.x % tiles_r, tc = blockIdx.x / tiles_r;
+    const int p = blockIdx.y;
+    const ProdDesc& P = a.prods[p];
+    const int nt = a.which == 0 ? P.nA : P.nB;
+    const SubRef* terms = a.which == 0 ? P.a : P.b;
+    const int tid = threadIdx.x;
+    const int64_t r = tr * TR + int64_t(tid % RT) * RPT;
+    if (r >= a.rows) return;
+    const T* X = static_cast<const T*>(a.X) + a.zoff + a.rs[r];
+    T* out = static_cast<T*>(a.out) + int64_t(p) * a.rows * a.cols + r;
+    int64_t base[kMaxTerms];
+    T coeff[kMaxTerms];
+#pragma unroll
+    for (int t = 0; t < kMaxTerms; t++)
+    {
+        base[t] = t < nt ? terms[t].base : 0;
+        coeff[t] = t < nt ? T(terms[t].coeff) : T(0);
+    }
+#pragma unroll
+    for (int j = 0; j < TC / CG; j++)
+    {
+        const int64_t c = tc * TC + tid / RT + int64_t(j) * CG;
+        if (c >= a.cols) break;
+        const int64_t cb = a.cs[c];
+        if constexpr (PAIR)
+        {
+            vec2 acc = {T(0), T(0)};
+#pragma unroll
+            for (int t = 0; t < kMaxTerms; t++)
+                if (t < nt)
+                {
+                    const vec2 v = *reinterpret_cast<const vec2*>(X + base[t] + cb);
+                    acc.x += coeff[t] * v.x;
+                    acc.y += coeff[t] * v.y;
+                }
+            *reinterpret_cast<vec2*>(out + c * a.rows) = acc;
+        }
+        else
+        {
+            T acc = T(0);
+#pragma unroll
+            for (int t = 0; t < kMaxTerms; t++)
+                if (t < nt) acc += coeff[t] * X[base[t] + cb];
+            out[c * a.rows] = acc;
+        }
+    }
+}
+
 hipError_t launch_operand_sums(int elem_size, const SumArgs& a, hipStream_t s)
 {
+    if (a.shifted && a.rows_fast)
+    {
+        const int64_t tiles = ((a.rows + 255) / 256) * ((a.cols + 15) / 16);
+        if (tiles == 0 || a.nprod == 0) return hipSuccess;
+        dim3 grid(unsigned(tiles), unsigned(a.nprod));
+        if (elem_size == 8)
+        {
+            if (a.pair) hipLaunchKernelGGL((operand_sums_rows_kernel<double, true>), grid, dim3(256), 0, s, a);
+            else hipLaunchKernelGGL((operand_sums_rows_kernel<double, false>), grid, dim3(256), 0, s, a);
+        }
+        else
+        {
+            if (a.pair) hipLaunchKernelGGL((operand_sums_rows_kernel<float, true>), grid, dim3(256), 0, s, a);
+            else hipLaunchKernelGGL((operand_sums_rows_kernel<float, false>), grid, dim3(256), 0, s, a);
+        }
+        return hipGetLastError();
+    }
     const int64_t tiles = ((a.rows + 63) / 64) * ((a.cols + 63) / 64);
     if (tiles == 0 || a.nprod == 0) return hipSuccess;
     dim3 grid(unsigned(tiles), unsigned(a.nprod));

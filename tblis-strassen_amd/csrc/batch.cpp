@@ -114,7 +114,7 @@ Status prepare(int count, const mistra_contraction* e, std::vector<std::shared_p
         std::shared_ptr<Plan> plan;
         Status st = get_plan(x.A, x.idx_A, x.B, x.idx_B, x.C, x.idx_C, level, IMPL_ABC, -1, 0, plan);
         if (!st.ok()) return st;
-        if (plan->general)
+        if (plan->general || plan->outer_staged)
         {
             singles.push_back(i);
             continue;

@@ -217,6 +217,121 @@ ProdDesc shifted(const Plan& p, const ProdDesc& pd)
     return f;
 }
 
+// Tables of the outer-staged form of a 2-level ABC plan (see Plan::outer_staged).  Outer product po
+// sums A over its 1-level terms into S_po (element (i, j) of the os_m x os_k block at
+// po*os_m*os_k + i*os_k + j: K-fast) and B into T_po (element (i, j) of os_k x os_n at
+// po*os_k*os_n + i + j*os_k: K-fast); product 7*po + pi of the 2-level table then multiplies the inner
+// sums of pi over S_po and T_po and updates the same C targets, with the same coefficients, as before.
+// Returns false when the form does not apply (C sub-blocks that are not translates of the first one, or
+// an inner sub-block that does not fit the 31-bit buffer range).
+bool build_outer_staged(Plan& p)
+{
+    const int64_t ms = p.ms, ns = p.ns, ks = p.ks, om = 2 * ms, on = 2 * ns, ok = 2 * ks;
+    for (int q = 0; q < p.nprod; q++)
+    {
+        const ProdDesc& pd = p.h_prods[size_t(q)];
+        for (int t = 0; t < pd.nC; t++)
+            if (!is_translate(p.h_c_m, pd.c[t].roff, ms) || !is_translate(p.h_c_n, pd.c[t].coff, ns)) return false;
+    }
+    const int TK = tile_kdepth(p.tile, p.elem);
+    const int64_t limit = (int64_t(1) << 31) / p.elem;
+    if ((ms - 1) * ok + ks + TK >= limit || (ns - 1) * ok + ks + TK >= limit) return false;
+    const auto& one = strassen_products_1level();
+    p.os_m = om;
+    p.os_n = on;
+    p.os_k = ok;
+    p.h_outer_sums.clear();
+    for (const Product& po : one)
+    {
+        ProdDesc sd;
+        std::memset(&sd, 0, sizeof(sd));
+        sd.nA = int(po.A.size());
+        sd.nB = int(po.B.size());
+        // A: rows are K (a_k), columns M (a_m); B: rows K (b_k), columns N (b_n)
+        for (size_t t = 0; t < po.A.size(); t++)
+            sd.a[t] = SubRef{0, (po.A[t].subid % 2) * ok, (po.A[t].subid / 2) * om, po.A[t].coeff};
+        for (size_t t = 0; t < po.B.size(); t++)
+            sd.b[t] = SubRef{0, (po.B[t].subid / 2) * ok, (po.B[t].subid % 2) * on, po.B[t].coeff};
+        p.h_outer_sums.push_back(sd);
+    }
+    // translate form of the outer sums (rows along K): shifts per term, and 2-row vector reads when
+    // K comes in contiguous pairs at even offsets
+    auto translate_form = [&](int which, const std::vector<int64_t>& rs, const std::vector<int64_t>& cs,
+                              int64_t rows, int64_t cols, int64_t zoff, bool& shift, bool& pair) {
+        shift = true;
+        bool even = zoff % 2 == 0 && rows % 2 == 0;
+        for (size_t q = 0; q < p.h_outer_sums.size(); q++)
+        {
+            const ProdDesc& sd = p.h_outer_sums[q];
+            const int nt = which == 0 ? sd.nA : sd.nB;
+            const SubRef* t = which == 0 ? sd.a : sd.b;
+            SubRef* f = which == 0 ? p.h_outer_shifted[q].a : p.h_outer_shifted[q].b;
+            for (int i = 0; i < nt; i++)
+            {
+                shift = shift && is_translate(rs, t[i].roff, rows) && is_translate(cs, t[i].coff, cols);
+                if (!shift) return;
+                f[i].base = (rs[size_t(t[i].roff)] - rs[0]) + (cs[size_t(t[i].coff)] - cs[0]);
+                even = even && f[i].base % 2 == 0;
+            }
+        }
+        for (int64_t r = 0; even && r + 1 < rows; r += 2) even = rs[size_t(r)] % 2 == 0 && rs[size_t(r + 1)] == rs[size_t(r)] + 1;
+        for (int64_t c = 0; even && c < cols; c++) even = cs[size_t(c)] % 2 == 0;
+        pair = even;
+    };
+    p.h_outer_shifted = p.h_outer_sums;
+    translate_form(0, p.h_a_k, p.h_a_m, ok, om, p.h_batch_a[0], p.outer_shift_a, p.outer_pair_a);
+    translate_form(1, p.h_b_k, p.h_b_n, ok, on, p.h_batch_b[0], p.outer_shift_b, p.outer_pair_b);
+    p.h_inner.clear();
+    for (size_t o = 0; o < one.size(); o++)
+        for (size_t i = 0; i < one.size(); i++)
+        {
+            const Product& pi = one[i];
+            ProdDesc pd = p.h_prods[o * one.size() + i];  // C terms of the 2-level product
+            pd.nA = int(pi.A.size());
+            pd.nB = int(pi.B.size());
+            std::memset(pd.a, 0, sizeof(pd.a));
+            std::memset(pd.b, 0, sizeof(pd.b));
+            // fast-body form: base = offset of the inner sub-block's first element
+            for (size_t t = 0; t < pi.A.size(); t++)
+            {
+                const int64_t r = (pi.A[t].subid / 2) * ms, c = (pi.A[t].subid % 2) * ks;
+                pd.a[t] = SubRef{int64_t(o) * om * ok + r * ok + c, r, c, pi.A[t].coeff};
+            }
+            for (size_t t = 0; t < pi.B.size(); t++)
+            {
+                const int64_t r = (pi.B[t].subid / 2) * ks, c = (pi.B[t].subid % 2) * ns;
+                pd.b[t] = SubRef{int64_t(o) * ok * on + r + c * ok, r, c, pi.B[t].coeff};
+            }
+            auto d = [](const std::vector<int64_t>& v, int64_t off) { return v[size_t(off)] - v[0]; };
+            for (int t = 0; t < pd.nC; t++) pd.c[t].base = d(p.h_c_m, pd.c[t].roff) + d(p.h_c_n, pd.c[t].coff);
+            p.h_inner.push_back(pd);
+        }
+    p.kf_inner.on = true;
+    p.kf_inner.mode = ks % TK == 0 ? KMODE_AFFINE_VEC : KMODE_AFFINE;
+    p.kf_inner.sa = 1;
+    p.kf_inner.sb = 1;
+    return true;
+}
+
+// Whether a 2-level ABC plan stages its outer sums: MISTRA_FLAG_FUSED_SUMS / MISTRA_FLAG_OUTER_STAGED
+// decide when given, else the cost model (the outer sums' HBM traffic against the slower 4-term leaf
+// launches of the fused form).
+void choose_outer_staged(Plan& p, unsigned flags)
+{
+    p.outer_staged = false;
+    if (p.level != 2 || !p.core || p.impl != IMPL_ABC || p.nbatch != 1 || (flags & MISTRA_FLAG_FUSED_SUMS)) return;
+    if (!build_outer_staged(p)) return;
+    if (flags & MISTRA_FLAG_OUTER_STAGED)
+    {
+        p.outer_staged = true;
+        return;
+    }
+    const double fused = model_ms(p);
+    p.outer_staged = true;
+    const double staged = model_ms(p);
+    p.outer_staged = staged < fused;
+}
+
 }  // namespace
 
 Plan::~Plan()
@@ -391,12 +506,12 @@ Status build_plan_host(const tblis::tblis_tensor* A, const label_type* idx_A, co
     p.h_b_n = std::move(b_n);
     p.h_c_m = std::move(c_m);
     p.h_c_n = std::move(c_n);
+    choose_outer_staged(p, flags);
     return Status();
 }
 
-// Greedy launch stages for ABC: product q goes to the first stage after the last stage of every earlier
-// product that shares a C sub-block with it.  Products of one stage write disjoint C sub-blocks, and
-// each C sub-block receives its updates in product-table order, so the result is bitwise the serial one.
+// Launch stages for ABC: products of one stage write disjoint C sub-blocks, so a stage is one launch;
+// the stages run in order, and the first writer of each C sub-block in that order applies beta.
 void make_stages(Plan& p, std::vector<int>& order)
 {
     // C target of a term: (staging base, row offset, column offset)
@@ -486,11 +601,32 @@ Status upload(Plan& p)
     std::vector<int> order;
     make_stages(p, order);
     std::vector<ProdDesc> staged, fast_staged;
+    std::vector<ProdDesc> inner_staged;
     for (size_t j = 0; j < order.size(); j++)
     {
         staged.push_back(p.h_prods[size_t(order[j])]);
         fast_staged.push_back(fast_prods[size_t(order[j])]);
         staged.back().first_mask = fast_staged.back().first_mask = p.h_stage_first[j];
+        if (p.outer_staged)
+        {
+            inner_staged.push_back(p.h_inner[size_t(order[j])]);
+            inner_staged.back().first_mask = p.h_stage_first[j];
+        }
+    }
+    p.h_inner_stage_terms.clear();
+    for (size_t st = 0; p.outer_staged && st + 1 < p.h_stage_begin.size(); st++)
+        p.h_inner_stage_terms.push_back(
+            max_terms(inner_staged.data() + p.h_stage_begin[st], p.h_stage_begin[st + 1] - p.h_stage_begin[st]));
+    // dense K-fast index vectors of the outer-staged buffers
+    std::vector<int64_t> os_sm, os_sk, os_tk, os_tn, os_skrun, os_tkrun;
+    for (int64_t i = 0; i < p.os_m; i++) os_sm.push_back(i * p.os_k);
+    for (int64_t i = 0; i < p.os_k; i++) os_sk.push_back(i);
+    for (int64_t i = 0; i < p.os_k; i++) os_tk.push_back(i);
+    for (int64_t i = 0; i < p.os_n; i++) os_tn.push_back(i * p.os_k);
+    if (p.outer_staged)
+    {
+        os_skrun = run_strides(os_sk, TK);
+        os_tkrun = run_strides(os_tk, TK);
     }
     if (p.core)
     {
@@ -544,7 +680,13 @@ Status upload(Plan& p)
          i_skr = add(s_krun.data(), s_krun.size() * 8), i_tk = add(t_k.data(), t_k.size() * 8),
          i_tkr = add(t_krun.data(), t_krun.size() * 8), i_tn = add(t_n.data(), t_n.size() * 8),
          i_dp = add(p.h_dense_prods.data(), p.h_dense_prods.size() * sizeof(ProdDesc)),
-         i_zero = add(nullptr, 256);
+         i_zero = add(nullptr, 256),
+         i_os = add(p.h_outer_sums.data(), p.h_outer_sums.size() * sizeof(ProdDesc)),
+         i_osf = add(p.h_outer_shifted.data(), p.h_outer_shifted.size() * sizeof(ProdDesc)),
+         i_ins = add(inner_staged.data(), inner_staged.size() * sizeof(ProdDesc)),
+         i_osm = add(os_sm.data(), os_sm.size() * 8), i_osk = add(os_sk.data(), os_sk.size() * 8),
+         i_oskr = add(os_skrun.data(), os_skrun.size() * 8), i_otk = add(os_tk.data(), os_tk.size() * 8),
+         i_otkr = add(os_tkrun.data(), os_tkrun.size() * 8), i_otn = add(os_tn.data(), os_tn.size() * 8);
 
     std::vector<unsigned char> staging(total, 0);
     for (const Item& it : items)
@@ -587,6 +729,15 @@ Status upload(Plan& p)
     p.t_n = reinterpret_cast<const int64_t*>(at(i_tn));
     p.dense_prods = reinterpret_cast<const ProdDesc*>(at(i_dp));
     p.zero = at(i_zero);
+    p.outer_sums = reinterpret_cast<const ProdDesc*>(at(i_os));
+    p.outer_shifted = reinterpret_cast<const ProdDesc*>(at(i_osf));
+    p.inner_staged = reinterpret_cast<const ProdDesc*>(at(i_ins));
+    p.os_sm = reinterpret_cast<const int64_t*>(at(i_osm));
+    p.os_sk = reinterpret_cast<const int64_t*>(at(i_osk));
+    p.os_skrun = reinterpret_cast<const int64_t*>(at(i_oskr));
+    p.os_tk = reinterpret_cast<const int64_t*>(at(i_otk));
+    p.os_tkrun = reinterpret_cast<const int64_t*>(at(i_otkr));
+    p.os_tn = reinterpret_cast<const int64_t*>(at(i_otn));
     return Status();
 }
 
@@ -600,7 +751,7 @@ Status get_plan(const tblis::tblis_tensor* A, const label_type* idx_A, const tbl
     HIP_TRY(hipGetDevice(&dev));
     std::ostringstream ks;
     ks << dev << '|' << int(A->type) << '|' << level << '|' << impl << '|' << tile << '|'
-       << (flags & MISTRA_FLAG_REF_FRINGE) << '|';
+       << (flags & (MISTRA_FLAG_REF_FRINGE | MISTRA_FLAG_FUSED_SUMS | MISTRA_FLAG_OUTER_STAGED)) << '|';
     std::string key = ks.str();
     append_key(key, A, idx_A);
     append_key(key, B, idx_B);
@@ -679,6 +830,7 @@ double leaf_flops(const Plan& p, int nprod, int64_t ms, int64_t ns, int64_t ks)
 Status plan_abc_jobs(const Plan& p, const Call& c, std::vector<LaunchJob>& out)
 {
     if (p.general) return Status::error(MISTRA_ERR_UNSUPPORTED, "general index classes run outside a batch");
+    if (p.outer_staged) return Status::error(MISTRA_ERR_UNSUPPORTED, "outer-staged plans run outside a batch");
     auto push = [&](GemmArgs g, int tile, int terms, int kmode, int round, double flops) {
         tile = effective_tile(tile, kmode, terms, p.a_mfast, p.b_nfast);
         const int te = tile_extent(tile);
@@ -719,13 +871,108 @@ Status plan_abc_jobs(const Plan& p, const Call& c, std::vector<LaunchJob>& out)
     return Status();
 }
 
+namespace {
+
+// Outer-staged 2-level ABC (Plan::outer_staged): the outer sums into S and T, then one launch per stage
+// over 2-term inner sums of S and T with the multi-C epilogue on C.
+hipError_t run_outer_staged(const Plan& p, const Call& c, hipStream_t stream)
+{
+    const size_t sbytes = size_t(7) * size_t(p.os_m) * size_t(p.os_k) * size_t(p.elem);
+    const size_t tbytes = size_t(7) * size_t(p.os_k) * size_t(p.os_n) * size_t(p.elem);
+    void *S = nullptr, *T = nullptr;
+    hipError_t e = hipMallocAsync(&S, sbytes, stream);
+    if (e != hipSuccess) return e;
+    e = hipMallocAsync(&T, tbytes, stream);
+    if (e != hipSuccess)
+    {
+        (void)hipFreeAsync(S, stream);
+        return e;
+    }
+    SumArgs sa;
+    std::memset(&sa, 0, sizeof(sa));
+    sa.X = c.A;
+    sa.zoff = p.h_batch_a[0];
+    sa.rs = p.a_k;
+    sa.cs = p.a_m;
+    sa.rows = p.os_k;
+    sa.cols = p.os_m;
+    sa.prods = p.outer_sums;
+    sa.nprod = 7;
+    sa.which = 0;
+    sa.rows_fast = !p.a_mfast;
+    sa.out = S;
+    // translate form: shifted bases; pairs also need a 2-element-aligned data pointer
+    const uintptr_t align = uintptr_t(2 * p.elem) - 1;
+    sa.shifted = p.outer_shift_a && sa.rows_fast;  // the translate-form kernel reads along rows only
+    sa.pair = p.outer_pair_a && (reinterpret_cast<uintptr_t>(c.A) & align) == 0;
+    if (sa.shifted) sa.prods = p.outer_shifted;
+    e = launch_operand_sums(p.elem, sa, stream);
+    if (e == hipSuccess)
+    {
+        sa.X = c.B;
+        sa.zoff = p.h_batch_b[0];
+        sa.rs = p.b_k;
+        sa.cs = p.b_n;
+        sa.rows = p.os_k;
+        sa.cols = p.os_n;
+        sa.which = 1;
+        sa.rows_fast = !p.b_nfast;
+        sa.out = T;
+        sa.shifted = p.outer_shift_b && sa.rows_fast;
+        sa.pair = p.outer_pair_b && (reinterpret_cast<uintptr_t>(c.B) & align) == 0;
+        sa.prods = sa.shifted ? p.outer_shifted : p.outer_sums;
+        e = launch_operand_sums(p.elem, sa, stream);
+    }
+    GemmArgs g;
+    std::memset(&g, 0, sizeof(g));
+    g.A = S;
+    g.B = T;
+    g.C = static_cast<char*>(c.C) + p.h_batch_c[0] * p.elem;
+    g.a_m = p.os_sm;
+    g.a_k = p.os_sk;
+    g.a_krun = p.os_skrun;
+    g.b_k = p.os_tk;
+    g.b_krun = p.os_tkrun;
+    g.b_n = p.os_tn;
+    g.c_m = p.c_m;
+    g.c_n = p.c_n;
+    g.ms = p.ms;
+    g.ns = p.ns;
+    g.ks = p.ks;
+    g.alpha = c.alpha;
+    g.beta = c.beta;
+    g.nbatch = 1;
+    g.zero = p.zero;
+    g.ka_stride = p.kf_inner.sa;
+    g.kb_stride = p.kf_inner.sb;
+    for (size_t st = 0; e == hipSuccess && st + 1 < p.h_stage_begin.size(); st++)
+    {
+        const int b = p.h_stage_begin[st], n = p.h_stage_begin[st + 1] - b;
+        g.prods = p.inner_staged + b;
+        g.nprod = n;
+        timing_begin(stream);
+        e = launch_leaf_gemm(p.elem, EPI_MULTI_C, p.tile, p.h_inner_stage_terms[st], p.kf_inner.mode, g, false,
+                             false, stream);
+        timing_end(stream, leaf_flops(p, n, p.ms, p.ns, p.ks));
+    }
+    const hipError_t f1 = hipFreeAsync(S, stream), f2 = hipFreeAsync(T, stream);
+    if (e != hipSuccess) return e;
+    return f1 != hipSuccess ? f1 : f2;
+}
+
+}  // namespace
+
 Status run_plan(const Plan& p, const Call& c, hipStream_t stream)
 {
     if (p.general) return Status::error(MISTRA_ERR_UNSUPPORTED, "contractions with A-only, B-only or C-only indices are not supported yet");
     if (p.nprod == 0) return Status();
     GemmArgs g = base_args(p, c);
 
-    if (!p.core || p.impl == IMPL_ABC || p.impl == IMPL_BLAS || p.impl == IMPL_REF)
+    if (p.outer_staged)
+    {
+        HIP_TRY(run_outer_staged(p, c, stream));
+    }
+    else if (!p.core || p.impl == IMPL_ABC || p.impl == IMPL_BLAS || p.impl == IMPL_REF)
     {
         g.ms = p.ms;
         g.ns = p.ns;

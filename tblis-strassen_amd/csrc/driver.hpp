@@ -88,6 +88,25 @@ struct Plan
     // fast-K eligibility of the core launch and of the Naive dense launch
     KFast kf_core, kf_dense;
 
+    // ABC at level 2 with the outer level's operand sums staged: the 7 outer sums of A and of B are
+    // written as dense K-fast buffers S (7 x os_m x os_k) and T (7 x os_k x os_n), and the stage launches
+    // read 2-term inner sums of those instead of 4-term sums of A and B.  C is still updated in the
+    // leaf kernel's epilogue, in the same stages and order as the fused form.
+    bool outer_staged = false;
+    int64_t os_m = 0, os_n = 0, os_k = 0;  // outer sub-block extents (2 ms, 2 ns, 2 ks)
+    std::vector<ProdDesc> h_outer_sums;    // sum-kernel tables: A with rows = K, cols = M; B rows = K, cols = N
+    std::vector<ProdDesc> h_outer_shifted; // the same with term bases = shifts from the first sub-block
+    bool outer_shift_a = false, outer_shift_b = false;  // every outer term of A (B) is a translate
+    bool outer_pair_a = false, outer_pair_b = false;    // rows in contiguous, even-aligned pairs
+    std::vector<ProdDesc> h_inner;         // the products over S / T in table order (fast-body form)
+    std::vector<int> h_inner_stage_terms;
+    KFast kf_inner;
+    const ProdDesc* outer_sums = nullptr;
+    const ProdDesc* outer_shifted = nullptr;
+    const ProdDesc* inner_staged = nullptr;
+    const int64_t *os_sm = nullptr, *os_sk = nullptr, *os_skrun = nullptr, *os_tk = nullptr, *os_tkrun = nullptr,
+                  *os_tn = nullptr;
+
     // device mirror (one allocation)
     void* dmem = nullptr;
     const int64_t *a_m = nullptr, *a_k = nullptr, *a_krun = nullptr, *b_k = nullptr, *b_krun = nullptr,

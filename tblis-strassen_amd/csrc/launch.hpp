@@ -55,6 +55,12 @@ struct SumArgs
     bool rows_fast;  // true if rs has unit stride (read along rows), else read along cols
     void* out;
     int64_t zoff;  // batch offset added to X
+    // Translate form (rows_fast only): every term's sub-block is the first one shifted, and `prods`
+    // holds the shifts as term bases (roff / coff unused): element (r, c) of term t is
+    // X[base_t + rs[r] + cs[c]].  `pair`: rows come in contiguous pairs at even element offsets, read
+    // with one vector load.
+    bool shifted;
+    bool pair;
 };
 hipError_t launch_operand_sums(int elem_size, const SumArgs& a, hipStream_t stream);
 

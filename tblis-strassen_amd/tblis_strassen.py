@@ -21,6 +21,8 @@ PREC = {"native": 0, "bf16x3": 1, "ozaki_bf16": 2, "fp8_block": 3}
 FLAG_REF_FRINGE = 1
 FLAG_SYNC = 2
 FLAG_GRAPH = 4
+FLAG_FUSED_SUMS = 8
+FLAG_OUTER_STAGED = 16
 
 
 class MistraError(RuntimeError):

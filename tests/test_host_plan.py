@@ -67,7 +67,7 @@ def _debug_plan(case, level, flags=0, strides=None, cap=8192):
         t.stride = ctypes.cast(t._stride, ctypes.POINTER(ctypes.c_ssize_t))
         t.data = None
         tensors.append(t)
-    dims = (ctypes.c_longlong * 13)()
+    dims = (ctypes.c_longlong * 15)()
     scat = (ctypes.c_longlong * (6 * cap))()
     prods = (ctypes.c_double * (64 * 40))()
     np_ = lib.mistra_debug_plan(ctypes.byref(tensors[0]), case["idx_A"].encode(), ctypes.byref(tensors[1]),
@@ -241,3 +241,27 @@ def test_autotuner_analytic_model():
     Z = [np.zeros(int(np.prod(small[f"shape_{x}"]))).reshape(small[f"shape_{x}"], order="F") for x in "ABC"]
     level, impl, tile = ts.autotune(Z[0], small["idx_A"], Z[1], small["idx_B"], Z[2], small["idx_C"])
     assert level == 0 and tile == 1  # TILE_64
+
+
+def test_abc_stage_counts():
+    # first-fit-decreasing packing by C targets: 4 launches at one level, 16 at two (optimal for 49
+    # products of equal size: every stage but one holds an even number of products)
+    case = cases.make_case(8, m=(32,), n=(32,), k=(32,))
+    for level, want in ((0, 1), (1, 4), (2, 16)):
+        _, dims, _, _ = _debug_plan(case, level, flags=ts.FLAG_FUSED_SUMS)
+        assert dims[14] == want
+
+
+def test_outer_staged_choice():
+    # the cost model stages the outer sums of a 2-level ABC plan at 8192^3 (sum traffic ~ n^2 against
+    # the slower 4-term leaf launches ~ n^3); the flags force either form; levels other than 2 and
+    # batched plans never stage
+    big = cases.make_case(9, m=(8192,), n=(8192,), k=(8192,))
+    small = cases.make_case(9, m=(64,), n=(64,), k=(64,))
+    batched = cases.make_case(9, m=(64,), n=(64,), k=(64,), batch=(2,))
+    assert _debug_plan(big, 2)[1][13] == 1
+    assert _debug_plan(big, 2, flags=ts.FLAG_FUSED_SUMS)[1][13] == 0
+    assert _debug_plan(big, 1)[1][13] == 0
+    assert _debug_plan(small, 2, flags=ts.FLAG_FUSED_SUMS)[1][13] == 0
+    assert _debug_plan(small, 2, flags=ts.FLAG_OUTER_STAGED)[1][13] == 1
+    assert _debug_plan(batched, 2, flags=ts.FLAG_OUTER_STAGED)[1][13] == 0
