@@ -7,9 +7,10 @@ seeded order; each tensor's index order is a seeded permutation; A, B ~ uniform[
 generated in HBM); C starts at zero and every step is C += A.B (beta = 1, as the reference's
 test_strassen does).  A step is one contraction call; inputs are resident in HBM before timing.
 
-Variants: L0 (classical MFMA TC), L1/L2 x {ABC, AB, Naive}; 2-level ABC in the library's default form
-(at this size: the outer level's 7 operand sums of A and of B staged in HBM, 2-term inner sums in the
-leaf kernel) and in the fused form (every 4-term sum in the leaf kernel, no workspace).  Errors are ||C - C_L0||_F / ||C_L0||_F of a
+Variants: L0 (classical MFMA TC), L1/L2 x {ABC, AB, Naive}; ABC in both of its forms: staged (the outer
+level's 7 operand sums of A and of B written to HBM, the inner sums -- none at L1, 2 terms at L2 -- formed
+in the leaf kernel; the library's default at this size) and fused (every sum formed in the leaf kernel,
+no workspace).  C is updated in the leaf kernel's epilogue in both.  Errors are ||C - C_L0||_F / ||C_L0||_F of a
 single call from C = 0.  The reported `value` is the fastest Strassen variant (L1 or L2, any impl) that
 meets its tolerance (1e-12 at L1, 5e-12 at L2); every variant, ABC included, is listed under `variants`.
 
@@ -317,7 +318,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS) + ["c3", "c4", "c5"])
-    ap.add_argument("--variants", default="0:abc,1:abc,2:abc,2:abc:fused,1:ab,2:ab,1:naive,2:naive",
+    ap.add_argument("--variants", default="0:abc,1:abc:staged,1:abc:fused,2:abc:staged,2:abc:fused,1:ab,2:ab,"
+                                          "1:naive,2:naive",
                     help="level:impl[:form], form = fused (every operand sum in the leaf kernel) or staged "
                          "(2-level ABC with the outer sums staged in HBM); default: the library's choice")
     ap.add_argument("--no-cpu-baseline", action="store_true")

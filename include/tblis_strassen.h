@@ -130,12 +130,13 @@ enum
     MISTRA_FLAG_GRAPH      = 1u << 2, /* mistra_batch_mult: capture the batch's launches in a HIP graph on
                                          the first call and replay it on later calls with the same entries
                                          (same pointers, shapes, scalars); needs a non-null stream        */
-    MISTRA_FLAG_FUSED_SUMS = 1u << 3, /* 2-level ABC: form every operand sum inside the leaf kernel (no
+    MISTRA_FLAG_FUSED_SUMS = 1u << 3, /* ABC: form every operand sum inside the leaf kernel (no
                                          workspace), never staging the outer level's sums                */
-    MISTRA_FLAG_OUTER_STAGED = 1u << 4 /* 2-level ABC: stage the outer level's 7 operand sums of A and of
-                                         B in HBM workspace (7/4 of A's and of B's size) and form only
-                                         the inner 2-term sums in the leaf kernel, whenever the shape
-                                         allows it.  Without either flag the cost model decides.        */
+    MISTRA_FLAG_OUTER_STAGED = 1u << 4 /* ABC (levels 1, 2): stage the outer level's 7 operand sums of A
+                                         and of B in HBM workspace (7/4 of A's and of B's size) and form
+                                         only the inner sums (2 terms at level 2, none at level 1) in the
+                                         leaf kernel, whenever the shape allows it (one batch entry).
+                                         Without either flag the cost model decides.                     */
 };
 
 /* Error codes (0 = success). */
@@ -206,7 +207,8 @@ typedef struct mistra_contraction
 
 /* Run count independent contractions with few launches: the ABC launches of all entries are grouped
  * by Strassen stage (round) and kernel variant, and each group is one grouped launch whose workgroups
- * find their contraction in a device job table.  Entries must not write overlapping C memory.
+ * find their contraction in a device job table (ABC in its fused form, MISTRA_FLAG_FUSED_SUMS, so that
+ * every launch is a leaf launch).  Entries must not write overlapping C memory.
  * level/impl -1 take the autotuner's choice; entries that are not device-resident ABC contractions
  * (host data, general index classes, AB/Naive) run as individual calls. Asynchronous on stream unless
  * MISTRA_FLAG_SYNC. No counterpart in the reference (its harness loops over contractions). */

@@ -120,7 +120,10 @@ double model_ms(const Plan& p0)
                 ra += sd.nA + 1;
                 rb += sd.nB + 1;
             }
-            s += (ra * double(p.os_m) * p.os_k + rb * double(p.os_k) * p.os_n) * e / kHbm + 2 * kLaunch;
+            // translate-form sums stream at about 5 TB/s, gathered ones at kHbm
+            const double ha = p.outer_shift_a && !p.a_mfast ? 5.0e12 : kHbm;
+            const double hb = p.outer_shift_b && !p.b_nfast ? 5.0e12 : kHbm;
+            s += ra * double(p.os_m) * p.os_k * e / ha + rb * double(p.os_k) * p.os_n * e / hb + 2 * kLaunch;
         }
     }
     return s * 1e3;

@@ -112,7 +112,9 @@ Status prepare(int count, const mistra_contraction* e, std::vector<std::shared_p
             continue;
         }
         std::shared_ptr<Plan> plan;
-        Status st = get_plan(x.A, x.idx_A, x.B, x.idx_B, x.C, x.idx_C, level, IMPL_ABC, -1, 0, plan);
+        // fused sums: every launch of the plan is a leaf launch, so it can join a grouped launch
+        Status st = get_plan(x.A, x.idx_A, x.B, x.idx_B, x.C, x.idx_C, level, IMPL_ABC, -1, MISTRA_FLAG_FUSED_SUMS,
+                             plan);
         if (!st.ok()) return st;
         if (plan->general || plan->outer_staged)
         {

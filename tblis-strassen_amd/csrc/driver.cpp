@@ -217,16 +217,18 @@ ProdDesc shifted(const Plan& p, const ProdDesc& pd)
     return f;
 }
 
-// Tables of the outer-staged form of a 2-level ABC plan (see Plan::outer_staged).  Outer product po
-// sums A over its 1-level terms into S_po (element (i, j) of the os_m x os_k block at
+// Tables of the outer-staged form of an ABC plan (see Plan::outer_staged).  Outer product po of the
+// 1-level table sums A over its terms into S_po (element (i, j) of the os_m x os_k block at
 // po*os_m*os_k + i*os_k + j: K-fast) and B into T_po (element (i, j) of os_k x os_n at
-// po*os_k*os_n + i + j*os_k: K-fast); product 7*po + pi of the 2-level table then multiplies the inner
-// sums of pi over S_po and T_po and updates the same C targets, with the same coefficients, as before.
-// Returns false when the form does not apply (C sub-blocks that are not translates of the first one, or
-// an inner sub-block that does not fit the 31-bit buffer range).
+// po*os_k*os_n + i + j*os_k: K-fast); product n_in*po + pi of the plan's table then multiplies the sums
+// of inner product pi (of the (level-1)-level table: the identity at level 1) over S_po and T_po and
+// updates the same C targets, with the same coefficients, as before.  Returns false when the form does
+// not apply (C sub-blocks that are not translates of the first one, or an inner sub-block that does not
+// fit the 31-bit buffer range).
 bool build_outer_staged(Plan& p)
 {
-    const int64_t ms = p.ms, ns = p.ns, ks = p.ks, om = 2 * ms, on = 2 * ns, ok = 2 * ks;
+    const int64_t f = p.d / 2;  // leaf blocks per outer sub-block side
+    const int64_t ms = p.ms, ns = p.ns, ks = p.ks, om = f * ms, on = f * ns, ok = f * ks;
     for (int q = 0; q < p.nprod; q++)
     {
         const ProdDesc& pd = p.h_prods[size_t(q)];
@@ -282,11 +284,13 @@ bool build_outer_staged(Plan& p)
     translate_form(0, p.h_a_k, p.h_a_m, ok, om, p.h_batch_a[0], p.outer_shift_a, p.outer_pair_a);
     translate_form(1, p.h_b_k, p.h_b_n, ok, on, p.h_batch_b[0], p.outer_shift_b, p.outer_pair_b);
     p.h_inner.clear();
+    const auto& inner = products_for_level(p.level - 1);
+    const int64_t din = f;  // inner split divisor
     for (size_t o = 0; o < one.size(); o++)
-        for (size_t i = 0; i < one.size(); i++)
+        for (size_t i = 0; i < inner.size(); i++)
         {
-            const Product& pi = one[i];
-            ProdDesc pd = p.h_prods[o * one.size() + i];  // C terms of the 2-level product
+            const Product& pi = inner[i];
+            ProdDesc pd = p.h_prods[o * inner.size() + i];  // C terms of the plan's product
             pd.nA = int(pi.A.size());
             pd.nB = int(pi.B.size());
             std::memset(pd.a, 0, sizeof(pd.a));
@@ -294,12 +298,12 @@ bool build_outer_staged(Plan& p)
             // fast-body form: base = offset of the inner sub-block's first element
             for (size_t t = 0; t < pi.A.size(); t++)
             {
-                const int64_t r = (pi.A[t].subid / 2) * ms, c = (pi.A[t].subid % 2) * ks;
+                const int64_t r = (pi.A[t].subid / din) * ms, c = (pi.A[t].subid % din) * ks;
                 pd.a[t] = SubRef{int64_t(o) * om * ok + r * ok + c, r, c, pi.A[t].coeff};
             }
             for (size_t t = 0; t < pi.B.size(); t++)
             {
-                const int64_t r = (pi.B[t].subid / 2) * ks, c = (pi.B[t].subid % 2) * ns;
+                const int64_t r = (pi.B[t].subid / din) * ks, c = (pi.B[t].subid % din) * ns;
                 pd.b[t] = SubRef{int64_t(o) * ok * on + r + c * ok, r, c, pi.B[t].coeff};
             }
             auto d = [](const std::vector<int64_t>& v, int64_t off) { return v[size_t(off)] - v[0]; };
@@ -313,13 +317,13 @@ bool build_outer_staged(Plan& p)
     return true;
 }
 
-// Whether a 2-level ABC plan stages its outer sums: MISTRA_FLAG_FUSED_SUMS / MISTRA_FLAG_OUTER_STAGED
+// Whether an ABC plan (level 1 or 2) stages its outer sums: MISTRA_FLAG_FUSED_SUMS / MISTRA_FLAG_OUTER_STAGED
 // decide when given, else the cost model (the outer sums' HBM traffic against the slower 4-term leaf
 // launches of the fused form).
 void choose_outer_staged(Plan& p, unsigned flags)
 {
     p.outer_staged = false;
-    if (p.level != 2 || !p.core || p.impl != IMPL_ABC || p.nbatch != 1 || (flags & MISTRA_FLAG_FUSED_SUMS)) return;
+    if (p.level < 1 || !p.core || p.impl != IMPL_ABC || p.nbatch != 1 || (flags & MISTRA_FLAG_FUSED_SUMS)) return;
     if (!build_outer_staged(p)) return;
     if (flags & MISTRA_FLAG_OUTER_STAGED)
     {
@@ -873,8 +877,8 @@ Status plan_abc_jobs(const Plan& p, const Call& c, std::vector<LaunchJob>& out)
 
 namespace {
 
-// Outer-staged 2-level ABC (Plan::outer_staged): the outer sums into S and T, then one launch per stage
-// over 2-term inner sums of S and T with the multi-C epilogue on C.
+// Outer-staged ABC (Plan::outer_staged): the outer sums into S and T, then one launch per stage over the
+// inner sums of S and T (2 terms at level 2, 1 at level 1) with the multi-C epilogue on C.
 hipError_t run_outer_staged(const Plan& p, const Call& c, hipStream_t stream)
 {
     const size_t sbytes = size_t(7) * size_t(p.os_m) * size_t(p.os_k) * size_t(p.elem);

@@ -88,12 +88,13 @@ struct Plan
     // fast-K eligibility of the core launch and of the Naive dense launch
     KFast kf_core, kf_dense;
 
-    // ABC at level 2 with the outer level's operand sums staged: the 7 outer sums of A and of B are
-    // written as dense K-fast buffers S (7 x os_m x os_k) and T (7 x os_k x os_n), and the stage launches
-    // read 2-term inner sums of those instead of 4-term sums of A and B.  C is still updated in the
-    // leaf kernel's epilogue, in the same stages and order as the fused form.
+    // ABC with the outer level's operand sums staged: the 7 outer sums of A and of B are written as
+    // dense K-fast buffers S (7 x os_m x os_k) and T (7 x os_k x os_n), and the stage launches read the
+    // inner sums of those (2 terms at level 2, a single term at level 1) instead of 4- (2-) term sums of
+    // A and B.  C is still updated in the leaf kernel's epilogue, in the same stages and order as the
+    // fused form.
     bool outer_staged = false;
-    int64_t os_m = 0, os_n = 0, os_k = 0;  // outer sub-block extents (2 ms, 2 ns, 2 ks)
+    int64_t os_m = 0, os_n = 0, os_k = 0;  // outer sub-block extents (m/2, n/2, k/2 rounded to leaves)
     std::vector<ProdDesc> h_outer_sums;    // sum-kernel tables: A with rows = K, cols = M; B rows = K, cols = N
     std::vector<ProdDesc> h_outer_shifted; // the same with term bases = shifts from the first sub-block
     bool outer_shift_a = false, outer_shift_b = false;  // every outer term of A (B) is a translate

@@ -52,9 +52,11 @@ def test_batch_equals_individual_calls(dtype, level):
     ents, bufs = make_entries(batch, dtype, level, 100)
     ents2, bufs2 = make_entries(batch, dtype, level, 100)
     ts.batch_mult(ents, flags=ts.FLAG_SYNC)
+    # batch entries run ABC in its fused form (every launch a leaf launch); the same form called one
+    # by one gives the same bits
     for e in ents2:
         ts.tensor_mult(e["alpha"], e["A"], e["idx_A"], e["B"], e["idx_B"], e["beta"], e["C"], e["idx_C"],
-                       level=level, impl="abc", flags=ts.FLAG_SYNC)
+                       level=level, impl="abc", flags=ts.FLAG_SYNC | ts.FLAG_FUSED_SUMS)
     for (c1, *_), (c2, *_) in zip(bufs, bufs2):
         assert torch.equal(c1, c2)
     tol = 1e-4 if dtype == torch.float32 else 1e-12
