@@ -18,6 +18,8 @@ namespace {
 
 std::mutex g_tune_mutex;
 std::map<std::string, TuneChoice> g_measured;
+std::map<std::string, int> g_best_level;  // modelled level per (shape, impl)
+constexpr size_t kMaxTuneEntries = 1 << 16;
 
 std::string shape_key(const tblis::tblis_tensor* A, const label_type* idx_A, const tblis::tblis_tensor* B,
                       const label_type* idx_B, const tblis::tblis_tensor* C, const label_type* idx_C)
@@ -237,6 +239,7 @@ Status autotune(const tblis::tblis_tensor* A, const label_type* idx_A, const tbl
     }
     {
         std::lock_guard<std::mutex> lk(g_tune_mutex);
+        if (g_measured.size() >= kMaxTuneEntries) g_measured.clear();
         g_measured[key] = best;
     }
     out = best;
@@ -246,6 +249,13 @@ Status autotune(const tblis::tblis_tensor* A, const label_type* idx_A, const tbl
 int best_level(const tblis::tblis_tensor* A, const label_type* idx_A, const tblis::tblis_tensor* B,
                const label_type* idx_B, const tblis::tblis_tensor* C, const label_type* idx_C, int impl)
 {
+    // cached per shape: batches and loops ask for the same shapes on every call
+    const std::string key = shape_key(A, idx_A, B, idx_B, C, idx_C) + '#' + std::to_string(impl);
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mutex);
+        auto it = g_best_level.find(key);
+        if (it != g_best_level.end()) return it->second;
+    }
     int best = 0;
     double best_ms = 1e300;
     for (int level = 0; level <= 2; level++)
@@ -260,6 +270,9 @@ int best_level(const tblis::tblis_tensor* A, const label_type* idx_A, const tbli
             best = level;
         }
     }
+    std::lock_guard<std::mutex> lk(g_tune_mutex);
+    if (g_best_level.size() >= kMaxTuneEntries) g_best_level.clear();
+    g_best_level[key] = best;
     return best;
 }
 

@@ -20,14 +20,17 @@
 #include <map>
 #include <mutex>
 #include <sstream>
+#include <unordered_map>
 
 namespace mistra {
 
 namespace {
 
 std::mutex g_plan_mutex;
-std::list<std::pair<std::string, std::shared_ptr<Plan>>> g_plans;  // LRU, most recent first
-constexpr size_t kMaxPlans = 64;
+using PlanList = std::list<std::pair<std::string, std::shared_ptr<Plan>>>;
+PlanList g_plans;                                                  // LRU, most recent first
+std::unordered_map<std::string, PlanList::iterator> g_plan_index;  // key -> its entry of g_plans
+constexpr size_t kMaxPlans = 1024;  // plans are small: index vectors and tables
 
 #define HIP_TRY(expr)                                                                             \
     do                                                                                            \
@@ -318,8 +321,8 @@ bool build_outer_staged(Plan& p)
 }
 
 // Whether an ABC plan (level 1 or 2) stages its outer sums: MISTRA_FLAG_FUSED_SUMS / MISTRA_FLAG_OUTER_STAGED
-// decide when given, else the cost model (the outer sums' HBM traffic against the slower 4-term leaf
-// launches of the fused form).
+// decide when given, else the cost model (the outer sums' HBM traffic against the slower multi-term leaf
+// launches of the fused form), for leaves of at least one register tile per CU.
 void choose_outer_staged(Plan& p, unsigned flags)
 {
     p.outer_staged = false;
@@ -330,6 +333,10 @@ void choose_outer_staged(Plan& p, unsigned flags)
         p.outer_staged = true;
         return;
     }
+    // small leaves: launches that do not fill the 256 CUs are latency-bound, so fewer sum terms gain
+    // little against the two extra sum launches and the workspace
+    const int64_t te = tile_extent(p.tile);
+    if (((p.ms + te - 1) / te) * ((p.ns + te - 1) / te) < 256) return;
     const double fused = model_ms(p);
     p.outer_staged = true;
     const double staged = model_ms(p);
@@ -762,13 +769,13 @@ Status get_plan(const tblis::tblis_tensor* A, const label_type* idx_A, const tbl
     append_key(key, C, idx_C);
     {
         std::lock_guard<std::mutex> lk(g_plan_mutex);
-        for (auto it = g_plans.begin(); it != g_plans.end(); ++it)
-            if (it->first == key)
-            {
-                out = it->second;
-                g_plans.splice(g_plans.begin(), g_plans, it);
-                return Status();
-            }
+        auto f = g_plan_index.find(key);
+        if (f != g_plan_index.end())
+        {
+            out = f->second->second;
+            g_plans.splice(g_plans.begin(), g_plans, f->second);
+            return Status();
+        }
     }
     auto p = std::make_shared<Plan>();
     Status st = build_plan_host(A, idx_A, B, idx_B, C, idx_C, level, impl, tile, flags, *p);
@@ -779,8 +786,19 @@ Status get_plan(const tblis::tblis_tensor* A, const label_type* idx_A, const tbl
         if (!st.ok()) return st;
     }
     std::lock_guard<std::mutex> lk(g_plan_mutex);
+    auto f = g_plan_index.find(key);
+    if (f != g_plan_index.end())  // built concurrently by another thread: keep that one
+    {
+        out = f->second->second;
+        return Status();
+    }
     g_plans.emplace_front(key, p);
-    while (g_plans.size() > kMaxPlans) g_plans.pop_back();
+    g_plan_index[key] = g_plans.begin();
+    while (g_plans.size() > kMaxPlans)
+    {
+        g_plan_index.erase(g_plans.back().first);
+        g_plans.pop_back();
+    }
     out = p;
     return Status();
 }
@@ -788,6 +806,7 @@ Status get_plan(const tblis::tblis_tensor* A, const label_type* idx_A, const tbl
 void release_all()
 {
     std::lock_guard<std::mutex> lk(g_plan_mutex);
+    g_plan_index.clear();
     g_plans.clear();
 }
 
