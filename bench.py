@@ -7,10 +7,10 @@ seeded order; each tensor's index order is a seeded permutation; A, B ~ uniform[
 generated in HBM); C starts at zero and every step is C += A.B (beta = 1, as the reference's
 test_strassen does).  A step is one contraction call; inputs are resident in HBM before timing.
 
-Variants: L0 (classical MFMA TC), L1/L2 x {ABC, AB, Naive}; ABC in both of its forms: staged (the outer
+Variants: L0 (classical MFMA TC), L1/L2 x {ABC, AB, Naive}; ABC in each of its forms: staged (the outer
 level's 7 operand sums of A and of B written to HBM, the inner sums -- none at L1, 2 terms at L2 -- formed
-in the leaf kernel; the library's default at this size) and fused (every sum formed in the leaf kernel,
-no workspace).  C is updated in the leaf kernel's epilogue in both.  Errors are ||C - C_L0||_F / ||C_L0||_F of a
+in the leaf kernel), staged_all (at L2 all 49 sums written to HBM) and fused (every sum formed in the
+leaf kernel, no workspace).  C is updated in the leaf kernel's epilogue in all of them.  Errors are ||C - C_L0||_F / ||C_L0||_F of a
 single call from C = 0.  The reported `value` is the fastest Strassen variant (L1 or L2, any impl) that
 meets its tolerance (1e-12 at L1, 5e-12 at L2); every variant, ABC included, is listed under `variants`.
 
@@ -318,10 +318,11 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS) + ["c3", "c4", "c5"])
-    ap.add_argument("--variants", default="0:abc,1:abc:staged,1:abc:fused,2:abc:staged,2:abc:fused,1:ab,2:ab,"
-                                          "1:naive,2:naive",
-                    help="level:impl[:form], form = fused (every operand sum in the leaf kernel) or staged "
-                         "(2-level ABC with the outer sums staged in HBM); default: the library's choice")
+    ap.add_argument("--variants", default="0:abc,1:abc:staged,1:abc:fused,2:abc:staged_all,2:abc:staged,"
+                                          "2:abc:fused,1:ab,2:ab,1:naive,2:naive",
+                    help="level:impl[:form], form = fused (every operand sum in the leaf kernel), staged (the "
+                         "outer level's sums staged in HBM) or staged_all (every level's sums staged); default: "
+                         "the library's choice")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run one device copy of MIB MiB (a kernel of known traffic for counter scaling)")
@@ -367,7 +368,7 @@ def main():
     C, cbuf = device_tensor(case["shape_C"], torch.float64, torch)
 
     variants = {}
-    forms = {"": 0, "fused": ts.FLAG_FUSED_SUMS, "staged": ts.FLAG_OUTER_STAGED}
+    forms = {"": 0, "fused": ts.FLAG_FUSED_SUMS, "staged": ts.FLAG_OUTER_STAGED, "staged_all": ts.FLAG_ALL_STAGED}
     for spec in args.variants.split(","):
         lv, impl, *form = spec.split(":")
         form = form[0] if form else ""

@@ -132,11 +132,14 @@ enum
                                          (same pointers, shapes, scalars); needs a non-null stream        */
     MISTRA_FLAG_FUSED_SUMS = 1u << 3, /* ABC: form every operand sum inside the leaf kernel (no
                                          workspace), never staging the outer level's sums                */
-    MISTRA_FLAG_OUTER_STAGED = 1u << 4 /* ABC (levels 1, 2): stage the outer level's 7 operand sums of A
-                                         and of B in HBM workspace (7/4 of A's and of B's size) and form
+    MISTRA_FLAG_OUTER_STAGED = 1u << 4, /* ABC (levels 1, 2): stage the outer level's 7 operand sums of
+                                         A and of B in HBM workspace (7/4 of A's and of B's size) and form
                                          only the inner sums (2 terms at level 2, none at level 1) in the
-                                         leaf kernel, whenever the shape allows it (one batch entry).
-                                         Without either flag the cost model decides.                     */
+                                         leaf kernel, whenever the shape allows it (one batch entry).   */
+    MISTRA_FLAG_ALL_STAGED = 1u << 5   /* ABC: stage the sums of every level (at level 2 the 49 sums of A
+                                         and of B: 49/16 of A's and of B's size); each leaf product then
+                                         reads one dense operand pair.  Without any of the three flags
+                                         the cost model decides.                                         */
 };
 
 /* Error codes (0 = success). */

@@ -14,7 +14,7 @@ extern "C" {
 #endif
 
 /* dims[0..14] = m, n, k, nbatch, pure, transposed, ms, ns, ks, nprod (core products), nfringe,
- *               a_mfast, b_nfast, outer_staged (2-level ABC with the outer operand sums staged),
+ *               a_mfast, b_nfast, staged_depth (ABC levels whose operand sums are staged; 0: fused),
  *               number of ABC launch stages.
  * scat[6*cap]  = a_m, a_k, b_k, b_n, c_m, c_n (in the plan's own orientation), -1 padded.
  * prods[cap_prods*40] per product (core then fringe): nA, nB, nC, first_mask, then 4 x (roff, coff,

@@ -38,10 +38,11 @@ std::string shape_key(const tblis::tblis_tensor* A, const label_type* idx_A, con
 }
 
 // Leaf-kernel rates (TFLOP/s) measured on MI355X with the 8192^3 FP64 workload, by register tile and
-// number of sum terms (1, 2, 4); FP32 runs the same kernels at about twice the rate.
+// number of sum terms (1, 2, 4; single-term launches over K-fast operands load K-tiles straight into
+// LDS); FP32 runs the same kernels at about twice the rate.
 double leaf_rate(int elem, int tile, int terms)
 {
-    static const double r128[3] = {62.0, 53.5, 45.0}, r64[3] = {57.0, 46.0, 38.0};
+    static const double r128[3] = {66.0, 53.5, 45.0}, r64[3] = {57.0, 46.0, 38.0};
     const int ti = terms <= 1 ? 0 : (terms == 2 ? 1 : 2);
     const double r = tile == TILE_64 ? r64[ti] : r128[ti];
     return elem == 8 ? r : 1.8 * r;
@@ -115,13 +116,17 @@ double model_ms(const Plan& p0)
         }
         if (p.outer_staged)
         {
-            // outer sums: read every outer term's sub-block, write the 7 dense sums, for A and for B
+            // staged sums: read every term's sub-block (one-pass form: every source sub-block once) and
+            // write the dense sums, for A and for B
             double ra = 0, rb = 0;
             for (const ProdDesc& sd : p.h_outer_sums)
             {
                 ra += sd.nA + 1;
                 rb += sd.nB + 1;
             }
+            const double nout = double(p.h_outer_sums.size());
+            if (p.msum_a && !p.a_mfast) ra = p.msum_nsrc + nout;
+            if (p.msum_b && !p.b_nfast) rb = p.msum_nsrc + nout;
             // translate-form sums stream at about 5 TB/s, gathered ones at kHbm
             const double ha = p.outer_shift_a && !p.a_mfast ? 5.0e12 : kHbm;
             const double hb = p.outer_shift_b && !p.b_nfast ? 5.0e12 : kHbm;

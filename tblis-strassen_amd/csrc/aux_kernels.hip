@@ -119,6 +119,81 @@ __global__ __launch_bounds__(256) void operand_sums_rows_kernel(SumArgs a)
     }
 }
 
+// One pass over the sources for all staged sums (MultiSumArgs): a thread owns RPT consecutive rows of
+// one column, loads its NSRC source values once and writes every output from registers.
+template <typename T, bool PAIR, int NSRC>
+__global__ __launch_bounds__(256) void operand_multisums_kernel(MultiSumArgs a)
+{
+    constexpr int RPT = PAIR ? 2 : 1;
+    typedef T vec2 __attribute__((ext_vector_type(2)));
+    const int64_t tiles_r = (a.rows + 256 * RPT - 1) / (256 * RPT);
+    const int64_t tr = blockIdx.x % tiles_r, c = blockIdx.x / tiles_r;
+    const int64_t r = (tr * 256 + threadIdx.x) * RPT;
+    if (r >= a.rows) return;
+    const T* X = static_cast<const T*>(a.X) + a.zoff + a.rs[r] + a.cs[c];
+    T v[NSRC][RPT];
+#pragma unroll
+    for (int s = 0; s < NSRC; s++)
+    {
+        const T* x = X + a.shift[s];
+        if constexpr (PAIR)
+        {
+            const vec2 w = *reinterpret_cast<const vec2*>(x);
+            v[s][0] = w.x;
+            v[s][1] = w.y;
+        }
+        else
+            v[s][0] = x[0];
+    }
+    T* out = static_cast<T*>(a.out) + c * a.rows + r;
+    const int64_t stride = a.rows * a.cols;
+    for (int q = 0; q < a.nout; q++)
+    {
+        T acc[RPT];
+#pragma unroll
+        for (int i = 0; i < RPT; i++) acc[i] = T(0);
+#pragma unroll
+        for (int s = 0; s < NSRC; s++)
+        {
+            const double cf = a.coef[q * NSRC + s];  // uniform
+            if (cf != 0.0)
+#pragma unroll
+                for (int i = 0; i < RPT; i++) acc[i] += T(cf) * v[s][i];
+        }
+        if constexpr (PAIR)
+            *reinterpret_cast<vec2*>(out + q * stride) = vec2{acc[0], acc[1]};
+        else
+            out[q * stride] = acc[0];
+    }
+}
+
+template <typename T>
+hipError_t multisums_typed(const MultiSumArgs& a, hipStream_t s)
+{
+    const int rpt = a.pair ? 2 : 1;
+    const int64_t blocks = ((a.rows + 256 * rpt - 1) / (256 * rpt)) * a.cols;
+    if (blocks == 0 || a.nout == 0) return hipSuccess;
+    const dim3 grid(static_cast<unsigned>(blocks));
+    if (a.nsrc == 4)
+    {
+        if (a.pair) hipLaunchKernelGGL((operand_multisums_kernel<T, true, 4>), grid, dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((operand_multisums_kernel<T, false, 4>), grid, dim3(256), 0, s, a);
+    }
+    else if (a.nsrc == 16)
+    {
+        if (a.pair) hipLaunchKernelGGL((operand_multisums_kernel<T, true, 16>), grid, dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((operand_multisums_kernel<T, false, 16>), grid, dim3(256), 0, s, a);
+    }
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_operand_multisums(int elem_size, const MultiSumArgs& a, hipStream_t s)
+{
+    return elem_size == 8 ? multisums_typed<double>(a, s) : multisums_typed<float>(a, s);
+}
+
 hipError_t launch_operand_sums(int elem_size, const SumArgs& a, hipStream_t s)
 {
     if (a.shifted && a.rows_fast)

@@ -220,17 +220,18 @@ ProdDesc shifted(const Plan& p, const ProdDesc& pd)
     return f;
 }
 
-// Tables of the outer-staged form of an ABC plan (see Plan::outer_staged).  Outer product po of the
-// 1-level table sums A over its terms into S_po (element (i, j) of the os_m x os_k block at
-// po*os_m*os_k + i*os_k + j: K-fast) and B into T_po (element (i, j) of os_k x os_n at
-// po*os_k*os_n + i + j*os_k: K-fast); product n_in*po + pi of the plan's table then multiplies the sums
-// of inner product pi (of the (level-1)-level table: the identity at level 1) over S_po and T_po and
-// updates the same C targets, with the same coefficients, as before.  Returns false when the form does
-// not apply (C sub-blocks that are not translates of the first one, or an inner sub-block that does not
-// fit the 31-bit buffer range).
-bool build_outer_staged(Plan& p)
+// Tables of the staged form of an ABC plan (see Plan::outer_staged), staging the sums of the first
+// `depth` levels.  Outer product po of the depth-level table sums A over its terms into S_po (element
+// (i, j) of the os_m x os_k block at po*os_m*os_k + i*os_k + j: K-fast) and B into T_po (element (i, j)
+// of os_k x os_n at po*os_k*os_n + i + j*os_k: K-fast); product n_in*po + pi of the plan's table then
+// multiplies the sums of inner product pi (of the (level-depth)-level table: the identity when every
+// level is staged) over S_po and T_po and updates the same C targets, with the same coefficients, as
+// before.  Returns false when the form does not apply (C sub-blocks that are not translates of the
+// first one, or an inner sub-block that does not fit the 31-bit buffer range).
+bool build_outer_staged(Plan& p, int depth)
 {
-    const int64_t f = p.d / 2;  // leaf blocks per outer sub-block side
+    const int64_t dout = int64_t(1) << depth;  // staged split divisor
+    const int64_t f = p.d / dout;              // leaf blocks per staged sub-block side
     const int64_t ms = p.ms, ns = p.ns, ks = p.ks, om = f * ms, on = f * ns, ok = f * ks;
     for (int q = 0; q < p.nprod; q++)
     {
@@ -241,7 +242,8 @@ bool build_outer_staged(Plan& p)
     const int TK = tile_kdepth(p.tile, p.elem);
     const int64_t limit = (int64_t(1) << 31) / p.elem;
     if ((ms - 1) * ok + ks + TK >= limit || (ns - 1) * ok + ks + TK >= limit) return false;
-    const auto& one = strassen_products_1level();
+    const auto& one = products_for_level(depth);
+    p.staged_depth = depth;
     p.os_m = om;
     p.os_n = on;
     p.os_k = ok;
@@ -254,9 +256,9 @@ bool build_outer_staged(Plan& p)
         sd.nB = int(po.B.size());
         // A: rows are K (a_k), columns M (a_m); B: rows K (b_k), columns N (b_n)
         for (size_t t = 0; t < po.A.size(); t++)
-            sd.a[t] = SubRef{0, (po.A[t].subid % 2) * ok, (po.A[t].subid / 2) * om, po.A[t].coeff};
+            sd.a[t] = SubRef{0, (po.A[t].subid % dout) * ok, (po.A[t].subid / dout) * om, po.A[t].coeff};
         for (size_t t = 0; t < po.B.size(); t++)
-            sd.b[t] = SubRef{0, (po.B[t].subid / 2) * ok, (po.B[t].subid % 2) * on, po.B[t].coeff};
+            sd.b[t] = SubRef{0, (po.B[t].subid / dout) * ok, (po.B[t].subid % dout) * on, po.B[t].coeff};
         p.h_outer_sums.push_back(sd);
     }
     // translate form of the outer sums (rows along K): shifts per term, and 2-row vector reads when
@@ -286,8 +288,34 @@ bool build_outer_staged(Plan& p)
     p.h_outer_shifted = p.h_outer_sums;
     translate_form(0, p.h_a_k, p.h_a_m, ok, om, p.h_batch_a[0], p.outer_shift_a, p.outer_pair_a);
     translate_form(1, p.h_b_k, p.h_b_n, ok, on, p.h_batch_b[0], p.outer_shift_b, p.outer_pair_b);
+    // one-pass form: source s = sub-block s (row-major) of the dout x dout split; A's rows (K) are its
+    // column blocks, B's rows (K) its row blocks
+    const int nsrc = int(dout * dout);
+    p.msum_nsrc = nsrc;
+    p.h_msum_shift_a.assign(size_t(nsrc), 0);
+    p.h_msum_shift_b.assign(size_t(nsrc), 0);
+    p.msum_a = p.msum_b = true;
+    for (int src = 0; src < nsrc; src++)
+    {
+        const int64_t rb = src / dout, cb = src % dout;
+        p.msum_a = p.msum_a && is_translate(p.h_a_k, cb * ok, ok) && is_translate(p.h_a_m, rb * om, om);
+        p.msum_b = p.msum_b && is_translate(p.h_b_k, rb * ok, ok) && is_translate(p.h_b_n, cb * on, on);
+        if (p.msum_a)
+            p.h_msum_shift_a[size_t(src)] =
+                (p.h_a_k[size_t(cb * ok)] - p.h_a_k[0]) + (p.h_a_m[size_t(rb * om)] - p.h_a_m[0]);
+        if (p.msum_b)
+            p.h_msum_shift_b[size_t(src)] =
+                (p.h_b_k[size_t(rb * ok)] - p.h_b_k[0]) + (p.h_b_n[size_t(cb * on)] - p.h_b_n[0]);
+    }
+    p.h_msum_coef_a.assign(one.size() * size_t(nsrc), 0.0);
+    p.h_msum_coef_b.assign(one.size() * size_t(nsrc), 0.0);
+    for (size_t o = 0; o < one.size(); o++)
+    {
+        for (const SubTerm& t : one[o].A) p.h_msum_coef_a[o * size_t(nsrc) + size_t(t.subid)] = t.coeff;
+        for (const SubTerm& t : one[o].B) p.h_msum_coef_b[o * size_t(nsrc) + size_t(t.subid)] = t.coeff;
+    }
     p.h_inner.clear();
-    const auto& inner = products_for_level(p.level - 1);
+    const auto& inner = products_for_level(p.level - depth);
     const int64_t din = f;  // inner split divisor
     for (size_t o = 0; o < one.size(); o++)
         for (size_t i = 0; i < inner.size(); i++)
@@ -320,27 +348,42 @@ bool build_outer_staged(Plan& p)
     return true;
 }
 
-// Whether an ABC plan (level 1 or 2) stages its outer sums: MISTRA_FLAG_FUSED_SUMS / MISTRA_FLAG_OUTER_STAGED
-// decide when given, else the cost model (the outer sums' HBM traffic against the slower multi-term leaf
-// launches of the fused form), for leaves of at least one register tile per CU.
+// Whether (and how deep) an ABC plan stages its operand sums: MISTRA_FLAG_FUSED_SUMS,
+// MISTRA_FLAG_OUTER_STAGED (the outer level) and MISTRA_FLAG_ALL_STAGED (every level) decide when given,
+// else the cost model (the staged sums' HBM traffic against the slower multi-term leaf launches), for
+// leaves of at least one register tile per CU.
 void choose_outer_staged(Plan& p, unsigned flags)
 {
     p.outer_staged = false;
+    p.staged_depth = 0;
     if (p.level < 1 || !p.core || p.impl != IMPL_ABC || p.nbatch != 1 || (flags & MISTRA_FLAG_FUSED_SUMS)) return;
-    if (!build_outer_staged(p)) return;
-    if (flags & MISTRA_FLAG_OUTER_STAGED)
+    auto set = [&](int depth) {
+        p.outer_staged = depth > 0 && build_outer_staged(p, depth);
+        if (!p.outer_staged) p.staged_depth = 0;
+    };
+    if (flags & (MISTRA_FLAG_OUTER_STAGED | MISTRA_FLAG_ALL_STAGED))
     {
-        p.outer_staged = true;
+        set(flags & MISTRA_FLAG_ALL_STAGED ? p.level : 1);
         return;
     }
     // small leaves: launches that do not fill the 256 CUs are latency-bound, so fewer sum terms gain
-    // little against the two extra sum launches and the workspace
+    // little against the extra sum launches and the workspace
     const int64_t te = tile_extent(p.tile);
     if (((p.ms + te - 1) / te) * ((p.ns + te - 1) / te) < 256) return;
-    const double fused = model_ms(p);
-    p.outer_staged = true;
-    const double staged = model_ms(p);
-    p.outer_staged = staged < fused;
+    int best = 0;
+    double best_ms = model_ms(p);
+    for (int depth = 1; depth <= p.level; depth++)
+    {
+        set(depth);
+        if (!p.outer_staged) break;
+        const double ms = model_ms(p);
+        if (ms < best_ms)
+        {
+            best_ms = ms;
+            best = depth;
+        }
+    }
+    set(best);
 }
 
 }  // namespace
@@ -693,6 +736,10 @@ Status upload(Plan& p)
          i_dp = add(p.h_dense_prods.data(), p.h_dense_prods.size() * sizeof(ProdDesc)),
          i_zero = add(nullptr, 256),
          i_os = add(p.h_outer_sums.data(), p.h_outer_sums.size() * sizeof(ProdDesc)),
+         i_mca = add(p.h_msum_coef_a.data(), p.h_msum_coef_a.size() * 8),
+         i_mcb = add(p.h_msum_coef_b.data(), p.h_msum_coef_b.size() * 8),
+         i_msa = add(p.h_msum_shift_a.data(), p.h_msum_shift_a.size() * 8),
+         i_msb = add(p.h_msum_shift_b.data(), p.h_msum_shift_b.size() * 8),
          i_osf = add(p.h_outer_shifted.data(), p.h_outer_shifted.size() * sizeof(ProdDesc)),
          i_ins = add(inner_staged.data(), inner_staged.size() * sizeof(ProdDesc)),
          i_osm = add(os_sm.data(), os_sm.size() * 8), i_osk = add(os_sk.data(), os_sk.size() * 8),
@@ -741,6 +788,10 @@ Status upload(Plan& p)
     p.dense_prods = reinterpret_cast<const ProdDesc*>(at(i_dp));
     p.zero = at(i_zero);
     p.outer_sums = reinterpret_cast<const ProdDesc*>(at(i_os));
+    p.msum_coef_a = reinterpret_cast<const double*>(at(i_mca));
+    p.msum_coef_b = reinterpret_cast<const double*>(at(i_mcb));
+    p.msum_shift_a = reinterpret_cast<const int64_t*>(at(i_msa));
+    p.msum_shift_b = reinterpret_cast<const int64_t*>(at(i_msb));
     p.outer_shifted = reinterpret_cast<const ProdDesc*>(at(i_osf));
     p.inner_staged = reinterpret_cast<const ProdDesc*>(at(i_ins));
     p.os_sm = reinterpret_cast<const int64_t*>(at(i_osm));
@@ -762,7 +813,9 @@ Status get_plan(const tblis::tblis_tensor* A, const label_type* idx_A, const tbl
     HIP_TRY(hipGetDevice(&dev));
     std::ostringstream ks;
     ks << dev << '|' << int(A->type) << '|' << level << '|' << impl << '|' << tile << '|'
-       << (flags & (MISTRA_FLAG_REF_FRINGE | MISTRA_FLAG_FUSED_SUMS | MISTRA_FLAG_OUTER_STAGED)) << '|';
+       << (flags & (MISTRA_FLAG_REF_FRINGE | MISTRA_FLAG_FUSED_SUMS | MISTRA_FLAG_OUTER_STAGED |
+                    MISTRA_FLAG_ALL_STAGED))
+       << '|';
     std::string key = ks.str();
     append_key(key, A, idx_A);
     append_key(key, B, idx_B);
@@ -900,8 +953,9 @@ namespace {
 // inner sums of S and T (2 terms at level 2, 1 at level 1) with the multi-C epilogue on C.
 hipError_t run_outer_staged(const Plan& p, const Call& c, hipStream_t stream)
 {
-    const size_t sbytes = size_t(7) * size_t(p.os_m) * size_t(p.os_k) * size_t(p.elem);
-    const size_t tbytes = size_t(7) * size_t(p.os_k) * size_t(p.os_n) * size_t(p.elem);
+    const size_t nouter = p.h_outer_sums.size();
+    const size_t sbytes = nouter * size_t(p.os_m) * size_t(p.os_k) * size_t(p.elem);
+    const size_t tbytes = nouter * size_t(p.os_k) * size_t(p.os_n) * size_t(p.elem);
     void *S = nullptr, *T = nullptr;
     hipError_t e = hipMallocAsync(&S, sbytes, stream);
     if (e != hipSuccess) return e;
@@ -911,41 +965,52 @@ hipError_t run_outer_staged(const Plan& p, const Call& c, hipStream_t stream)
         (void)hipFreeAsync(S, stream);
         return e;
     }
-    SumArgs sa;
-    std::memset(&sa, 0, sizeof(sa));
-    sa.X = c.A;
-    sa.zoff = p.h_batch_a[0];
-    sa.rs = p.a_k;
-    sa.cs = p.a_m;
-    sa.rows = p.os_k;
-    sa.cols = p.os_m;
-    sa.prods = p.outer_sums;
-    sa.nprod = 7;
-    sa.which = 0;
-    sa.rows_fast = !p.a_mfast;
-    sa.out = S;
-    // translate form: shifted bases; pairs also need a 2-element-aligned data pointer
+    // sums of one operand: one pass over its source sub-blocks when they are translates and K (the rows
+    // of the staged buffers) has unit stride, else one gather pass per sum
     const uintptr_t align = uintptr_t(2 * p.elem) - 1;
-    sa.shifted = p.outer_shift_a && sa.rows_fast;  // the translate-form kernel reads along rows only
-    sa.pair = p.outer_pair_a && (reinterpret_cast<uintptr_t>(c.A) & align) == 0;
-    if (sa.shifted) sa.prods = p.outer_shifted;
-    e = launch_operand_sums(p.elem, sa, stream);
-    if (e == hipSuccess)
-    {
-        sa.X = c.B;
-        sa.zoff = p.h_batch_b[0];
-        sa.rs = p.b_k;
-        sa.cs = p.b_n;
+    auto sums = [&](int which) {
+        const bool isA = which == 0;
+        const void* X = isA ? c.A : c.B;
+        const bool rows_fast = isA ? !p.a_mfast : !p.b_nfast;
+        const bool pair = (isA ? p.outer_pair_a : p.outer_pair_b) && (reinterpret_cast<uintptr_t>(X) & align) == 0;
+        if ((isA ? p.msum_a : p.msum_b) && rows_fast)
+        {
+            MultiSumArgs ma;
+            std::memset(&ma, 0, sizeof(ma));
+            ma.X = X;
+            ma.zoff = isA ? p.h_batch_a[0] : p.h_batch_b[0];
+            ma.rs = isA ? p.a_k : p.b_k;
+            ma.cs = isA ? p.a_m : p.b_n;
+            ma.rows = p.os_k;
+            ma.cols = isA ? p.os_m : p.os_n;
+            ma.nsrc = p.msum_nsrc;
+            ma.nout = int(nouter);
+            ma.coef = isA ? p.msum_coef_a : p.msum_coef_b;
+            ma.shift = isA ? p.msum_shift_a : p.msum_shift_b;
+            ma.out = isA ? S : T;
+            ma.pair = pair;
+            return launch_operand_multisums(p.elem, ma, stream);
+        }
+        SumArgs sa;
+        std::memset(&sa, 0, sizeof(sa));
+        sa.X = X;
+        sa.zoff = isA ? p.h_batch_a[0] : p.h_batch_b[0];
+        sa.rs = isA ? p.a_k : p.b_k;
+        sa.cs = isA ? p.a_m : p.b_n;
         sa.rows = p.os_k;
-        sa.cols = p.os_n;
-        sa.which = 1;
-        sa.rows_fast = !p.b_nfast;
-        sa.out = T;
-        sa.shifted = p.outer_shift_b && sa.rows_fast;
-        sa.pair = p.outer_pair_b && (reinterpret_cast<uintptr_t>(c.B) & align) == 0;
+        sa.cols = isA ? p.os_m : p.os_n;
+        sa.nprod = int(nouter);
+        sa.which = which;
+        sa.rows_fast = rows_fast;
+        sa.out = isA ? S : T;
+        // translate form (shifted bases) only in the kernel that reads along rows
+        sa.shifted = (isA ? p.outer_shift_a : p.outer_shift_b) && rows_fast;
+        sa.pair = pair;
         sa.prods = sa.shifted ? p.outer_shifted : p.outer_sums;
-        e = launch_operand_sums(p.elem, sa, stream);
-    }
+        return launch_operand_sums(p.elem, sa, stream);
+    };
+    e = sums(0);
+    if (e == hipSuccess) e = sums(1);
     GemmArgs g;
     std::memset(&g, 0, sizeof(g));
     g.A = S;

@@ -88,17 +88,26 @@ struct Plan
     // fast-K eligibility of the core launch and of the Naive dense launch
     KFast kf_core, kf_dense;
 
-    // ABC with the outer level's operand sums staged: the 7 outer sums of A and of B are written as
-    // dense K-fast buffers S (7 x os_m x os_k) and T (7 x os_k x os_n), and the stage launches read the
-    // inner sums of those (2 terms at level 2, a single term at level 1) instead of 4- (2-) term sums of
-    // A and B.  C is still updated in the leaf kernel's epilogue, in the same stages and order as the
+    // ABC with operand sums staged: the sums of the first staged_depth levels (7 or 49 of them) of A and
+    // of B are written as dense K-fast buffers S (n x os_m x os_k) and T (n x os_k x os_n), and the stage
+    // launches read the inner sums of those (2 terms when only the outer level of 2 is staged, else a
+    // single term) instead of 4- (2-) term sums of A and B.  C is still updated in the leaf kernel's epilogue, in the same stages and order as the
     // fused form.
     bool outer_staged = false;
-    int64_t os_m = 0, os_n = 0, os_k = 0;  // outer sub-block extents (m/2, n/2, k/2 rounded to leaves)
+    int staged_depth = 0;                   // levels whose sums are staged (1: the outer 7; 2: all 49)
+    int64_t os_m = 0, os_n = 0, os_k = 0;  // staged sub-block extents (m / 2^depth, ... rounded to leaves)
     std::vector<ProdDesc> h_outer_sums;    // sum-kernel tables: A with rows = K, cols = M; B rows = K, cols = N
     std::vector<ProdDesc> h_outer_shifted; // the same with term bases = shifts from the first sub-block
     bool outer_shift_a = false, outer_shift_b = false;  // every outer term of A (B) is a translate
     bool outer_pair_a = false, outer_pair_b = false;    // rows in contiguous, even-aligned pairs
+    // one-pass sums (launch_operand_multisums) when every source sub-block of A (B) is a translate:
+    // coefficient matrix [nout][nsrc] over the sub-blocks of the staged split, and their shifts
+    bool msum_a = false, msum_b = false;
+    int msum_nsrc = 0;
+    std::vector<double> h_msum_coef_a, h_msum_coef_b;
+    std::vector<int64_t> h_msum_shift_a, h_msum_shift_b;
+    const double *msum_coef_a = nullptr, *msum_coef_b = nullptr;
+    const int64_t *msum_shift_a = nullptr, *msum_shift_b = nullptr;
     std::vector<ProdDesc> h_inner;         // the products over S / T in table order (fast-body form)
     std::vector<int> h_inner_stage_terms;
     KFast kf_inner;

@@ -64,6 +64,26 @@ struct SumArgs
 };
 hipError_t launch_operand_sums(int elem_size, const SumArgs& a, hipStream_t stream);
 
+// All staged sums of one operand in one pass: every source sub-block (nsrc = 4 or 16, a translate of
+// the first: element (r, c) of source s at X[shift[s] + rs[r] + cs[c]]) is read once per element, and
+//   out[q*rows*cols + r + c*rows] = sum_s coef[q*nsrc + s] * source_s(r, c)   for q < nout,
+// skipping zero coefficients.  rs has unit stride; `pair`: rows come in contiguous pairs at even
+// element offsets (one vector load per source).
+struct MultiSumArgs
+{
+    const void* X;
+    int64_t zoff;
+    const int64_t* rs;
+    const int64_t* cs;
+    int64_t rows, cols;
+    int nsrc, nout;
+    const double* coef;
+    const int64_t* shift;
+    void* out;
+    bool pair;
+};
+hipError_t launch_operand_multisums(int elem_size, const MultiSumArgs& a, hipStream_t stream);
+
 // Combine staged products into C sub-blocks:
 //   C[c_m[roff_s + r] + c_n[coff_s + c]] = beta*C + alpha * sum_{(p,coef) in list_s} coef*M_p[r + c*ms]
 struct CombineTerm

@@ -23,6 +23,7 @@ FLAG_SYNC = 2
 FLAG_GRAPH = 4
 FLAG_FUSED_SUMS = 8
 FLAG_OUTER_STAGED = 16
+FLAG_ALL_STAGED = 32
 
 
 class MistraError(RuntimeError):

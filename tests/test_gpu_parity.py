@@ -245,14 +245,14 @@ def test_errors_are_reported():
         ts.tensor_mult(1.0, A, "ik", Bf, "kj", 1.0, C, "ij", level=1)
 
 
-# ---- ABC with the outer level's operand sums staged (MISTRA_FLAG_OUTER_STAGED) ---------------------
+# ---- ABC with staged operand sums (MISTRA_FLAG_OUTER_STAGED, MISTRA_FLAG_ALL_STAGED) ---------------
 # Same bars as the fused form: bit-exact on small integers, 1e-13 against the oracle's 2-level ABC on
 # uniform inputs; layouts that transpose the plan, strided operands, fringes, alpha/beta with beta = 0
 # over a NaN-filled C.  The batched case (nbatch > 1) does not apply and runs fused.
 
 @pytest.mark.parametrize("ci", range(len(CASES)))
 @pytest.mark.parametrize("level", [1, 2])
-@pytest.mark.parametrize("flags", [ts.FLAG_OUTER_STAGED, ts.FLAG_FUSED_SUMS])
+@pytest.mark.parametrize("flags", [ts.FLAG_OUTER_STAGED, ts.FLAG_ALL_STAGED, ts.FLAG_FUSED_SUMS])
 def test_outer_staged_exact(ci, level, flags):
     case = cases.make_case(10 + ci, **CASES[ci])
     say(f"[outer int] case={ci} L={level} flags={flags}")
@@ -263,42 +263,46 @@ def test_outer_staged_exact(ci, level, flags):
 
 @pytest.mark.parametrize("ci", range(len(CASES)))
 @pytest.mark.parametrize("level", [1, 2])
-def test_outer_staged_float(ci, level):
+@pytest.mark.parametrize("flags", [ts.FLAG_OUTER_STAGED, ts.FLAG_ALL_STAGED])
+def test_outer_staged_float(ci, level, flags):
     case = cases.make_case(20 + ci, **CASES[ci])
-    say(f"[outer f64] case={ci} L={level}")
+    say(f"[outer f64] case={ci} L={level} flags={flags}")
     a, b, c = inputs(case, 200 + ci)
-    got = run_gpu(case, a, b, c, 1.0, 1.0, level, "abc", flags=ts.FLAG_OUTER_STAGED)
+    got = run_gpu(case, a, b, c, 1.0, 1.0, level, "abc", flags=flags)
     ref = run_oracle(case, a, b, c, 1.0, 1.0, level=level, impl="abc")
     assert cases.rel_err(got, ref) <= 1e-13
 
 
 @pytest.mark.parametrize("level", [1, 2])
 @pytest.mark.parametrize("alpha,beta", [(0.5, 0.0), (-2.0, 3.0), (1.0, -1.0)])
-def test_outer_staged_alpha_beta(level, alpha, beta):
+@pytest.mark.parametrize("flags", [ts.FLAG_OUTER_STAGED, ts.FLAG_ALL_STAGED])
+def test_outer_staged_alpha_beta(level, alpha, beta, flags):
     case = cases.make_case(32, m=(8, 8), n=(16, 4), k=(4, 16))
-    say(f"[outer alpha/beta] L={level} alpha={alpha} beta={beta}")
+    say(f"[outer alpha/beta] L={level} alpha={alpha} beta={beta} flags={flags}")
     a, b, c = inputs(case, 300)
     if beta == 0.0:
         c = np.full_like(c, np.nan)
-    got = run_gpu(case, a, b, c, alpha, beta, level, "abc", flags=ts.FLAG_OUTER_STAGED)
+    got = run_gpu(case, a, b, c, alpha, beta, level, "abc", flags=flags)
     want = einsum(case, a, b, np.nan_to_num(c) if beta == 0.0 else c, alpha, beta)
     assert cases.rel_err(got, want) <= 1e-13
 
 
 @pytest.mark.parametrize("level", [1, 2])
-def test_outer_staged_fringes(level):
+@pytest.mark.parametrize("flags", [ts.FLAG_OUTER_STAGED, ts.FLAG_ALL_STAGED])
+def test_outer_staged_fringes(level, flags):
     case = cases.make_case(52, m=(37,), n=(3, 9), k=(45,))
-    say(f"[outer fringe] L={level}")
+    say(f"[outer fringe] L={level} flags={flags}")
     a, b, c = inputs(case, 500, ints=True)
-    got = run_gpu(case, a, b, c, 1.0, 1.0, level, "abc", flags=ts.FLAG_OUTER_STAGED)
+    got = run_gpu(case, a, b, c, 1.0, 1.0, level, "abc", flags=flags)
     assert np.array_equal(got, einsum(case, a, b, c, 1.0, 1.0))
 
 
 @pytest.mark.parametrize("level", [1, 2])
 @pytest.mark.parametrize("layout", ["c_rowmajor", "a_kfast", "b_nfast", "strided"])
-def test_outer_staged_layouts(layout, level):
+@pytest.mark.parametrize("flags", [ts.FLAG_OUTER_STAGED, ts.FLAG_ALL_STAGED])
+def test_outer_staged_layouts(layout, level, flags):
     m, n, k = 200, 168, 136
-    say(f"[outer layout] {layout} L={level}")
+    say(f"[outer layout] {layout} L={level} flags={flags}")
     a = cases.small_ints(m * k * 2, 11)
     b = cases.small_ints(k * n * 2, 12)
     c = cases.small_ints(m * n * 2, 13)
@@ -315,5 +319,5 @@ def test_outer_staged_layouts(layout, level):
     B = torch.from_numpy(b).cuda().as_strided((k, n), sB)
     C = torch.from_numpy(c).cuda().as_strided((m, n), sC)
     want = A.cpu().numpy() @ B.cpu().numpy() + C.cpu().numpy()
-    ts.tensor_mult(1.0, A, "ik", B, "kj", 1.0, C, "ij", level=level, flags=ts.FLAG_SYNC | ts.FLAG_OUTER_STAGED)
+    ts.tensor_mult(1.0, A, "ik", B, "kj", 1.0, C, "ij", level=level, flags=ts.FLAG_SYNC | flags)
     assert np.array_equal(C.cpu().numpy(), want)
