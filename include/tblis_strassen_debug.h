@@ -13,9 +13,9 @@ namespace tblis {
 extern "C" {
 #endif
 
-/* dims[0..14] = m, n, k, nbatch, pure, transposed, ms, ns, ks, nprod (core products), nfringe,
- *               a_mfast, b_nfast, staged_depth (ABC levels whose operand sums are staged; 0: fused),
- *               number of ABC launch stages.
+/* dims[0..15] = m, n, k, nbatch, pure, transposed, ms, ns, ks, nprod (core products), nfringe,
+ *               a_mfast, b_nfast, staged_depth (levels whose operand sums are staged), number of ABC
+ *               launch stages, staged (1: operand sums, or at level 0 packed copies, staged in HBM).
  * scat[6*cap]  = a_m, a_k, b_k, b_n, c_m, c_n (in the plan's own orientation), -1 padded.
  * prods[cap_prods*40] per product (core then fringe): nA, nB, nC, first_mask, then 4 x (roff, coff,
  *               coeff) for A, B and C (-1, -1, 0 padded).  Returns the number of products or < 0. */

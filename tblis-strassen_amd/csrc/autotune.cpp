@@ -40,23 +40,23 @@ std::string shape_key(const tblis::tblis_tensor* A, const label_type* idx_A, con
 // Leaf-kernel rates (TFLOP/s) measured on MI355X with the 8192^3 FP64 workload, by register tile and
 // number of sum terms (1, 2, 4; single-term launches over K-fast operands load K-tiles straight into
 // LDS); FP32 runs the same kernels at about twice the rate.
-double leaf_rate(int elem, int tile, int terms)
+double leaf_rate(int elem, int tile, int terms, bool dma)
 {
-    static const double r128[3] = {66.0, 53.5, 45.0}, r64[3] = {57.0, 46.0, 38.0};
+    static const double r128[3] = {58.5, 53.5, 45.0}, r64[3] = {57.0, 46.0, 38.0};
     const int ti = terms <= 1 ? 0 : (terms == 2 ? 1 : 2);
-    const double r = tile == TILE_64 ? r64[ti] : r128[ti];
+    const double r = tile == TILE_64 ? r64[ti] : (ti == 0 && dma ? 66.0 : r128[ti]);
     return elem == 8 ? r : 1.8 * r;
 }
 
 constexpr double kHbm = 2.7e12;       // bytes/s of the sum / combine kernels (measured: strided gathers)
 constexpr double kLaunch = 4.0e-6;    // seconds per launch (dispatch and tail)
 
-double launch_s(int elem, int tile, int terms, int64_t tiles, double flops)
+double launch_s(int elem, int tile, int terms, bool dma, int64_t tiles, double flops)
 {
     if (tiles <= 0 || flops <= 0) return 0.0;
     const int64_t slots = tile == TILE_64 ? 1024 : 512;  // resident workgroups on 256 CUs
     const double waves = std::ceil(double(tiles) / double(slots)) * double(slots) / double(tiles);
-    return flops / (leaf_rate(elem, tile, terms) * 1e12) * waves + kLaunch;
+    return flops / (leaf_rate(elem, tile, terms, dma) * 1e12) * waves + kLaunch;
 }
 
 int64_t ntiles(int tile, int64_t ms, int64_t ns)
@@ -74,16 +74,23 @@ double model_ms(const Plan& p0)
     const int e = p.elem;
     double s = 0.0;
     const int64_t nb = p.nbatch;
+    // single-term launches over K-fast operands (both K-fast, or staged copies) load straight into LDS
+    const bool kfast = p.outer_staged || (!p.a_mfast && !p.b_nfast);
     auto gemm = [&](int64_t ms, int64_t ns, int64_t ks, int nprod, int terms, int t) {
         // work of whole register tiles (a partly filled tile costs as much as a full one)
         const int64_t nt = ntiles(t, ms, ns) * nprod * nb;
         const double te = double(tile_extent(t));
-        return launch_s(e, t, terms, nt, 2.0 * te * te * double(ks) * double(nt));
+        return launch_s(e, t, terms, kfast, nt, 2.0 * te * te * double(ks) * double(nt));
     };
     for (const Pass& f : p.fringe) s += gemm(f.ms, f.ns, f.ks, 1, 1, tile);
     if (p.nprod == 0) return s * 1e3;
     if (!p.core)
-        return (s + gemm(p.ms, p.ns, p.ks, 1, 1, tile)) * 1e3;
+    {
+        s += gemm(p.ms, p.ns, p.ks, 1, 1, tile);
+        if (p.outer_staged)  // packed copies of A and B: read and written once each
+            s += 2.0 * (double(p.ms) * p.ks + double(p.ks) * p.ns) * e / kHbm + 2 * kLaunch;
+        return s * 1e3;
+    }
     const double cbytes = double(p.ms) * p.ns * e * nb;
     if (p.impl == IMPL_AB)
     {

@@ -174,7 +174,12 @@ hipError_t multisums_typed(const MultiSumArgs& a, hipStream_t s)
     const int64_t blocks = ((a.rows + 256 * rpt - 1) / (256 * rpt)) * a.cols;
     if (blocks == 0 || a.nout == 0) return hipSuccess;
     const dim3 grid(static_cast<unsigned>(blocks));
-    if (a.nsrc == 4)
+    if (a.nsrc == 1)
+    {
+        if (a.pair) hipLaunchKernelGGL((operand_multisums_kernel<T, true, 1>), grid, dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((operand_multisums_kernel<T, false, 1>), grid, dim3(256), 0, s, a);
+    }
+    else if (a.nsrc == 4)
     {
         if (a.pair) hipLaunchKernelGGL((operand_multisums_kernel<T, true, 4>), grid, dim3(256), 0, s, a);
         else hipLaunchKernelGGL((operand_multisums_kernel<T, false, 4>), grid, dim3(256), 0, s, a);

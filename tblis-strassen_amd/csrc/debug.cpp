@@ -17,9 +17,9 @@ int mistra_debug_plan(const tblis_tensor* A, const label_type* idx_A, const tbli
     if (!st.ok()) return st.code;
     std::vector<int> order;
     mistra::make_stages(p, order);
-    const long long d[15] = {p.m, p.n, p.k, p.nbatch, p.general ? 0 : 1, p.transposed, p.ms, p.ns, p.ks,
+    const long long d[16] = {p.m, p.n, p.k, p.nbatch, p.general ? 0 : 1, p.transposed, p.ms, p.ns, p.ks,
                              p.nprod, (long long)p.fringe.size(), p.a_mfast, p.b_nfast, p.staged_depth,
-                             (long long)p.h_stage_begin.size() - 1};
+                             (long long)p.h_stage_begin.size() - 1, p.outer_staged};
     std::memcpy(dims, d, sizeof(d));
     const std::vector<int64_t>* v[6] = {&p.h_a_m, &p.h_a_k, &p.h_b_k, &p.h_b_n, &p.h_c_m, &p.h_c_n};
     for (int i = 0; i < 6; i++)

@@ -348,31 +348,41 @@ bool build_outer_staged(Plan& p, int depth)
     return true;
 }
 
-// Whether (and how deep) an ABC plan stages its operand sums: MISTRA_FLAG_FUSED_SUMS,
-// MISTRA_FLAG_OUTER_STAGED (the outer level) and MISTRA_FLAG_ALL_STAGED (every level) decide when given,
-// else the cost model (the staged sums' HBM traffic against the slower multi-term leaf launches), for
-// leaves of at least one register tile per CU.
+// Whether (and how deep) a plan stages its operand sums: MISTRA_FLAG_FUSED_SUMS, MISTRA_FLAG_OUTER_STAGED
+// (the outer level; at level 0 the packed copies) and MISTRA_FLAG_ALL_STAGED (every level) decide when
+// given, else the cost model (the staged sums' HBM traffic against the slower multi-term or
+// register-staged leaf launches), for leaves of at least one register tile per CU.
 void choose_outer_staged(Plan& p, unsigned flags)
 {
     p.outer_staged = false;
     p.staged_depth = 0;
-    if (p.level < 1 || !p.core || p.impl != IMPL_ABC || p.nbatch != 1 || (flags & MISTRA_FLAG_FUSED_SUMS)) return;
+    if (p.nprod == 0 || p.general || p.nbatch != 1 || (flags & MISTRA_FLAG_FUSED_SUMS)) return;
+    // Level 0: staging at depth 0 packs A and B into K-fast dense copies (for the direct-to-LDS leaf
+    // path), worth it only when an operand is not K-fast.  Levels 1, 2: ABC only.
+    int lo = 1, hi = p.level;
+    if (p.level == 0)
+    {
+        if (!p.a_mfast && !p.b_nfast) return;
+        lo = hi = 0;
+    }
+    else if (!p.core || p.impl != IMPL_ABC)
+        return;
     auto set = [&](int depth) {
-        p.outer_staged = depth > 0 && build_outer_staged(p, depth);
+        p.outer_staged = depth >= 0 && build_outer_staged(p, depth);
         if (!p.outer_staged) p.staged_depth = 0;
     };
     if (flags & (MISTRA_FLAG_OUTER_STAGED | MISTRA_FLAG_ALL_STAGED))
     {
-        set(flags & MISTRA_FLAG_ALL_STAGED ? p.level : 1);
+        set(p.level == 0 ? 0 : (flags & MISTRA_FLAG_ALL_STAGED ? p.level : 1));
         return;
     }
     // small leaves: launches that do not fill the 256 CUs are latency-bound, so fewer sum terms gain
     // little against the extra sum launches and the workspace
     const int64_t te = tile_extent(p.tile);
     if (((p.ms + te - 1) / te) * ((p.ns + te - 1) / te) < 256) return;
-    int best = 0;
+    int best = -1;
     double best_ms = model_ms(p);
-    for (int depth = 1; depth <= p.level; depth++)
+    for (int depth = lo; depth <= hi; depth++)
     {
         set(depth);
         if (!p.outer_staged) break;

@@ -246,12 +246,13 @@ def test_errors_are_reported():
 
 
 # ---- ABC with staged operand sums (MISTRA_FLAG_OUTER_STAGED, MISTRA_FLAG_ALL_STAGED) ---------------
-# Same bars as the fused form: bit-exact on small integers, 1e-13 against the oracle's 2-level ABC on
-# uniform inputs; layouts that transpose the plan, strided operands, fringes, alpha/beta with beta = 0
-# over a NaN-filled C.  The batched case (nbatch > 1) does not apply and runs fused.
+# Same bars as the fused form: bit-exact on small integers, 1e-13 against the oracle's ABC on uniform
+# inputs; layouts that transpose the plan, strided operands, fringes, alpha/beta with beta = 0 over a
+# NaN-filled C.  At level 0 the flags pack A and B into K-fast copies.  The batched case (nbatch > 1)
+# does not apply and runs fused.
 
 @pytest.mark.parametrize("ci", range(len(CASES)))
-@pytest.mark.parametrize("level", [1, 2])
+@pytest.mark.parametrize("level", [0, 1, 2])
 @pytest.mark.parametrize("flags", [ts.FLAG_OUTER_STAGED, ts.FLAG_ALL_STAGED, ts.FLAG_FUSED_SUMS])
 def test_outer_staged_exact(ci, level, flags):
     case = cases.make_case(10 + ci, **CASES[ci])
@@ -297,7 +298,7 @@ def test_outer_staged_fringes(level, flags):
     assert np.array_equal(got, einsum(case, a, b, c, 1.0, 1.0))
 
 
-@pytest.mark.parametrize("level", [1, 2])
+@pytest.mark.parametrize("level", [0, 1, 2])
 @pytest.mark.parametrize("layout", ["c_rowmajor", "a_kfast", "b_nfast", "strided"])
 @pytest.mark.parametrize("flags", [ts.FLAG_OUTER_STAGED, ts.FLAG_ALL_STAGED])
 def test_outer_staged_layouts(layout, level, flags):

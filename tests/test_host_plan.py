@@ -67,7 +67,7 @@ def _debug_plan(case, level, flags=0, strides=None, cap=8192):
         t.stride = ctypes.cast(t._stride, ctypes.POINTER(ctypes.c_ssize_t))
         t.data = None
         tensors.append(t)
-    dims = (ctypes.c_longlong * 15)()
+    dims = (ctypes.c_longlong * 16)()
     scat = (ctypes.c_longlong * (6 * cap))()
     prods = (ctypes.c_double * (64 * 40))()
     np_ = lib.mistra_debug_plan(ctypes.byref(tensors[0]), case["idx_A"].encode(), ctypes.byref(tensors[1]),
@@ -268,3 +268,18 @@ def test_outer_staged_choice():
     assert _debug_plan(small, 2, flags=ts.FLAG_ALL_STAGED)[1][13] == 2
     assert _debug_plan(small, 1, flags=ts.FLAG_ALL_STAGED)[1][13] == 1
     assert _debug_plan(batched, 2, flags=ts.FLAG_OUTER_STAGED)[1][13] == 0
+    assert _debug_plan(batched, 2, flags=ts.FLAG_OUTER_STAGED)[1][15] == 0
+
+
+def test_packed_classical_choice():
+    # level 0: A and B are packed into K-fast copies when an operand is not K-fast and the model says
+    # so (8192^3 with a column-major B, i.e. N-fast after the plan's orientation), never when both are
+    # K-fast already, and on request for small shapes
+    big = cases.make_case(9, m=(8192,), n=(8192,), k=(8192,))
+    strides = {"A": [8192, 1], "B": [8192, 1], "C": [1, 8192]}  # A: k fast; B[k][n]: n fast; C column-major
+    assert _debug_plan(big, 0, strides=strides)[1][15] == 1
+    kfast = {"A": [8192, 1], "B": [1, 8192], "C": [1, 8192]}
+    assert _debug_plan(big, 0, strides=kfast)[1][15] == 0
+    small = cases.make_case(9, m=(64,), n=(64,), k=(64,))
+    assert _debug_plan(small, 0, flags=ts.FLAG_OUTER_STAGED)[1][15] == 1
+    assert _debug_plan(small, 0, flags=ts.FLAG_FUSED_SUMS)[1][15] == 0
