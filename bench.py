@@ -22,9 +22,9 @@ cpu_baseline: the CPU oracle (a port of the reference algorithm) on a bounded sa
 Other workloads (one JSON line each, same fields; --workload NAME):
   c1  the same measurement at m=n=k=1024 (2 indices of 32 per group);
   c3  precision modes at m=n=16384, k=1024 (groups 128x128, 128x128, 32x32; A, B uniform times 2^u,
-      u in [-8, 8]): FP64 classical and Strassen, BF16 hi/lo split, Ozaki BF16 slicing and FP8 block
-      scaling, each with its error against the FP64 classical result; value = the fastest mode within
-      1e-10;
+      u in [-8, 8]): FP64 classical and Strassen, BF16 hi/lo split, Ozaki BF16 and INT8 slicing and FP8
+      block scaling, each with its error against the FP64 classical result; value = the fastest mode
+      within 1e-10;
   c4  256 seeded FP32 contractions of uneven shape (tests/cases.py:c4_batch): a loop of individual
       calls, the grouped batch, and its HIP-graph replay; value = the batch throughput;
   c5  C[abce] = A[dcaf] B[dfbe], all lengths 64, FP32: forward + dA + dB with 1-level ABC (and the
@@ -151,7 +151,7 @@ def common(args, metric, value, unit, ms, dtype, config, roofline, extra=None):
     return out
 
 
-PEAK = {"f64": 78.6, "f32": 157.3, "bf16": 2500.0, "fp8": 5000.0}
+PEAK = {"f64": 78.6, "f32": 157.3, "bf16": 2500.0, "fp8": 5000.0, "int8": 5000.0}
 
 
 def bench_c3(ts, torch, args, stream):
@@ -171,7 +171,8 @@ def bench_c3(ts, torch, args, stream):
     C, cbuf = device_tensor(case["shape_C"], torch.float64, torch)
     log(f"workload c3: m={m} n={n} k={k}")
     modes = [("fp64", 0), ("fp64", 1), ("fp64", 2), ("bf16x3", 1), ("bf16x3", 2), ("ozaki_bf16", 0),
-             ("ozaki_bf16", 1), ("ozaki_bf16", 2), ("fp8_block", 1)]
+             ("ozaki_bf16", 1), ("ozaki_bf16", 2), ("ozaki_int8", 0), ("ozaki_int8", 1), ("ozaki_int8", 2),
+             ("fp8_block", 1)]
     res = {}
     for prec, level in modes:
         pr = "native" if prec == "fp64" else prec
@@ -188,7 +189,7 @@ def bench_c3(ts, torch, args, stream):
         kms, launches, kflops = ts.kernel_timing_read(reset=True)
         ts.kernel_timing(False)
         name = f"{prec}_L{level}"
-        peak = PEAK["f64"] if prec == "fp64" else (PEAK["fp8"] if prec == "fp8_block" else PEAK["bf16"])
+        peak = {"fp64": PEAK["f64"], "fp8_block": PEAK["fp8"], "ozaki_int8": PEAK["int8"]}.get(prec, PEAK["bf16"])
         leaf = kflops / kms / 1e9 if kms else None
         res[name] = {"precision": prec, "level": level, "ms_per_step": round(ms, 4),
                      "eff_tflops": round(eff / ms / 1e9, 3), "rel_err_vs_fp64": err,

@@ -117,8 +117,11 @@ enum
                                    3 products, FP32 accumulate, FP64 combine                          */
     MISTRA_PREC_OZAKI_BF16 = 2, /* double only: Ozaki BF16 digit slicing (5 digits, 15 products; env
                                    MISTRA_OZAKI_SLICES = 2..7), exact FP32 chunks, FP64 sum           */
-    MISTRA_PREC_FP8_BLOCK  = 3  /* double only: FP8 E4M3 with one scale per row (column) and 128-long
+    MISTRA_PREC_FP8_BLOCK  = 3, /* double only: FP8 E4M3 with one scale per row (column) and 128-long
                                    K block, FP32 accumulate within a block, FP64 across blocks        */
+    MISTRA_PREC_OZAKI_INT8 = 4  /* double only: Ozaki INT8 digit slicing (6 digits of 7 bits, 21
+                                   products; env MISTRA_OZAKI_I8_SLICES = 2..6), exact INT32
+                                   accumulation, FP64 sum                                             */
 };
 
 /* Flags for mistra_tensor_mult. */

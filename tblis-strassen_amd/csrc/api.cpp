@@ -175,6 +175,18 @@ int ozaki_slices()
     return n;
 }
 
+// Digit slices of the INT8 Ozaki mode: 6 (21 products, about 41 bits) unless MISTRA_OZAKI_I8_SLICES
+// says 2..6.
+int ozaki_i8_slices()
+{
+    static const int n = [] {
+        const char* v = std::getenv("MISTRA_OZAKI_I8_SLICES");
+        const int x = v ? std::atoi(v) : 6;
+        return x >= 2 && x <= 6 ? x : 6;
+    }();
+    return n;
+}
+
 }  // namespace
 
 void timing_begin(hipStream_t s)
@@ -401,7 +413,7 @@ Status tensor_mult(const tblis::tblis_tensor* A, const label_type* idx_A, const 
 {
     Status st = validate(A, idx_A, B, idx_B, C, idx_C);
     if (!st.ok()) return st;
-    if (precision < MISTRA_PREC_NATIVE || precision > MISTRA_PREC_FP8_BLOCK)
+    if (precision < MISTRA_PREC_NATIVE || precision > MISTRA_PREC_OZAKI_INT8)
         return Status::error(MISTRA_ERR_ARG, "unknown precision mode");
     if (precision != MISTRA_PREC_NATIVE && C->type != tblis::TYPE_DOUBLE)
         return Status::error(MISTRA_ERR_UNSUPPORTED, "low-precision modes take FP64 tensors");
@@ -502,7 +514,10 @@ Status tensor_mult(const tblis::tblis_tensor* A, const label_type* idx_A, const 
                     st = run_plan(*plan, call, stream);
                 else
                     st = run_plan_lowp(*plan, call, precision,
-                                       precision == MISTRA_PREC_BF16X3 ? 2 : ozaki_slices(), stream);
+                                       precision == MISTRA_PREC_BF16X3       ? 2
+                                       : precision == MISTRA_PREC_OZAKI_INT8 ? ozaki_i8_slices()
+                                                                             : ozaki_slices(),
+                                       stream);
             }
         }
         if (!st.ok())

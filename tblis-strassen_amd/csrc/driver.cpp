@@ -1253,14 +1253,16 @@ Status run_plan_lowp(const Plan& p, const Call& c, int precision, int nslices, h
     if (p.general) return Status::error(MISTRA_ERR_UNSUPPORTED, "low-precision modes need a pure contraction");
     if (precision != MISTRA_PREC_FP8_BLOCK && (nslices < 2 || nslices > 7))
         return Status::error(MISTRA_ERR_ARG, "Ozaki slice count must be in 2..7");
+    if (precision == MISTRA_PREC_OZAKI_INT8 && nslices > 6)
+        return Status::error(MISTRA_ERR_ARG, "INT8 Ozaki slice count must be in 2..6");
     if (p.nprod > 0)
     {
-        const bool fp8 = precision == MISTRA_PREC_FP8_BLOCK;
+        const bool fp8 = precision == MISTRA_PREC_FP8_BLOCK, i8 = precision == MISTRA_PREC_OZAKI_INT8;
         const int64_t ms = p.ms, ns = p.ns, ks = p.ks, nq = p.nprod;
         auto up = [](int64_t x, int64_t m) { return (x + m - 1) / m * m; };
         const int64_t mp = up(ms, 128), np = up(ns, 64), kp = up(ks, fp8 ? 128 : 64);
         const int nsl = fp8 ? 1 : nslices;
-        const size_t esz = fp8 ? 1 : 2;
+        const size_t esz = fp8 || i8 ? 1 : 2;
         struct Buf
         {
             void* p = nullptr;
@@ -1337,10 +1339,11 @@ Status run_plan_lowp(const Plan& p, const Call& c, int precision, int nslices, h
             else
             {
                 e = launch_lowp_exponents(la, stream);
-                if (e == hipSuccess) e = launch_ozaki_slices(nsl, la, stream);
+                if (e == hipSuccess) e = i8 ? launch_ozaki_i8_slices(nsl, la, stream) : launch_ozaki_slices(nsl, la, stream);
                 if (e != hipSuccess) break;
                 timing_begin(stream);
-                e = launch_ozaki_gemm(nsl, XA.p, XB.p, la.ea, la.eb, static_cast<double*>(M.p), lg, stream);
+                e = i8 ? launch_ozaki_i8_gemm(nsl, XA.p, XB.p, la.ea, la.eb, static_cast<double*>(M.p), lg, stream)
+                       : launch_ozaki_gemm(nsl, XA.p, XB.p, la.ea, la.eb, static_cast<double*>(M.p), lg, stream);
                 timing_end(stream, 2.0 * double(nq) * double(mp) * double(np) * double(kp) * (nsl * (nsl + 1) / 2));
             }
             if (e != hipSuccess) break;

@@ -1,4 +1,5 @@
-// Low-precision leaf products (lowp_kernels.hip): Ozaki BF16 digit slicing and FP8 E4M3 block scaling.
+// Low-precision leaf products (lowp_kernels.hip): Ozaki BF16 and INT8 digit slicing and FP8 E4M3 block
+// scaling.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -38,6 +39,9 @@ hipError_t launch_lowp_exponents(const LowpSliceArgs& a, hipStream_t s);
 hipError_t launch_ozaki_slices(int nslices, const LowpSliceArgs& a, hipStream_t s);
 hipError_t launch_ozaki_gemm(int nslices, const void* As, const void* Bs, const int* ea, const int* eb, double* M,
                              const LowpGemmArgs& g, hipStream_t s);
+hipError_t launch_ozaki_i8_slices(int nslices, const LowpSliceArgs& a, hipStream_t s);
+hipError_t launch_ozaki_i8_gemm(int nslices, const void* As, const void* Bs, const int* ea, const int* eb, double* M,
+                                const LowpGemmArgs& g, hipStream_t s);
 hipError_t launch_fp8_slices(const LowpSliceArgs& a, float* sa, float* sb, hipStream_t s);
 hipError_t launch_fp8_gemm(const void* As, const void* Bs, const float* sa, const float* sb, double* M,
                            const LowpGemmArgs& g, hipStream_t s);

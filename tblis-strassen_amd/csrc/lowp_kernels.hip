@@ -10,6 +10,11 @@
 //    nsl = 5 keeps 15 products and about 40 bits.
 //    (Ozaki, Ogita, Oishi, Rump, "Error-free transformations of matrix multiplication by using fast
 //    routines of matrix multiplication and its applications", Numer. Algorithms 59, 2012.)
+//  * Ozaki-style INT8 slicing: the same scheme with base 2^7 digits |d_s| <= 64 (exact in int8), x =
+//    sum_s d_s 2^(-6-7s).  A digit product is at most 2^12 in magnitude, so the INT32 MFMA accumulation
+//    of a weight group is exact over 2^31 / (NSL 2^12) K elements; groups are folded into FP64 every
+//    1024 K-steps of 64.  6 digits keep about 41 bits with 21 products, on the INT8 MFMA (twice the
+//    BF16 rate).
 //  * FP8 E4M3 with one FP32 scale per row (column) and 128-long K block: FP32 MFMA accumulation inside a
 //    block, FP64 across blocks.
 //
@@ -133,6 +138,74 @@ __global__ __launch_bounds__(256) void slice_cols_kernel(const double* __restric
         if (j < cols && k < kk) v = ldexp(T[(q * cols + j) * kk + k], -e[q * cols + j]);
         __bf16 d[NSL];
         digits<NSL>(v, d);
+#pragma unroll
+        for (int sl = 0; sl < NSL; sl++) out[sl * slab + x] = d[sl];
+    }
+}
+
+// ---- Ozaki INT8 digits ------------------------------------------------------------------------------
+
+template <int NSL>
+__device__ __forceinline__ void digits_i8(double x, int8_t (&d)[NSL])
+{
+    // x in (-1, 1): r = x 2^6, d_0 = round(r) in [-64, 64], then r = (r - d) 2^7 in [-64, 64], ... (exact)
+    double r = x * 64.0;
+#pragma unroll
+    for (int s = 0; s < NSL; s++)
+    {
+        const double q = rint(r);
+        d[s] = int8_t(int(q));
+        r = (r - q) * 128.0;
+    }
+}
+
+// S (rows fastest) -> out[s][q][i][k] (k fastest, int8), through a 64 x 64 LDS transpose
+template <int NSL>
+__global__ __launch_bounds__(256) void slice_rows_i8_kernel(const double* __restrict__ S, int64_t rows, int64_t cols,
+                                                            const int* __restrict__ e, int8_t* __restrict__ out,
+                                                            int64_t rp, int64_t kp, int nmat)
+{
+    __shared__ double tile[64][65];
+    const int64_t tr = blockIdx.x, tk = blockIdx.y, q = blockIdx.z;
+    const int tid = threadIdx.x;
+    const double* s = S + q * rows * cols;
+    for (int x = tid; x < 64 * 64; x += 256)
+    {
+        const int r = x % 64, c = x / 64;
+        const int64_t i = tr * 64 + r, k = tk * 64 + c;
+        double v = 0.0;
+        if (i < rows && k < cols) v = ldexp(s[i + k * rows], -e[q * rows + i]);
+        tile[r][c] = v;
+    }
+    __syncthreads();
+    const int64_t slab = int64_t(nmat) * rp * kp;
+    for (int x = tid; x < 64 * 64; x += 256)
+    {
+        const int c = x % 64, r = x / 64;
+        const int64_t i = tr * 64 + r, k = tk * 64 + c;
+        if (i >= rp || k >= kp) continue;
+        int8_t d[NSL];
+        digits_i8<NSL>(tile[r][c], d);
+        int8_t* o = out + (q * rp + i) * kp + k;
+#pragma unroll
+        for (int sl = 0; sl < NSL; sl++) o[sl * slab] = d[sl];
+    }
+}
+
+// T (k fastest, [q][j][k]) -> out[s][q][j][k] (int8)
+template <int NSL>
+__global__ __launch_bounds__(256) void slice_cols_i8_kernel(const double* __restrict__ T, int64_t kk, int64_t cols,
+                                                            const int* __restrict__ e, int8_t* __restrict__ out,
+                                                            int64_t cp, int64_t kp, int nmat)
+{
+    const int64_t n = int64_t(nmat) * cp * kp, slab = n;
+    for (int64_t x = int64_t(blockIdx.x) * 256 + threadIdx.x; x < n; x += int64_t(gridDim.x) * 256)
+    {
+        const int64_t k = x % kp, j = (x / kp) % cp, q = x / (kp * cp);
+        double v = 0.0;
+        if (j < cols && k < kk) v = ldexp(T[(q * cols + j) * kk + k], -e[q * cols + j]);
+        int8_t d[NSL];
+        digits_i8<NSL>(v, d);
 #pragma unroll
         for (int sl = 0; sl < NSL; sl++) out[sl * slab + x] = d[sl];
     }
@@ -375,6 +448,155 @@ __global__ __launch_bounds__(512) void ozaki_gemm_lds_kernel(const __bf16* __res
         }
 }
 
+// ---- Ozaki INT8 GEMM ---------------------------------------------------------------------------------
+
+// 512 threads (8 waves as 4 x 2, 32 x 32 each) per 128 x 64 tile of M_q, K-steps of 64.  The digit tiles
+// are double-buffered in LDS as [slice][row][64 bytes] with the four 16-byte chunks of row r
+// XOR-swizzled by (r >> 1) & 3, so the MFMA operand reads (16 consecutive rows at one chunk) spread each
+// 8 rows over the 8 chunk slots of a 128-byte bank window.  Group d (digit pairs s + t = d, weight
+// 2^(-12-7d)) accumulates exactly in INT32; with FOLD (kp > 64 kI8Fold) it is folded into FP64 every
+// kI8Fold K-steps, else once at the end (which keeps the FP64 accumulators out of the loop's registers).
+// Needs mp % 128 == 0, np % 64 == 0, kp % 64 == 0, NSL <= 6 (LDS).
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+constexpr int kI8Fold = 1024;  // K-steps per INT32 fold: 1024 * 64 * NSL * 2^12 < 2^31
+template <int NSL, bool FOLD>
+__global__ __launch_bounds__(512) void ozaki_i8_gemm_kernel(const int8_t* __restrict__ As, const int8_t* __restrict__ Bs,
+                                                            const int* __restrict__ ea, const int* __restrict__ eb,
+                                                            double* __restrict__ M, LowpGemmArgs g)
+{
+    static_assert(NSL <= 6, "LDS: 2 x NSL x 12 KiB");
+    __shared__ __attribute__((aligned(16))) int8_t sa[2][NSL][128 * 64];
+    __shared__ __attribute__((aligned(16))) int8_t sb[2][NSL][64 * 64];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tiles_m = int(g.mp / 128);
+    const int tm = blockIdx.x % tiles_m, tn = blockIdx.x / tiles_m;
+    const int q = blockIdx.y;
+    const int64_t row0 = int64_t(tm) * 128, col0 = int64_t(tn) * 64;
+    const int wm = (wave & 3) * 32, wn = (wave >> 2) * 32;
+    const int64_t slab_a = int64_t(g.nprod) * g.mp * g.kp, slab_b = int64_t(g.nprod) * g.np * g.kp;
+    // loader: A tile = 128 rows x 64 bytes = 512 x 16 B per slice (one per thread); B = 64 x 64 = 256 x 16 B
+    const int lr = tid >> 2, lc = tid & 3;
+    const int8_t* ga = As + (int64_t(q) * g.mp + row0 + lr) * g.kp + lc * 16;
+    const int8_t* gb = Bs + (int64_t(q) * g.np + col0 + (lr & 63)) * g.kp + lc * 16;
+    const bool bload = tid < 256;
+    const int soff = lr * 64 + ((lc ^ ((lr >> 1) & 3)) * 16);  // this thread's swizzled LDS chunk
+
+    double acc[2][2][4];
+    i32x4 a32[NSL][2][2];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+        {
+#pragma unroll
+            for (int v = 0; v < 4; v++) acc[i][j][v] = 0.0;
+#pragma unroll
+            for (int d = 0; d < NSL; d++) a32[d][i][j] = i32x4{0, 0, 0, 0};
+        }
+    auto fold = [&]() {
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++)
+#pragma unroll
+                for (int v = 0; v < 4; v++)
+                {
+                    // small weights first
+                    double t = 0.0;
+#pragma unroll
+                    for (int d = NSL - 1; d >= 0; d--) t += ldexp(double(a32[d][i][j][v]), -12 - 7 * d);
+                    acc[i][j][v] += t;
+                }
+#pragma unroll
+        for (int d = 0; d < NSL; d++)
+#pragma unroll
+            for (int i = 0; i < 2; i++)
+#pragma unroll
+                for (int j = 0; j < 2; j++) a32[d][i][j] = i32x4{0, 0, 0, 0};
+    };
+
+    i32x4 ra[NSL], rb[NSL];
+    auto gload = [&](int64_t st) {
+#pragma unroll
+        for (int s = 0; s < NSL; s++)
+        {
+            ra[s] = *reinterpret_cast<const i32x4*>(ga + s * slab_a + st * 64);
+            rb[s] = bload ? *reinterpret_cast<const i32x4*>(gb + s * slab_b + st * 64) : i32x4{0, 0, 0, 0};
+        }
+    };
+    auto lstore = [&](int buf) {
+#pragma unroll
+        for (int s = 0; s < NSL; s++)
+        {
+            *reinterpret_cast<i32x4*>(&sa[buf][s][soff]) = ra[s];
+            if (bload) *reinterpret_cast<i32x4*>(&sb[buf][s][soff]) = rb[s];
+        }
+    };
+
+    // operand reads: 16 consecutive rows (lane % 16) at chunk lane / 16
+    const int fr = lane & 15, fc = lane >> 4;
+    int aoff[2], boff[2];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+    {
+        const int ra_ = wm + i * 16 + fr, rb_ = wn + i * 16 + fr;
+        aoff[i] = ra_ * 64 + ((fc ^ ((ra_ >> 1) & 3)) * 16);
+        boff[i] = rb_ * 64 + ((fc ^ ((rb_ >> 1) & 3)) * 16);
+    }
+
+    const int64_t nsteps = g.kp / 64;
+    gload(0);
+    lstore(0);
+    __syncthreads();
+    for (int64_t st = 0; st < nsteps; st++)
+    {
+        const int cur = int(st & 1);
+        if (st + 1 < nsteps) gload(st + 1);
+        i32x4 fa[NSL][2];
+#pragma unroll
+        for (int s = 0; s < NSL; s++)
+#pragma unroll
+            for (int i = 0; i < 2; i++) fa[s][i] = *reinterpret_cast<const i32x4*>(&sa[cur][s][aoff[i]]);
+#pragma unroll
+        for (int t = 0; t < NSL; t++)
+        {
+            i32x4 fb[2];
+#pragma unroll
+            for (int j = 0; j < 2; j++) fb[j] = *reinterpret_cast<const i32x4*>(&sb[cur][t][boff[j]]);
+#pragma unroll
+            for (int s = 0; s + t < NSL; s++)
+#pragma unroll
+                for (int i = 0; i < 2; i++)
+#pragma unroll
+                    for (int j = 0; j < 2; j++)
+                        a32[s + t][i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[s][i], fb[j], a32[s + t][i][j], 0, 0, 0);
+        }
+        if constexpr (FOLD)
+            if ((st + 1) % kI8Fold == 0) fold();
+        if (st + 1 < nsteps) lstore(cur ^ 1);
+        __syncthreads();
+    }
+    fold();
+
+    double* Mq = M + int64_t(q) * g.ms * g.ns;
+    const int64_t r0 = row0 + wm, c0 = col0 + wn;
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+        {
+            const int64_t col = c0 + j * 16 + (lane & 15);
+            if (col >= g.ns) continue;
+            const int ec = eb[int64_t(q) * g.ns + col];
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+            {
+                const int64_t row = r0 + i * 16 + 4 * (lane >> 4) + v;
+                if (row < g.ms) put_out(g, Mq, row, col, ldexp(acc[i][j][v], ea[int64_t(q) * g.ms + row] + ec));
+            }
+        }
+}
+
 // ---- FP8 E4M3 with per-128 K-block scales -------------------------------------------------------------
 
 constexpr float kE4M3Max = 240.0f;  // largest finite E4M3 value common to the OCP and FNUZ encodings
@@ -546,7 +768,59 @@ hipError_t gemm_t(const void* As, const void* Bs, const int* ea, const int* eb, 
     return hipGetLastError();
 }
 
+template <int NSL>
+hipError_t slices_i8_t(const LowpSliceArgs& a, hipStream_t s)
+{
+    hipLaunchKernelGGL(slice_rows_i8_kernel<NSL>, dim3(unsigned(a.mp / 64), unsigned(a.kp / 64), unsigned(a.nprod)),
+                       dim3(256), 0, s, a.S, a.ms, a.ks, a.ea, static_cast<int8_t*>(a.outA), a.mp, a.kp, a.nprod);
+    hipLaunchKernelGGL(slice_cols_i8_kernel<NSL>, dim3(grid_of(int64_t(a.nprod) * a.np * a.kp, 256)), dim3(256), 0, s,
+                       a.T, a.ks, a.ns, a.eb, static_cast<int8_t*>(a.outB), a.np, a.kp, a.nprod);
+    return hipGetLastError();
+}
+
+template <int NSL>
+hipError_t gemm_i8_t(const void* As, const void* Bs, const int* ea, const int* eb, double* M, const LowpGemmArgs& g,
+                     hipStream_t s)
+{
+    if (g.mp % 128 || g.np % 64 || g.kp % 64) return hipErrorInvalidValue;
+    const dim3 grid(unsigned((g.mp / 128) * (g.np / 64)), unsigned(g.nprod));
+    if (g.kp > int64_t(64) * kI8Fold)
+        hipLaunchKernelGGL((ozaki_i8_gemm_kernel<NSL, true>), grid, dim3(512), 0, s, static_cast<const int8_t*>(As),
+                           static_cast<const int8_t*>(Bs), ea, eb, M, g);
+    else
+        hipLaunchKernelGGL((ozaki_i8_gemm_kernel<NSL, false>), grid, dim3(512), 0, s, static_cast<const int8_t*>(As),
+                           static_cast<const int8_t*>(Bs), ea, eb, M, g);
+    return hipGetLastError();
+}
+
 }  // namespace
+
+hipError_t launch_ozaki_i8_slices(int nsl, const LowpSliceArgs& a, hipStream_t s)
+{
+    switch (nsl)
+    {
+    case 2: return slices_i8_t<2>(a, s);
+    case 3: return slices_i8_t<3>(a, s);
+    case 4: return slices_i8_t<4>(a, s);
+    case 5: return slices_i8_t<5>(a, s);
+    case 6: return slices_i8_t<6>(a, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_ozaki_i8_gemm(int nsl, const void* As, const void* Bs, const int* ea, const int* eb, double* M,
+                                const LowpGemmArgs& g, hipStream_t s)
+{
+    switch (nsl)
+    {
+    case 2: return gemm_i8_t<2>(As, Bs, ea, eb, M, g, s);
+    case 3: return gemm_i8_t<3>(As, Bs, ea, eb, M, g, s);
+    case 4: return gemm_i8_t<4>(As, Bs, ea, eb, M, g, s);
+    case 5: return gemm_i8_t<5>(As, Bs, ea, eb, M, g, s);
+    case 6: return gemm_i8_t<6>(As, Bs, ea, eb, M, g, s);
+    default: return hipErrorInvalidValue;
+    }
+}
 
 hipError_t launch_lowp_exponents(const LowpSliceArgs& a, hipStream_t s)
 {

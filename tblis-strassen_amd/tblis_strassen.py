@@ -17,7 +17,7 @@ LIB_PATH = os.path.join(HERE, "libtblis_strassen_amd.so")
 
 TYPE_FLOAT, TYPE_DOUBLE = 0, 1
 IMPL = {"abc": 0, "blis": 0, "blas": 1, "ref": 2, "naive": 3, "ab": 4, "auto": -1}
-PREC = {"native": 0, "bf16x3": 1, "ozaki_bf16": 2, "fp8_block": 3}
+PREC = {"native": 0, "bf16x3": 1, "ozaki_bf16": 2, "fp8_block": 3, "ozaki_int8": 4}
 FLAG_REF_FRINGE = 1
 FLAG_SYNC = 2
 FLAG_GRAPH = 4
