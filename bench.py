@@ -25,6 +25,8 @@ Other workloads (one JSON line each, same fields; --workload NAME):
       u in [-8, 8]): FP64 classical and Strassen, BF16 hi/lo split, Ozaki BF16 and INT8 slicing and FP8
       block scaling, each with its error against the FP64 classical result; value = the fastest mode
       within 1e-10;
+  c2p the precision modes on the c2 contraction (m=n=k=8192, uniform[-1,1)): FP64 classical and
+      Strassen, BF16 hi/lo, Ozaki BF16 and INT8 slicing; value = the fastest mode within 1e-10;
   c4  256 seeded FP32 contractions of uneven shape (tests/cases.py:c4_batch): a loop of individual
       calls, the grouped batch, and its HIP-graph replay; value = the batch throughput;
   c5  C[abce] = A[dcaf] B[dfbe], all lengths 64, FP32: forward + dA + dB with 1-level ABC (and the
@@ -154,25 +156,35 @@ def common(args, metric, value, unit, ms, dtype, config, roofline, extra=None):
 PEAK = {"f64": 78.6, "f32": 157.3, "bf16": 2500.0, "fp8": 5000.0, "int8": 5000.0}
 
 
-def bench_c3(ts, torch, args, stream):
-    """Precision modes at m=n=16384, k=1024 with 2^u-scaled inputs; errors vs FP64 classical."""
+def bench_c3(ts, torch, args, stream, workload="c3"):
+    """Precision modes; errors vs FP64 classical.  c3: m=n=16384, k=1024 with 2^u-scaled inputs; c2p: the
+    c2 contraction (m=n=k=8192, uniform[-1,1) inputs), where the split-precision Strassen target is set."""
     import cases
 
-    case = cases.make_case(4242, m=(128, 128), n=(128, 128), k=(32, 32))
+    if workload == "c3":
+        case = cases.make_case(4242, m=(128, 128), n=(128, 128), k=(32, 32))
+        emax = 8
+        modes = [("fp64", 0), ("fp64", 1), ("fp64", 2), ("bf16x3", 1), ("bf16x3", 2), ("ozaki_bf16", 0),
+                 ("ozaki_bf16", 1), ("ozaki_bf16", 2), ("ozaki_int8", 0), ("ozaki_int8", 1), ("ozaki_int8", 2),
+                 ("fp8_block", 1)]
+    else:
+        case = make_workload("c2")
+        emax = 0
+        modes = [("fp64", 0), ("fp64", 1), ("fp64", 2), ("bf16x3", 1), ("ozaki_bf16", 0), ("ozaki_bf16", 1),
+                 ("ozaki_int8", 0), ("ozaki_int8", 1), ("ozaki_int8", 2)]
+    if args.modes:
+        modes = [(x.split(":")[0], int(x.split(":")[1])) for x in args.modes.split(",")]
     m, n, k = cases.mnk(case)
     eff = 2.0 * m * n * k
     A, abuf = device_tensor(case["shape_A"], torch.float64, torch)
     B, bbuf = device_tensor(case["shape_B"], torch.float64, torch)
-    ts.fill_uniform(abuf, 1, emax=8, stream=stream)
-    ts.fill_uniform(bbuf, 2, emax=8, stream=stream)
+    ts.fill_uniform(abuf, 1, emax=emax, stream=stream)
+    ts.fill_uniform(bbuf, 2, emax=emax, stream=stream)
     Cref, crefbuf = device_tensor(case["shape_C"], torch.float64, torch)
     crefbuf.zero_()
     ts.tensor_mult(1.0, A, case["idx_A"], B, case["idx_B"], 0.0, Cref, case["idx_C"], level=0, stream=stream)
     C, cbuf = device_tensor(case["shape_C"], torch.float64, torch)
-    log(f"workload c3: m={m} n={n} k={k}")
-    modes = [("fp64", 0), ("fp64", 1), ("fp64", 2), ("bf16x3", 1), ("bf16x3", 2), ("ozaki_bf16", 0),
-             ("ozaki_bf16", 1), ("ozaki_bf16", 2), ("ozaki_int8", 0), ("ozaki_int8", 1), ("ozaki_int8", 2),
-             ("fp8_block", 1)]
+    log(f"workload {workload}: m={m} n={n} k={k}")
     res = {}
     for prec, level in modes:
         pr = "native" if prec == "fp64" else prec
@@ -197,13 +209,19 @@ def bench_c3(ts, torch, args, stream):
                      "leaf_peak_tflops": peak, "leaf_frac": round(leaf / peak, 4) if leaf else None}
         log(f"{name:14s} {ms:8.3f} ms {eff / ms / 1e9:8.2f} eff TFLOP/s  err {err:.2e}  leaf {leaf or 0:.1f} TFLOP/s")
     ok = [v for v in res.values() if v["rel_err_vs_fp64"] <= 1e-10 and v["precision"] != "fp64"]
+    if "fp64_L0" not in res:  # a --modes subset: time the classical reference too
+        res["fp64_L0"] = {"eff_tflops": float("nan")}
     head = max(ok, key=lambda v: v["eff_tflops"]) if ok else res["fp64_L0"]
     hname = f"{head['precision']}_L{head['level']}"
+    desc = ("c3: m=n=16384, k=1024, A,B uniform*2^u (u in [-8,8])" if workload == "c3" else
+            "c2p: the c2 contraction, m=n=k=8192, A,B uniform[-1,1)")
+    dtype = "bf16" if "bf16" in hname else ("int8" if "int8" in hname else "f64")
     return common(args, "effective TFLOP/s (2mnk/t) at <=1e-10 rel-err vs FP64, low-precision leaves",
-                  head["eff_tflops"], "TFLOP/s", head["ms_per_step"], "bf16" if "bf16" in hname else "f64",
-                  {"workload": f"c3: m=n=16384, k=1024, A,B uniform*2^u (u in [-8,8]); reported mode {hname}",
+                  head["eff_tflops"], "TFLOP/s", head["ms_per_step"], dtype,
+                  {"workload": f"{desc}; reported mode {hname}",
                    "m": m, "n": n, "k": k, "reported_mode": hname,
-                   "fp64_classical_eff_tflops": res["fp64_L0"]["eff_tflops"]},
+                   "fp64_classical_eff_tflops": res["fp64_L0"]["eff_tflops"],
+                   "speedup_vs_fp64_classical": round(head["eff_tflops"] / res["fp64_L0"]["eff_tflops"], 4)},
                   {"bound": "mfma", "achieved": head["leaf_kernel_tflops"], "peak": head["leaf_peak_tflops"],
                    "unit": "TFLOP/s", "frac": head["leaf_frac"], "traffic": None},
                   {"modes": res})
@@ -318,12 +336,13 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS) + ["c3", "c4", "c5"])
+    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS) + ["c2p", "c3", "c4", "c5"])
     ap.add_argument("--variants", default="0:abc,1:abc:staged,1:abc:fused,2:abc:staged_all,2:abc:staged,"
                                           "2:abc:fused,1:ab,2:ab,1:naive,2:naive",
                     help="level:impl[:form], form = fused (every operand sum in the leaf kernel), staged (the "
                          "outer level's sums staged in HBM) or staged_all (every level's sums staged); default: "
                          "the library's choice")
+    ap.add_argument("--modes", default="", help="c2p / c3: precision:level list (default: all modes)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run one device copy of MIB MiB (a kernel of known traffic for counter scaling)")
@@ -348,8 +367,11 @@ def main():
         ts.copy_bytes(dst, src, nbytes, stream=stream)
         torch.cuda.synchronize()
         del src, dst
-    if args.workload in ("c3", "c4", "c5"):
-        out = {"c3": bench_c3, "c4": bench_c4, "c5": bench_c5}[args.workload](ts, torch, args, stream)
+    if args.workload in ("c2p", "c3", "c4", "c5"):
+        if args.workload == "c2p":
+            out = bench_c3(ts, torch, args, stream, "c2p")
+        else:
+            out = {"c3": bench_c3, "c4": bench_c4, "c5": bench_c5}[args.workload](ts, torch, args, stream)
         print(json.dumps(out))
         return
     case = make_workload(args.workload)

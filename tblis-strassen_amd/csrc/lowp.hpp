@@ -13,7 +13,7 @@ namespace mistra {
 // kp (a multiple of 64 for digits, 128 for FP8).
 struct LowpSliceArgs
 {
-    const double* S;
+    const double* S;   // ms x ks rows fastest, or with s_kfast [q][i][k] (K fastest, as T)
     const double* T;
     int64_t ms, ns, ks, mp, np, kp;
     int nprod;
@@ -21,18 +21,21 @@ struct LowpSliceArgs
     int* eb;  // [q][ns] column exponents of T
     void* outA;
     void* outB;
+    bool s_kfast;
 };
 
 struct LowpGemmArgs
 {
     int64_t ms, ns, mp, np, kp;
-    int nprod;
+    int nprod;  // products in the sliced operands (their slab stride)
     // direct epilogue (a single product feeding all of C with coefficient 1): C[c_m[r] + c_n[c]] =
     // alpha * M + beta * C instead of staging M
     double* C;
     const int64_t* c_m;
     const int64_t* c_n;
     double alpha, beta;
+    // this launch: products [q0, q0 + nlaunch)
+    int q0, nlaunch;
 };
 
 hipError_t launch_lowp_exponents(const LowpSliceArgs& a, hipStream_t s);

@@ -213,9 +213,11 @@ __global__ __launch_bounds__(256) void slice_cols_i8_kernel(const double* __rest
 
 // ---- epilogue -----------------------------------------------------------------------------------------
 
-// M_q[row + col * ms] = v, or with the direct epilogue C = alpha * v + beta * C at the scattered element
-__device__ __forceinline__ void put_out(const LowpGemmArgs& g, double* Mq, int64_t row, int64_t col, double v)
+// Element (row, col) of product q: staged into M_q[row + col * ms]; or, with the direct epilogue, C =
+// alpha * v + beta * C at the scattered element.
+__device__ __forceinline__ void put_out(const LowpGemmArgs& g, int q, double* Mq, int64_t row, int64_t col, double v)
 {
+    (void)q;
     if (g.C)
     {
         double* c = g.C + g.c_m[row] + g.c_n[col];
@@ -223,6 +225,51 @@ __device__ __forceinline__ void put_out(const LowpGemmArgs& g, double* Mq, int64
     }
     else
         Mq[row + col * g.ms] = v;
+}
+
+// Output of a 128 x 64 tile held by 8 waves (4 x 2, 32 x 32 each, element v of a lane at row
+// 4 (lane / 16) + v, column lane % 16 of its 16 x 16 block), through LDS in two halves of 64 rows:
+// the waves holding a half write it to `stage` (64 columns x 65 doubles), then all 512 threads write it
+// out with consecutive threads on consecutive rows (C's unit-stride dimension after the plan's
+// orientation, and M's), applying the row / column scale exponents.  `stage` aliases the K-loop's LDS
+// buffers, which are free after the loop.
+__device__ __forceinline__ void tile_out_128x64(const LowpGemmArgs& g, int q, const double (&acc)[2][2][4],
+                                                double* stage, int64_t row0, int64_t col0, int wm, int wn,
+                                                const int* __restrict__ ea, const int* __restrict__ eb,
+                                                double* __restrict__ M)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    double* Mq = M + int64_t(q) * g.ms * g.ns;
+    const int r = tid & 63, cg = tid >> 6;
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+    {
+        __syncthreads();  // the K loop (or the previous half) is done with the LDS
+        if (((wave & 3) >> 1) == h)
+        {
+#pragma unroll
+            for (int i = 0; i < 2; i++)
+#pragma unroll
+                for (int j = 0; j < 2; j++)
+#pragma unroll
+                    for (int v = 0; v < 4; v++)
+                        stage[(wn + j * 16 + (lane & 15)) * 65 + (wm - 64 * h + i * 16 + 4 * (lane >> 4) + v)] =
+                            acc[i][j][v];
+        }
+        __syncthreads();
+        const int64_t row = row0 + 64 * h + r;
+        if (row < g.ms)
+        {
+            const int er = ea[int64_t(q) * g.ms + row];
+#pragma unroll
+            for (int e = 0; e < 8; e++)
+            {
+                const int c = cg + 8 * e;
+                const int64_t col = col0 + c;
+                if (col < g.ns) put_out(g, q, Mq, row, col, ldexp(stage[c * 65 + r], er + eb[int64_t(q) * g.ns + col]));
+            }
+        }
+    }
 }
 
 // ---- Ozaki BF16 GEMM --------------------------------------------------------------------------------
@@ -247,7 +294,7 @@ __global__ __launch_bounds__(256) void ozaki_gemm_kernel(const __bf16* __restric
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int tiles_m = int(g.mp / 64);
     const int tm = blockIdx.x % tiles_m, tn = blockIdx.x / tiles_m;
-    const int q = blockIdx.y;
+    const int q = g.q0 + int(blockIdx.y);  // product (the launch covers [q0, q0 + nlaunch))
     const int64_t row0 = int64_t(tm) * 64 + (wave & 1) * 32, col0 = int64_t(tn) * 64 + (wave >> 1) * 32;
     const int64_t slab_a = int64_t(g.nprod) * g.mp * g.kp, slab_b = int64_t(g.nprod) * g.np * g.kp;
     // lane's operand rows / columns and k offset
@@ -321,7 +368,7 @@ __global__ __launch_bounds__(256) void ozaki_gemm_kernel(const __bf16* __restric
             for (int v = 0; v < 4; v++)
             {
                 const int64_t row = row0 + i * 16 + 4 * (lane >> 4) + v;
-                if (row < g.ms) put_out(g, Mq, row, col, ldexp(acc[i][j][v], ea[int64_t(q) * g.ms + row] + ec));
+                if (row < g.ms) put_out(g, q, Mq, row, col, ldexp(acc[i][j][v], ea[int64_t(q) * g.ms + row] + ec));
             }
         }
 }
@@ -341,7 +388,7 @@ __global__ __launch_bounds__(512) void ozaki_gemm_lds_kernel(const __bf16* __res
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int tiles_m = int(g.mp / 128);
     const int tm = blockIdx.x % tiles_m, tn = blockIdx.x / tiles_m;
-    const int q = blockIdx.y;
+    const int q = g.q0 + int(blockIdx.y);  // product (the launch covers [q0, q0 + nlaunch))
     const int64_t row0 = int64_t(tm) * 128, col0 = int64_t(tn) * 64;
     const int wm = (wave & 3) * 32, wn = (wave >> 2) * 32;
     const int64_t slab_a = int64_t(g.nprod) * g.mp * g.kp, slab_b = int64_t(g.nprod) * g.np * g.kp;
@@ -429,23 +476,8 @@ __global__ __launch_bounds__(512) void ozaki_gemm_lds_kernel(const __bf16* __res
 #pragma unroll
     for (int d = 0; d < NSL; d++) flush(d);
 
-    double* Mq = M + int64_t(q) * g.ms * g.ns;
-    const int64_t r0 = row0 + wm, c0 = col0 + wn;
-#pragma unroll
-    for (int i = 0; i < 2; i++)
-#pragma unroll
-        for (int j = 0; j < 2; j++)
-        {
-            const int64_t col = c0 + j * 16 + (lane & 15);
-            if (col >= g.ns) continue;
-            const int ec = eb[int64_t(q) * g.ns + col];
-#pragma unroll
-            for (int v = 0; v < 4; v++)
-            {
-                const int64_t row = r0 + i * 16 + 4 * (lane >> 4) + v;
-                if (row < g.ms) put_out(g, Mq, row, col, ldexp(acc[i][j][v], ea[int64_t(q) * g.ms + row] + ec));
-            }
-        }
+    static_assert(sizeof(sa) >= 64 * 65 * sizeof(double), "output staging fits the A buffers");
+    tile_out_128x64(g, q, acc, reinterpret_cast<double*>(&sa[0][0][0]), row0, col0, wm, wn, ea, eb, M);
 }
 
 // ---- Ozaki INT8 GEMM ---------------------------------------------------------------------------------
@@ -465,12 +497,13 @@ __global__ __launch_bounds__(512) void ozaki_i8_gemm_kernel(const int8_t* __rest
                                                             double* __restrict__ M, LowpGemmArgs g)
 {
     static_assert(NSL <= 6, "LDS: 2 x NSL x 12 KiB");
-    __shared__ __attribute__((aligned(16))) int8_t sa[2][NSL][128 * 64];
+    constexpr int NSA = NSL < 3 ? 3 : NSL;  // at least the 33 KiB of the output staging tile
+    __shared__ __attribute__((aligned(16))) int8_t sa[2][NSA][128 * 64];
     __shared__ __attribute__((aligned(16))) int8_t sb[2][NSL][64 * 64];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int tiles_m = int(g.mp / 128);
     const int tm = blockIdx.x % tiles_m, tn = blockIdx.x / tiles_m;
-    const int q = blockIdx.y;
+    const int q = g.q0 + int(blockIdx.y);  // product (the launch covers [q0, q0 + nlaunch))
     const int64_t row0 = int64_t(tm) * 128, col0 = int64_t(tn) * 64;
     const int wm = (wave & 3) * 32, wn = (wave >> 2) * 32;
     const int64_t slab_a = int64_t(g.nprod) * g.mp * g.kp, slab_b = int64_t(g.nprod) * g.np * g.kp;
@@ -578,23 +611,8 @@ __global__ __launch_bounds__(512) void ozaki_i8_gemm_kernel(const int8_t* __rest
     }
     fold();
 
-    double* Mq = M + int64_t(q) * g.ms * g.ns;
-    const int64_t r0 = row0 + wm, c0 = col0 + wn;
-#pragma unroll
-    for (int i = 0; i < 2; i++)
-#pragma unroll
-        for (int j = 0; j < 2; j++)
-        {
-            const int64_t col = c0 + j * 16 + (lane & 15);
-            if (col >= g.ns) continue;
-            const int ec = eb[int64_t(q) * g.ns + col];
-#pragma unroll
-            for (int v = 0; v < 4; v++)
-            {
-                const int64_t row = r0 + i * 16 + 4 * (lane >> 4) + v;
-                if (row < g.ms) put_out(g, Mq, row, col, ldexp(acc[i][j][v], ea[int64_t(q) * g.ms + row] + ec));
-            }
-        }
+    static_assert(sizeof(sa) >= 64 * 65 * sizeof(double), "output staging fits the A buffers");
+    tile_out_128x64(g, q, acc, reinterpret_cast<double*>(&sa[0][0][0]), row0, col0, wm, wn, ea, eb, M);
 }
 
 // ---- FP8 E4M3 with per-128 K-block scales -------------------------------------------------------------
@@ -666,7 +684,7 @@ __global__ __launch_bounds__(256) void fp8_gemm_kernel(const uint8_t* __restrict
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int tiles_m = int(g.mp / 64);
     const int tm = blockIdx.x % tiles_m, tn = blockIdx.x / tiles_m;
-    const int q = blockIdx.y;
+    const int q = g.q0 + int(blockIdx.y);  // product (the launch covers [q0, q0 + nlaunch))
     const int64_t row0 = int64_t(tm) * 64 + (wave & 1) * 32, col0 = int64_t(tn) * 64 + (wave >> 1) * 32;
     const int64_t nkb = g.kp / 128;
     const uint8_t* pa = As + (int64_t(q) * g.mp + row0 + (lane & 15)) * g.kp + 8 * (lane >> 4);
@@ -728,7 +746,7 @@ __global__ __launch_bounds__(256) void fp8_gemm_kernel(const uint8_t* __restrict
             for (int v = 0; v < 4; v++)
             {
                 const int64_t row = row0 + i * 16 + 4 * (lane >> 4) + v;
-                if (row < g.ms) put_out(g, Mq, row, col, acc[i][j][v]);
+                if (row < g.ms) put_out(g, q, Mq, row, col, acc[i][j][v]);
             }
         }
 }
@@ -742,8 +760,12 @@ unsigned grid_of(int64_t n, int per)
 template <int NSL>
 hipError_t slices_t(const LowpSliceArgs& a, hipStream_t s)
 {
-    hipLaunchKernelGGL(slice_rows_kernel<NSL>, dim3(unsigned(a.mp / 64), unsigned(a.kp / 64), unsigned(a.nprod)),
-                       dim3(256), 0, s, a.S, a.ms, a.ks, a.ea, static_cast<__bf16*>(a.outA), a.mp, a.kp, a.nprod);
+    if (a.s_kfast)  // S laid out as T: rows of contiguous K
+        hipLaunchKernelGGL(slice_cols_kernel<NSL>, dim3(grid_of(int64_t(a.nprod) * a.mp * a.kp, 256)), dim3(256), 0, s,
+                           a.S, a.ks, a.ms, a.ea, static_cast<__bf16*>(a.outA), a.mp, a.kp, a.nprod);
+    else
+        hipLaunchKernelGGL(slice_rows_kernel<NSL>, dim3(unsigned(a.mp / 64), unsigned(a.kp / 64), unsigned(a.nprod)),
+                           dim3(256), 0, s, a.S, a.ms, a.ks, a.ea, static_cast<__bf16*>(a.outA), a.mp, a.kp, a.nprod);
     hipLaunchKernelGGL(slice_cols_kernel<NSL>, dim3(grid_of(int64_t(a.nprod) * a.np * a.kp, 256)), dim3(256), 0, s,
                        a.T, a.ks, a.ns, a.eb, static_cast<__bf16*>(a.outB), a.np, a.kp, a.nprod);
     return hipGetLastError();
@@ -757,13 +779,13 @@ hipError_t gemm_t(const void* As, const void* Bs, const int* ea, const int* eb, 
     {
         if (g.mp % 128 == 0 && g.np % 64 == 0 && g.kp % 32 == 0)
         {
-            hipLaunchKernelGGL(ozaki_gemm_lds_kernel<NSL>, dim3(unsigned((g.mp / 128) * (g.np / 64)), unsigned(g.nprod)),
+            hipLaunchKernelGGL(ozaki_gemm_lds_kernel<NSL>, dim3(unsigned((g.mp / 128) * (g.np / 64)), unsigned(g.nlaunch)),
                                dim3(512), 0, s, static_cast<const __bf16*>(As), static_cast<const __bf16*>(Bs), ea, eb,
                                M, g);
             return hipGetLastError();
         }
     }
-        hipLaunchKernelGGL(ozaki_gemm_kernel<NSL>, dim3(unsigned((g.mp / 64) * (g.np / 64)), unsigned(g.nprod)),
+        hipLaunchKernelGGL(ozaki_gemm_kernel<NSL>, dim3(unsigned((g.mp / 64) * (g.np / 64)), unsigned(g.nlaunch)),
                            dim3(256), 0, s, static_cast<const __bf16*>(As), static_cast<const __bf16*>(Bs), ea, eb, M, g);
     return hipGetLastError();
 }
@@ -771,8 +793,12 @@ hipError_t gemm_t(const void* As, const void* Bs, const int* ea, const int* eb, 
 template <int NSL>
 hipError_t slices_i8_t(const LowpSliceArgs& a, hipStream_t s)
 {
-    hipLaunchKernelGGL(slice_rows_i8_kernel<NSL>, dim3(unsigned(a.mp / 64), unsigned(a.kp / 64), unsigned(a.nprod)),
-                       dim3(256), 0, s, a.S, a.ms, a.ks, a.ea, static_cast<int8_t*>(a.outA), a.mp, a.kp, a.nprod);
+    if (a.s_kfast)
+        hipLaunchKernelGGL(slice_cols_i8_kernel<NSL>, dim3(grid_of(int64_t(a.nprod) * a.mp * a.kp, 256)), dim3(256), 0,
+                           s, a.S, a.ks, a.ms, a.ea, static_cast<int8_t*>(a.outA), a.mp, a.kp, a.nprod);
+    else
+        hipLaunchKernelGGL(slice_rows_i8_kernel<NSL>, dim3(unsigned(a.mp / 64), unsigned(a.kp / 64), unsigned(a.nprod)),
+                           dim3(256), 0, s, a.S, a.ms, a.ks, a.ea, static_cast<int8_t*>(a.outA), a.mp, a.kp, a.nprod);
     hipLaunchKernelGGL(slice_cols_i8_kernel<NSL>, dim3(grid_of(int64_t(a.nprod) * a.np * a.kp, 256)), dim3(256), 0, s,
                        a.T, a.ks, a.ns, a.eb, static_cast<int8_t*>(a.outB), a.np, a.kp, a.nprod);
     return hipGetLastError();
@@ -783,7 +809,7 @@ hipError_t gemm_i8_t(const void* As, const void* Bs, const int* ea, const int* e
                      hipStream_t s)
 {
     if (g.mp % 128 || g.np % 64 || g.kp % 64) return hipErrorInvalidValue;
-    const dim3 grid(unsigned((g.mp / 128) * (g.np / 64)), unsigned(g.nprod));
+    const dim3 grid(unsigned((g.mp / 128) * (g.np / 64)), unsigned(g.nlaunch));
     if (g.kp > int64_t(64) * kI8Fold)
         hipLaunchKernelGGL((ozaki_i8_gemm_kernel<NSL, true>), grid, dim3(512), 0, s, static_cast<const int8_t*>(As),
                            static_cast<const int8_t*>(Bs), ea, eb, M, g);
@@ -824,8 +850,12 @@ hipError_t launch_ozaki_i8_gemm(int nsl, const void* As, const void* Bs, const i
 
 hipError_t launch_lowp_exponents(const LowpSliceArgs& a, hipStream_t s)
 {
-    hipLaunchKernelGGL(row_exp_kernel, dim3(grid_of(int64_t(a.nprod) * a.ms, 256)), dim3(256), 0, s, a.S, a.ms, a.ks,
-                       a.nprod, a.ea);
+    if (a.s_kfast)  // one wave per row of contiguous K
+        hipLaunchKernelGGL(col_exp_kernel, dim3(grid_of(int64_t(a.nprod) * a.ms, 4)), dim3(256), 0, s, a.S, a.ks, a.ms,
+                           a.nprod, a.ea);
+    else
+        hipLaunchKernelGGL(row_exp_kernel, dim3(grid_of(int64_t(a.nprod) * a.ms, 256)), dim3(256), 0, s, a.S, a.ms,
+                           a.ks, a.nprod, a.ea);
     hipLaunchKernelGGL(col_exp_kernel, dim3(grid_of(int64_t(a.nprod) * a.ns, 4)), dim3(256), 0, s, a.T, a.ks, a.ns,
                        a.nprod, a.eb);
     return hipGetLastError();
@@ -872,7 +902,7 @@ hipError_t launch_fp8_slices(const LowpSliceArgs& a, float* sa, float* sb, hipSt
 hipError_t launch_fp8_gemm(const void* As, const void* Bs, const float* sa, const float* sb, double* M,
                            const LowpGemmArgs& g, hipStream_t s)
 {
-    hipLaunchKernelGGL(fp8_gemm_kernel, dim3(unsigned((g.mp / 64) * (g.np / 64)), unsigned(g.nprod)), dim3(256), 0, s,
+    hipLaunchKernelGGL(fp8_gemm_kernel, dim3(unsigned((g.mp / 64) * (g.np / 64)), unsigned(g.nlaunch)), dim3(256), 0, s,
                        static_cast<const uint8_t*>(As), static_cast<const uint8_t*>(Bs), sa, sb, M, g);
     return hipGetLastError();
 }
