@@ -220,6 +220,40 @@ ProdDesc shifted(const Plan& p, const ProdDesc& pd)
     return f;
 }
 
+// One-pass sum tables (launch_operand_multisums) for the sums of the first `depth` levels over sub-blocks
+// of extent om x ok (A) and ok x on (B): source s = sub-block s (row-major) of the 2^depth split; A's
+// rows (K) are its column blocks, B's rows (K) its row blocks.  msum_a / msum_b say whether every source
+// sub-block is a translate of the first.
+void build_msum(Plan& p, int depth, int64_t om, int64_t on, int64_t ok)
+{
+    const int64_t dout = int64_t(1) << depth;
+    const auto& one = products_for_level(depth);
+    const int nsrc = int(dout * dout);
+    p.msum_nsrc = nsrc;
+    p.h_msum_shift_a.assign(size_t(nsrc), 0);
+    p.h_msum_shift_b.assign(size_t(nsrc), 0);
+    p.msum_a = p.msum_b = true;
+    for (int src = 0; src < nsrc; src++)
+    {
+        const int64_t rb = src / dout, cb = src % dout;
+        p.msum_a = p.msum_a && is_translate(p.h_a_k, cb * ok, ok) && is_translate(p.h_a_m, rb * om, om);
+        p.msum_b = p.msum_b && is_translate(p.h_b_k, rb * ok, ok) && is_translate(p.h_b_n, cb * on, on);
+        if (p.msum_a)
+            p.h_msum_shift_a[size_t(src)] =
+                (p.h_a_k[size_t(cb * ok)] - p.h_a_k[0]) + (p.h_a_m[size_t(rb * om)] - p.h_a_m[0]);
+        if (p.msum_b)
+            p.h_msum_shift_b[size_t(src)] =
+                (p.h_b_k[size_t(rb * ok)] - p.h_b_k[0]) + (p.h_b_n[size_t(cb * on)] - p.h_b_n[0]);
+    }
+    p.h_msum_coef_a.assign(one.size() * size_t(nsrc), 0.0);
+    p.h_msum_coef_b.assign(one.size() * size_t(nsrc), 0.0);
+    for (size_t o = 0; o < one.size(); o++)
+    {
+        for (const SubTerm& t : one[o].A) p.h_msum_coef_a[o * size_t(nsrc) + size_t(t.subid)] = t.coeff;
+        for (const SubTerm& t : one[o].B) p.h_msum_coef_b[o * size_t(nsrc) + size_t(t.subid)] = t.coeff;
+    }
+}
+
 // Tables of the staged form of an ABC plan (see Plan::outer_staged), staging the sums of the first
 // `depth` levels.  Outer product po of the depth-level table sums A over its terms into S_po (element
 // (i, j) of the os_m x os_k block at po*os_m*os_k + i*os_k + j: K-fast) and B into T_po (element (i, j)
@@ -288,32 +322,7 @@ bool build_outer_staged(Plan& p, int depth)
     p.h_outer_shifted = p.h_outer_sums;
     translate_form(0, p.h_a_k, p.h_a_m, ok, om, p.h_batch_a[0], p.outer_shift_a, p.outer_pair_a);
     translate_form(1, p.h_b_k, p.h_b_n, ok, on, p.h_batch_b[0], p.outer_shift_b, p.outer_pair_b);
-    // one-pass form: source s = sub-block s (row-major) of the dout x dout split; A's rows (K) are its
-    // column blocks, B's rows (K) its row blocks
-    const int nsrc = int(dout * dout);
-    p.msum_nsrc = nsrc;
-    p.h_msum_shift_a.assign(size_t(nsrc), 0);
-    p.h_msum_shift_b.assign(size_t(nsrc), 0);
-    p.msum_a = p.msum_b = true;
-    for (int src = 0; src < nsrc; src++)
-    {
-        const int64_t rb = src / dout, cb = src % dout;
-        p.msum_a = p.msum_a && is_translate(p.h_a_k, cb * ok, ok) && is_translate(p.h_a_m, rb * om, om);
-        p.msum_b = p.msum_b && is_translate(p.h_b_k, rb * ok, ok) && is_translate(p.h_b_n, cb * on, on);
-        if (p.msum_a)
-            p.h_msum_shift_a[size_t(src)] =
-                (p.h_a_k[size_t(cb * ok)] - p.h_a_k[0]) + (p.h_a_m[size_t(rb * om)] - p.h_a_m[0]);
-        if (p.msum_b)
-            p.h_msum_shift_b[size_t(src)] =
-                (p.h_b_k[size_t(rb * ok)] - p.h_b_k[0]) + (p.h_b_n[size_t(cb * on)] - p.h_b_n[0]);
-    }
-    p.h_msum_coef_a.assign(one.size() * size_t(nsrc), 0.0);
-    p.h_msum_coef_b.assign(one.size() * size_t(nsrc), 0.0);
-    for (size_t o = 0; o < one.size(); o++)
-    {
-        for (const SubTerm& t : one[o].A) p.h_msum_coef_a[o * size_t(nsrc) + size_t(t.subid)] = t.coeff;
-        for (const SubTerm& t : one[o].B) p.h_msum_coef_b[o * size_t(nsrc) + size_t(t.subid)] = t.coeff;
-    }
+    build_msum(p, depth, om, on, ok);
     p.h_inner.clear();
     const auto& inner = products_for_level(p.level - depth);
     const int64_t din = f;  // inner split divisor
@@ -571,6 +580,8 @@ Status build_plan_host(const tblis::tblis_tensor* A, const label_type* idx_A, co
     p.h_c_m = std::move(c_m);
     p.h_c_n = std::move(c_n);
     choose_outer_staged(p, flags);
+    // the low-precision leaves stage every sum of a Naive plan (depth = level) in one pass when they can
+    if (p.core && p.impl == IMPL_NAIVE && !p.outer_staged) build_msum(p, p.level, p.ms, p.ns, p.ks);
     return Status();
 }
 
@@ -1294,6 +1305,26 @@ Status run_plan_lowp(const Plan& p, const Call& c, int precision, int nslices, h
             sa.which = 0;
             sa.out = S.p;
             sa.zoff = p.h_batch_a[size_t(z)];
+            // one pass over the source sub-blocks when the tables are there (Naive core plans)
+            const bool ms_a = p.core && p.msum_a && p.msum_nsrc == int(p.d * p.d);
+            const bool ms_b = p.core && p.msum_b && p.msum_nsrc == int(p.d * p.d);
+            auto multisums = [&](bool isA) {
+                MultiSumArgs ma;
+                std::memset(&ma, 0, sizeof(ma));
+                ma.X = isA ? c.A : c.B;
+                ma.zoff = isA ? p.h_batch_a[size_t(z)] : p.h_batch_b[size_t(z)];
+                ma.rs = isA ? p.a_k : p.b_k;
+                ma.cs = isA ? p.a_m : p.b_n;
+                ma.rows = ks;
+                ma.cols = isA ? ms : ns;
+                ma.nsrc = p.msum_nsrc;
+                ma.nout = p.nprod;
+                ma.coef = isA ? p.msum_coef_a : p.msum_coef_b;
+                ma.shift = isA ? p.msum_shift_a : p.msum_shift_b;
+                ma.out = isA ? S.p : T.p;
+                ma.pair = false;
+                return launch_operand_multisums(8, ma, stream);
+            };
             if (s_kfast)
             {
                 sa.rs = p.a_k;
@@ -1312,7 +1343,7 @@ Status run_plan_lowp(const Plan& p, const Call& c, int precision, int nslices, h
                 sa.cols = ks;
                 sa.rows_fast = p.a_mfast;
             }
-            e = launch_operand_sums(8, sa, stream);
+            e = s_kfast && ms_a ? multisums(true) : launch_operand_sums(8, sa, stream);
             if (e != hipSuccess) break;
             sa.X = c.B;
             sa.rs = p.b_k;
@@ -1325,7 +1356,7 @@ Status run_plan_lowp(const Plan& p, const Call& c, int precision, int nslices, h
             sa.prods = t_shift ? p.prods_fast : p.prods;
             sa.out = T.p;
             sa.zoff = p.h_batch_b[size_t(z)];
-            e = launch_operand_sums(8, sa, stream);
+            e = t_shift && ms_b ? multisums(false) : launch_operand_sums(8, sa, stream);
             if (e != hipSuccess) break;
 
             LowpSliceArgs la;
