@@ -378,8 +378,10 @@ __global__ __launch_bounds__(256) void ozaki_gemm_kernel(const __bf16* __restric
 // (rows of 32 BF16 padded to 40, so the 16-byte fragment reads of 16 consecutive rows spread over the
 // banks); the waves read their MFMA fragments from LDS.  Needs mp % 128 == 0, np % 64 == 0, kp % 32 == 0.
 constexpr int kOzLd = 40;  // LDS row stride in BF16 elements
+// Two workgroups per CU need at most 128 VGPRs: asked for with 2 slices (the 3-product split), where
+// the output staging took one register over and halved the occupancy.
 template <int NSL>
-__global__ __launch_bounds__(512) void ozaki_gemm_lds_kernel(const __bf16* __restrict__ As, const __bf16* __restrict__ Bs,
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(NSL <= 2 ? 4 : 1))) void ozaki_gemm_lds_kernel(const __bf16* __restrict__ As, const __bf16* __restrict__ Bs,
                                                              const int* __restrict__ ea, const int* __restrict__ eb,
                                                              double* __restrict__ M, LowpGemmArgs g)
 {
