@@ -538,22 +538,31 @@ __device__ __forceinline__ void leaf_gemm_body(const GemmArgs& g, int bx, int by
                 (void)ko;
 #endif
             };
+            // The MFMA sums over whichever k each lane group supplies, as long as A and B agree: lane
+            // group g (lane / 16) takes chunk 4 grp + g of the row for the CH consecutive MFMA steps of
+            // group grp, element e of the chunk in step CH grp + e, so one 16-byte LDS read feeds CH
+            // MFMA steps (the 16 k of a K-tile are still each used once).
+            typedef T vec16 __attribute__((ext_vector_type(CH)));
             auto mfma_dma = [&](const T* __restrict__ la, const T* __restrict__ lb) {
                 const int sw = lane & 7;  // row & 7 of every fragment row this lane reads
+                static_assert(K4 % CH == 0 && (K4 / CH) * 4 * CH == TK, "chunks of a K-tile");
 #pragma unroll
-                for (int k4 = 0; k4 < K4; k4++)
+                for (int grp = 0; grp < K4 / CH; grp++)
                 {
-                    const int kr = k4 * 4 + (lane >> 4);
-                    const int ko = (((kr / CH) ^ sw) * CH) + kr % CH;
-                    T af[FM], bf[FN];
-#pragma unroll
-                    for (int i = 0; i < FM; i++) af[i] = la[(wm0 + i * 16 + (lane & 15)) * TK + ko];
-#pragma unroll
-                    for (int j = 0; j < FN; j++) bf[j] = lb[(wn0 + j * 16 + (lane & 15)) * TK + ko];
+                    const int ko = ((grp * 4 + (lane >> 4)) ^ sw) * CH;
+                    vec16 af[FM], bf[FN];
 #pragma unroll
                     for (int i = 0; i < FM; i++)
+                        af[i] = *reinterpret_cast<const vec16*>(&la[(wm0 + i * 16 + (lane & 15)) * TK + ko]);
 #pragma unroll
-                        for (int j = 0; j < FN; j++) acc[i][j] = Mfma<T>::run(af[i], bf[j], acc[i][j]);
+                    for (int j = 0; j < FN; j++)
+                        bf[j] = *reinterpret_cast<const vec16*>(&lb[(wn0 + j * 16 + (lane & 15)) * TK + ko]);
+#pragma unroll
+                    for (int e = 0; e < CH; e++)
+#pragma unroll
+                        for (int i = 0; i < FM; i++)
+#pragma unroll
+                            for (int j = 0; j < FN; j++) acc[i][j] = Mfma<T>::run(af[i][e], bf[j][e], acc[i][j]);
                 }
             };
             fetch_kinfo(0);
