@@ -167,6 +167,77 @@ __global__ __launch_bounds__(256) void operand_multisums_kernel(MultiSumArgs a)
     }
 }
 
+// The 2-level one-pass sums through the Kronecker factorization of their coefficients (MultiSumArgs::
+// coef1): source (a, b) -- outer sub-block a, inner b -- is sub-block 4 (2 (a / 2) + b / 2) + 2 (a % 2) +
+// b % 2 of the 4 x 4 split.
+template <typename T, bool PAIR>
+__global__ __launch_bounds__(256) void operand_multisums_kron_kernel(MultiSumArgs a)
+{
+    constexpr int RPT = PAIR ? 2 : 1;
+    typedef T vec2 __attribute__((ext_vector_type(2)));
+    const int64_t tiles_r = (a.rows + 256 * RPT - 1) / (256 * RPT);
+    const int64_t tr = blockIdx.x % tiles_r, c = blockIdx.x / tiles_r;
+    const int64_t r = (tr * 256 + threadIdx.x) * RPT;
+    if (r >= a.rows) return;
+    const T* X = static_cast<const T*>(a.X) + a.zoff + a.rs[r] + a.cs[c];
+    T v[16][RPT];
+#pragma unroll
+    for (int s = 0; s < 16; s++)
+    {
+        const T* x = X + a.shift[s];
+        if constexpr (PAIR)
+        {
+            const vec2 w = *reinterpret_cast<const vec2*>(x);
+            v[s][0] = w.x;
+            v[s][1] = w.y;
+        }
+        else
+            v[s][0] = x[0];
+    }
+    T* out = static_cast<T*>(a.out) + c * a.rows + r;
+    const int64_t stride = a.rows * a.cols;
+    for (int o = 0; o < 7; o++)
+    {
+        T u[4][RPT];
+#pragma unroll
+        for (int b = 0; b < 4; b++)
+        {
+#pragma unroll
+            for (int i = 0; i < RPT; i++) u[b][i] = T(0);
+#pragma unroll
+            for (int q = 0; q < 4; q++)  // outer sub-block
+            {
+                const double cf = a.coef1[o * 4 + q];  // uniform
+                if (cf != 0.0)
+                {
+                    const int src = 4 * (2 * (q / 2) + b / 2) + 2 * (q % 2) + b % 2;
+#pragma unroll
+                    for (int i = 0; i < RPT; i++) u[b][i] += T(cf) * v[src][i];
+                }
+            }
+        }
+        for (int p = 0; p < 7; p++)
+        {
+            T acc[RPT];
+#pragma unroll
+            for (int i = 0; i < RPT; i++) acc[i] = T(0);
+#pragma unroll
+            for (int b = 0; b < 4; b++)
+            {
+                const double cf = a.coef1[p * 4 + b];
+                if (cf != 0.0)
+#pragma unroll
+                    for (int i = 0; i < RPT; i++) acc[i] += T(cf) * u[b][i];
+            }
+            T* o_ = out + (7 * o + p) * stride;
+            if constexpr (PAIR)
+                *reinterpret_cast<vec2*>(o_) = vec2{acc[0], acc[1]};
+            else
+                o_[0] = acc[0];
+        }
+    }
+}
+
 template <typename T>
 hipError_t multisums_typed(const MultiSumArgs& a, hipStream_t s)
 {
@@ -183,6 +254,11 @@ hipError_t multisums_typed(const MultiSumArgs& a, hipStream_t s)
     {
         if (a.pair) hipLaunchKernelGGL((operand_multisums_kernel<T, true, 4>), grid, dim3(256), 0, s, a);
         else hipLaunchKernelGGL((operand_multisums_kernel<T, false, 4>), grid, dim3(256), 0, s, a);
+    }
+    else if (a.nsrc == 16 && a.coef1 && a.nout == 49)
+    {
+        if (a.pair) hipLaunchKernelGGL((operand_multisums_kron_kernel<T, true>), grid, dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((operand_multisums_kron_kernel<T, false>), grid, dim3(256), 0, s, a);
     }
     else if (a.nsrc == 16)
     {

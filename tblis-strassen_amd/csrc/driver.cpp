@@ -252,6 +252,15 @@ void build_msum(Plan& p, int depth, int64_t om, int64_t on, int64_t ok)
         for (const SubTerm& t : one[o].A) p.h_msum_coef_a[o * size_t(nsrc) + size_t(t.subid)] = t.coeff;
         for (const SubTerm& t : one[o].B) p.h_msum_coef_b[o * size_t(nsrc) + size_t(t.subid)] = t.coeff;
     }
+    // the 2-level coefficients are the Kronecker square of the 1-level ones (plan.cpp: kron)
+    const auto& l1 = strassen_products_1level();
+    p.h_msum_coef1_a.assign(depth == 2 ? 28 : 0, 0.0);
+    p.h_msum_coef1_b.assign(depth == 2 ? 28 : 0, 0.0);
+    for (size_t o = 0; depth == 2 && o < l1.size(); o++)
+    {
+        for (const SubTerm& t : l1[o].A) p.h_msum_coef1_a[o * 4 + size_t(t.subid)] = t.coeff;
+        for (const SubTerm& t : l1[o].B) p.h_msum_coef1_b[o * 4 + size_t(t.subid)] = t.coeff;
+    }
 }
 
 // Tables of the staged form of an ABC plan (see Plan::outer_staged), staging the sums of the first
@@ -760,6 +769,8 @@ Status upload(Plan& p)
          i_mca = add(p.h_msum_coef_a.data(), p.h_msum_coef_a.size() * 8),
          i_mcb = add(p.h_msum_coef_b.data(), p.h_msum_coef_b.size() * 8),
          i_msa = add(p.h_msum_shift_a.data(), p.h_msum_shift_a.size() * 8),
+         i_mc1a = add(p.h_msum_coef1_a.data(), p.h_msum_coef1_a.size() * 8),
+         i_mc1b = add(p.h_msum_coef1_b.data(), p.h_msum_coef1_b.size() * 8),
          i_msb = add(p.h_msum_shift_b.data(), p.h_msum_shift_b.size() * 8),
          i_osf = add(p.h_outer_shifted.data(), p.h_outer_shifted.size() * sizeof(ProdDesc)),
          i_ins = add(inner_staged.data(), inner_staged.size() * sizeof(ProdDesc)),
@@ -812,6 +823,8 @@ Status upload(Plan& p)
     p.msum_coef_a = reinterpret_cast<const double*>(at(i_mca));
     p.msum_coef_b = reinterpret_cast<const double*>(at(i_mcb));
     p.msum_shift_a = reinterpret_cast<const int64_t*>(at(i_msa));
+    p.msum_coef1_a = p.h_msum_coef1_a.empty() ? nullptr : reinterpret_cast<const double*>(at(i_mc1a));
+    p.msum_coef1_b = p.h_msum_coef1_b.empty() ? nullptr : reinterpret_cast<const double*>(at(i_mc1b));
     p.msum_shift_b = reinterpret_cast<const int64_t*>(at(i_msb));
     p.outer_shifted = reinterpret_cast<const ProdDesc*>(at(i_osf));
     p.inner_staged = reinterpret_cast<const ProdDesc*>(at(i_ins));
@@ -1007,6 +1020,7 @@ hipError_t run_outer_staged(const Plan& p, const Call& c, hipStream_t stream)
             ma.nsrc = p.msum_nsrc;
             ma.nout = int(nouter);
             ma.coef = isA ? p.msum_coef_a : p.msum_coef_b;
+            ma.coef1 = isA ? p.msum_coef1_a : p.msum_coef1_b;
             ma.shift = isA ? p.msum_shift_a : p.msum_shift_b;
             ma.out = isA ? S : T;
             ma.pair = pair;
@@ -1320,6 +1334,7 @@ Status run_plan_lowp(const Plan& p, const Call& c, int precision, int nslices, h
                 ma.nsrc = p.msum_nsrc;
                 ma.nout = p.nprod;
                 ma.coef = isA ? p.msum_coef_a : p.msum_coef_b;
+                ma.coef1 = isA ? p.msum_coef1_a : p.msum_coef1_b;
                 ma.shift = isA ? p.msum_shift_a : p.msum_shift_b;
                 ma.out = isA ? S.p : T.p;
                 ma.pair = false;

@@ -105,6 +105,8 @@ struct Plan
     bool msum_a = false, msum_b = false;
     int msum_nsrc = 0;
     std::vector<double> h_msum_coef_a, h_msum_coef_b;
+    std::vector<double> h_msum_coef1_a, h_msum_coef1_b;  // 1-level 7 x 4 matrices (Kronecker factor, depth 2)
+    const double *msum_coef1_a = nullptr, *msum_coef1_b = nullptr;
     std::vector<int64_t> h_msum_shift_a, h_msum_shift_b;
     const double *msum_coef_a = nullptr, *msum_coef_b = nullptr;
     const int64_t *msum_shift_a = nullptr, *msum_shift_b = nullptr;

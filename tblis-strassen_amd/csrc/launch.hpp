@@ -81,6 +81,10 @@ struct MultiSumArgs
     const int64_t* shift;
     void* out;
     bool pair;
+    // 2-level sums (nsrc = 16, nout = 49) whose coefficient matrix is the Kronecker square of this
+    // 7 x 4 matrix (products o, i; sub-blocks a, b of the 2 x 2 split): computed in two steps,
+    // U[o][b] = sum_a coef1[o][a] src(a, b), then out[7 o + i] = sum_b coef1[i][b] U[o][b]
+    const double* coef1;
 };
 hipError_t launch_operand_multisums(int elem_size, const MultiSumArgs& a, hipStream_t stream);
 
