@@ -720,11 +720,17 @@ Status upload(Plan& p)
         p.kf_dense.mode = !p.kf_dense.on ? KMODE_GENERAL : (p.ks % TK == 0 ? KMODE_AFFINE_VEC : KMODE_AFFINE);
         p.kf_dense.sa = p.ms;
         p.kf_dense.sb = 1;
+        // S K-fast ([q][i][k]): both operands K-fast, the direct-to-LDS path when K-tiles are whole
+        p.kf_dense_k.on = (p.ms - 1) * p.ks + TK < limit && (p.ns - 1) * p.ks + TK < limit;
+        p.kf_dense_k.mode = !p.kf_dense_k.on ? KMODE_GENERAL : (p.ks % TK == 0 ? KMODE_AFFINE_VEC : KMODE_AFFINE);
+        p.kf_dense_k.sa = 1;
+        p.kf_dense_k.sb = 1;
     }
-    std::vector<int64_t> s_m, s_k, t_k, t_n, s_krun, t_krun;
+    std::vector<int64_t> s_m, s_k, t_k, t_n, s_krun, t_krun, sk_m;
     if (p.core)
     {
         for (int64_t i = 0; i < p.ms; i++) s_m.push_back(i);
+        for (int64_t i = 0; i < p.ms; i++) sk_m.push_back(i * p.ks);
         for (int64_t i = 0; i < p.ks; i++) s_k.push_back(i * p.ms);
         for (int64_t i = 0; i < p.ks; i++) t_k.push_back(i);
         for (int64_t i = 0; i < p.ns; i++) t_n.push_back(i * p.ks);
@@ -761,6 +767,7 @@ Status upload(Plan& p)
          i_tb = add(p.h_term_begin.data(), p.h_term_begin.size() * sizeof(int)),
          i_te = add(p.h_terms.data(), p.h_terms.size() * sizeof(CombineTerm)),
          i_sm = add(s_m.data(), s_m.size() * 8), i_sk = add(s_k.data(), s_k.size() * 8),
+         i_skm = add(sk_m.data(), sk_m.size() * 8),
          i_skr = add(s_krun.data(), s_krun.size() * 8), i_tk = add(t_k.data(), t_k.size() * 8),
          i_tkr = add(t_krun.data(), t_krun.size() * 8), i_tn = add(t_n.data(), t_n.size() * 8),
          i_dp = add(p.h_dense_prods.data(), p.h_dense_prods.size() * sizeof(ProdDesc)),
@@ -812,6 +819,7 @@ Status upload(Plan& p)
     p.term_begin = reinterpret_cast<const int*>(at(i_tb));
     p.terms = reinterpret_cast<const CombineTerm*>(at(i_te));
     p.s_m = reinterpret_cast<const int64_t*>(at(i_sm));
+    p.sk_m = reinterpret_cast<const int64_t*>(at(i_skm));
     p.s_k = reinterpret_cast<const int64_t*>(at(i_sk));
     p.s_krun = reinterpret_cast<const int64_t*>(at(i_skr));
     p.t_k = reinterpret_cast<const int64_t*>(at(i_tk));
@@ -1167,8 +1175,30 @@ Status run_plan(const Plan& p, const Call& c, hipStream_t stream)
         HIP_TRY(hipMallocAsync(&T, tbytes, stream));
         HIP_TRY(hipMallocAsync(&M, mbytes, stream));
         hipError_t e = hipSuccess;
+        // one-pass sums (every source sub-block read once) when the sub-blocks are translates and K has
+        // unit stride: S then comes out K-fast ([q][i][k], like T) and the dense GEMM reads both operands
+        // K-fast (straight into LDS)
+        const bool d2 = p.msum_nsrc == int(p.d * p.d);
+        const bool s_k = d2 && p.msum_a && !p.a_mfast && p.kf_dense_k.on, t_one = d2 && p.msum_b && !p.b_nfast;
         for (int64_t z = 0; z < p.nbatch && e == hipSuccess; z++)
         {
+            auto multisums = [&](bool isA) {
+                MultiSumArgs ma;
+                std::memset(&ma, 0, sizeof(ma));
+                ma.X = isA ? c.A : c.B;
+                ma.zoff = isA ? p.h_batch_a[size_t(z)] : p.h_batch_b[size_t(z)];
+                ma.rs = isA ? p.a_k : p.b_k;
+                ma.cs = isA ? p.a_m : p.b_n;
+                ma.rows = p.ks;
+                ma.cols = isA ? p.ms : p.ns;
+                ma.nsrc = p.msum_nsrc;
+                ma.nout = p.nprod;
+                ma.coef = isA ? p.msum_coef_a : p.msum_coef_b;
+                ma.coef1 = isA ? p.msum_coef1_a : p.msum_coef1_b;
+                ma.shift = isA ? p.msum_shift_a : p.msum_shift_b;
+                ma.out = isA ? S : T;
+                return launch_operand_multisums(p.elem, ma, stream);
+            };
             SumArgs sa;
             std::memset(&sa, 0, sizeof(sa));
             sa.X = c.A;
@@ -1182,7 +1212,7 @@ Status run_plan(const Plan& p, const Call& c, hipStream_t stream)
             sa.rows_fast = p.a_mfast;
             sa.out = S;
             sa.zoff = p.h_batch_a[size_t(z)];
-            e = launch_operand_sums(p.elem, sa, stream);
+            e = s_k ? multisums(true) : launch_operand_sums(p.elem, sa, stream);
             if (e != hipSuccess) break;
             sa.X = c.B;
             sa.rs = p.b_k;
@@ -1193,16 +1223,16 @@ Status run_plan(const Plan& p, const Call& c, hipStream_t stream)
             sa.rows_fast = !p.b_nfast;
             sa.out = T;
             sa.zoff = p.h_batch_b[size_t(z)];
-            e = launch_operand_sums(p.elem, sa, stream);
+            e = t_one ? multisums(false) : launch_operand_sums(p.elem, sa, stream);
             if (e != hipSuccess) break;
 
             GemmArgs gd;
             std::memset(&gd, 0, sizeof(gd));
             gd.A = S;
             gd.B = T;
-            gd.a_m = p.s_m;
-            gd.a_k = p.s_k;
-            gd.a_krun = p.s_krun;
+            gd.a_m = s_k ? p.sk_m : p.s_m;
+            gd.a_k = s_k ? p.t_k : p.s_k;
+            gd.a_krun = s_k ? p.t_krun : p.s_krun;
             gd.b_k = p.t_k;
             gd.b_krun = p.t_krun;
             gd.b_n = p.t_n;
@@ -1214,10 +1244,11 @@ Status run_plan(const Plan& p, const Call& c, hipStream_t stream)
             gd.nbatch = 1;
             gd.M = M;
             gd.zero = p.zero;
-            gd.ka_stride = p.kf_dense.sa;
-            gd.kb_stride = p.kf_dense.sb;
+            const KFast& kd = s_k ? p.kf_dense_k : p.kf_dense;
+            gd.ka_stride = kd.sa;
+            gd.kb_stride = kd.sb;
             timing_begin(stream);
-            e = launch_leaf_gemm(p.elem, EPI_STORE, dense_tile(p), 1, p.kf_dense.mode, gd, true, false, stream);
+            e = launch_leaf_gemm(p.elem, EPI_STORE, dense_tile(p), 1, kd.mode, gd, !s_k, false, stream);
             timing_end(stream, 2.0 * double(p.nprod) * double(p.ms) * double(p.ns) * double(p.ks));
             if (e != hipSuccess) break;
 

@@ -132,6 +132,9 @@ struct Plan
     const int64_t *s_m = nullptr, *s_k = nullptr, *s_krun = nullptr, *t_k = nullptr, *t_krun = nullptr,
                   *t_n = nullptr;
     const ProdDesc* dense_prods = nullptr;
+    // Naive with S written K-fast (one-pass sums): row offsets i * ks of S; its K vector is t_k
+    const int64_t* sk_m = nullptr;
+    KFast kf_dense_k;
     const void* zero = nullptr;  // 256 zeroed bytes
 
     // general (non-pure) contractions: reduce/broadcast descriptors
