@@ -999,6 +999,7 @@ hipError_t run_outer_staged(const Plan& p, const Call& c, hipStream_t stream)
     const size_t sbytes = nouter * size_t(p.os_m) * size_t(p.os_k) * size_t(p.elem);
     const size_t tbytes = nouter * size_t(p.os_k) * size_t(p.os_n) * size_t(p.elem);
     void *S = nullptr, *T = nullptr;
+    // an allocation failure returns before anything is launched (the caller then takes the fused form)
     hipError_t e = hipMallocAsync(&S, sbytes, stream);
     if (e != hipSuccess) return e;
     e = hipMallocAsync(&T, tbytes, stream);
@@ -1099,9 +1100,22 @@ Status run_plan(const Plan& p, const Call& c, hipStream_t stream)
     if (p.nprod == 0) return Status();
     GemmArgs g = base_args(p, c);
 
+    // staged form; when its workspace cannot be allocated (nothing has run yet), the fused form below,
+    // which needs none
+    bool staged_done = false;
     if (p.outer_staged)
     {
-        HIP_TRY(run_outer_staged(p, c, stream));
+        const hipError_t e = run_outer_staged(p, c, stream);
+        if (e == hipErrorOutOfMemory)
+            (void)hipGetLastError();
+        else
+        {
+            HIP_TRY(e);
+            staged_done = true;
+        }
+    }
+    if (staged_done)
+    {
     }
     else if (!p.core || p.impl == IMPL_ABC || p.impl == IMPL_BLAS || p.impl == IMPL_REF)
     {
