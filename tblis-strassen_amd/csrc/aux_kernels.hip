@@ -312,34 +312,170 @@ hipError_t launch_operand_sums(int elem_size, const SumArgs& a, hipStream_t s)
 
 // ---- combine ----------------------------------------------------------------------------------------
 
+// One workgroup covers kCombineRows x (4 * kCombineCols) elements of one C sub-block: each wave takes 64
+// consecutive rows (M is row-fast, so every load of a wave is one 512-byte run for FP64) and each lane
+// kCombineCols columns, whose loads are issued together.  No integer division per element; the row's C
+// offset is read once per lane, the column offsets once per column.  The terms of a sub-block are summed
+// in table order in FP64, the same arithmetic as the per-element form it replaces.
+constexpr int kCombineRows = 64;
+constexpr int kCombineCols = 8;
+
 template <typename T>
 __global__ __launch_bounds__(256) void combine_kernel(CombineArgs a)
 {
     const int s = blockIdx.y;
-    const int64_t z = blockIdx.z;
     const int64_t nel = a.ms * a.ns;
+    const int64_t row_tiles = (a.ms + kCombineRows - 1) / kCombineRows;
+    const int64_t rt = int64_t(blockIdx.x) % row_tiles, ct = int64_t(blockIdx.x) / row_tiles;
+    const int64_t r = rt * kCombineRows + (threadIdx.x & (kCombineRows - 1));
+    const int64_t c0 = (ct * 4 + (threadIdx.x >> 6)) * kCombineCols;
+    if (r >= a.ms || c0 >= a.ns) return;  // no barrier below
+    const int ncol = a.ns - c0 < kCombineCols ? int(a.ns - c0) : kCombineCols;
     const int tb = a.term_begin[s], te = a.term_begin[s + 1];
-    T* C = static_cast<T*>(a.C) + (a.batch_c ? a.batch_c[z] : 0);
-    const T* M = static_cast<const T*>(a.M) + z * int64_t(a.nprod) * nel;
     const int64_t roff = a.sub_roff[s], coff = a.sub_coff[s];
-    for (int64_t e = int64_t(blockIdx.x) * 256 + threadIdx.x; e < nel; e += int64_t(gridDim.x) * 256)
+    const int64_t cm = a.c_m[roff + r];
+    for (int64_t z = blockIdx.z; z < a.nbatch; z += gridDim.z)
     {
-        const int64_t r = e % a.ms, c = e / a.ms;
-        double acc = 0.0;
-        for (int t = tb; t < te; t++) acc += a.terms[t].coeff * double(M[int64_t(a.terms[t].prod) * nel + e]);
-        T* ptr = C + a.c_m[roff + r] + a.c_n[coff + c];
-        const double v = a.alpha * acc;
-        *ptr = a.beta == 0.0 ? T(v) : T(a.beta * double(*ptr) + v);
+        T* C = static_cast<T*>(a.C) + (a.batch_c ? a.batch_c[z] : 0) + cm;
+        const T* M = static_cast<const T*>(a.M) + z * int64_t(a.nprod) * nel + c0 * a.ms + r;
+        double acc[kCombineCols];
+#pragma unroll
+        for (int j = 0; j < kCombineCols; j++) acc[j] = 0.0;
+        if (ncol == kCombineCols)
+        {
+            for (int t = tb; t < te; t++)
+            {
+                const double w = a.terms[t].coeff;
+                const T* Mp = M + int64_t(a.terms[t].prod) * nel;
+                T v[kCombineCols];
+#pragma unroll
+                for (int j = 0; j < kCombineCols; j++) v[j] = Mp[int64_t(j) * a.ms];
+#pragma unroll
+                for (int j = 0; j < kCombineCols; j++) acc[j] += w * double(v[j]);
+            }
+        }
+        else
+        {
+            for (int t = tb; t < te; t++)
+            {
+                const double w = a.terms[t].coeff;
+                const T* Mp = M + int64_t(a.terms[t].prod) * nel;
+#pragma unroll
+                for (int j = 0; j < kCombineCols; j++)
+                    if (j < ncol) acc[j] += w * double(Mp[int64_t(j) * a.ms]);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < kCombineCols; j++)
+        {
+            if (j >= ncol) break;
+            T* ptr = C + a.c_n[coff + c0 + j];
+            const double v = a.alpha * acc[j];
+            *ptr = a.beta == 0.0 ? T(v) : T(a.beta * double(*ptr) + v);
+        }
+    }
+}
+
+// All sub-blocks in one pass (CombineArgs::pmask, NSUB = nsub): a lane reads M_p at its NC positions once
+// per product and adds it to every sub-block the product feeds, so M is read once rather than once per C
+// target (12 of 7 products' worth at one level, 144 of 49 at two).  Products are visited in order and a
+// sub-block the product does not feed is skipped (a uniform branch on the kernel-argument mask), so each
+// sub-block sums the same terms in the same order as the per-sub-block form.  The next product's values
+// are loaded before the current ones are used (the last product re-reads itself), with no branch around
+// the loads.
+template <typename T, int NSUB, int NC>
+__global__ __launch_bounds__(256) void combine_all_kernel(CombineArgs a)
+{
+    const int64_t nel = a.ms * a.ns;
+    const int64_t row_tiles = (a.ms + kCombineRows - 1) / kCombineRows;
+    const int64_t rt = int64_t(blockIdx.x) % row_tiles, ct = int64_t(blockIdx.x) / row_tiles;
+    const int64_t r = rt * kCombineRows + (threadIdx.x & (kCombineRows - 1));
+    const int64_t c0 = (ct * 4 + (threadIdx.x >> 6)) * NC;
+    if (r >= a.ms || c0 >= a.ns) return;  // no barrier below
+    const int ncol = a.ns - c0 < NC ? int(a.ns - c0) : NC;
+    const int last = a.nprod - 1;
+    for (int64_t z = blockIdx.z; z < a.nbatch; z += gridDim.z)
+    {
+        const T* M = static_cast<const T*>(a.M) + z * int64_t(a.nprod) * nel + c0 * a.ms + r;
+        double acc[NSUB][NC];
+#pragma unroll
+        for (int s = 0; s < NSUB; s++)
+#pragma unroll
+            for (int j = 0; j < NC; j++) acc[s][j] = 0.0;
+        // columns past the edge read column c0 again (in bounds) and are never stored
+        int64_t coff_j[NC];
+#pragma unroll
+        for (int j = 0; j < NC; j++) coff_j[j] = (j < ncol ? j : 0) * a.ms;
+        T v[NC], vn[NC];
+#pragma unroll
+        for (int j = 0; j < NC; j++) v[j] = M[coff_j[j]];
+        for (int p = 0; p < a.nprod; p++)
+        {
+            const T* Mn = M + int64_t(p < last ? p + 1 : last) * nel;
+#pragma unroll
+            for (int j = 0; j < NC; j++) vn[j] = Mn[coff_j[j]];
+            const unsigned pm = a.pmask[p];
+#pragma unroll
+            for (int s = 0; s < NSUB; s++)
+            {
+                if (pm & (1u << s))
+                {
+                    const double ws = (pm & (1u << (16 + s))) ? -1.0 : 1.0;
+#pragma unroll
+                    for (int j = 0; j < NC; j++) acc[s][j] += ws * double(v[j]);
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < NC; j++) v[j] = vn[j];
+        }
+        T* C = static_cast<T*>(a.C) + (a.batch_c ? a.batch_c[z] : 0);
+#pragma unroll
+        for (int s = 0; s < NSUB; s++)
+        {
+            T* Cs = C + a.c_m[a.sub_roff[s] + r];
+            const int64_t coff = a.sub_coff[s] + c0;
+#pragma unroll
+            for (int j = 0; j < NC; j++)
+            {
+                if (j >= ncol) break;
+                T* ptr = Cs + a.c_n[coff + j];
+                const double v = a.alpha * acc[s][j];
+                *ptr = a.beta == 0.0 ? T(v) : T(a.beta * double(*ptr) + v);
+            }
+        }
     }
 }
 
 hipError_t launch_combine(int elem_size, const CombineArgs& a, hipStream_t s)
 {
     const int64_t nel = a.ms * a.ns;
-    if (nel == 0 || a.nsub == 0 || a.nbatch == 0) return hipSuccess;
-    int64_t blocks = (nel + 255) / 256;
-    if (blocks > 4096) blocks = 4096;
-    dim3 grid(unsigned(blocks), unsigned(a.nsub), unsigned(a.nbatch));
+    if (nel == 0 || a.nsub == 0 || a.nbatch == 0 || a.nprod == 0) return hipSuccess;
+    const int64_t row_tiles = (a.ms + kCombineRows - 1) / kCombineRows;
+    const int64_t zb = a.nbatch < 65535 ? a.nbatch : 65535;
+    auto blocks_for = [&](int cols_per_lane) {
+        return row_tiles * ((a.ns + 4 * cols_per_lane - 1) / (4 * cols_per_lane));
+    };
+    if (a.dense && (a.nsub == 1 || a.nsub == 4 || a.nsub == 16) && a.nprod <= kCombineMaxProds)
+    {
+        const int nc = a.nsub == 16 ? 2 : 8;
+        const int64_t blocks = blocks_for(nc);
+        if (blocks > 0x7fffffff) return hipErrorInvalidValue;
+        const dim3 grid(static_cast<unsigned>(blocks), 1u, static_cast<unsigned>(zb));
+#define MISTRA_COMBINE_ALL(NSUB, NC)                                                                         \
+    do                                                                                                       \
+    {                                                                                                        \
+        if (elem_size == 8) hipLaunchKernelGGL((combine_all_kernel<double, NSUB, NC>), grid, dim3(256), 0, s, a); \
+        else hipLaunchKernelGGL((combine_all_kernel<float, NSUB, NC>), grid, dim3(256), 0, s, a);            \
+    } while (0)
+        if (a.nsub == 16) MISTRA_COMBINE_ALL(16, 2);
+        else if (a.nsub == 4) MISTRA_COMBINE_ALL(4, 8);
+        else MISTRA_COMBINE_ALL(1, 8);
+#undef MISTRA_COMBINE_ALL
+        return hipGetLastError();
+    }
+    const int64_t blocks = blocks_for(kCombineCols);
+    if (blocks > 0x7fffffff || a.nsub > 65535) return hipErrorInvalidValue;
+    const dim3 grid(static_cast<unsigned>(blocks), static_cast<unsigned>(a.nsub), static_cast<unsigned>(zb));
     if (elem_size == 8) hipLaunchKernelGGL((combine_kernel<double>), grid, dim3(256), 0, s, a);
     else hipLaunchKernelGGL((combine_kernel<float>), grid, dim3(256), 0, s, a);
     return hipGetLastError();

@@ -414,6 +414,32 @@ void choose_outer_staged(Plan& p, unsigned flags)
     set(best);
 }
 
+// CombineArgs::pmask of the plan's combine table, or empty when the table does not fit that form
+void build_comb_mask(Plan& p)
+{
+    p.h_comb_mask.assign(size_t(p.nprod), 0u);
+    const bool fits = (p.nsub == 1 || p.nsub == 4 || p.nsub == 16) && p.nprod >= 1 && p.nprod <= kCombineMaxProds;
+    for (int s = 0; fits && s < p.nsub; s++)
+        for (int t = p.h_term_begin[size_t(s)]; t < p.h_term_begin[size_t(s) + 1]; t++)
+        {
+            const CombineTerm& ct = p.h_terms[size_t(t)];
+            unsigned& m = p.h_comb_mask[size_t(ct.prod)];
+            if ((m >> s) & 1u || (ct.coeff != 1.0 && ct.coeff != -1.0))
+            {
+                p.h_comb_mask.clear();
+                return;
+            }
+            m |= (1u << s) | (ct.coeff < 0 ? 1u << (16 + s) : 0u);
+        }
+    if (!fits) p.h_comb_mask.clear();
+}
+
+void set_comb_mask(const Plan& p, CombineArgs& ca)
+{
+    ca.dense = p.h_comb_mask.empty() ? 0 : 1;
+    for (size_t q = 0; q < p.h_comb_mask.size(); q++) ca.pmask[q] = p.h_comb_mask[q];
+}
+
 }  // namespace
 
 Plan::~Plan()
@@ -532,6 +558,7 @@ Status build_plan_host(const tblis::tblis_tensor* A, const label_type* idx_A, co
         p.h_sub_roff = {0};
         p.h_sub_coff = {0};
         p.h_terms = {CombineTerm{0, 1.0}};
+        build_comb_mask(p);
     }
     else
     {
@@ -572,6 +599,9 @@ Status build_plan_host(const tblis::tblis_tensor* A, const label_type* idx_A, co
                     if (t.subid == s) p.h_terms.push_back(CombineTerm{int(q), t.coeff});
         }
         p.h_term_begin[size_t(p.nsub)] = int(p.h_terms.size());
+        // one-pass form: each (product, sub-block) pair at most once with coefficient +-1, so that visiting
+        // the products in order sums every sub-block's terms in its list order
+        build_comb_mask(p);
         for (int q = 0; q < p.nprod; q++)
         {
             ProdDesc pd;
@@ -1168,6 +1198,7 @@ Status run_plan(const Plan& p, const Call& c, hipStream_t stream)
             ca.sub_coff = p.sub_coff;
             ca.term_begin = p.term_begin;
             ca.terms = p.terms;
+            set_comb_mask(p, ca);
             ca.nprod = p.nprod;
             ca.alpha = c.alpha;
             ca.beta = c.beta;
@@ -1279,6 +1310,7 @@ Status run_plan(const Plan& p, const Call& c, hipStream_t stream)
             ca.sub_coff = p.sub_coff;
             ca.term_begin = p.term_begin;
             ca.terms = p.terms;
+            set_comb_mask(p, ca);
             ca.nprod = p.nprod;
             ca.alpha = c.alpha;
             ca.beta = c.beta;
@@ -1480,6 +1512,7 @@ Status run_plan_lowp(const Plan& p, const Call& c, int precision, int nslices, h
             ca.sub_coff = p.sub_coff;
             ca.term_begin = p.term_begin;
             ca.terms = p.terms;
+            set_comb_mask(p, ca);
             ca.nprod = p.nprod;
             ca.alpha = c.alpha;
             ca.beta = c.beta;

@@ -74,6 +74,7 @@ struct Plan
     std::vector<int64_t> h_sub_roff, h_sub_coff;
     std::vector<int> h_term_begin;
     std::vector<CombineTerm> h_terms;
+    std::vector<unsigned> h_comb_mask;  // the same as CombineArgs::pmask (empty if not representable)
 
     // Naive: dense products over S_p (ms x ks) and T_p (ks x ns)
     std::vector<ProdDesc> h_dense_prods;

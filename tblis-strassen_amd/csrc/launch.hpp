@@ -90,6 +90,7 @@ hipError_t launch_operand_multisums(int elem_size, const MultiSumArgs& a, hipStr
 
 // Combine staged products into C sub-blocks:
 //   C[c_m[roff_s + r] + c_n[coff_s + c]] = beta*C + alpha * sum_{(p,coef) in list_s} coef*M_p[r + c*ms]
+constexpr int kCombineMaxProds = 49;
 struct CombineTerm
 {
     int prod;
@@ -111,6 +112,11 @@ struct CombineArgs
     double alpha, beta;
     int64_t nbatch;
     const int64_t* batch_c;
+    // Optional one-pass form (dense != 0; nsub in {1, 4, 16}, nprod <= kCombineMaxProds, every coefficient
+    // +-1): pmask[p] bit s says product p feeds sub-block s, bit 16 + s that it does with coefficient -1.
+    // Every M_p is then read once and added to all its sub-blocks.
+    int dense;
+    unsigned pmask[kCombineMaxProds];
 };
 hipError_t launch_combine(int elem_size, const CombineArgs& a, hipStream_t stream);
 
