@@ -569,15 +569,66 @@ __device__ __forceinline__ void leaf_gemm_body(const GemmArgs& g, int bx, int by
             dma(0, read_kinfo());
             fetch_kinfo(1);
             __syncthreads();  // waits for the loads (vmcnt) before the barrier
-            for (int64_t kt = 0; kt < nkt; kt++)
+            if constexpr (K4 / CH == 2)
             {
-                const int cur = int(kt & 1);
-                const KOff kn = read_kinfo();
-                fetch_kinfo(kt + 2);
-                // buffer cur ^ 1 was last read in iteration kt - 1, before its closing barrier
-                if (kt + 1 < nkt) dma(cur ^ 1, kn);
-                mfma_dma(sA[cur], sB[cur]);
-                __syncthreads();
+                // Two fragment groups per K-tile, software-pipelined: group 1 of tile kt is read while
+                // group 0's MFMAs run, and group 0 of tile kt + 1 right after the barrier that publishes
+                // it, while group 1's MFMAs run, so no MFMA waits for an LDS read it could have had
+                // earlier.  After the last barrier the read of the other buffer is stale and unused
+                // (it stays unconditional: no branch around reads).
+                auto read_grp = [&](const T* __restrict__ la, const T* __restrict__ lb, int grp, vec16* af,
+                                    vec16* bf) {
+                    const int ko = ((grp * 4 + (lane >> 4)) ^ (lane & 7)) * CH;
+#pragma unroll
+                    for (int i = 0; i < FM; i++)
+                        af[i] = *reinterpret_cast<const vec16*>(&la[(wm0 + i * 16 + (lane & 15)) * TK + ko]);
+#pragma unroll
+                    for (int j = 0; j < FN; j++)
+                        bf[j] = *reinterpret_cast<const vec16*>(&lb[(wn0 + j * 16 + (lane & 15)) * TK + ko]);
+                };
+                auto mma_grp = [&](const vec16* af, const vec16* bf) {
+#pragma unroll
+                    for (int e = 0; e < CH; e++)
+#pragma unroll
+                        for (int i = 0; i < FM; i++)
+#pragma unroll
+                            for (int j = 0; j < FN; j++) acc[i][j] = Mfma<T>::run(af[i][e], bf[j][e], acc[i][j]);
+                };
+                vec16 a0[FM], b0[FN], a1[FM], b1[FN];
+                read_grp(sA[0], sB[0], 0, a0, b0);
+                for (int64_t kt = 0; kt < nkt; kt++)
+                {
+                    const int cur = int(kt & 1);
+                    const KOff kn = read_kinfo();
+                    fetch_kinfo(kt + 2);
+                    // buffer cur ^ 1 was last read in iteration kt - 1, before its closing barrier
+                    if (kt + 1 < nkt) dma(cur ^ 1, kn);
+                    read_grp(sA[cur], sB[cur], 1, a1, b1);
+                    // scheduling fences keep this order: left alone, the scheduler sinks the reads after
+                    // the barrier into the next iteration and hoists the barrier (and its wait for all
+                    // reads) above group 0's MFMAs
+                    __builtin_amdgcn_sched_barrier(0);
+                    mma_grp(a0, b0);
+                    __builtin_amdgcn_sched_barrier(0);
+                    __syncthreads();
+                    read_grp(sA[cur ^ 1], sB[cur ^ 1], 0, a0, b0);
+                    __builtin_amdgcn_sched_barrier(0);
+                    mma_grp(a1, b1);
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+            }
+            else
+            {
+                for (int64_t kt = 0; kt < nkt; kt++)
+                {
+                    const int cur = int(kt & 1);
+                    const KOff kn = read_kinfo();
+                    fetch_kinfo(kt + 2);
+                    // buffer cur ^ 1 was last read in iteration kt - 1, before its closing barrier
+                    if (kt + 1 < nkt) dma(cur ^ 1, kn);
+                    mfma_dma(sA[cur], sB[cur]);
+                    __syncthreads();
+                }
             }
             epilogue(P, p, acc, true, double(acoef[0]) * double(bcoef[0]));
             return;
