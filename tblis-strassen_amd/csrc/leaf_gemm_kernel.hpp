@@ -492,13 +492,12 @@ __device__ __forceinline__ void leaf_gemm_body(const GemmArgs& g, int bx, int by
         // 128-byte bank window.  The operands are loaded as stored, so the term coefficients (+-1) move
         // to the epilogue.
         //
-        // The next tile's loads are issued before the current tile's fragment reads, and the loop keeps
-        // the buffer index a run-time value and the loads unconditional: that way the compiler leaves
-        // the reads without a wait for the loads (measured: 69.5 TFLOP/s classical at 8192^3), while a
-        // branch around the loads, an unrolled loop with constant buffers, reads merged into
-        // ds_read2st64, or the reads moved before the loads all made it wait, or issue the loads late,
-        // and cost 8-13%.  Correctness never depends on this: the vmcnt(0) before each barrier waits
-        // for the loads.
+        // The loop keeps the buffer index a run-time value, and the fragment reads that follow a load
+        // into LDS in program order sit in another basic block than the load: that way the compiler
+        // leaves the reads without a wait for the loads, while an unrolled loop with constant buffers,
+        // reads merged into ds_read2st64, or reads in the loads' own block all made it wait, or issue
+        // the loads late, and cost 8-13%.  Correctness never depends on this: an explicit vmcnt(0)
+        // before each barrier that publishes a tile waits for the loads.
         if constexpr (kDma)
         {
             typedef __attribute__((address_space(3))) void* lds_ptr;
@@ -538,6 +537,15 @@ __device__ __forceinline__ void leaf_gemm_body(const GemmArgs& g, int bx, int by
                 (void)ko;
 #endif
             };
+            // Every barrier that publishes loaded tiles is preceded by this wait for the wave's own loads
+            // into LDS (vmcnt counts them).  It is written out rather than left to the compiler, whose
+            // wait insertion does not reliably see a later LDS read depending on an earlier load into LDS
+            // across loop iterations.
+            auto dma_landed = [&]() {
+#if defined(__HIP_DEVICE_COMPILE__)
+                __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0); expcnt and lgkmcnt unconstrained
+#endif
+            };
             // The MFMA sums over whichever k each lane group supplies, as long as A and B agree: lane
             // group g (lane / 16) takes chunk 4 grp + g of the row for the CH consecutive MFMA steps of
             // group grp, element e of the chunk in step CH grp + e, so one 16-byte LDS read feeds CH
@@ -565,17 +573,17 @@ __device__ __forceinline__ void leaf_gemm_body(const GemmArgs& g, int bx, int by
                             for (int j = 0; j < FN; j++) acc[i][j] = Mfma<T>::run(af[i][e], bf[j][e], acc[i][j]);
                 }
             };
-            fetch_kinfo(0);
-            dma(0, read_kinfo());
-            fetch_kinfo(1);
-            __syncthreads();  // waits for the loads (vmcnt) before the barrier
             if constexpr (K4 / CH == 2)
             {
-                // Two fragment groups per K-tile, software-pipelined: group 1 of tile kt is read while
-                // group 0's MFMAs run, and group 0 of tile kt + 1 right after the barrier that publishes
-                // it, while group 1's MFMAs run, so no MFMA waits for an LDS read it could have had
-                // earlier.  After the last barrier the read of the other buffer is stale and unused
-                // (it stays unconditional: no branch around reads).
+                // Two fragment groups per K-tile, software-pipelined.  Iteration kt reads group 1 of
+                // tile kt while group 0's MFMAs run; after the barrier (tile kt fully read, tile kt + 1
+                // landed) it reads group 0 of tile kt + 1 and loads tile kt + 2 into the buffer tile kt
+                // leaves, while group 1's MFMAs run.  A load thus has a whole iteration (64 MFMAs per
+                // wave) to land before the barrier that publishes it, and no MFMA waits for an LDS read
+                // it could have had earlier.  The loads sit in their own block, after the reads: a read
+                // after a load into LDS in the same block makes the compiler wait for the load (it cannot
+                // tell the buffers apart).  After the last barrier the read of the other buffer is stale
+                // and unused (it stays unconditional: no branch around reads).
                 auto read_grp = [&](const T* __restrict__ la, const T* __restrict__ lb, int grp, vec16* af,
                                     vec16* bf) {
                     const int ko = ((grp * 4 + (lane >> 4)) ^ (lane & 7)) * CH;
@@ -594,31 +602,50 @@ __device__ __forceinline__ void leaf_gemm_body(const GemmArgs& g, int bx, int by
 #pragma unroll
                             for (int j = 0; j < FN; j++) acc[i][j] = Mfma<T>::run(af[i][e], bf[j][e], acc[i][j]);
                 };
+                fetch_kinfo(0);
+                dma(0, read_kinfo());
+                fetch_kinfo(1);
+                dma(1, read_kinfo());  // with one tile the offsets clamp to it: a repeat, waited for below
+                fetch_kinfo(2);
+                dma_landed();
+                __syncthreads();
                 vec16 a0[FM], b0[FN], a1[FM], b1[FN];
                 read_grp(sA[0], sB[0], 0, a0, b0);
                 for (int64_t kt = 0; kt < nkt; kt++)
                 {
                     const int cur = int(kt & 1);
-                    const KOff kn = read_kinfo();
-                    fetch_kinfo(kt + 2);
-                    // buffer cur ^ 1 was last read in iteration kt - 1, before its closing barrier
-                    if (kt + 1 < nkt) dma(cur ^ 1, kn);
                     read_grp(sA[cur], sB[cur], 1, a1, b1);
-                    // scheduling fences keep this order: left alone, the scheduler sinks the reads after
-                    // the barrier into the next iteration and hoists the barrier (and its wait for all
-                    // reads) above group 0's MFMAs
-                    __builtin_amdgcn_sched_barrier(0);
                     mma_grp(a0, b0);
+                    // group 1's reads go between group 0's first MFMAs (each MFMA occupies the pipe for
+                    // many cycles; the wave issues the reads in its shadow)
+#pragma unroll
+                    for (int i = 0; i < FM + FN; i++)
+                    {
+                        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+                        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // DS read
+                    }
+                    // scheduling fences keep the rest in this order: left alone, the scheduler sinks
+                    // the reads after the barrier into the next iteration and hoists the barrier (and
+                    // its wait for all reads) above group 0's MFMAs
                     __builtin_amdgcn_sched_barrier(0);
+                    dma_landed();
                     __syncthreads();
                     read_grp(sA[cur ^ 1], sB[cur ^ 1], 0, a0, b0);
                     __builtin_amdgcn_sched_barrier(0);
+                    const KOff kn = read_kinfo();
+                    fetch_kinfo(kt + 3);
+                    if (kt + 2 < nkt) dma(cur, kn);
                     mma_grp(a1, b1);
                     __builtin_amdgcn_sched_barrier(0);
                 }
             }
             else
             {
+                fetch_kinfo(0);
+                dma(0, read_kinfo());
+                fetch_kinfo(1);
+                dma_landed();
+                __syncthreads();
                 for (int64_t kt = 0; kt < nkt; kt++)
                 {
                     const int cur = int(kt & 1);
@@ -627,6 +654,7 @@ __device__ __forceinline__ void leaf_gemm_body(const GemmArgs& g, int bx, int by
                     // buffer cur ^ 1 was last read in iteration kt - 1, before its closing barrier
                     if (kt + 1 < nkt) dma(cur ^ 1, kn);
                     mfma_dma(sA[cur], sB[cur]);
+                    dma_landed();
                     __syncthreads();
                 }
             }
