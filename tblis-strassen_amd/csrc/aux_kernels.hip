@@ -121,7 +121,7 @@ __global__ __launch_bounds__(256) void operand_sums_rows_kernel(SumArgs a)
 
 // One pass over the sources for all staged sums (MultiSumArgs): a thread owns RPT consecutive rows of
 // one column, loads its NSRC source values once and writes every output from registers.
-template <typename T, bool PAIR, int NSRC>
+template <typename T, bool PAIR, int NSRC, bool BYVAL>
 __global__ __launch_bounds__(256) void operand_multisums_kernel(MultiSumArgs a)
 {
     constexpr int RPT = PAIR ? 2 : 1;
@@ -155,7 +155,7 @@ __global__ __launch_bounds__(256) void operand_multisums_kernel(MultiSumArgs a)
 #pragma unroll
         for (int s = 0; s < NSRC; s++)
         {
-            const double cf = a.coef[q * NSRC + s];  // uniform
+            const double cf = BYVAL ? a.cv[q * NSRC + s] : a.coef[q * NSRC + s];  // uniform
             if (cf != 0.0)
 #pragma unroll
                 for (int i = 0; i < RPT; i++) acc[i] += T(cf) * v[s][i];
@@ -170,7 +170,7 @@ __global__ __launch_bounds__(256) void operand_multisums_kernel(MultiSumArgs a)
 // The 2-level one-pass sums through the Kronecker factorization of their coefficients (MultiSumArgs::
 // coef1): source (a, b) -- outer sub-block a, inner b -- is sub-block 4 (2 (a / 2) + b / 2) + 2 (a % 2) +
 // b % 2 of the 4 x 4 split.
-template <typename T, bool PAIR>
+template <typename T, bool PAIR, bool BYVAL>
 __global__ __launch_bounds__(256) void operand_multisums_kron_kernel(MultiSumArgs a)
 {
     constexpr int RPT = PAIR ? 2 : 1;
@@ -207,7 +207,7 @@ __global__ __launch_bounds__(256) void operand_multisums_kron_kernel(MultiSumArg
 #pragma unroll
             for (int q = 0; q < 4; q++)  // outer sub-block
             {
-                const double cf = a.coef1[o * 4 + q];  // uniform
+                const double cf = BYVAL ? a.cv[o * 4 + q] : a.coef1[o * 4 + q];  // uniform
                 if (cf != 0.0)
                 {
                     const int src = 4 * (2 * (q / 2) + b / 2) + 2 * (q % 2) + b % 2;
@@ -224,7 +224,7 @@ __global__ __launch_bounds__(256) void operand_multisums_kron_kernel(MultiSumArg
 #pragma unroll
             for (int b = 0; b < 4; b++)
             {
-                const double cf = a.coef1[p * 4 + b];
+                const double cf = BYVAL ? a.cv[p * 4 + b] : a.coef1[p * 4 + b];
                 if (cf != 0.0)
 #pragma unroll
                     for (int i = 0; i < RPT; i++) acc[i] += T(cf) * u[b][i];
@@ -238,6 +238,18 @@ __global__ __launch_bounds__(256) void operand_multisums_kron_kernel(MultiSumArg
     }
 }
 
+template <typename T, bool PAIR, bool BYVAL>
+hipError_t multisums_launch(const MultiSumArgs& a, dim3 grid, hipStream_t s)
+{
+    if (a.nsrc == 1) hipLaunchKernelGGL((operand_multisums_kernel<T, PAIR, 1, BYVAL>), grid, dim3(256), 0, s, a);
+    else if (a.nsrc == 4) hipLaunchKernelGGL((operand_multisums_kernel<T, PAIR, 4, BYVAL>), grid, dim3(256), 0, s, a);
+    else if (a.nsrc == 16 && a.coef1 && a.nout == 49)
+        hipLaunchKernelGGL((operand_multisums_kron_kernel<T, PAIR, BYVAL>), grid, dim3(256), 0, s, a);
+    else if (a.nsrc == 16) hipLaunchKernelGGL((operand_multisums_kernel<T, PAIR, 16, BYVAL>), grid, dim3(256), 0, s, a);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
 template <typename T>
 hipError_t multisums_typed(const MultiSumArgs& a, hipStream_t s)
 {
@@ -245,29 +257,10 @@ hipError_t multisums_typed(const MultiSumArgs& a, hipStream_t s)
     const int64_t blocks = ((a.rows + 256 * rpt - 1) / (256 * rpt)) * a.cols;
     if (blocks == 0 || a.nout == 0) return hipSuccess;
     const dim3 grid(static_cast<unsigned>(blocks));
-    if (a.nsrc == 1)
-    {
-        if (a.pair) hipLaunchKernelGGL((operand_multisums_kernel<T, true, 1>), grid, dim3(256), 0, s, a);
-        else hipLaunchKernelGGL((operand_multisums_kernel<T, false, 1>), grid, dim3(256), 0, s, a);
-    }
-    else if (a.nsrc == 4)
-    {
-        if (a.pair) hipLaunchKernelGGL((operand_multisums_kernel<T, true, 4>), grid, dim3(256), 0, s, a);
-        else hipLaunchKernelGGL((operand_multisums_kernel<T, false, 4>), grid, dim3(256), 0, s, a);
-    }
-    else if (a.nsrc == 16 && a.coef1 && a.nout == 49)
-    {
-        if (a.pair) hipLaunchKernelGGL((operand_multisums_kron_kernel<T, true>), grid, dim3(256), 0, s, a);
-        else hipLaunchKernelGGL((operand_multisums_kron_kernel<T, false>), grid, dim3(256), 0, s, a);
-    }
-    else if (a.nsrc == 16)
-    {
-        if (a.pair) hipLaunchKernelGGL((operand_multisums_kernel<T, true, 16>), grid, dim3(256), 0, s, a);
-        else hipLaunchKernelGGL((operand_multisums_kernel<T, false, 16>), grid, dim3(256), 0, s, a);
-    }
-    else
-        return hipErrorInvalidValue;
-    return hipGetLastError();
+    const bool kron = a.nsrc == 16 && a.coef1 && a.nout == 49;
+    const bool byval = kron ? a.ncv == 28 : (a.ncv > 0 && a.ncv == a.nsrc * a.nout);
+    if (a.pair) return byval ? multisums_launch<T, true, true>(a, grid, s) : multisums_launch<T, true, false>(a, grid, s);
+    return byval ? multisums_launch<T, false, true>(a, grid, s) : multisums_launch<T, false, false>(a, grid, s);
 }
 
 hipError_t launch_operand_multisums(int elem_size, const MultiSumArgs& a, hipStream_t s)

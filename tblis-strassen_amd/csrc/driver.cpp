@@ -434,6 +434,16 @@ void build_comb_mask(Plan& p)
     if (!fits) p.h_comb_mask.clear();
 }
 
+// MultiSumArgs::cv: the 1-level factor when the sums go through it, else the whole matrix if it fits
+void set_msum_cv(const Plan& p, bool isA, MultiSumArgs& ma)
+{
+    const std::vector<double>& c1 = isA ? p.h_msum_coef1_a : p.h_msum_coef1_b;
+    const std::vector<double>& c = isA ? p.h_msum_coef_a : p.h_msum_coef_b;
+    const std::vector<double>& src = c1.empty() ? c : c1;
+    ma.ncv = src.size() <= size_t(kMsumMaxCv) ? int(src.size()) : 0;
+    for (int i = 0; i < ma.ncv; i++) ma.cv[i] = src[size_t(i)];
+}
+
 void set_comb_mask(const Plan& p, CombineArgs& ca)
 {
     ca.dense = p.h_comb_mask.empty() ? 0 : 1;
@@ -1060,6 +1070,7 @@ hipError_t run_outer_staged(const Plan& p, const Call& c, hipStream_t stream)
             ma.nout = int(nouter);
             ma.coef = isA ? p.msum_coef_a : p.msum_coef_b;
             ma.coef1 = isA ? p.msum_coef1_a : p.msum_coef1_b;
+            set_msum_cv(p, isA, ma);
             ma.shift = isA ? p.msum_shift_a : p.msum_shift_b;
             ma.out = isA ? S : T;
             ma.pair = pair;
@@ -1240,6 +1251,7 @@ Status run_plan(const Plan& p, const Call& c, hipStream_t stream)
                 ma.nout = p.nprod;
                 ma.coef = isA ? p.msum_coef_a : p.msum_coef_b;
                 ma.coef1 = isA ? p.msum_coef1_a : p.msum_coef1_b;
+                set_msum_cv(p, isA, ma);
                 ma.shift = isA ? p.msum_shift_a : p.msum_shift_b;
                 ma.out = isA ? S : T;
                 return launch_operand_multisums(p.elem, ma, stream);
@@ -1412,6 +1424,7 @@ Status run_plan_lowp(const Plan& p, const Call& c, int precision, int nslices, h
                 ma.nout = p.nprod;
                 ma.coef = isA ? p.msum_coef_a : p.msum_coef_b;
                 ma.coef1 = isA ? p.msum_coef1_a : p.msum_coef1_b;
+                set_msum_cv(p, isA, ma);
                 ma.shift = isA ? p.msum_shift_a : p.msum_shift_b;
                 ma.out = isA ? S.p : T.p;
                 ma.pair = false;

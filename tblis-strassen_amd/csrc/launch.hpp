@@ -69,6 +69,7 @@ hipError_t launch_operand_sums(int elem_size, const SumArgs& a, hipStream_t stre
 //   out[q*rows*cols + r + c*rows] = sum_s coef[q*nsrc + s] * source_s(r, c)   for q < nout,
 // skipping zero coefficients.  rs has unit stride; `pair`: rows come in contiguous pairs at even
 // element offsets (one vector load per source).
+constexpr int kMsumMaxCv = 28;
 struct MultiSumArgs
 {
     const void* X;
@@ -85,6 +86,11 @@ struct MultiSumArgs
     // 7 x 4 matrix (products o, i; sub-blocks a, b of the 2 x 2 split): computed in two steps,
     // U[o][b] = sum_a coef1[o][a] src(a, b), then out[7 o + i] = sum_b coef1[i][b] U[o][b]
     const double* coef1;
+    // Kernel-argument copy of coef1 when it is set, else of coef when nsrc * nout <= kMsumMaxCv (ncv
+    // entries; 0: use the pointers).  Read with scalar loads: the pointer forms are vector loads after
+    // the output stores, and on this target each wait for them also waits for those stores.
+    int ncv;
+    double cv[kMsumMaxCv];
 };
 hipError_t launch_operand_multisums(int elem_size, const MultiSumArgs& a, hipStream_t stream);
 
