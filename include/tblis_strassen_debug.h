@@ -28,6 +28,12 @@ int mistra_debug_plan(const tblis_tensor* A, const label_type* idx_A,
 /* Run strides of a scatter vector (see run_strides in the plan code); out has n entries. */
 int mistra_debug_run_strides(const long long* scat, long long n, int run, long long* out);
 
+/* Combine pass of the staged products into C (AB, Naive, low precision): 0 = the one-pass form where the
+ * plan allows it (every staged product read once, all C sub-blocks updated), 1 = always the per-sub-block
+ * form.  Both sum each sub-block's terms in the same order, so results are identical; the switch exists
+ * to test that.  Process-wide; returns the previous setting. */
+int mistra_debug_set_combine_form(int per_sub_block);
+
 #ifdef __cplusplus
 } /* extern "C" */
 } /* namespace tblis */

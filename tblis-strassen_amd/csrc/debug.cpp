@@ -61,5 +61,10 @@ int mistra_debug_run_strides(const long long* scat, long long n, int run, long l
     return 0;
 }
 
+int mistra_debug_set_combine_form(int per_sub_block)
+{
+    return mistra::g_combine_per_sub.exchange(per_sub_block ? 1 : 0);
+}
+
 }  // extern "C"
 }  // namespace tblis

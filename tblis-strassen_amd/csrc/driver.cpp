@@ -446,11 +446,13 @@ void set_msum_cv(const Plan& p, bool isA, MultiSumArgs& ma)
 
 void set_comb_mask(const Plan& p, CombineArgs& ca)
 {
-    ca.dense = p.h_comb_mask.empty() ? 0 : 1;
+    ca.dense = p.h_comb_mask.empty() || g_combine_per_sub.load() ? 0 : 1;
     for (size_t q = 0; q < p.h_comb_mask.size(); q++) ca.pmask[q] = p.h_comb_mask[q];
 }
 
 }  // namespace
+
+std::atomic<int> g_combine_per_sub{0};
 
 Plan::~Plan()
 {

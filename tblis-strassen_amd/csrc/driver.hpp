@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <memory>
 #include <string>
@@ -210,6 +211,9 @@ double model_ms(const Plan& p);
 // Batched contractions (batch.cpp).
 Status batch_mult(int count, const mistra_contraction* entries, unsigned flags, hipStream_t stream);
 void release_batch_graphs();
+
+// 1: the combine pass always takes its per-sub-block form (mistra_debug_set_combine_form)
+extern std::atomic<int> g_combine_per_sub;
 double scalar_value(const tblis::tblis_tensor* t);
 
 // Kernel timing hook of the dominant leaf kernel.

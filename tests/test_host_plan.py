@@ -36,6 +36,15 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
 
 
+def test_combine_form_switch_returns_previous_setting():
+    lib = ts.load()
+    lib.mistra_debug_set_combine_form.restype = ctypes.c_int
+    lib.mistra_debug_set_combine_form.argtypes = [ctypes.c_int]
+    old = lib.mistra_debug_set_combine_form(1)
+    assert lib.mistra_debug_set_combine_form(7) == 1      # any non-zero value means the per-sub-block form
+    assert lib.mistra_debug_set_combine_form(old) == 1
+
+
 def test_cxx_impl_global_is_exported():
     # tblis::internal::impl keeps its C++ name for drop-in users of the reference's global
     lib = ts.load()
