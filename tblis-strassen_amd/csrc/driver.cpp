@@ -1033,6 +1033,60 @@ Status plan_abc_jobs(const Plan& p, const Call& c, std::vector<LaunchJob>& out)
 
 namespace {
 
+// A second stream of the current device, for an independent pass beside the caller's stream: the sums
+// of A and of B read and write disjoint buffers, and each alone leaves HBM bandwidth unused.  One per
+// thread and device, created on first use with the default priority and kept for the thread's life.
+struct SideFork
+{
+    hipStream_t side = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+};
+
+// The side stream waits for everything issued to `main` so far.  false (run everything on `main`) when
+// the stream is being captured into a graph or a call fails.
+bool fork_begin(hipStream_t main, SideFork& f)
+{
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    int dev = 0;
+    if (hipStreamIsCapturing(main, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone ||
+        hipGetDevice(&dev) != hipSuccess)
+    {
+        (void)hipGetLastError();
+        return false;
+    }
+    thread_local std::map<int, SideFork> forks;
+    auto it = forks.find(dev);
+    if (it == forks.end())
+    {
+        SideFork n;
+        if (hipStreamCreateWithFlags(&n.side, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&n.ev[0], hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&n.ev[1], hipEventDisableTiming) != hipSuccess)
+        {
+            (void)hipGetLastError();
+            if (n.ev[0]) (void)hipEventDestroy(n.ev[0]);
+            if (n.side) (void)hipStreamDestroy(n.side);
+            return false;
+        }
+        it = forks.emplace(dev, n).first;
+    }
+    f = it->second;
+    if (hipEventRecord(f.ev[0], main) != hipSuccess || hipStreamWaitEvent(f.side, f.ev[0], 0) != hipSuccess)
+    {
+        (void)hipGetLastError();
+        return false;
+    }
+    return true;
+}
+
+// `main` waits for everything issued to the side stream so far.
+hipError_t fork_end(hipStream_t main, const SideFork& f)
+{
+    hipError_t e = hipEventRecord(f.ev[1], f.side);
+    if (e == hipSuccess) e = hipStreamWaitEvent(main, f.ev[1], 0);
+    return e;
+}
+
 // Outer-staged ABC (Plan::outer_staged): the outer sums into S and T, then one launch per stage over the
 // inner sums of S and T (2 terms at level 2, 1 at level 1) with the multi-C epilogue on C.
 hipError_t run_outer_staged(const Plan& p, const Call& c, hipStream_t stream)
@@ -1053,7 +1107,7 @@ hipError_t run_outer_staged(const Plan& p, const Call& c, hipStream_t stream)
     // sums of one operand: one pass over its source sub-blocks when they are translates and K (the rows
     // of the staged buffers) has unit stride, else one gather pass per sum
     const uintptr_t align = uintptr_t(2 * p.elem) - 1;
-    auto sums = [&](int which) {
+    auto sums = [&](int which, hipStream_t st) {
         const bool isA = which == 0;
         const void* X = isA ? c.A : c.B;
         const bool rows_fast = isA ? !p.a_mfast : !p.b_nfast;
@@ -1076,7 +1130,7 @@ hipError_t run_outer_staged(const Plan& p, const Call& c, hipStream_t stream)
             ma.shift = isA ? p.msum_shift_a : p.msum_shift_b;
             ma.out = isA ? S : T;
             ma.pair = pair;
-            return launch_operand_multisums(p.elem, ma, stream);
+            return launch_operand_multisums(p.elem, ma, st);
         }
         SumArgs sa;
         std::memset(&sa, 0, sizeof(sa));
@@ -1094,10 +1148,19 @@ hipError_t run_outer_staged(const Plan& p, const Call& c, hipStream_t stream)
         sa.shifted = (isA ? p.outer_shift_a : p.outer_shift_b) && rows_fast;
         sa.pair = pair;
         sa.prods = sa.shifted ? p.outer_shifted : p.outer_sums;
-        return launch_operand_sums(p.elem, sa, stream);
+        return launch_operand_sums(p.elem, sa, st);
     };
-    e = sums(0);
-    if (e == hipSuccess) e = sums(1);
+    // B's sums on the side stream, A's on the caller's, joined before the stage launches
+    SideFork fk;
+    const bool forked = fork_begin(stream, fk);
+    e = sums(0, stream);
+    const hipError_t eb = sums(1, forked ? fk.side : stream);
+    if (e == hipSuccess) e = eb;
+    if (forked)
+    {
+        const hipError_t ej = fork_end(stream, fk);
+        if (e == hipSuccess) e = ej;
+    }
     GemmArgs g;
     std::memset(&g, 0, sizeof(g));
     g.A = S;
@@ -1240,7 +1303,7 @@ Status run_plan(const Plan& p, const Call& c, hipStream_t stream)
         const bool s_k = d2 && p.msum_a && !p.a_mfast && p.kf_dense_k.on, t_one = d2 && p.msum_b && !p.b_nfast;
         for (int64_t z = 0; z < p.nbatch && e == hipSuccess; z++)
         {
-            auto multisums = [&](bool isA) {
+            auto multisums = [&](bool isA, hipStream_t st) {
                 MultiSumArgs ma;
                 std::memset(&ma, 0, sizeof(ma));
                 ma.X = isA ? c.A : c.B;
@@ -1256,7 +1319,7 @@ Status run_plan(const Plan& p, const Call& c, hipStream_t stream)
                 set_msum_cv(p, isA, ma);
                 ma.shift = isA ? p.msum_shift_a : p.msum_shift_b;
                 ma.out = isA ? S : T;
-                return launch_operand_multisums(p.elem, ma, stream);
+                return launch_operand_multisums(p.elem, ma, st);
             };
             SumArgs sa;
             std::memset(&sa, 0, sizeof(sa));
@@ -1271,8 +1334,11 @@ Status run_plan(const Plan& p, const Call& c, hipStream_t stream)
             sa.rows_fast = p.a_mfast;
             sa.out = S;
             sa.zoff = p.h_batch_a[size_t(z)];
-            e = s_k ? multisums(true) : launch_operand_sums(p.elem, sa, stream);
-            if (e != hipSuccess) break;
+            // B's sums on the side stream, A's on the caller's, joined before the dense GEMM
+            SideFork fk;
+            const bool forked = fork_begin(stream, fk);
+            const hipStream_t sb = forked ? fk.side : stream;
+            e = s_k ? multisums(true, stream) : launch_operand_sums(p.elem, sa, stream);
             sa.X = c.B;
             sa.rs = p.b_k;
             sa.cs = p.b_n;
@@ -1282,7 +1348,14 @@ Status run_plan(const Plan& p, const Call& c, hipStream_t stream)
             sa.rows_fast = !p.b_nfast;
             sa.out = T;
             sa.zoff = p.h_batch_b[size_t(z)];
-            e = t_one ? multisums(false) : launch_operand_sums(p.elem, sa, stream);
+            const hipError_t eb = e != hipSuccess ? hipSuccess
+                                                  : (t_one ? multisums(false, sb) : launch_operand_sums(p.elem, sa, sb));
+            if (e == hipSuccess) e = eb;
+            if (forked)
+            {
+                const hipError_t ej = fork_end(stream, fk);
+                if (e == hipSuccess) e = ej;
+            }
             if (e != hipSuccess) break;
 
             GemmArgs gd;
