@@ -170,7 +170,13 @@ __device__ __forceinline__ void leaf_gemm_body(const GemmArgs& g, int bx, int by
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
+    // the same in every lane of a wave: made a scalar, so the per-wave LDS addresses of the direct-to-LDS
+    // loads (M0) and the fragment offsets are scalar arithmetic rather than vector adds and read-backs
+#if defined(__HIP_DEVICE_COMPILE__)
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+#else
     const int wave = tid >> 6;
+#endif
     const int wm0 = (wave & 1) * WM;
     const int wn0 = (wave >> 1) * WN;
 
