@@ -541,6 +541,8 @@ Status build_plan_host(const tblis::tblis_tensor* A, const label_type* idx_A, co
     p.m = int64_t(a_m.size());
     p.n = int64_t(b_n.size());
     p.k = int64_t(a_k.size());
+    if (p.k / 16 >= int64_t(INT32_MAX))
+        return Status::error(MISTRA_ERR_UNSUPPORTED, "contracted extent of 2^35 or more (K-tiles are counted in 32 bits)");
     p.a_mfast = unit(a_m) || !unit(a_k);
     p.b_nfast = unit(b_n) || !unit(b_k);
 

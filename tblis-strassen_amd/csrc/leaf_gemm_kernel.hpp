@@ -356,6 +356,10 @@ __device__ __forceinline__ void leaf_gemm_body(const GemmArgs& g, int bx, int by
         }
 
         const int64_t nkt = (ks + TK - 1) / TK;
+        // K-tile counters are 32-bit (the host rejects K of 2^31 tiles or more): the scalar unit has no
+        // signed 64-bit compare, so 64-bit tile counters put the loop tests and clamps on the vector
+        // unit, which does not run beside the MFMAs
+        const int nkt32 = int(nkt);
         // K offsets of a tile.  Affine: the tile's first K offsets (lane 0: A, lane 1: B), one vector
         // load, read into SGPRs with v_readlane.  Gathered: each lane's K offsets for its own K
         // positions (positions past ks clamp to ks - 1; their elements read 0).  Fetched one tile
@@ -368,8 +372,8 @@ __device__ __forceinline__ void leaf_gemm_body(const GemmArgs& g, int bx, int by
         };
         int64_t kinfo = 0;
         KOff kgat;
-        auto fetch_kinfo = [&](int64_t kt) {
-            const int64_t k0 = lmin(kt, nkt - 1) * TK;
+        auto fetch_kinfo = [&](int kt) {
+            const int64_t k0 = int64_t(kt < nkt32 - 1 ? kt : nkt32 - 1) * TK;
             if constexpr (KG)
             {
 #pragma unroll
@@ -404,9 +408,9 @@ __device__ __forceinline__ void leaf_gemm_body(const GemmArgs& g, int bx, int by
         // out of range, i.e. zeros with no memory traffic): the load count between any two points of
         // the loop is then the same on every path, so each wait covers exactly the loads it needs
         // (a skipped load on one path would make the waits count conservatively, down to vmcnt(0)).
-        auto issue = [&](auto set, int t, int64_t kt, const KOff& ko) {
+        auto issue = [&](auto set, int t, int kt, const KOff& ko) {
             constexpr int S = decltype(set)::value;
-            const int klim = kt < nkt ? int(lmin(ks - 1 - kt * TK, TK - 1)) : -1;  // k > klim reads 0
+            const int klim = kt < nkt32 ? int(lmin(ks - 1 - int64_t(kt) * TK, TK - 1)) : -1;  // k > klim reads 0
             const int kla = t < nA ? klim : -1, klb = t < nB ? klim : -1;
             const __amdgpu_buffer_rsrc_t ra_ = make_rsrc(Ag + ash[t] + (KG ? 0 : ko.a[0]));
 #pragma unroll
@@ -617,9 +621,9 @@ __device__ __forceinline__ void leaf_gemm_body(const GemmArgs& g, int bx, int by
                 __syncthreads();
                 vec16 a0[FM], b0[FN], a1[FM], b1[FN];
                 read_grp(sA[0], sB[0], 0, a0, b0);
-                for (int64_t kt = 0; kt < nkt; kt++)
+                for (int kt = 0; kt < nkt32; kt++)
                 {
-                    const int cur = int(kt & 1);
+                    const int cur = kt & 1;
                     read_grp(sA[cur], sB[cur], 1, a1, b1);
                     mma_grp(a0, b0);
                     // group 1's reads go between group 0's first MFMAs (each MFMA occupies the pipe for
@@ -640,7 +644,7 @@ __device__ __forceinline__ void leaf_gemm_body(const GemmArgs& g, int bx, int by
                     __builtin_amdgcn_sched_barrier(0);
                     const KOff kn = read_kinfo();
                     fetch_kinfo(kt + 3);
-                    if (kt + 2 < nkt) dma(cur, kn);
+                    if (kt + 2 < nkt32) dma(cur, kn);
                     mma_grp(a1, b1);
                     __builtin_amdgcn_sched_barrier(0);
                 }
@@ -652,13 +656,13 @@ __device__ __forceinline__ void leaf_gemm_body(const GemmArgs& g, int bx, int by
                 fetch_kinfo(1);
                 dma_landed();
                 __syncthreads();
-                for (int64_t kt = 0; kt < nkt; kt++)
+                for (int kt = 0; kt < nkt32; kt++)
                 {
-                    const int cur = int(kt & 1);
+                    const int cur = kt & 1;
                     const KOff kn = read_kinfo();
                     fetch_kinfo(kt + 2);
                     // buffer cur ^ 1 was last read in iteration kt - 1, before its closing barrier
-                    if (kt + 1 < nkt) dma(cur ^ 1, kn);
+                    if (kt + 1 < nkt32) dma(cur ^ 1, kn);
                     mfma_dma(sA[cur], sB[cur]);
                     dma_landed();
                     __syncthreads();
@@ -683,9 +687,9 @@ __device__ __forceinline__ void leaf_gemm_body(const GemmArgs& g, int bx, int by
             }
             fetch_kinfo(1);
             __syncthreads();
-            for (int64_t kt = 0; kt < nkt; kt++)
+            for (int kt = 0; kt < nkt32; kt++)
             {
-                const int cur = int(kt & 1);
+                const int cur = kt & 1;
                 const KOff kn = read_kinfo();
                 fetch_kinfo(kt + 2);
 #pragma unroll
@@ -693,7 +697,7 @@ __device__ __forceinline__ void leaf_gemm_body(const GemmArgs& g, int bx, int by
                 {
                     issue(S0(), ph, kt + 1, kn);
                     mfma_steps(sA[cur], sB[cur], ph * KPH, (ph + 1) * KPH);
-                    if (kt + 1 < nkt) put(S0(), ph, cur ^ 1);
+                    if (kt + 1 < nkt32) put(S0(), ph, cur ^ 1);
                 }
                 __syncthreads();
             }
@@ -715,9 +719,9 @@ __device__ __forceinline__ void leaf_gemm_body(const GemmArgs& g, int bx, int by
             issue(S0(), 0, 1, kcur);  // job TERMS: set TERMS & 1 = 0
             __syncthreads();
             static_assert(TERMS % 2 == 0, "register-set choice must not depend on the tile parity");
-            for (int64_t kt = 0; kt < nkt; kt++)
+            for (int kt = 0; kt < nkt32; kt++)
             {
-                const int cur = int(kt & 1);
+                const int cur = kt & 1;
                 const KOff knext = read_kinfo();
                 fetch_kinfo(kt + 3);
 #pragma unroll
@@ -734,7 +738,7 @@ __device__ __forceinline__ void leaf_gemm_body(const GemmArgs& g, int bx, int by
                     else
                         issue(S0(), 0, kt + 2, knext);
                     mfma_steps(sA[cur], sB[cur], ph * KPH, (ph + 1) * KPH);
-                    if (kt + 1 < nkt)
+                    if (kt + 1 < nkt32)
                     {
                         if ((ph & 1) == 0) put(S0(), ph, cur ^ 1);
                         else put(S1(), ph, cur ^ 1);
