@@ -492,6 +492,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(NSL <= 2 ? 
 // kI8Fold K-steps, else once at the end (which keeps the FP64 accumulators out of the loop's registers).
 // Needs mp % 128 == 0, np % 64 == 0, kp % 64 == 0, NSL <= 6 (LDS).
 typedef int i32x4 __attribute__((ext_vector_type(4)));
+constexpr uint32_t kI8Oob = 0x80000000u;  // buffer offsets at or past this read 0
 constexpr int kI8Fold = 1024;  // K-steps per INT32 fold: 1024 * 64 * NSL * 2^12 < 2^31
 template <int NSL, bool FOLD>
 __global__ __launch_bounds__(512) void ozaki_i8_gemm_kernel(const int8_t* __restrict__ As, const int8_t* __restrict__ Bs,
@@ -509,11 +510,24 @@ __global__ __launch_bounds__(512) void ozaki_i8_gemm_kernel(const int8_t* __rest
     const int64_t row0 = int64_t(tm) * 128, col0 = int64_t(tn) * 64;
     const int wm = (wave & 3) * 32, wn = (wave >> 2) * 32;
     const int64_t slab_a = int64_t(g.nprod) * g.mp * g.kp, slab_b = int64_t(g.nprod) * g.np * g.kp;
-    // loader: A tile = 128 rows x 64 bytes = 512 x 16 B per slice (one per thread); B = 64 x 64 = 256 x 16 B
+    // loader: A tile = 128 rows x 64 bytes = 512 x 16 B per slice (one per thread); B = 64 x 64 = 256 x 16 B.
+    // Buffer loads: a scalar resource per slice at the tile's first row, a fixed 32-bit lane offset and the
+    // K-step as the scalar offset, so the loop spends no vector instructions on addresses (the vector unit
+    // does not run beside the MFMAs); lanes past the B tile read 0 through an out-of-range offset.
+    // Offsets stay below 128 kp < 2^31 (host-checked).
     const int lr = tid >> 2, lc = tid & 3;
-    const int8_t* ga = As + (int64_t(q) * g.mp + row0 + lr) * g.kp + lc * 16;
-    const int8_t* gb = Bs + (int64_t(q) * g.np + col0 + (lr & 63)) * g.kp + lc * 16;
     const bool bload = tid < 256;
+    __amdgpu_buffer_rsrc_t rsa[NSL], rsb[NSL];
+#pragma unroll
+    for (int s = 0; s < NSL; s++)
+    {
+        rsa[s] = __builtin_amdgcn_make_buffer_rsrc(const_cast<int8_t*>(As + s * slab_a + (int64_t(q) * g.mp + row0) * g.kp),
+                                                   (short)0, int(kI8Oob), 0x00020000);
+        rsb[s] = __builtin_amdgcn_make_buffer_rsrc(const_cast<int8_t*>(Bs + s * slab_b + (int64_t(q) * g.np + col0) * g.kp),
+                                                   (short)0, int(kI8Oob), 0x00020000);
+    }
+    const uint32_t voa = uint32_t(lr * g.kp + lc * 16);
+    const uint32_t vob = bload ? uint32_t((lr & 63) * g.kp + lc * 16) : kI8Oob;
     const int soff = lr * 64 + ((lc ^ ((lr >> 1) & 3)) * 16);  // this thread's swizzled LDS chunk
 
     double acc[2][2][4];
@@ -551,12 +565,12 @@ __global__ __launch_bounds__(512) void ozaki_i8_gemm_kernel(const int8_t* __rest
     };
 
     i32x4 ra[NSL], rb[NSL];
-    auto gload = [&](int64_t st) {
+    auto gload = [&](int st) {
 #pragma unroll
         for (int s = 0; s < NSL; s++)
         {
-            ra[s] = *reinterpret_cast<const i32x4*>(ga + s * slab_a + st * 64);
-            rb[s] = bload ? *reinterpret_cast<const i32x4*>(gb + s * slab_b + st * 64) : i32x4{0, 0, 0, 0};
+            ra[s] = __builtin_amdgcn_raw_buffer_load_b128(rsa[s], voa, st * 64, 0);
+            rb[s] = __builtin_amdgcn_raw_buffer_load_b128(rsb[s], vob, st * 64, 0);
         }
     };
     auto lstore = [&](int buf) {
@@ -579,13 +593,13 @@ __global__ __launch_bounds__(512) void ozaki_i8_gemm_kernel(const int8_t* __rest
         boff[i] = rb_ * 64 + ((fc ^ ((rb_ >> 1) & 3)) * 16);
     }
 
-    const int64_t nsteps = g.kp / 64;
+    const int nsteps = int(g.kp / 64);
     gload(0);
     lstore(0);
     __syncthreads();
-    for (int64_t st = 0; st < nsteps; st++)
+    for (int st = 0; st < nsteps; st++)
     {
-        const int cur = int(st & 1);
+        const int cur = st & 1;
         if (st + 1 < nsteps) gload(st + 1);
         i32x4 fa[NSL][2];
 #pragma unroll
@@ -811,6 +825,7 @@ hipError_t gemm_i8_t(const void* As, const void* Bs, const int* ea, const int* e
                      hipStream_t s)
 {
     if (g.mp % 128 || g.np % 64 || g.kp % 64) return hipErrorInvalidValue;
+    if (g.kp >= (int64_t(1) << 24)) return hipErrorInvalidValue;  // tile offsets below 128 kp < 2^31
     const dim3 grid(unsigned((g.mp / 128) * (g.np / 64)), unsigned(g.nlaunch));
     if (g.kp > int64_t(64) * kI8Fold)
         hipLaunchKernelGGL((ozaki_i8_gemm_kernel<NSL, true>), grid, dim3(512), 0, s, static_cast<const int8_t*>(As),
