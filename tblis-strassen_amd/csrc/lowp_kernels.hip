@@ -373,6 +373,8 @@ __global__ __launch_bounds__(256) void ozaki_gemm_kernel(const __bf16* __restric
         }
 }
 
+constexpr uint32_t kBufOob = 0x80000000u;  // buffer-load offsets at or past this read 0
+
 // LDS-staged form: 512 threads (8 waves as 4 x 2, 32 x 32 each) per 128 x 64 tile of M_q.  Every digit
 // element of a K-step (32) is read from HBM/L2 once per workgroup into a double-buffered LDS tile
 // (rows of 32 BF16 padded to 40, so the 16-byte fragment reads of 16 consecutive rows spread over the
@@ -394,11 +396,23 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(NSL <= 2 ? 
     const int64_t row0 = int64_t(tm) * 128, col0 = int64_t(tn) * 64;
     const int wm = (wave & 3) * 32, wn = (wave >> 2) * 32;
     const int64_t slab_a = int64_t(g.nprod) * g.mp * g.kp, slab_b = int64_t(g.nprod) * g.np * g.kp;
-    // loader: A tile = 128 rows x 32 k = 512 x 16 B per slice (one per thread); B tile = 64 x 32 k = 256 x 16 B
+    // loader: A tile = 128 rows x 32 k = 512 x 16 B per slice (one per thread); B tile = 64 x 32 k = 256 x 16 B.
+    // Buffer loads as in the INT8 GEMM: a scalar resource per slice at the tile's first row, a fixed lane
+    // offset and the K-step as the scalar offset (no vector instructions on addresses in the loop); lanes
+    // past the B tile read 0.  Byte offsets stay below 256 kp < 2^31 (host-checked).
     const int lr = tid >> 2, lc = (tid & 3) * 8;  // row, k offset of this thread's 16-byte chunk
-    const __bf16* ga = As + (int64_t(q) * g.mp + row0 + lr) * g.kp + lc;
-    const __bf16* gb = Bs + (int64_t(q) * g.np + col0 + (lr & 63)) * g.kp + lc;
     const bool bload = tid < 256;
+    __amdgpu_buffer_rsrc_t rsa[NSL], rsb[NSL];
+#pragma unroll
+    for (int s = 0; s < NSL; s++)
+    {
+        rsa[s] = __builtin_amdgcn_make_buffer_rsrc(const_cast<__bf16*>(As + s * slab_a + (int64_t(q) * g.mp + row0) * g.kp),
+                                                   (short)0, int(kBufOob), 0x00020000);
+        rsb[s] = __builtin_amdgcn_make_buffer_rsrc(const_cast<__bf16*>(Bs + s * slab_b + (int64_t(q) * g.np + col0) * g.kp),
+                                                   (short)0, int(kBufOob), 0x00020000);
+    }
+    const uint32_t voa = uint32_t((lr * g.kp + lc) * 2);
+    const uint32_t vob = bload ? uint32_t(((lr & 63) * g.kp + lc) * 2) : kBufOob;
 
     double acc[2][2][4];
     f32x4 a32[NSL][2][2];
@@ -426,12 +440,12 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(NSL <= 2 ? 
     };
 
     bf16x8 ra[NSL], rb[NSL];
-    auto gload = [&](int64_t st) {
+    auto gload = [&](int st) {
 #pragma unroll
         for (int s = 0; s < NSL; s++)
         {
-            ra[s] = *reinterpret_cast<const bf16x8*>(ga + s * slab_a + st * 32);
-            rb[s] = bload ? *reinterpret_cast<const bf16x8*>(gb + s * slab_b + st * 32) : bf16x8{};
+            ra[s] = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(rsa[s], voa, st * 64, 0));
+            rb[s] = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(rsb[s], vob, st * 64, 0));
         }
     };
     auto lstore = [&](int buf) {
@@ -443,13 +457,13 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(NSL <= 2 ? 
         }
     };
 
-    const int64_t nsteps = g.kp / 32;
+    const int nsteps = int(g.kp / 32);
     gload(0);
     lstore(0);
     __syncthreads();
-    for (int64_t st = 0; st < nsteps; st++)
+    for (int st = 0; st < nsteps; st++)
     {
-        const int cur = int(st & 1);
+        const int cur = st & 1;
         if (st + 1 < nsteps) gload(st + 1);
         bf16x8 fa[NSL][2], fb[NSL][2];
 #pragma unroll
@@ -492,7 +506,6 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(NSL <= 2 ? 
 // kI8Fold K-steps, else once at the end (which keeps the FP64 accumulators out of the loop's registers).
 // Needs mp % 128 == 0, np % 64 == 0, kp % 64 == 0, NSL <= 6 (LDS).
 typedef int i32x4 __attribute__((ext_vector_type(4)));
-constexpr uint32_t kI8Oob = 0x80000000u;  // buffer offsets at or past this read 0
 constexpr int kI8Fold = 1024;  // K-steps per INT32 fold: 1024 * 64 * NSL * 2^12 < 2^31
 template <int NSL, bool FOLD>
 __global__ __launch_bounds__(512) void ozaki_i8_gemm_kernel(const int8_t* __restrict__ As, const int8_t* __restrict__ Bs,
@@ -522,12 +535,12 @@ __global__ __launch_bounds__(512) void ozaki_i8_gemm_kernel(const int8_t* __rest
     for (int s = 0; s < NSL; s++)
     {
         rsa[s] = __builtin_amdgcn_make_buffer_rsrc(const_cast<int8_t*>(As + s * slab_a + (int64_t(q) * g.mp + row0) * g.kp),
-                                                   (short)0, int(kI8Oob), 0x00020000);
+                                                   (short)0, int(kBufOob), 0x00020000);
         rsb[s] = __builtin_amdgcn_make_buffer_rsrc(const_cast<int8_t*>(Bs + s * slab_b + (int64_t(q) * g.np + col0) * g.kp),
-                                                   (short)0, int(kI8Oob), 0x00020000);
+                                                   (short)0, int(kBufOob), 0x00020000);
     }
     const uint32_t voa = uint32_t(lr * g.kp + lc * 16);
-    const uint32_t vob = bload ? uint32_t((lr & 63) * g.kp + lc * 16) : kI8Oob;
+    const uint32_t vob = bload ? uint32_t((lr & 63) * g.kp + lc * 16) : kBufOob;
     const int soff = lr * 64 + ((lc ^ ((lr >> 1) & 3)) * 16);  // this thread's swizzled LDS chunk
 
     double acc[2][2][4];
@@ -793,7 +806,8 @@ hipError_t gemm_t(const void* As, const void* Bs, const int* ea, const int* eb, 
 {
     if constexpr (NSL <= 5)  // the double-buffered digit tiles of 6 or 7 slices exceed the 160 KiB of LDS
     {
-        if (g.mp % 128 == 0 && g.np % 64 == 0 && g.kp % 32 == 0)
+        // (tile byte offsets below 256 kp < 2^31 for its buffer loads; longer K takes the other kernel)
+        if (g.mp % 128 == 0 && g.np % 64 == 0 && g.kp % 32 == 0 && g.kp < (int64_t(1) << 23))
         {
             hipLaunchKernelGGL(ozaki_gemm_lds_kernel<NSL>, dim3(unsigned((g.mp / 128) * (g.np / 64)), unsigned(g.nlaunch)),
                                dim3(512), 0, s, static_cast<const __bf16*>(As), static_cast<const __bf16*>(Bs), ea, eb,
