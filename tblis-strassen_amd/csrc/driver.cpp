@@ -201,7 +201,8 @@ KFast kfast_check(const Plan& p, const std::vector<ProdDesc>& prods, int first, 
         ak = std::max(ak, p.h_a_k[size_t(k)]);
         bk = std::max(bk, p.h_b_k[size_t(k)]);
     }
-    if (am + ak < limit && bn + bk < limit)
+    // (and the kernel's 32-bit byte offsets into the K tables: ks < 2^28)
+    if (am + ak < limit && bn + bk < limit && ks < (int64_t(1) << 28))
     {
         kf.on = true;
         kf.mode = KMODE_GATHER;

@@ -63,6 +63,7 @@ hipError_t launch_leaf_gemm(int elem_size, int epi, int tile, int terms, int kmo
     if (g.tiles_m == 0 || g.tiles_n == 0 || g.nbatch == 0 || g.nprod == 0) return hipSuccess;
     if (g.ks <= 0) return hipErrorInvalidValue;  // callers route empty K to a scale of C
     if (g.ks / 16 >= int64_t(INT32_MAX)) return hipErrorInvalidValue;  // the kernels count K-tiles in 32 bits
+    if (kmode == KMODE_GATHER && g.ks >= (int64_t(1) << 28)) return hipErrorInvalidValue;  // 32-bit K offsets
     // grid.y indexes the launch's products; the products of a multi-C launch must write disjoint C
     // sub-blocks (the caller groups them so), since their read-modify-write updates are not ordered
     dim3 grid(unsigned(g.tiles_m) * unsigned(g.tiles_n), unsigned(g.nprod), unsigned(g.nbatch));

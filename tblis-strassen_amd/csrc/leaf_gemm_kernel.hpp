@@ -373,16 +373,29 @@ __device__ __forceinline__ void leaf_gemm_body(const GemmArgs& g, int bx, int by
         int64_t kinfo = 0;
         KOff kgat;
         auto fetch_kinfo = [&](int kt) {
-            const int64_t k0 = int64_t(kt < nkt32 - 1 ? kt : nkt32 - 1) * TK;
+            const int kc = kt < nkt32 - 1 ? kt : nkt32 - 1;
             if constexpr (KG)
             {
+                // 32-bit positions and byte offsets (the host limits gathered K to ks < 2^28): a load from a
+                // scalar base plus a lane offset, with no 64-bit vector arithmetic
+                const int k0 = kc * TK, kmax = int(ks) - 1;
+                const char* ak = reinterpret_cast<const char*>(g.a_k);
+                const char* bk = reinterpret_cast<const char*>(g.b_k);
 #pragma unroll
-                for (int i = 0; i < NKA; i++) kgat.a[i] = KT(g.a_k[lmin(k0 + a_k + i * A_KSTEP, ks - 1)]);
+                for (int i = 0; i < NKA; i++)
+                {
+                    const int kk = min(k0 + a_k + i * A_KSTEP, kmax);
+                    kgat.a[i] = KT(*reinterpret_cast<const int64_t*>(ak + uint32_t(kk) * 8u));
+                }
 #pragma unroll
-                for (int i = 0; i < NKB; i++) kgat.b[i] = KT(g.b_k[lmin(k0 + b_k + i * B_KSTEP, ks - 1)]);
+                for (int i = 0; i < NKB; i++)
+                {
+                    const int kk = min(k0 + b_k + i * B_KSTEP, kmax);
+                    kgat.b[i] = KT(*reinterpret_cast<const int64_t*>(bk + uint32_t(kk) * 8u));
+                }
             }
             else
-                kinfo = (lane == 1 ? g.b_k : g.a_k)[k0];
+                kinfo = (lane == 1 ? g.b_k : g.a_k)[int64_t(kc) * TK];
         };
         auto read_kinfo = [&]() {
             if constexpr (KG)
@@ -438,6 +451,9 @@ __device__ __forceinline__ void leaf_gemm_body(const GemmArgs& g, int bx, int by
         // Term sums: term 0 stores its scaled values into the LDS buffer, later terms read-modify-write
         // their own elements (a running sum in registers measured slower: it costs the registers that
         // hide load latency).
+        // A single term is stored as loaded and its coefficient (+-1) applied in the epilogue, as on the
+        // direct-to-LDS path: no multiply per element in the loop (vector instructions there take MFMA
+        // issue time)
         auto put = [&](auto set, int t, int buf) {
             constexpr int S = decltype(set)::value;
             if (t < nA)
@@ -448,7 +464,7 @@ __device__ __forceinline__ void leaf_gemm_body(const GemmArgs& g, int bx, int by
                     for (int j = 0; j < WA; j++)
                     {
                         T& dst = sA[buf][(a_k + j + e * A_KSTEP) * LDA + a_m + e * A_MSTEP];
-                        const T v = acoef[t] * ra[S][e * WA + j];
+                        const T v = TERMS == 1 ? ra[S][e * WA + j] : acoef[t] * ra[S][e * WA + j];
                         dst = t == 0 ? v : dst + v;
                     }
             }
@@ -460,7 +476,7 @@ __device__ __forceinline__ void leaf_gemm_body(const GemmArgs& g, int bx, int by
                     for (int j = 0; j < WB; j++)
                     {
                         T& dst = sB[buf][(b_k + j + e * B_KSTEP) * LDB + b_n + e * B_NSTEP];
-                        const T v = bcoef[t] * rb[S][e * WB + j];
+                        const T v = TERMS == 1 ? rb[S][e * WB + j] : bcoef[t] * rb[S][e * WB + j];
                         dst = t == 0 ? v : dst + v;
                     }
             }
@@ -748,7 +764,7 @@ __device__ __forceinline__ void leaf_gemm_body(const GemmArgs& g, int bx, int by
                 __syncthreads();
             }
         }
-        epilogue(P, p, acc, true, 1.0);
+        epilogue(P, p, acc, true, TERMS == 1 ? double(acoef[0]) * double(bcoef[0]) : 1.0);
         return;
     }
     else
