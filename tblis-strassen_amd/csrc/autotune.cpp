@@ -38,25 +38,33 @@ std::string shape_key(const tblis::tblis_tensor* A, const label_type* idx_A, con
 }
 
 // Leaf-kernel rates (TFLOP/s) measured on MI355X with the 8192^3 FP64 workload, by register tile and
-// number of sum terms (1, 2, 4; single-term launches over K-fast operands load K-tiles straight into
-// LDS); FP32 runs the same kernels at about twice the rate.
-double leaf_rate(int elem, int tile, int terms, bool dma)
+// number of sum terms (1, 2, 4); FP32 runs the same kernels at about 1.8 times the rate.  Single-term
+// launches over K-fast operands (K-tiles straight into LDS) run at 72.5 - 7645 / ks: 71.6 at a leaf
+// depth of 8192 (the classical kernel), 68.8 at 2048 (the L=2 Naive dense launch) -- the per-tile
+// start and end weigh more on shorter K loops.
+double leaf_rate(int elem, int tile, int terms, bool dma, int64_t ks)
 {
-    static const double r128[3] = {58.5, 53.5, 45.0}, r64[3] = {57.0, 46.0, 38.0};
+    static const double r128[3] = {61.0, 57.5, 46.0}, r64[3] = {57.0, 46.0, 38.0};
     const int ti = terms <= 1 ? 0 : (terms == 2 ? 1 : 2);
-    const double r = tile == TILE_64 ? r64[ti] : (ti == 0 && dma ? 66.0 : r128[ti]);
+    double r = tile == TILE_64 ? r64[ti] : r128[ti];
+    if (tile != TILE_64 && ti == 0 && dma) r = std::max(r, 72.5 - 7645.0 / double(std::max<int64_t>(ks, 512)));
     return elem == 8 ? r : 1.8 * r;
 }
 
-constexpr double kHbm = 2.7e12;       // bytes/s of the sum / combine kernels (measured: strided gathers)
+constexpr double kHbm = 2.7e12;       // bytes/s of the gathering sum / combine kernels (strided gathers)
+constexpr double kHbmSums = 4.5e12;   // the one-pass sum kernels (translate sub-blocks, unit-stride K)
+constexpr double kHbmComb = 4.0e12;   // the one-pass combine (every staged product read once)
 constexpr double kLaunch = 4.0e-6;    // seconds per launch (dispatch and tail)
 
-double launch_s(int elem, int tile, int terms, bool dma, int64_t tiles, double flops)
+// `cpen`: the multi-C epilogue's share of the launch (C targets per product over the leaf depth): the
+// stages of an ABC launch sequence end all their workgroups' read-modify-writes of C at once; measured
+// -9% at L=2 (about 3 targets per product, 2048-deep leaves), nothing visible at L=1.
+double launch_s(int elem, int tile, int terms, bool dma, int64_t tiles, double flops, int64_t ks, double cpen = 0.0)
 {
     if (tiles <= 0 || flops <= 0) return 0.0;
     const int64_t slots = tile == TILE_64 ? 1024 : 512;  // resident workgroups on 256 CUs
     const double waves = std::ceil(double(tiles) / double(slots)) * double(slots) / double(tiles);
-    return flops / (leaf_rate(elem, tile, terms, dma) * 1e12) * waves + kLaunch;
+    return flops / (leaf_rate(elem, tile, terms, dma, ks) * 1e12 * (1.0 - cpen)) * waves + kLaunch;
 }
 
 int64_t ntiles(int tile, int64_t ms, int64_t ns)
@@ -76,35 +84,50 @@ double model_ms(const Plan& p0)
     const int64_t nb = p.nbatch;
     // single-term launches over K-fast operands (both K-fast, or staged copies) load straight into LDS
     const bool kfast = p.outer_staged || (!p.a_mfast && !p.b_nfast);
-    auto gemm = [&](int64_t ms, int64_t ns, int64_t ks, int nprod, int terms, int t) {
+    auto gemm = [&](int64_t ms, int64_t ns, int64_t ks, int nprod, int terms, int t, bool dma, double cpen) {
         // work of whole register tiles (a partly filled tile costs as much as a full one)
         const int64_t nt = ntiles(t, ms, ns) * nprod * nb;
         const double te = double(tile_extent(t));
-        return launch_s(e, t, terms, kfast, nt, 2.0 * te * te * double(ks) * double(nt));
+        return launch_s(e, t, terms, dma, nt, 2.0 * te * te * double(ks) * double(nt), ks, cpen);
     };
-    for (const Pass& f : p.fringe) s += gemm(f.ms, f.ns, f.ks, 1, 1, tile);
+    for (const Pass& f : p.fringe) s += gemm(f.ms, f.ns, f.ks, 1, 1, tile, kfast, 0.0);
     if (p.nprod == 0) return s * 1e3;
     if (!p.core)
     {
-        s += gemm(p.ms, p.ns, p.ks, 1, 1, tile);
+        s += gemm(p.ms, p.ns, p.ks, 1, 1, tile, kfast, 0.0);
         if (p.outer_staged)  // packed copies of A and B: read and written once each
             s += 2.0 * (double(p.ms) * p.ks + double(p.ks) * p.ns) * e / kHbm + 2 * kLaunch;
         return s * 1e3;
     }
     const double cbytes = double(p.ms) * p.ns * e * nb;
+    // combine: read every M_p once and update every C sub-block (one-pass form), or per sub-block
+    const double comb = p.h_comb_mask.empty() ? (double(p.h_terms.size()) + 2.0 * p.nsub) * cbytes / kHbm
+                                              : (double(p.nprod) + 2.0 * p.nsub) * cbytes / kHbmComb;
     if (p.impl == IMPL_AB)
     {
-        s += gemm(p.ms, p.ns, p.ks, p.nprod, max_terms(p.h_prods.data(), p.nprod), tile);
-        s += (double(p.nprod) + 2.0 * p.nsub) * cbytes / kHbm + kLaunch;  // combine: read M, update C
+        s += gemm(p.ms, p.ns, p.ks, p.nprod, max_terms(p.h_prods.data(), p.nprod), tile, false, 0.0);
+        s += comb + kLaunch;
     }
     else if (p.impl == IMPL_NAIVE)
     {
-        double terms = 0;
-        for (int q = 0; q < p.nprod; q++) terms += p.h_prods[q].nA * double(p.ms) * p.ks + p.h_prods[q].nB * double(p.ks) * p.ns;
-        const double sums = (terms + p.nprod * (double(p.ms) * p.ks + double(p.ks) * p.ns)) * e * nb;
-        s += sums / kHbm + 2 * kLaunch;
-        s += gemm(p.ms, p.ns, p.ks, p.nprod, 1, p.ms * p.ns >= int64_t(1024) * 1024 ? TILE_128 : TILE_64);
-        s += (double(p.nprod) + 2.0 * p.nsub) * cbytes / kHbm + kLaunch;
+        // sums: one pass over each operand when the tables allow it (S then K-fast, the dense GEMM loads
+        // straight into LDS), else one gathering pass per sum
+        const bool d2 = p.msum_nsrc == int(p.d * p.d);
+        // (the dense launch's K check runs at upload; a host plan assumes it passes with K-fast sums)
+        const bool one_a = d2 && p.msum_a && !p.a_mfast, one_b = d2 && p.msum_b && !p.b_nfast;
+        const double sa = double(p.ms) * p.ks, sb = double(p.ks) * p.ns;
+        double ta = 0, tb = 0;
+        for (int q = 0; q < p.nprod; q++)
+        {
+            ta += p.h_prods[q].nA;
+            tb += p.h_prods[q].nB;
+        }
+        s += (one_a ? (double(p.d * p.d) + p.nprod) * sa * e * nb / kHbmSums : (ta + p.nprod) * sa * e * nb / kHbm);
+        s += (one_b ? (double(p.d * p.d) + p.nprod) * sb * e * nb / kHbmSums : (tb + p.nprod) * sb * e * nb / kHbm);
+        s += 2 * kLaunch;
+        s += gemm(p.ms, p.ns, p.ks, p.nprod, 1, p.ms * p.ns >= int64_t(1024) * 1024 ? TILE_128 : TILE_64,
+                  one_a && one_b, 0.0);
+        s += comb + kLaunch;
     }
     else
     {
@@ -119,7 +142,14 @@ double model_ms(const Plan& p0)
                 for (int j = p.h_stage_begin[st]; j < p.h_stage_begin[st + 1]; j++)
                     terms = std::max(terms, max_terms(&p.h_inner[size_t(order[size_t(j)])], 1));
             }
-            s += gemm(p.ms, p.ns, p.ks, p.h_stage_begin[st + 1] - p.h_stage_begin[st], terms, tile);
+            int nct = 0;
+            for (int j = p.h_stage_begin[st]; j < p.h_stage_begin[st + 1]; j++) nct += p.h_prods[size_t(order[size_t(j)])].nC;
+            const double per = double(nct) / double(p.h_stage_begin[st + 1] - p.h_stage_begin[st]);
+            // (80 per / ks - 0.033: 0 at L=1 of 8192^3, about 1.7 targets per product over 4096; 0.08 at
+            // L=2, about 3 over 2048)
+            const double cpen = std::min(0.3, 80.0 * per / double(std::max<int64_t>(p.ks, 1)) - 0.033);
+            s += gemm(p.ms, p.ns, p.ks, p.h_stage_begin[st + 1] - p.h_stage_begin[st], terms, tile, kfast && terms == 1,
+                      std::max(0.0, cpen));
         }
         if (p.outer_staged)
         {
@@ -135,8 +165,8 @@ double model_ms(const Plan& p0)
             if (p.msum_a && !p.a_mfast) ra = p.msum_nsrc + nout;
             if (p.msum_b && !p.b_nfast) rb = p.msum_nsrc + nout;
             // translate-form sums stream at about 5 TB/s, gathered ones at kHbm
-            const double ha = p.outer_shift_a && !p.a_mfast ? 5.0e12 : kHbm;
-            const double hb = p.outer_shift_b && !p.b_nfast ? 5.0e12 : kHbm;
+            const double ha = p.outer_shift_a && !p.a_mfast ? kHbmSums : kHbm;
+            const double hb = p.outer_shift_b && !p.b_nfast ? kHbmSums : kHbm;
             s += ra * double(p.os_m) * p.os_k * e / ha + rb * double(p.os_k) * p.os_n * e / hb + 2 * kLaunch;
         }
     }

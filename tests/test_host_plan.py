@@ -252,6 +252,20 @@ def test_autotuner_analytic_model():
     assert level == 0 and tile == 1  # TILE_64
 
 
+def test_autotuner_model_picks_measured_fastest_at_c2():
+    # with c2's real (column-major, seeded-permutation) layout the model's costs follow the measured
+    # ones (classical 15.4 ms, L1 ABC staged 14.5, L2 ABC staged 14.5, L2 Naive 14.2 on MI355X) and it
+    # picks L2 Naive with 128 x 128 tiles; only the descriptors are read
+    import bench
+
+    case = bench.make_workload("c2")
+    buf = np.zeros(1)
+    A, B, C = (np.lib.stride_tricks.as_strided(buf, shape=case[f"shape_{x}"],
+                                               strides=[8 * s for s in cases.colmajor_strides(case[f"shape_{x}"])])
+               for x in "ABC")
+    assert ts.autotune(A, case["idx_A"], B, case["idx_B"], C, case["idx_C"]) == (2, ts.IMPL["naive"], 0)
+
+
 def test_abc_stage_counts():
     # first-fit-decreasing packing by C targets: 4 launches at one level, 16 at two (optimal for 49
     # products of equal size: every stage but one holds an even number of products)
