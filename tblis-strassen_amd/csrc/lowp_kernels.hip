@@ -1,13 +1,13 @@
 // Low-precision leaf products with FP64 accuracy won back (gfx950 MFMA):
 //
 //  * Ozaki-style BF16 slicing.  Each row i of an FP64 operand S (each column j of T) is scaled by a power
-//    of two 2^e so that |x| < 1, then split into `nsl` signed integer digits d_s, x = sum_s d_s 2^(-F-8s)
-//    up to the last digit's rounding, with F = 7 (|d_s| <= 128) for three or more digits and F = 8
-//    (|d_0| <= 256) for two.  A digit is an integer of at most 8 significant bits, exact in BF16, and
-//    the product of two is an integer, so an FP32 MFMA accumulation of digit pairs with the same weight
-//    2^(-2F-8d) is exact up to a bounded number of K elements (flush_period); each such chunk is then
-//    added to an FP64 accumulator.  Pairs with s + t >= nsl are dropped (Ozaki truncation): nsl = 2 is
-//    the hi/lo split with 3 products, nsl = 5 keeps 15 products and about 39 bits.
+//    of two 2^e so that |x| < 1, then split into `nsl` signed integer digits d_s (|d_0| <= 256,
+//    |d_s| <= 128), x = sum_s d_s 2^(-8(s+1)) up to the last digit's rounding.  A digit is an integer of
+//    at most 8 significant bits, exact in BF16.  The product of two digits is an integer below 2^16,
+//    so an FP32 MFMA accumulation of up to 2^24 / ((d+1) 2^16) of them (d+1 digit pairs with the same
+//    weight 2^(-8(d+2)) summed together) is exact; each such chunk is then added to an FP64 accumulator.
+//    Pairs with s + t >= nsl are dropped (Ozaki truncation): nsl = 2 is the hi/lo split with 3 products,
+//    nsl = 5 keeps 15 products and about 40 bits.
 //    (Ozaki, Ogita, Oishi, Rump, "Error-free transformations of matrix multiplication by using fast
 //    routines of matrix multiplication and its applications", Numer. Algorithms 59, 2012.)
 //  * Ozaki-style INT8 slicing: the same scheme with base 2^7 digits |d_s| <= 64 (exact in int8), x =
@@ -76,24 +76,15 @@ __global__ __launch_bounds__(256) void col_exp_kernel(const double* __restrict__
 
 // ---- Ozaki digits -----------------------------------------------------------------------------------
 
-// Bits of the first BF16 digit: 7 with three or more digits (every digit then fits 8 bits with its sign,
-// which halves the FP64 flushes of the GEMM), 8 for the two-digit hi/lo split, which needs the bit.
-template <int NSL>
-__host__ __device__ constexpr int first_bits()
-{
-    return NSL <= 2 ? 8 : 7;
-}
-
 template <int NSL>
 __device__ __forceinline__ void digits(double x, __bf16 (&d)[NSL])
 {
-    // x in (-1, 1): d_0 = round(x 2^F), r = x 2^F - d_0 (exact), d_1 = round(r 2^8), ...; so
-    // x = sum_s d_s 2^(-F-8s) with F = first_bits<NSL>(), |d_0| <= 2^F and |d_s| <= 128 after it
+    // x in (-1, 1): d_0 = round(x 2^8), r = x 2^8 - d_0 (exact), d_1 = round(r 2^8), ...
     double r = x;
 #pragma unroll
     for (int s = 0; s < NSL; s++)
     {
-        r *= s == 0 ? double(1 << first_bits<NSL>()) : 256.0;
+        r *= 256.0;
         const double q = rint(r);
         r -= q;
         d[s] = __bf16(float(q));
@@ -283,17 +274,15 @@ __device__ __forceinline__ void tile_out_128x64(const LowpGemmArgs& g, int q, co
 
 // ---- Ozaki BF16 GEMM --------------------------------------------------------------------------------
 
-// Digit pairs (s, t) with s + t = d share the weight 2^(-2F-8d) and are summed in one FP32 accumulator,
-// which stays exact while every partial sum is an integer of magnitude <= 2^24.
-//  * F = 7 (three or more digits): every |d_s| <= 128, a pair adds at most 2^14 per K element, so group d
-//    (d + 1 pairs) holds 1024 / (d + 1) K elements: flushed to FP64 every 32 / (d + 1) K-steps of 32
-//    (32, 16, 10, 8, 6, 5, 4 for d = 0..6).
-//  * F = 8 (the two-digit hi/lo split, which needs the extra bit): |d_0| <= 256, so group 0 adds up to
-//    2^16 and group d >= 1 up to (d + 3) 2^14 per K element: every 8 K-steps (d = 0, 1).
-template <int NSL>
+// Digit pairs (s, t) with s + t = d share the weight 2^(-8(d+2)) and are summed in one FP32 accumulator,
+// which stays exact while every partial sum is an integer of magnitude <= 2^24.  With |d_0| <= 256 and
+// |d_s| <= 128 (s >= 1), a K element adds at most 256^2 = 2^16 to group 0 and at most
+// 2 * 256 * 128 + (d - 1) * 128^2 = (d + 3) 2^14 to group d >= 1, so group 0 holds 256 K elements and
+// group d >= 1 holds 1024 / (d + 3): flushed to FP64 every 8 K-steps of 32 (d = 0, 1), 6 (d = 2),
+// 5 (d = 3), 4 (d = 4, 5), 3 (d = 6).
 __host__ __device__ constexpr int flush_period(int d)
 {
-    return first_bits<NSL>() == 7 ? 32 / (d + 1) : (d == 0 ? 8 : 32 / (d + 3));
+    return d == 0 ? 8 : 32 / (d + 3);
 }
 
 // One 64 x 64 tile of M_q per workgroup (4 waves, 32 x 32 each as 2 x 2 MFMA blocks of 16 x 16).
@@ -325,7 +314,7 @@ __global__ __launch_bounds__(256) void ozaki_gemm_kernel(const __bf16* __restric
             for (int d = 0; d < NSL; d++) a32[d][i][j] = f32x4{0, 0, 0, 0};
         }
     auto flush = [&](int d) {
-        const double w = ldexp(1.0, -2 * first_bits<NSL>() - 8 * d);
+        const double w = ldexp(1.0, -8 * (d + 2));
 #pragma unroll
         for (int i = 0; i < 2; i++)
 #pragma unroll
@@ -360,7 +349,7 @@ __global__ __launch_bounds__(256) void ozaki_gemm_kernel(const __bf16* __restric
                         a32[s + t][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[s][i], fb[t][j], a32[s + t][i][j], 0, 0, 0);
 #pragma unroll
         for (int d = 0; d < NSL; d++)
-            if ((st + 1) % flush_period<NSL>(d) == 0) flush(d);
+            if ((st + 1) % flush_period(d) == 0) flush(d);
     }
 #pragma unroll
     for (int d = 0; d < NSL; d++) flush(d);
@@ -438,7 +427,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(NSL <= 2 ? 
             for (int d = 0; d < NSL; d++) a32[d][i][j] = f32x4{0, 0, 0, 0};
         }
     auto flush = [&](int d) {
-        const double w = ldexp(1.0, -2 * first_bits<NSL>() - 8 * d);
+        const double w = ldexp(1.0, -8 * (d + 2));
 #pragma unroll
         for (int i = 0; i < 2; i++)
 #pragma unroll
@@ -496,7 +485,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(NSL <= 2 ? 
                         a32[s + t][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[s][i], fb[t][j], a32[s + t][i][j], 0, 0, 0);
 #pragma unroll
         for (int d = 0; d < NSL; d++)
-            if ((st + 1) % flush_period<NSL>(d) == 0) flush(d);
+            if ((st + 1) % flush_period(d) == 0) flush(d);
         if (st + 1 < nsteps) lstore(cur ^ 1);
         __syncthreads();
     }
