@@ -376,10 +376,11 @@ __global__ __launch_bounds__(256) void ozaki_gemm_kernel(const __bf16* __restric
 constexpr uint32_t kBufOob = 0x80000000u;  // buffer-load offsets at or past this read 0
 
 // LDS-staged form: 512 threads (8 waves as 4 x 2, 32 x 32 each) per 128 x 64 tile of M_q.  Every digit
-// element of a K-step (32) is read from HBM/L2 once per workgroup into a double-buffered LDS tile
-// (rows of 32 BF16 padded to 40, so the 16-byte fragment reads of 16 consecutive rows spread over the
-// banks); the waves read their MFMA fragments from LDS.  Needs mp % 128 == 0, np % 64 == 0, kp % 32 == 0.
-constexpr int kOzLd = 40;  // LDS row stride in BF16 elements
+// element of a K-step (32) is read from HBM/L2 once per workgroup into a double-buffered LDS tile laid
+// out as [slice][row][64 bytes] with the four 16-byte chunks of row r XOR-swizzled by (r >> 1) & 3, the
+// INT8 GEMM's layout: the MFMA operand reads (16 consecutive rows at one chunk) spread each 8 rows over
+// the 8 chunk slots of a 128-byte bank window (rows padded to 80 bytes instead cost twice the LDS
+// cycles in bank conflicts).  Needs mp % 128 == 0, np % 64 == 0, kp % 32 == 0.
 // Two workgroups per CU need at most 128 VGPRs: asked for with 2 slices (the 3-product split), where
 // the output staging took one register over and halved the occupancy.
 template <int NSL>
@@ -387,8 +388,9 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(NSL <= 2 ? 
                                                              const int* __restrict__ ea, const int* __restrict__ eb,
                                                              double* __restrict__ M, LowpGemmArgs g)
 {
-    __shared__ __attribute__((aligned(16))) __bf16 sa[2][NSL][128 * kOzLd];
-    __shared__ __attribute__((aligned(16))) __bf16 sb[2][NSL][64 * kOzLd];
+    constexpr int NSA = NSL < 3 ? 3 : NSL;  // at least the 33 KiB of the output staging tile
+    __shared__ __attribute__((aligned(16))) char sa[2][NSA][128 * 64];
+    __shared__ __attribute__((aligned(16))) char sb[2][NSL][64 * 64];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int tiles_m = int(g.mp / 128);
     const int tm = blockIdx.x % tiles_m, tn = blockIdx.x / tiles_m;
@@ -400,7 +402,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(NSL <= 2 ? 
     // Buffer loads as in the INT8 GEMM: a scalar resource per slice at the tile's first row, a fixed lane
     // offset and the K-step as the scalar offset (no vector instructions on addresses in the loop); lanes
     // past the B tile read 0.  Byte offsets stay below 256 kp < 2^31 (host-checked).
-    const int lr = tid >> 2, lc = (tid & 3) * 8;  // row, k offset of this thread's 16-byte chunk
+    const int lr = tid >> 2, lc = tid & 3;  // row, 16-byte chunk (8 k) of this thread's load
     const bool bload = tid < 256;
     __amdgpu_buffer_rsrc_t rsa[NSL], rsb[NSL];
 #pragma unroll
@@ -411,8 +413,9 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(NSL <= 2 ? 
         rsb[s] = __builtin_amdgcn_make_buffer_rsrc(const_cast<__bf16*>(Bs + s * slab_b + (int64_t(q) * g.np + col0) * g.kp),
                                                    (short)0, int(kBufOob), 0x00020000);
     }
-    const uint32_t voa = uint32_t((lr * g.kp + lc) * 2);
-    const uint32_t vob = bload ? uint32_t(((lr & 63) * g.kp + lc) * 2) : kBufOob;
+    const uint32_t voa = uint32_t((lr * g.kp + lc * 8) * 2);
+    const uint32_t vob = bload ? uint32_t(((lr & 63) * g.kp + lc * 8) * 2) : kBufOob;
+    const int soff = lr * 64 + ((lc ^ ((lr >> 1) & 3)) * 16);  // this thread's swizzled LDS chunk
 
     double acc[2][2][4];
     f32x4 a32[NSL][2][2];
@@ -452,10 +455,21 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(NSL <= 2 ? 
 #pragma unroll
         for (int s = 0; s < NSL; s++)
         {
-            *reinterpret_cast<bf16x8*>(&sa[buf][s][lr * kOzLd + lc]) = ra[s];
-            if (bload) *reinterpret_cast<bf16x8*>(&sb[buf][s][lr * kOzLd + lc]) = rb[s];
+            *reinterpret_cast<bf16x8*>(&sa[buf][s][soff]) = ra[s];
+            if (bload) *reinterpret_cast<bf16x8*>(&sb[buf][s][soff]) = rb[s];
         }
     };
+
+    // operand reads: 16 consecutive rows (lane % 16) at chunk lane / 16
+    const int fr = lane & 15, fc = lane >> 4;
+    int aoff[2], boff[2];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+    {
+        const int ra_ = wm + i * 16 + fr, rb_ = wn + i * 16 + fr;
+        aoff[i] = ra_ * 64 + ((fc ^ ((ra_ >> 1) & 3)) * 16);
+        boff[i] = rb_ * 64 + ((fc ^ ((rb_ >> 1) & 3)) * 16);
+    }
 
     const int nsteps = int(g.kp / 32);
     gload(0);
@@ -471,8 +485,8 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(NSL <= 2 ? 
 #pragma unroll
             for (int i = 0; i < 2; i++)
             {
-                fa[s][i] = *reinterpret_cast<const bf16x8*>(&sa[cur][s][(wm + i * 16 + (lane & 15)) * kOzLd + 8 * (lane >> 4)]);
-                fb[s][i] = *reinterpret_cast<const bf16x8*>(&sb[cur][s][(wn + i * 16 + (lane & 15)) * kOzLd + 8 * (lane >> 4)]);
+                fa[s][i] = *reinterpret_cast<const bf16x8*>(&sa[cur][s][aoff[i]]);
+                fb[s][i] = *reinterpret_cast<const bf16x8*>(&sb[cur][s][boff[i]]);
             }
 #pragma unroll
         for (int s = 0; s < NSL; s++)
