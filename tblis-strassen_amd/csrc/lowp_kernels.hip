@@ -229,10 +229,12 @@ __device__ __forceinline__ void put_out(const LowpGemmArgs& g, int q, double* Mq
 
 // Output of a 128 x 64 tile held by 8 waves (4 x 2, 32 x 32 each, element v of a lane at row
 // 4 (lane / 16) + v, column lane % 16 of its 16 x 16 block), through LDS in two halves of 64 rows:
-// the waves holding a half write it to `stage` (64 columns x 65 doubles), then all 512 threads write it
-// out with consecutive threads on consecutive rows (C's unit-stride dimension after the plan's
-// orientation, and M's), applying the row / column scale exponents.  `stage` aliases the K-loop's LDS
-// buffers, which are free after the loop.
+// the waves holding a half write it to `stage` (64 columns x 64 doubles, row r of column c at position
+// (r + c) % 64, which spreads both the writes and the reads over the banks), then all 512 threads write
+// it out with consecutive threads on consecutive rows (C's unit-stride dimension after the plan's
+// orientation, and M's), applying the row / column scale exponents.  `stage` (32 KiB) aliases the
+// K-loop's LDS buffers, which are free after the loop.
+constexpr int kStageBytes = 64 * 64 * 8;
 __device__ __forceinline__ void tile_out_128x64(const LowpGemmArgs& g, int q, const double (&acc)[2][2][4],
                                                 double* stage, int64_t row0, int64_t col0, int wm, int wn,
                                                 const int* __restrict__ ea, const int* __restrict__ eb,
@@ -253,8 +255,10 @@ __device__ __forceinline__ void tile_out_128x64(const LowpGemmArgs& g, int q, co
                 for (int j = 0; j < 2; j++)
 #pragma unroll
                     for (int v = 0; v < 4; v++)
-                        stage[(wn + j * 16 + (lane & 15)) * 65 + (wm - 64 * h + i * 16 + 4 * (lane >> 4) + v)] =
-                            acc[i][j][v];
+                    {
+                        const int c = wn + j * 16 + (lane & 15), rr = wm - 64 * h + i * 16 + 4 * (lane >> 4) + v;
+                        stage[c * 64 + ((rr + c) & 63)] = acc[i][j][v];
+                    }
         }
         __syncthreads();
         const int64_t row = row0 + 64 * h + r;
@@ -266,7 +270,8 @@ __device__ __forceinline__ void tile_out_128x64(const LowpGemmArgs& g, int q, co
             {
                 const int c = cg + 8 * e;
                 const int64_t col = col0 + c;
-                if (col < g.ns) put_out(g, q, Mq, row, col, ldexp(stage[c * 65 + r], er + eb[int64_t(q) * g.ns + col]));
+                if (col < g.ns)
+                    put_out(g, q, Mq, row, col, ldexp(stage[c * 64 + ((r + c) & 63)], er + eb[int64_t(q) * g.ns + col]));
             }
         }
     }
@@ -376,34 +381,43 @@ __global__ __launch_bounds__(256) void ozaki_gemm_kernel(const __bf16* __restric
 constexpr uint32_t kBufOob = 0x80000000u;  // buffer-load offsets at or past this read 0
 
 // LDS-staged form: 512 threads (8 waves as 4 x 2, 32 x 32 each) per 128 x 64 tile of M_q.  Every digit
-// element of a K-step (32) is read from HBM/L2 once per workgroup into a double-buffered LDS tile laid
-// out as [slice][row][64 bytes] with the four 16-byte chunks of row r XOR-swizzled by (r >> 1) & 3, the
-// INT8 GEMM's layout: the MFMA operand reads (16 consecutive rows at one chunk) spread each 8 rows over
-// the 8 chunk slots of a 128-byte bank window (rows padded to 80 bytes instead cost twice the LDS
-// cycles in bank conflicts).  Needs mp % 128 == 0, np % 64 == 0, kp % 32 == 0.
-// Two workgroups per CU need at most 128 VGPRs: asked for with 2 slices (the 3-product split), where
-// the output staging took one register over and halved the occupancy.
+// element of a K-step (32) is loaded from HBM/L2 once per workgroup straight into LDS (buffer loads with
+// the LDS destination: no staging registers), laid out as [slice][row][64 bytes] with the four 16-byte
+// chunks of row r XOR-swizzled by (r >> 1) & 3, the INT8 GEMM's layout: the MFMA operand reads (16
+// consecutive rows at one chunk) spread each 8 rows over the 8 chunk slots of a 128-byte bank window
+// (rows padded to 80 bytes instead cost twice the LDS cycles in bank conflicts).  A tiles are loaded two
+// K-steps ahead into three buffers, B tiles one step ahead into two (5 slices: 120 + 40 KiB, all of the
+// LDS): the 8 waves of the one workgroup per CU meet at every step's barrier together, so a step's MFMAs
+// must cover the loads' latency by themselves.  Needs mp % 128 == 0, np % 64 == 0, kp % 32 == 0, NSL <= 5.
+// Two workgroups per CU need at most 128 VGPRs: asked for with 2 slices (the 3-product split).
 template <int NSL>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(NSL <= 2 ? 4 : 1))) void ozaki_gemm_lds_kernel(const __bf16* __restrict__ As, const __bf16* __restrict__ Bs,
                                                              const int* __restrict__ ea, const int* __restrict__ eb,
                                                              double* __restrict__ M, LowpGemmArgs g)
 {
-    constexpr int NSA = NSL < 3 ? 3 : NSL;  // at least the 33 KiB of the output staging tile
-    __shared__ __attribute__((aligned(16))) char sa[2][NSA][128 * 64];
-    __shared__ __attribute__((aligned(16))) char sb[2][NSL][64 * 64];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    static_assert(NSL >= 2 && NSL <= 5, "LDS: 3 x NSL x 8 KiB (A) + 2 x NSL x 4 KiB (B) <= 160 KiB");
+    // separate arrays per buffer, indexed by compile-time buffer numbers: the compiler's wait insertion
+    // then knows that a load into one buffer does not feed reads of another (with one array and a
+    // run-time index it waits for every load into LDS before the next step's reads)
+    // (the first A buffer, at least 4 slices long, doubles as the output staging tile; 2 slices take
+    // 80 KiB, two workgroups per CU)
+    __shared__ __attribute__((aligned(16))) char sa0[NSL < 4 ? 4 : NSL][128 * 64];
+    __shared__ __attribute__((aligned(16))) char sa1[NSL][128 * 64], sa2[NSL][128 * 64];
+    __shared__ __attribute__((aligned(16))) char sb0[NSL][64 * 64], sb1[NSL][64 * 64];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int tiles_m = int(g.mp / 128);
     const int tm = blockIdx.x % tiles_m, tn = blockIdx.x / tiles_m;
     const int q = g.q0 + int(blockIdx.y);  // product (the launch covers [q0, q0 + nlaunch))
     const int64_t row0 = int64_t(tm) * 128, col0 = int64_t(tn) * 64;
     const int wm = (wave & 3) * 32, wn = (wave >> 2) * 32;
     const int64_t slab_a = int64_t(g.nprod) * g.mp * g.kp, slab_b = int64_t(g.nprod) * g.np * g.kp;
-    // loader: A tile = 128 rows x 32 k = 512 x 16 B per slice (one per thread); B tile = 64 x 32 k = 256 x 16 B.
-    // Buffer loads as in the INT8 GEMM: a scalar resource per slice at the tile's first row, a fixed lane
-    // offset and the K-step as the scalar offset (no vector instructions on addresses in the loop); lanes
-    // past the B tile read 0.  Byte offsets stay below 256 kp < 2^31 (host-checked).
-    const int lr = tid >> 2, lc = tid & 3;  // row, 16-byte chunk (8 k) of this thread's load
-    const bool bload = tid < 256;
+    // Loads: a scalar resource per slice at the tile's first row, a fixed lane offset and the K-step as
+    // the scalar offset (no vector instructions on addresses in the loop).  Wave w fills rows
+    // 16 w .. 16 w + 15 of every A slice tile (waves 0-3 also of every B slice tile), one 16-byte chunk
+    // per lane: LDS slot `lane` of the wave's 1 KiB is row 16 w + lane / 4 at position lane % 4, which
+    // holds chunk (lane % 4) ^ ((row >> 1) & 3) of the row.  Byte offsets stay below 256 kp < 2^31
+    // (host-checked).
     __amdgpu_buffer_rsrc_t rsa[NSL], rsb[NSL];
 #pragma unroll
     for (int s = 0; s < NSL; s++)
@@ -413,9 +427,9 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(NSL <= 2 ? 
         rsb[s] = __builtin_amdgcn_make_buffer_rsrc(const_cast<__bf16*>(Bs + s * slab_b + (int64_t(q) * g.np + col0) * g.kp),
                                                    (short)0, int(kBufOob), 0x00020000);
     }
-    const uint32_t voa = uint32_t((lr * g.kp + lc * 8) * 2);
-    const uint32_t vob = bload ? uint32_t(((lr & 63) * g.kp + lc * 8) * 2) : kBufOob;
-    const int soff = lr * 64 + ((lc ^ ((lr >> 1) & 3)) * 16);  // this thread's swizzled LDS chunk
+    const int dr = wave * 16 + (lane >> 2);
+    const int dch = (lane & 3) ^ ((dr >> 1) & 3);
+    const uint32_t vo = uint32_t((dr * g.kp + dch * 8) * 2);  // the B tile's rows are those of waves 0-3
 
     double acc[2][2][4];
     f32x4 a32[NSL][2][2];
@@ -442,22 +456,41 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(NSL <= 2 ? 
             }
     };
 
-    bf16x8 ra[NSL], rb[NSL];
-    auto gload = [&](int st) {
+    typedef __attribute__((address_space(3))) void* lds_ptr;
+    auto dma_a = [&](char (*buf)[128 * 64], int st) {
+#if defined(__HIP_DEVICE_COMPILE__)  // the builtin exists for the gfx950 target only, not the host pass
 #pragma unroll
         for (int s = 0; s < NSL; s++)
-        {
-            ra[s] = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(rsa[s], voa, st * 64, 0));
-            rb[s] = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(rsb[s], vob, st * 64, 0));
-        }
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(rsa[s], (lds_ptr)(&buf[s][wave * 1024]), 16, vo, st * 64, 0, 0);
+#else
+        (void)buf;
+        (void)st;
+#endif
     };
-    auto lstore = [&](int buf) {
+    auto dma_b = [&](char (*buf)[64 * 64], int st) {
+#if defined(__HIP_DEVICE_COMPILE__)
+        if (wave < 4)
 #pragma unroll
-        for (int s = 0; s < NSL; s++)
-        {
-            *reinterpret_cast<bf16x8*>(&sa[buf][s][soff]) = ra[s];
-            if (bload) *reinterpret_cast<bf16x8*>(&sb[buf][s][soff]) = rb[s];
-        }
+            for (int s = 0; s < NSL; s++)
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(rsb[s], (lds_ptr)(&buf[s][wave * 1024]), 16, vo, st * 64, 0, 0);
+#else
+        (void)buf;
+        (void)st;
+#endif
+    };
+    // Before a barrier that publishes a step's tiles: wait for this wave's loads into LDS (vmcnt counts
+    // them) except, with `keep`, the NSL newest ones (A of two steps ahead, issued after everything the
+    // barrier publishes).  Written out: the compiler's wait insertion does not see LDS reads depending on
+    // earlier loads into LDS.
+    auto landed = [&](bool keep) {
+#if defined(__HIP_DEVICE_COMPILE__)
+        if (keep)
+            __builtin_amdgcn_s_waitcnt(0x0F70 | NSL);  // vmcnt(NSL); expcnt and lgkmcnt unconstrained
+        else
+            __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+#else
+        (void)keep;
+#endif
     };
 
     // operand reads: 16 consecutive rows (lane % 16) at chunk lane / 16
@@ -471,23 +504,33 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(NSL <= 2 ? 
         boff[i] = rb_ * 64 + ((fc ^ ((rb_ >> 1) & 3)) * 16);
     }
 
+    // The step barrier: the wave's LDS reads done (their fragments are consumed by then anyway), then
+    // s_barrier, with no memory fence: __syncthreads() would also wait for every load into LDS, the A
+    // tiles two steps ahead included.  The "memory" clobber keeps the compiler from moving LDS accesses
+    // across it.
+    auto step_barrier = [&]() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); };
+
     const int nsteps = int(g.kp / 32);
-    gload(0);
-    lstore(0);
-    __syncthreads();
-    for (int st = 0; st < nsteps; st++)
-    {
-        const int cur = st & 1;
-        if (st + 1 < nsteps) gload(st + 1);
+    // One K-step.  A of step st is in buffer st % 3 (sa0, sa1, sa2), B in buffer st % 2 (sb0, sb1).
+    auto step = [&](int st, auto abuf_c) {
+        constexpr int ab = decltype(abuf_c)::value;
+        const int bb = st & 1;
+        char(*const ca)[128 * 64] = ab == 0 ? sa0 : ab == 1 ? sa1 : sa2;
+        char(*const cb)[64 * 64] = bb == 0 ? sb0 : sb1;
         bf16x8 fa[NSL][2], fb[NSL][2];
 #pragma unroll
         for (int s = 0; s < NSL; s++)
 #pragma unroll
             for (int i = 0; i < 2; i++)
             {
-                fa[s][i] = *reinterpret_cast<const bf16x8*>(&sa[cur][s][aoff[i]]);
-                fb[s][i] = *reinterpret_cast<const bf16x8*>(&sb[cur][s][boff[i]]);
+                fa[s][i] = *reinterpret_cast<const bf16x8*>(&ca[s][aoff[i]]);
+                fb[s][i] = *reinterpret_cast<const bf16x8*>(&cb[s][boff[i]]);
             }
+        // B of step st + 1 into the other B buffer and A of step st + 2 into the A buffer after next: both
+        // last read in step st - 1, before the barrier that ended it
+        __builtin_amdgcn_sched_barrier(0);
+        if (st + 1 < nsteps) dma_b(bb == 0 ? sb1 : sb0, st + 1);
+        if (st + 2 < nsteps) dma_a(ab == 0 ? sa2 : ab == 1 ? sa0 : sa1, st + 2);
 #pragma unroll
         for (int s = 0; s < NSL; s++)
 #pragma unroll
@@ -500,14 +543,32 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(NSL <= 2 ? 
 #pragma unroll
         for (int d = 0; d < NSL; d++)
             if ((st + 1) % flush_period(d) == 0) flush(d);
-        if (st + 1 < nsteps) lstore(cur ^ 1);
-        __syncthreads();
+        landed(st + 2 < nsteps);
+        step_barrier();
+    };
+    typedef std::integral_constant<int, 0> I0;
+    typedef std::integral_constant<int, 1> I1;
+    typedef std::integral_constant<int, 2> I2;
+
+    dma_a(sa0, 0);
+    dma_b(sb0, 0);
+    if (nsteps > 1) dma_a(sa1, 1);
+    landed(nsteps > 1);
+    step_barrier();
+    // three steps per trip, one per A buffer
+    for (int st = 0; st < nsteps; st += 3)
+    {
+        step(st, I0());
+        if (st + 1 >= nsteps) break;
+        step(st + 1, I1());
+        if (st + 2 >= nsteps) break;
+        step(st + 2, I2());
     }
 #pragma unroll
     for (int d = 0; d < NSL; d++) flush(d);
 
-    static_assert(sizeof(sa) >= 64 * 65 * sizeof(double), "output staging fits the A buffers");
-    tile_out_128x64(g, q, acc, reinterpret_cast<double*>(&sa[0][0][0]), row0, col0, wm, wn, ea, eb, M);
+    static_assert(sizeof(sa0) >= kStageBytes, "output staging fits the first A buffer");
+    tile_out_128x64(g, q, acc, reinterpret_cast<double*>(&sa0[0][0]), row0, col0, wm, wn, ea, eb, M);
 }
 
 // ---- Ozaki INT8 GEMM ---------------------------------------------------------------------------------
@@ -527,8 +588,7 @@ __global__ __launch_bounds__(512) void ozaki_i8_gemm_kernel(const int8_t* __rest
                                                             double* __restrict__ M, LowpGemmArgs g)
 {
     static_assert(NSL <= 6, "LDS: 2 x NSL x 12 KiB");
-    constexpr int NSA = NSL < 3 ? 3 : NSL;  // at least the 33 KiB of the output staging tile
-    __shared__ __attribute__((aligned(16))) int8_t sa[2][NSA][128 * 64];
+    __shared__ __attribute__((aligned(16))) int8_t sa[2][NSL][128 * 64];  // (>= 2 slices: the 32 KiB output stage)
     __shared__ __attribute__((aligned(16))) int8_t sb[2][NSL][64 * 64];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int tiles_m = int(g.mp / 128);
@@ -654,7 +714,7 @@ __global__ __launch_bounds__(512) void ozaki_i8_gemm_kernel(const int8_t* __rest
     }
     fold();
 
-    static_assert(sizeof(sa) >= 64 * 65 * sizeof(double), "output staging fits the A buffers");
+    static_assert(sizeof(sa) >= kStageBytes, "output staging fits the A buffers");
     tile_out_128x64(g, q, acc, reinterpret_cast<double*>(&sa[0][0][0]), row0, col0, wm, wn, ea, eb, M);
 }
 
