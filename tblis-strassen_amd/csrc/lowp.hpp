@@ -38,6 +38,8 @@ struct LowpGemmArgs
     int q0, nlaunch;
 };
 
+// Scale exponents, then digits: launch_lowp_exponents fills ea for an S with rows fastest; the slice
+// launches fill the exponents of K-fast operands (T, and S with s_kfast) in the same pass as the digits.
 hipError_t launch_lowp_exponents(const LowpSliceArgs& a, hipStream_t s);
 hipError_t launch_ozaki_slices(int nslices, const LowpSliceArgs& a, hipStream_t s);
 hipError_t launch_ozaki_gemm(int nslices, const void* As, const void* Bs, const int* ea, const int* eb, double* M,

@@ -24,6 +24,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "lowp.hpp"
 
@@ -55,22 +56,6 @@ __global__ __launch_bounds__(256) void row_exp_kernel(const double* __restrict__
         double m = 0.0;
         for (int64_t k = 0; k < cols; k++) m = fmax(m, fabs(s[k * rows]));
         e[x] = exp_of(m);
-    }
-}
-
-// T: [q][k + j * kk] (k fastest): one wave per (q, j)
-__global__ __launch_bounds__(256) void col_exp_kernel(const double* __restrict__ T, int64_t kk, int64_t cols,
-                                                      int nmat, int* __restrict__ e)
-{
-    const int lane = threadIdx.x & 63;
-    const int64_t n = int64_t(nmat) * cols;
-    for (int64_t x = (int64_t(blockIdx.x) * 256 + threadIdx.x) / 64; x < n; x += int64_t(gridDim.x) * 4)
-    {
-        const double* t = T + x * kk;
-        double m = 0.0;
-        for (int64_t k = lane; k < kk; k += 64) m = fmax(m, fabs(t[k]));
-        for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
-        if (lane == 0) e[x] = exp_of(m);
     }
 }
 
@@ -124,25 +109,6 @@ __global__ __launch_bounds__(256) void slice_rows_kernel(const double* __restric
     }
 }
 
-// T (k fastest, [q][j][k]) -> out[s][q][j][k]
-template <int NSL>
-__global__ __launch_bounds__(256) void slice_cols_kernel(const double* __restrict__ T, int64_t kk, int64_t cols,
-                                                         const int* __restrict__ e, __bf16* __restrict__ out,
-                                                         int64_t cp, int64_t kp, int nmat)
-{
-    const int64_t n = int64_t(nmat) * cp * kp, slab = n;
-    for (int64_t x = int64_t(blockIdx.x) * 256 + threadIdx.x; x < n; x += int64_t(gridDim.x) * 256)
-    {
-        const int64_t k = x % kp, j = (x / kp) % cp, q = x / (kp * cp);
-        double v = 0.0;
-        if (j < cols && k < kk) v = ldexp(T[(q * cols + j) * kk + k], -e[q * cols + j]);
-        __bf16 d[NSL];
-        digits<NSL>(v, d);
-#pragma unroll
-        for (int sl = 0; sl < NSL; sl++) out[sl * slab + x] = d[sl];
-    }
-}
-
 // ---- Ozaki INT8 digits ------------------------------------------------------------------------------
 
 template <int NSL>
@@ -193,21 +159,42 @@ __global__ __launch_bounds__(256) void slice_rows_i8_kernel(const double* __rest
 }
 
 // T (k fastest, [q][j][k]) -> out[s][q][j][k] (int8)
-template <int NSL>
-__global__ __launch_bounds__(256) void slice_cols_i8_kernel(const double* __restrict__ T, int64_t kk, int64_t cols,
-                                                            const int* __restrict__ e, int8_t* __restrict__ out,
-                                                            int64_t cp, int64_t kp, int nmat)
+// ---- K-fast operands: scale exponent and digits in one pass -------------------------------------------
+
+// T (k fastest, [q][j][k], kk long) -> e[q][j] and out[s][q][j][k] (rows padded to cp, k to kp, padding
+// zero): one wave per padded row, which takes the row's largest magnitude (exponent e, so |x| 2^-e < 1)
+// and then reads the row again for the digits -- the second read mostly from cache, where a separate
+// exponent pass read the operand from HBM once more.  D = __bf16 (BF16 digits) or int8_t (INT8 digits).
+template <int NSL, typename D>
+__global__ __launch_bounds__(256) void slice_kfast_kernel(const double* __restrict__ T, int64_t kk, int64_t cols,
+                                                          int* __restrict__ e, D* __restrict__ out, int64_t cp,
+                                                          int64_t kp, int nmat)
 {
-    const int64_t n = int64_t(nmat) * cp * kp, slab = n;
-    for (int64_t x = int64_t(blockIdx.x) * 256 + threadIdx.x; x < n; x += int64_t(gridDim.x) * 256)
+    const int lane = threadIdx.x & 63;
+    const int64_t n = int64_t(nmat) * cp, slab = n * kp;
+    for (int64_t x = (int64_t(blockIdx.x) * 256 + threadIdx.x) / 64; x < n; x += int64_t(gridDim.x) * 4)
     {
-        const int64_t k = x % kp, j = (x / kp) % cp, q = x / (kp * cp);
-        double v = 0.0;
-        if (j < cols && k < kk) v = ldexp(T[(q * cols + j) * kk + k], -e[q * cols + j]);
-        int8_t d[NSL];
-        digits_i8<NSL>(v, d);
+        const int64_t q = x / cp, j = x % cp;
+        const bool real = j < cols;
+        const double* t = T + (q * cols + j) * kk;
+        double m = 0.0;
+        if (real)
+            for (int64_t k = lane; k < kk; k += 64) m = fmax(m, fabs(t[k]));
+        for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+        const int ex = exp_of(m);
+        if (real && lane == 0) e[q * cols + j] = ex;
+        D* o = out + x * kp;
+        for (int64_t k = lane; k < kp; k += 64)
+        {
+            const double v = real && k < kk ? ldexp(t[k], -ex) : 0.0;
+            D d[NSL];
+            if constexpr (std::is_same<D, int8_t>::value)
+                digits_i8<NSL>(v, d);
+            else
+                digits<NSL>(v, d);
 #pragma unroll
-        for (int sl = 0; sl < NSL; sl++) out[sl * slab + x] = d[sl];
+            for (int sl = 0; sl < NSL; sl++) o[sl * slab + k] = d[sl];
+        }
     }
 }
 
@@ -864,12 +851,12 @@ template <int NSL>
 hipError_t slices_t(const LowpSliceArgs& a, hipStream_t s)
 {
     if (a.s_kfast)  // S laid out as T: rows of contiguous K
-        hipLaunchKernelGGL(slice_cols_kernel<NSL>, dim3(grid_of(int64_t(a.nprod) * a.mp * a.kp, 256)), dim3(256), 0, s,
-                           a.S, a.ks, a.ms, a.ea, static_cast<__bf16*>(a.outA), a.mp, a.kp, a.nprod);
+        hipLaunchKernelGGL((slice_kfast_kernel<NSL, __bf16>), dim3(grid_of(int64_t(a.nprod) * a.mp, 4)), dim3(256), 0,
+                           s, a.S, a.ks, a.ms, a.ea, static_cast<__bf16*>(a.outA), a.mp, a.kp, a.nprod);
     else
         hipLaunchKernelGGL(slice_rows_kernel<NSL>, dim3(unsigned(a.mp / 64), unsigned(a.kp / 64), unsigned(a.nprod)),
                            dim3(256), 0, s, a.S, a.ms, a.ks, a.ea, static_cast<__bf16*>(a.outA), a.mp, a.kp, a.nprod);
-    hipLaunchKernelGGL(slice_cols_kernel<NSL>, dim3(grid_of(int64_t(a.nprod) * a.np * a.kp, 256)), dim3(256), 0, s,
+    hipLaunchKernelGGL((slice_kfast_kernel<NSL, __bf16>), dim3(grid_of(int64_t(a.nprod) * a.np, 4)), dim3(256), 0, s,
                        a.T, a.ks, a.ns, a.eb, static_cast<__bf16*>(a.outB), a.np, a.kp, a.nprod);
     return hipGetLastError();
 }
@@ -898,12 +885,12 @@ template <int NSL>
 hipError_t slices_i8_t(const LowpSliceArgs& a, hipStream_t s)
 {
     if (a.s_kfast)
-        hipLaunchKernelGGL(slice_cols_i8_kernel<NSL>, dim3(grid_of(int64_t(a.nprod) * a.mp * a.kp, 256)), dim3(256), 0,
+        hipLaunchKernelGGL((slice_kfast_kernel<NSL, int8_t>), dim3(grid_of(int64_t(a.nprod) * a.mp, 4)), dim3(256), 0,
                            s, a.S, a.ks, a.ms, a.ea, static_cast<int8_t*>(a.outA), a.mp, a.kp, a.nprod);
     else
         hipLaunchKernelGGL(slice_rows_i8_kernel<NSL>, dim3(unsigned(a.mp / 64), unsigned(a.kp / 64), unsigned(a.nprod)),
                            dim3(256), 0, s, a.S, a.ms, a.ks, a.ea, static_cast<int8_t*>(a.outA), a.mp, a.kp, a.nprod);
-    hipLaunchKernelGGL(slice_cols_i8_kernel<NSL>, dim3(grid_of(int64_t(a.nprod) * a.np * a.kp, 256)), dim3(256), 0, s,
+    hipLaunchKernelGGL((slice_kfast_kernel<NSL, int8_t>), dim3(grid_of(int64_t(a.nprod) * a.np, 4)), dim3(256), 0, s,
                        a.T, a.ks, a.ns, a.eb, static_cast<int8_t*>(a.outB), a.np, a.kp, a.nprod);
     return hipGetLastError();
 }
@@ -955,15 +942,14 @@ hipError_t launch_ozaki_i8_gemm(int nsl, const void* As, const void* Bs, const i
 
 hipError_t launch_lowp_exponents(const LowpSliceArgs& a, hipStream_t s)
 {
-    if (a.s_kfast)  // one wave per row of contiguous K
-        hipLaunchKernelGGL(col_exp_kernel, dim3(grid_of(int64_t(a.nprod) * a.ms, 4)), dim3(256), 0, s, a.S, a.ks, a.ms,
-                           a.nprod, a.ea);
-    else
+    // K-fast operands (T always, S when s_kfast) get their exponents in the slice pass itself
+    if (!a.s_kfast)
+    {
         hipLaunchKernelGGL(row_exp_kernel, dim3(grid_of(int64_t(a.nprod) * a.ms, 256)), dim3(256), 0, s, a.S, a.ms,
                            a.ks, a.nprod, a.ea);
-    hipLaunchKernelGGL(col_exp_kernel, dim3(grid_of(int64_t(a.nprod) * a.ns, 4)), dim3(256), 0, s, a.T, a.ks, a.ns,
-                       a.nprod, a.eb);
-    return hipGetLastError();
+        return hipGetLastError();
+    }
+    return hipSuccess;
 }
 
 hipError_t launch_ozaki_slices(int nsl, const LowpSliceArgs& a, hipStream_t s)
