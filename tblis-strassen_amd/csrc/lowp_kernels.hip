@@ -537,8 +537,11 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(NSL <= 2 ? 
     typedef std::integral_constant<int, 1> I1;
     typedef std::integral_constant<int, 2> I2;
 
-    dma_a(sa0, 0);
-    dma_b(sb0, 0);
+    if (nsteps > 0)  // (no K: no loads; the tile is written as zeros)
+    {
+        dma_a(sa0, 0);
+        dma_b(sb0, 0);
+    }
     if (nsteps > 1) dma_a(sa1, 1);
     landed(nsteps > 1);
     step_barrier();
@@ -668,8 +671,11 @@ __global__ __launch_bounds__(512) void ozaki_i8_gemm_kernel(const int8_t* __rest
     }
 
     const int nsteps = int(g.kp / 64);
-    gload(0);
-    lstore(0);
+    if (nsteps > 0)  // (no K: no loads; the tile is written as zeros)
+    {
+        gload(0);
+        lstore(0);
+    }
     __syncthreads();
     for (int st = 0; st < nsteps; st++)
     {
