@@ -175,7 +175,7 @@ def bench_c3(ts, torch, args, stream, workload="c3"):
         case = make_workload("c2")
         emax = 0
         modes = [("fp64", 0), ("fp64", 1), ("fp64", 2), ("bf16x3", 1), ("ozaki_bf16", 0), ("ozaki_bf16", 1),
-                 ("ozaki_int8", 0), ("ozaki_int8", 1), ("ozaki_int8", 2)]
+                 ("ozaki_bf16", 2), ("ozaki_int8", 0), ("ozaki_int8", 1), ("ozaki_int8", 2)]
     if args.modes:
         modes = [(x.split(":")[0], int(x.split(":")[1])) for x in args.modes.split(",")]
     m, n, k = cases.mnk(case)
