@@ -34,6 +34,13 @@ int mistra_debug_run_strides(const long long* scat, long long n, int run, long l
  * to test that.  Process-wide; returns the previous setting. */
 int mistra_debug_set_combine_form(int per_sub_block);
 
+/* Digit counts of the Ozaki modes for every later call of this process, overriding the environment
+ * (MISTRA_OZAKI_SLICES, MISTRA_OZAKI_I8_SLICES): bf16_slices 2..7 (6 and 7 take the GEMM kernel that
+ * stages through registers), int8_slices 2..6, and int8_fold_ksteps 1..1024: the INT8 GEMM folds its
+ * exact INT32 sums into FP64 every that many 64-deep K-steps (by default only for K beyond 65,536).
+ * 0 restores a default.  Returns 0, or MISTRA_ERR_ARG for a value out of range. */
+int mistra_debug_set_lowp(int bf16_slices, int int8_slices, int int8_fold_ksteps);
+
 #ifdef __cplusplus
 } /* extern "C" */
 } /* namespace tblis */

@@ -43,8 +43,23 @@ COMMON = [
 ]
 
 
+def up_to_date(src, obj):
+    """The object is newer than its source and every header of csrc/ and include/ (a coarse rule)."""
+    if not os.path.exists(obj):
+        return False
+    t = os.path.getmtime(obj)
+    deps = [os.path.join(CSRC, src)] + [os.path.join(d, f) for d in (CSRC, os.path.join(ROOT, "include"))
+                                         for f in os.listdir(d) if f.endswith((".hpp", ".h"))]
+    return all(os.path.getmtime(x) < t for x in deps)
+
+
+INCREMENTAL = False
+
+
 def compile_one(src):
     obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
+    if INCREMENTAL and up_to_date(src, obj):
+        return src, 0, "", obj
     cmd = [HIPCC] + COMMON + ["-c", os.path.join(CSRC, src), "-o", obj]
     if src.endswith(".cpp"):
         cmd = [HIPCC] + COMMON + ["-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
@@ -76,5 +91,7 @@ if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=4)
     ap.add_argument("-v", "--verbose", action="store_true")
+    ap.add_argument("--incremental", action="store_true", help="skip objects newer than their sources and headers")
     a = ap.parse_args()
+    INCREMENTAL = a.incremental
     print(build(a.jobs, a.verbose))

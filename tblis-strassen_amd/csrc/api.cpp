@@ -132,17 +132,16 @@ void set_scalar_one(tblis::tblis_tensor* t)
     t->conj = 0;
 }
 
-Status scale_c(const tblis::tblis_tensor* C, void* data, double beta, hipStream_t s)
+Status merged_dims_c(const tblis::tblis_tensor* t, const label_type* idx, StridedDims& d);
+
+// C = beta*C over the elements that C[idx_C] addresses: a repeated label addresses a diagonal
+// (src/stra_iface/3t/stra_mult.cxx:47-50 applies diagonal() to C before :212-231 scales it), so the
+// elements off that diagonal keep their values.
+Status scale_c(const tblis::tblis_tensor* C, const label_type* idx_C, void* data, double beta, hipStream_t s)
 {
     StridedDims d;
-    std::memset(&d, 0, sizeof(d));
-    if (C->ndim > 8) return Status::error(MISTRA_ERR_UNSUPPORTED, "C has more than 8 dimensions");
-    d.ndim = int(C->ndim);
-    for (unsigned i = 0; i < C->ndim; i++)
-    {
-        d.len[i] = C->len[i];
-        d.stride[i] = C->stride[i];
-    }
+    Status st = merged_dims_c(C, idx_C, d);
+    if (!st.ok()) return st;
     if (beta == 1.0) return Status();
     hipError_t e = launch_scale(C->type == tblis::TYPE_DOUBLE ? 8 : 4, data, d, beta, s);
     if (e != hipSuccess) return Status::error(MISTRA_ERR_HIP, hipGetErrorString(e));
@@ -167,6 +166,7 @@ Status validate(const tblis::tblis_tensor* A, const label_type* idx_A, const tbl
 // Digit slices of the Ozaki mode: 5 (15 products, about 40 bits) unless MISTRA_OZAKI_SLICES says 2..7.
 int ozaki_slices()
 {
+    if (const int o = g_dbg_ozaki_slices.load()) return o;
     static const int n = [] {
         const char* v = std::getenv("MISTRA_OZAKI_SLICES");
         const int x = v ? std::atoi(v) : 5;
@@ -179,6 +179,7 @@ int ozaki_slices()
 // says 2..6.
 int ozaki_i8_slices()
 {
+    if (const int o = g_dbg_i8_slices.load()) return o;
     static const int n = [] {
         const char* v = std::getenv("MISTRA_OZAKI_I8_SLICES");
         const int x = v ? std::atoi(v) : 6;
@@ -238,6 +239,21 @@ Status merged_dims(const tblis::tblis_tensor* t, const label_type* idx, std::vec
         if (!found) out.push_back(Dim{idx[i], t->len[i], t->stride[i]});
     }
     if (out.size() > 8) return Status::error(MISTRA_ERR_UNSUPPORTED, "more than 8 distinct labels in one tensor");
+    return Status();
+}
+
+Status merged_dims_c(const tblis::tblis_tensor* t, const label_type* idx, StridedDims& d)
+{
+    std::vector<Dim> v;
+    Status st = merged_dims(t, idx, v);
+    if (!st.ok()) return st;
+    std::memset(&d, 0, sizeof(d));
+    d.ndim = int(v.size());
+    for (size_t i = 0; i < v.size(); i++)
+    {
+        d.len[i] = v[i].len;
+        d.stride[i] = v[i].stride;
+    }
     return Status();
 }
 
@@ -407,6 +423,71 @@ Status general_mult(const tblis::tblis_tensor* A, const label_type* idx_A, const
     return st;
 }
 
+namespace {
+
+// FP32 tensors in a low-precision mode: the modes form their operand sums, split / scale computation and
+// combine in FP64, so A, B and C are copied to FP64 (the touched span of each, same strides), the FP64
+// contraction runs on the copies, and the addressed elements of C are rounded back to FP32.  Device
+// tensors only.
+Status lowp_float(const tblis::tblis_tensor* A, const label_type* idx_A, const tblis::tblis_tensor* B,
+                  const label_type* idx_B, tblis::tblis_tensor* C, const label_type* idx_C, int level, int impl,
+                  int precision, unsigned flags, hipStream_t stream, bool synchronous, int tile)
+{
+    if (A->type != tblis::TYPE_FLOAT || B->type != tblis::TYPE_FLOAT)
+        return Status::error(MISTRA_ERR_ARG, "A, B and C must have the same type");
+    if (!is_device_pointer(A->data) || !is_device_pointer(B->data) || !is_device_pointer(C->data))
+        return Status::error(MISTRA_ERR_UNSUPPORTED, "low-precision modes on FP32 tensors need device memory");
+    StridedDims cd;
+    Status st = merged_dims_c(C, idx_C, cd);
+    if (!st.ok()) return st;
+    const tblis::tblis_tensor* src[3] = {A, B, C};
+    tblis::tblis_tensor t64[3];
+    void* buf[3] = {nullptr, nullptr, nullptr};
+    auto release = [&]() {
+        for (void* b : buf)
+            if (b) (void)hipFreeAsync(b, stream);
+    };
+    for (int i = 0; i < 3; i++)
+    {
+        t64[i] = *src[i];
+        t64[i].type = tblis::TYPE_DOUBLE;
+        set_scalar(&t64[i], scalar_as_double(src[i]));
+        if (empty_tensor(src[i])) continue;
+        int64_t lo = 0, hi = 0;
+        span(src[i], lo, hi);
+        const int64_t n = hi - lo + 1;
+        hipError_t e = hipMallocAsync(&buf[i], size_t(n) * 8, stream);
+        if (e == hipSuccess)
+            e = launch_f32_to_f64(static_cast<const float*>(src[i]->data) + lo, static_cast<double*>(buf[i]), n,
+                                  stream);
+        if (e != hipSuccess)
+        {
+            release();
+            return Status::error(e == hipErrorOutOfMemory ? MISTRA_ERR_NOMEM : MISTRA_ERR_HIP,
+                                 std::string("FP64 copies of FP32 tensors: ") + hipGetErrorString(e));
+        }
+        t64[i].data = static_cast<double*>(buf[i]) - lo;
+    }
+    st = tensor_mult(&t64[0], idx_A, &t64[1], idx_B, &t64[2], idx_C, level, impl, precision,
+                     flags & ~unsigned(MISTRA_FLAG_SYNC), stream, false, tile);
+    if (st.ok() && buf[2])
+    {
+        hipError_t e = launch_f64_to_f32_strided(static_cast<const double*>(t64[2].data), static_cast<float*>(C->data),
+                                                 cd, stream);
+        if (e != hipSuccess) st = Status::error(MISTRA_ERR_HIP, hipGetErrorString(e));
+    }
+    release();
+    if (st.ok() && (synchronous || (flags & MISTRA_FLAG_SYNC)))
+    {
+        hipError_t e = hipStreamSynchronize(stream);
+        if (e != hipSuccess) st = Status::error(MISTRA_ERR_HIP, hipGetErrorString(e));
+    }
+    if (st.ok()) set_scalar_one(C);
+    return st;
+}
+
+}  // namespace
+
 Status tensor_mult(const tblis::tblis_tensor* A, const label_type* idx_A, const tblis::tblis_tensor* B,
                    const label_type* idx_B, tblis::tblis_tensor* C, const label_type* idx_C, int level, int impl,
                    int precision, unsigned flags, hipStream_t stream, bool synchronous, int tile)
@@ -415,8 +496,10 @@ Status tensor_mult(const tblis::tblis_tensor* A, const label_type* idx_A, const 
     if (!st.ok()) return st;
     if (precision < MISTRA_PREC_NATIVE || precision > MISTRA_PREC_OZAKI_INT8)
         return Status::error(MISTRA_ERR_ARG, "unknown precision mode");
+    if (precision != MISTRA_PREC_NATIVE && C->type == tblis::TYPE_FLOAT)
+        return lowp_float(A, idx_A, B, idx_B, C, idx_C, level, impl, precision, flags, stream, synchronous, tile);
     if (precision != MISTRA_PREC_NATIVE && C->type != tblis::TYPE_DOUBLE)
-        return Status::error(MISTRA_ERR_UNSUPPORTED, "low-precision modes take FP64 tensors");
+        return Status::error(MISTRA_ERR_UNSUPPORTED, "low-precision modes take FP32 or FP64 tensors");
     if (level < 0 && impl != MISTRA_IMPL_AUTO)
         level = best_level(A, idx_A, B, idx_B, C, idx_C, impl);
     else if (level < 0 || impl == MISTRA_IMPL_AUTO)
@@ -436,10 +519,21 @@ Status tensor_mult(const tblis::tblis_tensor* A, const label_type* idx_A, const 
     const double alpha = scalar_as_double(A) * scalar_as_double(B);
     const double beta = scalar_as_double(C);
 
+    // A caller may record the call into a HIP graph: then the plan must exist already (building one
+    // uploads its tables synchronously) and stays resident for the graph's replays; host tensors (staged
+    // with copies from pageable memory) cannot be captured.
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(stream, &cap) != hipSuccess)
+    {
+        (void)hipGetLastError();
+        cap = hipStreamCaptureStatusNone;
+    }
+    const bool capturing = cap != hipStreamCaptureStatusNone;
+
     std::shared_ptr<Plan> plan;
     if (!empty_tensor(C) && alpha != 0.0 && !empty_tensor(A) && !empty_tensor(B))
     {
-        st = get_plan(A, idx_A, B, idx_B, C, idx_C, level, impl, tile, flags, plan);
+        st = get_plan(A, idx_A, B, idx_B, C, idx_C, level, impl, tile, flags, plan, capturing);
         if (!st.ok()) return st;
     }
     else if (A->type != C->type || B->type != C->type)
@@ -449,6 +543,10 @@ Status tensor_mult(const tblis::tblis_tensor* A, const label_type* idx_A, const 
     if (!elem) return Status::error(MISTRA_ERR_UNSUPPORTED, "only TYPE_FLOAT and TYPE_DOUBLE are supported");
 
     const bool dev = is_device_pointer(A->data) && is_device_pointer(B->data) && is_device_pointer(C->data);
+    if (capturing && !dev)
+        return Status::error(MISTRA_ERR_UNSUPPORTED, "host tensors cannot be used inside a HIP graph capture");
+    if (capturing && synchronous)
+        return Status::error(MISTRA_ERR_UNSUPPORTED, "a synchronous call cannot be recorded into a HIP graph");
     const void *dA = A->data, *dB = B->data;
     void* dC = C->data;
     void *bufA = nullptr, *bufB = nullptr, *bufC = nullptr;
@@ -499,7 +597,7 @@ Status tensor_mult(const tblis::tblis_tensor* A, const label_type* idx_A, const 
         if (!plan)
         {
             // alpha == 0 or an empty operand: C = beta*C (src/stra_iface/3t/stra_mult.cxx:212-231)
-            st = scale_c(C, dC, beta, stream);
+            st = scale_c(C, idx_C, dC, beta, stream);
         }
         else
         {

@@ -654,6 +654,51 @@ hipError_t launch_scale(int elem_size, void* C, const StridedDims& d, double bet
     return hipGetLastError();
 }
 
+// ---- FP32 tensors in the low-precision modes: FP64 staging copies -------------------------------------
+
+__global__ __launch_bounds__(256) void f32_to_f64_kernel(const float* __restrict__ src, double* __restrict__ dst,
+                                                         int64_t n)
+{
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256)
+        dst[i] = double(src[i]);
+}
+
+// dst[off] = float(src[off]) for the offsets a strided tensor addresses (elements between them untouched)
+__global__ __launch_bounds__(256) void f64_to_f32_strided_kernel(const double* src, float* dst, StridedDims d,
+                                                                 int64_t n)
+{
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256)
+    {
+        int64_t rem = i, off = 0;
+        for (int k = 0; k < d.ndim; k++)
+        {
+            off += (rem % d.len[k]) * d.stride[k];
+            rem /= d.len[k];
+        }
+        dst[off] = float(src[off]);
+    }
+}
+
+hipError_t launch_f32_to_f64(const float* src, double* dst, int64_t n, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(f32_to_f64_kernel, dim3(unsigned(blocks)), dim3(256), 0, s, src, dst, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_f64_to_f32_strided(const double* src, float* dst, const StridedDims& d, hipStream_t s)
+{
+    int64_t n = 1;
+    for (int k = 0; k < d.ndim; k++) n *= d.len[k];
+    if (n <= 0) return hipSuccess;
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(f64_to_f32_strided_kernel, dim3(unsigned(blocks)), dim3(256), 0, s, src, dst, d, n);
+    return hipGetLastError();
+}
+
 // ---- general index classes -----------------------------------------------------------------------------
 
 template <typename T>

@@ -66,5 +66,16 @@ int mistra_debug_set_combine_form(int per_sub_block)
     return mistra::g_combine_per_sub.exchange(per_sub_block ? 1 : 0);
 }
 
+int mistra_debug_set_lowp(int bf16_slices, int int8_slices, int int8_fold_ksteps)
+{
+    if (bf16_slices != 0 && (bf16_slices < 2 || bf16_slices > 7)) return MISTRA_ERR_ARG;
+    if (int8_slices != 0 && (int8_slices < 2 || int8_slices > 6)) return MISTRA_ERR_ARG;
+    if (int8_fold_ksteps < 0 || int8_fold_ksteps > 1024) return MISTRA_ERR_ARG;
+    mistra::g_dbg_ozaki_slices.store(bf16_slices);
+    mistra::g_dbg_i8_slices.store(int8_slices);
+    mistra::g_dbg_i8_fold.store(int8_fold_ksteps);
+    return 0;
+}
+
 }  // extern "C"
 }  // namespace tblis

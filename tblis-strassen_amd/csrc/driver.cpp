@@ -31,6 +31,10 @@ using PlanList = std::list<std::pair<std::string, std::shared_ptr<Plan>>>;
 PlanList g_plans;                                                  // LRU, most recent first
 std::unordered_map<std::string, PlanList::iterator> g_plan_index;  // key -> its entry of g_plans
 constexpr size_t kMaxPlans = 1024;  // plans are small: index vectors and tables
+// Plans whose device tables a caller's HIP graph has captured: a graph keeps the table addresses in its
+// kernel nodes and may be replayed at any later time, so these plans stay resident until the process
+// ends (neither LRU eviction nor mistra_release_workspace frees them).
+std::unordered_map<std::string, std::shared_ptr<Plan>> g_pinned;
 
 #define HIP_TRY(expr)                                                                             \
     do                                                                                            \
@@ -454,6 +458,7 @@ void set_comb_mask(const Plan& p, CombineArgs& ca)
 }  // namespace
 
 std::atomic<int> g_combine_per_sub{0};
+std::atomic<int> g_dbg_ozaki_slices{0}, g_dbg_i8_slices{0}, g_dbg_i8_fold{0};
 
 Plan::~Plan()
 {
@@ -894,7 +899,7 @@ Status upload(Plan& p)
 
 Status get_plan(const tblis::tblis_tensor* A, const label_type* idx_A, const tblis::tblis_tensor* B,
                 const label_type* idx_B, const tblis::tblis_tensor* C, const label_type* idx_C, int level,
-                int impl, int tile, unsigned flags, std::shared_ptr<Plan>& out)
+                int impl, int tile, unsigned flags, std::shared_ptr<Plan>& out, bool capturing)
 {
     int dev = 0;
     HIP_TRY(hipGetDevice(&dev));
@@ -914,9 +919,21 @@ Status get_plan(const tblis::tblis_tensor* A, const label_type* idx_A, const tbl
         {
             out = f->second->second;
             g_plans.splice(g_plans.begin(), g_plans, f->second);
+            if (capturing) g_pinned.emplace(key, out);
+            return Status();
+        }
+        auto pin = g_pinned.find(key);
+        if (pin != g_pinned.end())
+        {
+            out = pin->second;
             return Status();
         }
     }
+    // building a plan allocates and uploads its tables synchronously, which a stream capture forbids
+    if (capturing)
+        return Status::error(MISTRA_ERR_UNSUPPORTED,
+                             "no plan for this contraction yet: make one call with the same shapes, strides, level, "
+                             "impl and flags outside the graph capture first");
     auto p = std::make_shared<Plan>();
     Status st = build_plan_host(A, idx_A, B, idx_B, C, idx_C, level, impl, tile, flags, *p);
     if (!st.ok()) return st;
@@ -945,6 +962,7 @@ Status get_plan(const tblis::tblis_tensor* A, const label_type* idx_A, const tbl
 
 void release_all()
 {
+    // pinned plans (captured by a caller's graph) are kept: their graphs may still be replayed
     std::lock_guard<std::mutex> lk(g_plan_mutex);
     g_plan_index.clear();
     g_plans.clear();
@@ -1295,10 +1313,16 @@ Status run_plan(const Plan& p, const Call& c, hipStream_t stream)
         const size_t tbytes = size_t(p.nprod) * size_t(p.ks) * size_t(p.ns) * size_t(p.elem);
         const size_t mbytes = size_t(p.nprod) * size_t(p.ms) * size_t(p.ns) * size_t(p.elem);
         void *S = nullptr, *T = nullptr, *M = nullptr;
-        HIP_TRY(hipMallocAsync(&S, sbytes, stream));
-        HIP_TRY(hipMallocAsync(&T, tbytes, stream));
-        HIP_TRY(hipMallocAsync(&M, mbytes, stream));
-        hipError_t e = hipSuccess;
+        // on a failed allocation the ones already made go back to the pool before the error returns
+        hipError_t e = hipMallocAsync(&S, sbytes, stream);
+        if (e == hipSuccess) e = hipMallocAsync(&T, tbytes, stream);
+        if (e == hipSuccess) e = hipMallocAsync(&M, mbytes, stream);
+        if (e != hipSuccess)
+        {
+            for (void* x : {S, T, M})
+                if (x) (void)hipFreeAsync(x, stream);
+            return Status::error(MISTRA_ERR_NOMEM, std::string("Naive workspace: ") + hipGetErrorString(e));
+        }
         // one-pass sums (every source sub-block read once) when the sub-blocks are translates and K has
         // unit stride: S then comes out K-fast ([q][i][k], like T) and the dense GEMM reads both operands
         // K-fast (straight into LDS)
@@ -1557,7 +1581,8 @@ Status run_plan_lowp(const Plan& p, const Call& c, int precision, int nslices, h
             la.outA = XA.p;
             la.outB = XB.p;
             la.s_kfast = s_kfast;
-            LowpGemmArgs lg{ms, ns, mp, np, kp, p.nprod, nullptr, nullptr, nullptr, c.alpha, c.beta, 0, p.nprod};
+            LowpGemmArgs lg{ms, ns, mp, np, kp, p.nprod, nullptr, nullptr, nullptr, c.alpha, c.beta, 0, p.nprod,
+                            g_dbg_i8_fold.load()};
             if (direct)
             {
                 lg.C = reinterpret_cast<double*>(static_cast<char*>(c.C) + p.h_batch_c[size_t(z)] * p.elem);

@@ -153,10 +153,12 @@ struct Call
     double alpha, beta;
 };
 
-// Build (or fetch from the cache) the plan of a contraction.
+// Build (or fetch from the cache) the plan of a contraction.  capturing: the call is being recorded
+// into a HIP graph; then the plan must already exist (an error otherwise) and is pinned for the life of
+// the process, since the graph holds its device table addresses.
 Status get_plan(const tblis::tblis_tensor* A, const label_type* idx_A, const tblis::tblis_tensor* B,
                 const label_type* idx_B, const tblis::tblis_tensor* C, const label_type* idx_C, int level,
-                int impl, int tile, unsigned flags, std::shared_ptr<Plan>& out);
+                int impl, int tile, unsigned flags, std::shared_ptr<Plan>& out, bool capturing = false);
 
 // Build a plan without touching the device (index analysis and tables only; for tests and tuning).
 Status build_plan_host(const tblis::tblis_tensor* A, const label_type* idx_A, const tblis::tblis_tensor* B,
@@ -214,6 +216,9 @@ void release_batch_graphs();
 
 // 1: the combine pass always takes its per-sub-block form (mistra_debug_set_combine_form)
 extern std::atomic<int> g_combine_per_sub;
+// test overrides of the Ozaki modes (mistra_debug_set_lowp; 0 = default): BF16 and INT8 digit counts,
+// and the INT8 GEMM's fold period in K-steps
+extern std::atomic<int> g_dbg_ozaki_slices, g_dbg_i8_slices, g_dbg_i8_fold;
 double scalar_value(const tblis::tblis_tensor* t);
 
 // Kernel timing hook of the dominant leaf kernel.

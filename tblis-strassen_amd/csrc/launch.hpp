@@ -141,6 +141,10 @@ struct StridedDims
     int64_t stride[8];
 };
 hipError_t launch_scale(int elem_size, void* C, const StridedDims& d, double beta, hipStream_t stream);
+// FP32 tensors in the low-precision modes run through FP64 copies: dst[i] = src[i] over n elements, and
+// back, dst[off] = float(src[off]) over the offsets of a strided tensor.
+hipError_t launch_f32_to_f64(const float* src, double* dst, int64_t n, hipStream_t stream);
+hipError_t launch_f64_to_f32_strided(const double* src, float* dst, const StridedDims& d, hipStream_t stream);
 
 // General index classes (the classical fallback of src/stra_internal/3t/stra_mult.cxx:1311-1382):
 // dst (dense, first index fastest over `out`) = sum over the `red` dims of src.

@@ -36,6 +36,9 @@ struct LowpGemmArgs
     double alpha, beta;
     // this launch: products [q0, q0 + nlaunch)
     int q0, nlaunch;
+    // INT8 GEMM: fold the exact INT32 sums into FP64 every fold_steps K-steps of 64 (0: the default,
+    // 1024, and only when K needs it; at most 1024)
+    int fold_steps;
 };
 
 // Scale exponents, then digits: launch_lowp_exponents fills ea for an S with rows fastest; the slice

@@ -701,7 +701,7 @@ __global__ __launch_bounds__(512) void ozaki_i8_gemm_kernel(const int8_t* __rest
                         a32[s + t][i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[s][i], fb[j], a32[s + t][i][j], 0, 0, 0);
         }
         if constexpr (FOLD)
-            if ((st + 1) % kI8Fold == 0) fold();
+            if ((st + 1) % g.fold_steps == 0) fold();
         if (st + 1 < nsteps) lstore(cur ^ 1);
         __syncthreads();
     }
@@ -908,9 +908,14 @@ hipError_t gemm_i8_t(const void* As, const void* Bs, const int* ea, const int* e
     if (g.mp % 128 || g.np % 64 || g.kp % 64) return hipErrorInvalidValue;
     if (g.kp >= (int64_t(1) << 24)) return hipErrorInvalidValue;  // tile offsets below 128 kp < 2^31
     const dim3 grid(unsigned((g.mp / 128) * (g.np / 64)), unsigned(g.nlaunch));
-    if (g.kp > int64_t(64) * kI8Fold)
+    if (g.fold_steps < 0 || g.fold_steps > kI8Fold) return hipErrorInvalidValue;
+    if (g.fold_steps > 0 || g.kp > int64_t(64) * kI8Fold)
+    {
+        LowpGemmArgs gf = g;
+        if (gf.fold_steps == 0) gf.fold_steps = kI8Fold;
         hipLaunchKernelGGL((ozaki_i8_gemm_kernel<NSL, true>), grid, dim3(512), 0, s, static_cast<const int8_t*>(As),
-                           static_cast<const int8_t*>(Bs), ea, eb, M, g);
+                           static_cast<const int8_t*>(Bs), ea, eb, M, gf);
+    }
     else
         hipLaunchKernelGGL((ozaki_i8_gemm_kernel<NSL, false>), grid, dim3(512), 0, s, static_cast<const int8_t*>(As),
                            static_cast<const int8_t*>(Bs), ea, eb, M, g);

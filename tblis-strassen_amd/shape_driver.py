@@ -72,7 +72,9 @@ def group_sizes(idx_A, idx_B, idx_C, lengths):
     return m, n, k, prod(sorted(lengths))
 
 
-def run_line(ts, torch, idx_C, idx_A, idx_B, lengths, niter, level, impl, dtype, seed, trash):
+def run_line(ts, torch, idx_C, idx_A, idx_B, lengths, niter, level, impl, dtype, seed, trash, keep=False):
+    """Time one shape line (classical, then the chosen level and impl); keep=True also returns the flat
+    device buffers of A, B, C (Strassen) and C_ref (classical) under "buffers"."""
     def colmajor(shape):
         st, s = [], 1
         for L in shape:
@@ -91,8 +93,8 @@ def run_line(ts, torch, idx_C, idx_A, idx_B, lengths, niter, level, impl, dtype,
             buf.add_(1.0).mul_(0.5)  # uniform [0,1), the reference's drand48 range
         return buf.as_strided(shape, colmajor(shape)), buf
 
-    A, _ = dev(idx_A, seed * 2 + 1)
-    B, _ = dev(idx_B, seed * 2 + 2)
+    A, abuf = dev(idx_A, seed * 2 + 1)
+    B, bbuf = dev(idx_B, seed * 2 + 2)
     C, cbuf = dev(idx_C)
     Cref, crbuf = dev(idx_C)
 
@@ -115,7 +117,10 @@ def run_line(ts, torch, idx_C, idx_A, idx_B, lengths, niter, level, impl, dtype,
     err2 = diff / (ref if ref != 0.0 else 1.0)
     m, n, k, ntot = group_sizes(idx_A, idx_B, idx_C, lengths)
     gflops = 2.0 * ntot * 1e-9
-    return dict(m=m, n=n, k=k, dt=dt, dt2=dt2, gflops=gflops, rate=gflops / dt, rate2=gflops / dt2, err2=err2)
+    out = dict(m=m, n=n, k=k, dt=dt, dt2=dt2, gflops=gflops, rate=gflops / dt, rate2=gflops / dt2, err2=err2)
+    if keep:
+        out["buffers"] = dict(A=abuf, B=bbuf, C=cbuf, Cref=crbuf)
+    return out
 
 
 def format_row(r):

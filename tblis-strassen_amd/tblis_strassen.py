@@ -327,6 +327,22 @@ def copy_bytes(dst, src, nbytes, stream=None):
     _check(load().mistra_copy_bytes(dst.data_ptr(), src.data_ptr(), nbytes, _stream_ptr(stream)))
 
 
+def debug_set_lowp(bf16_slices=0, int8_slices=0, int8_fold_ksteps=0):
+    """Test hook (include/tblis_strassen_debug.h): Ozaki digit counts and the INT8 fold period for the
+    later calls of this process; 0 restores a default."""
+    lib = load()
+    lib.mistra_debug_set_lowp.argtypes = [ctypes.c_int] * 3
+    rc = lib.mistra_debug_set_lowp(bf16_slices, int8_slices, int8_fold_ksteps)
+    if rc != 0:
+        raise MistraError(f"mistra_debug_set_lowp: error {rc}")
+
+
+def release_workspace():
+    """mistra_release_workspace: drop the plan cache and the batch graphs (plans captured by a caller's
+    graph stay resident)."""
+    _check(load().mistra_release_workspace())
+
+
 def rel_error(X, Y, stream=None):
     """||X-Y||_F / ||Y||_F over two contiguous device tensors of equal size (FP64 reduction on device)."""
     out = ctypes.c_double()

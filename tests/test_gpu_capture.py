@@ -65,3 +65,60 @@ def test_graph_replays_of_a_staged_call_match_eager(level, impl, flags):
                             cases.host_tensor(case["shape_C"], np.zeros(n[2])), 1.0, 0.0)
     got = cases.host_tensor(case["shape_C"], eager.cpu().numpy())
     assert cases.rel_err(got, want) <= 1e-13
+
+
+def test_captured_plan_survives_release_workspace():
+    """The graph holds the plan's device tables: releasing the plan cache (and evicting the plan) must not
+    free them while the graph can still be replayed."""
+    case = cases.make_case(913, m=(16, 18), n=(12, 22), k=(20, 13))
+    n = [int(np.prod(case[f"shape_{x}"])) for x in "ABC"]
+    a, b = cases.uniform(n[0], 61), cases.uniform(n[1], 62)
+    A, _ = dev(case["shape_A"], a)
+    B, _ = dev(case["shape_B"], b)
+    C, cbuf = dev(case["shape_C"], np.zeros(n[2]))
+    stream = torch.cuda.Stream()
+    say("capture, release_workspace, replay: L1 abc staged")
+    with torch.cuda.stream(stream):
+        ts.tensor_mult(1.0, A, case["idx_A"], B, case["idx_B"], 0.0, C, case["idx_C"], level=1, impl="abc",
+                       flags=ts.FLAG_OUTER_STAGED, stream=stream)
+    torch.cuda.synchronize()
+    eager = cbuf.clone()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=stream):
+        ts.tensor_mult(1.0, A, case["idx_A"], B, case["idx_B"], 0.0, C, case["idx_C"], level=1, impl="abc",
+                       flags=ts.FLAG_OUTER_STAGED, stream=stream)
+    ts.release_workspace()
+    # other plans built after the release reuse freed device memory if the captured tables were freed
+    for j in range(4):
+        other = cases.make_case(950 + j, m=(9 + j, 7), n=(11, 5 + j), k=(6, 13))
+        nn = [int(np.prod(other[f"shape_{x}"])) for x in "ABC"]
+        X, _ = dev(other["shape_A"], cases.uniform(nn[0], 70 + j))
+        Y, _ = dev(other["shape_B"], cases.uniform(nn[1], 80 + j))
+        Z, _ = dev(other["shape_C"], np.zeros(nn[2]))
+        ts.tensor_mult(1.0, X, other["idx_A"], Y, other["idx_B"], 0.0, Z, other["idx_C"], level=1, impl="abc",
+                       flags=ts.FLAG_SYNC)
+    for rep in range(3):
+        say("replay after release", rep)
+        cbuf.zero_()
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(cbuf, eager), rep
+
+
+def test_capture_without_a_plan_is_refused():
+    """A shape never called outside a capture has no plan; building one would upload tables synchronously,
+    which the capture forbids: the call returns an error and enqueues nothing."""
+    case = cases.make_case(977, m=(13, 6), n=(7, 9), k=(11, 5))
+    n = [int(np.prod(case[f"shape_{x}"])) for x in "ABC"]
+    A, _ = dev(case["shape_A"], cases.uniform(n[0], 1))
+    B, _ = dev(case["shape_B"], cases.uniform(n[1], 2))
+    C, cbuf = dev(case["shape_C"], np.ones(n[2]))
+    stream = torch.cuda.Stream()
+    g = torch.cuda.CUDAGraph()
+    with pytest.raises(ts.MistraError, match="outside the graph capture"):
+        with torch.cuda.graph(g, stream=stream):
+            ts.tensor_mult(1.0, A, case["idx_A"], B, case["idx_B"], 0.0, C, case["idx_C"], level=2, impl="naive",
+                           stream=stream)
+    torch.cuda.synchronize()
+    assert torch.equal(cbuf, torch.ones_like(cbuf))

@@ -110,3 +110,29 @@ def test_matrix_api_errors():
     C = torch.zeros(4, 6, dtype=torch.float64, device="cuda")
     with pytest.raises(ts.MistraError):
         ts.matrix_mult(1.0, A, B, 0.0, C)
+
+
+@pytest.mark.parametrize("alpha,empty", [(0.0, False), (1.0, True)])
+def test_scale_only_touches_the_diagonal_of_c(alpha, empty):
+    """alpha == 0 (or an empty operand) leaves C = beta*C over the elements C[idx_C] addresses: with a
+    repeated label in idx_C that is a diagonal (the reference applies diagonal() to C before it scales,
+    src/stra_iface/3t/stra_mult.cxx:47-50, :212-231); the elements off the diagonal keep their values."""
+    L, K = 7, (0 if empty else 5)
+    rng = np.random.default_rng(5)
+    c = rng.uniform(-1, 1, size=L * L * 3)
+    a = rng.uniform(-1, 1, size=max(1, L * K))[: L * K]
+    b = rng.uniform(-1, 1, size=max(1, K * 3))[: K * 3]
+    A, _ = dev_t((L, K), a)
+    B, _ = dev_t((K, 3), b)
+    C, cbuf = dev_t((L, L, 3), c)
+    ts.tensor_mult(alpha, A, "ik", B, "kj", 0.5, C, "iij", level=1, flags=ts.FLAG_SYNC)
+    got = cbuf.cpu().numpy().reshape((L, L, 3), order="F")
+    want = c.reshape((L, L, 3), order="F").copy()
+    for i in range(L):
+        want[i, i, :] *= 0.5
+    assert np.array_equal(got, want)
+
+
+def dev_t(shape, flat):
+    buf = torch.from_numpy(np.ascontiguousarray(flat, dtype=np.float64)).to("cuda")
+    return buf.as_strided(tuple(shape), tuple(cases.colmajor_strides(shape))), buf

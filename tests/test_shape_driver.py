@@ -1,6 +1,7 @@
 """Shape-file driver: parsing (CPU) against the reference's own shape files, kept as data fixtures
 (tests/golden/tcb.txt and shape.txt are copies of the reference's bin/tcb.txt and bin/shape.txt), and a
-GPU run of scaled-down shape lines through the C-ABI."""
+GPU run of scaled-down shape lines through the C-ABI, whose classical and Strassen results are both
+checked against FP64 sums over the oracle's scatter vectors (tests/sampled.py)."""
 import os
 
 import pytest
@@ -48,14 +49,24 @@ def test_driver_runs_scaled_reference_shapes(tmp_path, capsys):
         for idx_C, idx_A, idx_B, lengths in lines:
             ls = "".join(f"{c}:{max(1, v // 8)};" for c, v in sorted(lengths.items()))
             f.write(f"{idx_C} {idx_A} {idx_B} & {ls}\n")
+    import sampled
     import tblis_strassen as ts
 
     ts.load()
     for level, impl in ((1, "abc"), (2, "ab"), (1, "naive"), (2, "abc")):
         for idx_C, idx_A, idx_B, lengths in sd.parse_shape_lines(open(scaled)):
-            r = sd.run_line(ts, torch, idx_C, idx_A, idx_B, {c: max(1, v) for c, v in lengths.items()}, 1, level,
-                            impl, torch.float64, 0, None)
+            r = sd.run_line(ts, torch, idx_C, idx_A, idx_B, lengths, 1, level, impl, torch.float64, 0, None,
+                            keep=True)
             assert r["err2"] <= 1e-24, (idx_C, idx_A, idx_B, level, impl, r)
+            # both results against FP64 sums over the oracle's scatter vectors (one call: C = A.B)
+            case = dict(idx_A=idx_A, idx_B=idx_B, idx_C=idx_C, lengths=lengths,
+                        **{f"shape_{x}": [lengths[c] for c in ix] for x, ix in zip("ABC", (idx_A, idx_B, idx_C))})
+            buf = r["buffers"]
+            off, ref = sampled.sample(case, buf["A"].cpu().numpy(), buf["B"].cpu().numpy(), nsamp=1024, seed=level)
+            idx = torch.from_numpy(off).to("cuda")
+            for name in ("C", "Cref"):
+                err = sampled.rel_err(buf[name][idx].cpu().numpy(), ref)
+                assert err <= 1e-13, (idx_C, idx_A, idx_B, level, impl, name, err)
     sd.main(["-file", str(scaled), "-niter", "1", "-level", "1", "-impl", "1", "-notrash"])
     rows = [r for r in capsys.readouterr().out.splitlines() if r.strip()]
     assert len(rows) == len(lines) and all(len(r.split()) == 9 for r in rows)
