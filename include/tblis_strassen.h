@@ -162,6 +162,22 @@ int mistra_tensor_mult(const tblis_tensor* A, const label_type* idx_A,
                              tblis_tensor* C, const label_type* idx_C,
                        int level, int impl, int precision, unsigned flags, void* stream);
 
+/* mistra_tensor_mult that allocates nothing: no counterpart in the reference; it is for callers that
+ * cannot allocate device memory (code between allocations of a tight serving loop, harnesses that forbid
+ * hipMalloc).  The plan's tables and the staged workspaces of its launches live in a fixed 64 MiB device
+ * block per GPU, held from the first call until the process ends; no hipMalloc, hipFree or pool
+ * allocation happens in the call.  Same arguments and results as mistra_tensor_mult, restricted to:
+ * MISTRA_PREC_NATIVE, level 0..2, impl 0..4 (no autotuning), device tensors, pure contractions.  ABC's
+ * staged sums fall back to the fused form when they do not fit; AB and Naive return MISTRA_ERR_NOMEM
+ * when their staged products (AB: nprod * ms * ns elements; Naive: nprod * (ms*ks + ks*ns + ms*ns))
+ * exceed what the block leaves beside the tables.  Calls that overlap on one GPU (other streams or
+ * threads) race on the block: serialize them.  Not capturable into a HIP graph.  Returns 0 or a negative
+ * code (message in mistra_last_error). */
+int mistra_tensor_mult_noalloc(const tblis_tensor* A, const label_type* idx_A,
+                               const tblis_tensor* B, const label_type* idx_B,
+                                     tblis_tensor* C, const label_type* idx_C,
+                               int level, int impl, int precision, unsigned flags, void* stream);
+
 /* Thread-local description of the last error of this thread ("" if none). */
 const char* mistra_last_error(void);
 

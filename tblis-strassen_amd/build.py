@@ -22,6 +22,7 @@ SOURCES = [
     "debug.cpp",
     "autotune.cpp",
     "batch.cpp",
+    "noalloc.cpp",
     "leaf_gemm.cpp",
     "leaf_f64_multi.hip",
     "leaf_f64_store.hip",

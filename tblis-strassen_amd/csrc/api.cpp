@@ -11,6 +11,8 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <list>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -641,6 +643,152 @@ Status tensor_mult(const tblis::tblis_tensor* A, const label_type* idx_A, const 
     return Status();
 }
 
+namespace {
+
+// Plans of the allocation-free entry point: built on the host, their tables bound to the device's
+// scratch block (the image is copied there when a call needs a plan other than the one resident).
+struct ScratchPlan
+{
+    std::shared_ptr<Plan> plan;  // device tables point into the scratch block (dmem stays null)
+    PlanImage img;
+    size_t table_bytes = 0;      // the image, rounded up to 256 bytes: workspace starts there
+};
+std::mutex g_noalloc_mutex;
+std::list<std::pair<std::string, std::shared_ptr<ScratchPlan>>> g_noalloc_plans;  // LRU, most recent first
+constexpr size_t kMaxScratchPlans = 64;
+std::map<int, std::shared_ptr<ScratchPlan>> g_resident;  // device -> the plan whose tables are in its scratch
+
+}  // namespace
+
+// mistra_tensor_mult without device allocation: the plan's tables and the staged workspaces of its
+// launches live in the fixed scratch block (noalloc.cpp), nothing is allocated or freed on the device.
+// Native precision, explicit level and impl, device tensors, pure contractions (no A-only / B-only /
+// C-only labels), not under graph capture.  Staged ABC falls back to the fused form when its sums do
+// not fit; AB and Naive need their staged products to fit and report an error otherwise.
+Status tensor_mult_noalloc(const tblis::tblis_tensor* A, const label_type* idx_A, const tblis::tblis_tensor* B,
+                           const label_type* idx_B, tblis::tblis_tensor* C, const label_type* idx_C, int level,
+                           int impl, int precision, unsigned flags, hipStream_t stream)
+{
+    Status st = validate(A, idx_A, B, idx_B, C, idx_C);
+    if (!st.ok()) return st;
+    if (precision != MISTRA_PREC_NATIVE)
+        return Status::error(MISTRA_ERR_UNSUPPORTED, "the allocation-free entry point runs native precision only");
+    if (level < 0 || level > 2 || impl < 0 || impl > 4)
+        return Status::error(MISTRA_ERR_ARG, "the allocation-free entry point needs level 0..2 and impl 0..4");
+    if (A->type != C->type || B->type != C->type)
+        return Status::error(MISTRA_ERR_ARG, "A, B and C must have the same type");
+    const int elem = C->type == tblis::TYPE_DOUBLE ? 8 : (C->type == tblis::TYPE_FLOAT ? 4 : 0);
+    if (!elem) return Status::error(MISTRA_ERR_UNSUPPORTED, "only TYPE_FLOAT and TYPE_DOUBLE are supported");
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(stream, &cap) != hipSuccess)
+    {
+        (void)hipGetLastError();
+        cap = hipStreamCaptureStatusNone;
+    }
+    if (cap != hipStreamCaptureStatusNone)
+        return Status::error(MISTRA_ERR_UNSUPPORTED,
+                             "the allocation-free entry point cannot be captured (its scratch contents change "
+                             "from call to call)");
+    if (!is_device_pointer(A->data) || !is_device_pointer(B->data) || !is_device_pointer(C->data))
+        return Status::error(MISTRA_ERR_UNSUPPORTED, "the allocation-free entry point takes device tensors only");
+    const double alpha = scalar_as_double(A) * scalar_as_double(B);
+    const double beta = scalar_as_double(C);
+    if (empty_tensor(C))
+    {
+        set_scalar_one(C);
+        return Status();
+    }
+    if (alpha == 0.0 || empty_tensor(A) || empty_tensor(B))
+    {
+        st = scale_c(C, idx_C, C->data, beta, stream);
+        if (st.ok() && (flags & MISTRA_FLAG_SYNC))
+        {
+            const hipError_t e = hipStreamSynchronize(stream);
+            if (e != hipSuccess) st = Status::error(MISTRA_ERR_HIP, hipGetErrorString(e));
+        }
+        if (st.ok()) set_scalar_one(C);
+        return st;
+    }
+
+    void* base = nullptr;
+    size_t cap_bytes = 0;
+    st = noalloc_scratch(&base, &cap_bytes);
+    if (!st.ok()) return st;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return Status::error(MISTRA_ERR_HIP, hipGetErrorString(e));
+    const std::string key = plan_key(A, idx_A, B, idx_B, C, idx_C, level, impl, -1, flags, dev);
+
+    std::lock_guard<std::mutex> lk(g_noalloc_mutex);
+    std::shared_ptr<ScratchPlan> sp;
+    for (auto it = g_noalloc_plans.begin(); it != g_noalloc_plans.end(); ++it)
+        if (it->first == key)
+        {
+            sp = it->second;
+            g_noalloc_plans.splice(g_noalloc_plans.begin(), g_noalloc_plans, it);
+            break;
+        }
+    if (!sp)
+    {
+        auto n = std::make_shared<ScratchPlan>();
+        n->plan = std::make_shared<Plan>();
+        st = build_plan_host(A, idx_A, B, idx_B, C, idx_C, level, impl, -1, flags, *n->plan);
+        if (!st.ok()) return st;
+        if (n->plan->general)
+            return Status::error(MISTRA_ERR_UNSUPPORTED,
+                                 "the allocation-free entry point takes pure contractions only (no labels that "
+                                 "appear in A, B or C alone)");
+        build_image(*n->plan, n->img);
+        n->table_bytes = (n->img.bytes.size() + 255) & ~size_t(255);
+        if (n->table_bytes > cap_bytes)
+            return Status::error(MISTRA_ERR_NOMEM, "the plan's tables need " + std::to_string(n->img.bytes.size()) +
+                                                       " bytes, more than the allocation-free scratch block of " +
+                                                       std::to_string(cap_bytes) + " bytes");
+        bind_image(*n->plan, n->img, base);
+        g_noalloc_plans.emplace_front(key, n);
+        while (g_noalloc_plans.size() > kMaxScratchPlans) g_noalloc_plans.pop_back();
+        sp = n;
+    }
+    const Plan& p = *sp->plan;
+    // staged products of AB / Naive must fit beside the tables (ABC's staged sums fall back to the fused form)
+    const size_t room = cap_bytes - sp->table_bytes;
+    if (p.core && (p.impl == IMPL_AB || p.impl == IMPL_NAIVE))
+    {
+        const size_t q = size_t(p.nprod) * size_t(p.elem), up = 255;
+        const size_t need = p.impl == IMPL_AB
+                                ? ((size_t(p.nbatch) * q * size_t(p.ms) * size_t(p.ns) + up) & ~up)
+                                : ((q * size_t(p.ms) * size_t(p.ks) + up) & ~up) +
+                                      ((q * size_t(p.ks) * size_t(p.ns) + up) & ~up) +
+                                      ((q * size_t(p.ms) * size_t(p.ns) + up) & ~up);
+        if (need > room)
+            return Status::error(MISTRA_ERR_NOMEM,
+                                 std::string(p.impl == IMPL_AB ? "AB" : "Naive") + " needs " + std::to_string(need) +
+                                     " bytes of staged products, more than the " + std::to_string(room) +
+                                     " bytes the allocation-free scratch block has left beside the plan's tables");
+    }
+    // the tables of this plan into the scratch block, unless they are there already
+    auto res = g_resident.find(dev);
+    if (res == g_resident.end() || res->second != sp)
+    {
+        e = hipMemcpyAsync(base, sp->img.bytes.data(), sp->img.bytes.size(), hipMemcpyHostToDevice, stream);
+        if (e != hipSuccess) return Status::error(MISTRA_ERR_HIP, std::string("scratch upload: ") + hipGetErrorString(e));
+        g_resident[dev] = sp;  // keeps the image alive while it is the resident one
+    }
+    Arena ws;
+    ws.base = static_cast<char*>(base) + sp->table_bytes;
+    ws.size = room;
+    Call call{A->data, B->data, C->data, alpha, beta, &ws};
+    if (p.transposed) std::swap(call.A, call.B);
+    st = run_plan(p, call, stream);
+    if (st.ok() && (flags & MISTRA_FLAG_SYNC))
+    {
+        e = hipStreamSynchronize(stream);
+        if (e != hipSuccess) st = Status::error(MISTRA_ERR_HIP, hipGetErrorString(e));
+    }
+    if (st.ok()) set_scalar_one(C);
+    return st;
+}
+
 }  // namespace mistra
 
 // ---- C-ABI -----------------------------------------------------------------------------------------------
@@ -763,6 +911,16 @@ int mistra_tensor_mult(const tblis_tensor* A, const label_type* idx_A, const tbl
     mistra::g_last_error.clear();
     Status s = mistra::tensor_mult(A, idx_A, B, idx_B, C, idx_C, level, impl, precision, flags,
                                    static_cast<hipStream_t>(stream), false);
+    return s.ok() ? 0 : fail(s);
+}
+
+int mistra_tensor_mult_noalloc(const tblis_tensor* A, const label_type* idx_A, const tblis_tensor* B,
+                               const label_type* idx_B, tblis_tensor* C, const label_type* idx_C, int level, int impl,
+                               int precision, unsigned flags, void* stream)
+{
+    mistra::g_last_error.clear();
+    Status s = mistra::tensor_mult_noalloc(A, idx_A, B, idx_B, C, idx_C, level, impl, precision, flags,
+                                           static_cast<hipStream_t>(stream));
     return s.ok() ? 0 : fail(s);
 }
 

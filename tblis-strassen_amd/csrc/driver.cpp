@@ -721,9 +721,9 @@ void make_stages(Plan& p, std::vector<int>& order)
     }
 }
 
-namespace {
-
-Status upload(Plan& p)
+// Builds the device image of a plan (the byte layout of every table) on the host; bind_image then points
+// the plan's device-table members into a copy of it at `base`.
+void build_image(Plan& p, PlanImage& img)
 {
     const int TK = tile_kdepth(p.tile, p.elem);
     std::vector<int64_t> a_krun = run_strides(p.h_a_k, TK), b_krun = run_strides(p.h_b_k, TK);
@@ -835,20 +835,29 @@ Status upload(Plan& p)
          i_oskr = add(os_skrun.data(), os_skrun.size() * 8), i_otk = add(os_tk.data(), os_tk.size() * 8),
          i_otkr = add(os_tkrun.data(), os_tkrun.size() * 8), i_otn = add(os_tn.data(), os_tn.size() * 8);
 
-    std::vector<unsigned char> staging(total, 0);
+    img.bytes.assign(total, 0);
     for (const Item& it : items)
-        if (it.bytes && it.src) std::memcpy(staging.data() + it.off, it.src, it.bytes);
-    void* d = nullptr;
-    hipError_t e = hipMalloc(&d, total);
-    if (e != hipSuccess) return Status::error(MISTRA_ERR_NOMEM, std::string("plan allocation: ") + hipGetErrorString(e));
-    e = hipMemcpy(d, staging.data(), total, hipMemcpyHostToDevice);
-    if (e != hipSuccess)
+        if (it.bytes && it.src) std::memcpy(img.bytes.data() + it.off, it.src, it.bytes);
+    const size_t idx[] = {i_a_m, i_a_k, i_a_kr, i_b_k, i_b_kr, i_b_n, i_c_m, i_c_n, i_ba, i_bb, i_bc, i_pr, i_prf,
+                          i_pst, i_pfst, i_sr, i_sc, i_tb, i_te, i_sm, i_sk, i_skm, i_skr, i_tk, i_tkr, i_tn, i_dp,
+                          i_zero, i_os, i_mca, i_mcb, i_msa, i_mc1a, i_mc1b, i_msb, i_osf, i_ins, i_osm, i_osk,
+                          i_oskr, i_otk, i_otkr, i_otn};
+    static_assert(sizeof(idx) / sizeof(idx[0]) == PlanImage::kParts, "one offset per table");
+    for (int i = 0; i < PlanImage::kParts; i++) img.off[i] = items[idx[i]].off;
+}
+
+// Points the plan's device tables at `d`, which holds a copy of the image build_image made.
+void bind_image(Plan& p, const PlanImage& img, void* d)
+{
+    // the parts in the order build_image records their offsets
+    enum
     {
-        (void)hipFree(d);
-        return Status::error(MISTRA_ERR_HIP, std::string("plan upload: ") + hipGetErrorString(e));
-    }
-    p.dmem = d;
-    auto at = [&](size_t i) { return reinterpret_cast<unsigned char*>(d) + items[i].off; };
+        i_a_m, i_a_k, i_a_kr, i_b_k, i_b_kr, i_b_n, i_c_m, i_c_n, i_ba, i_bb, i_bc, i_pr, i_prf, i_pst, i_pfst,
+        i_sr, i_sc, i_tb, i_te, i_sm, i_sk, i_skm, i_skr, i_tk, i_tkr, i_tn, i_dp, i_zero, i_os, i_mca, i_mcb,
+        i_msa, i_mc1a, i_mc1b, i_msb, i_osf, i_ins, i_osm, i_osk, i_oskr, i_otk, i_otkr, i_otn, n_parts
+    };
+    static_assert(n_parts == PlanImage::kParts, "one offset per table");
+    auto at = [&](size_t i) { return reinterpret_cast<unsigned char*>(d) + img.off[i]; };
     p.a_m = reinterpret_cast<const int64_t*>(at(i_a_m));
     p.a_k = reinterpret_cast<const int64_t*>(at(i_a_k));
     p.a_krun = reinterpret_cast<const int64_t*>(at(i_a_kr));
@@ -892,17 +901,34 @@ Status upload(Plan& p)
     p.os_tk = reinterpret_cast<const int64_t*>(at(i_otk));
     p.os_tkrun = reinterpret_cast<const int64_t*>(at(i_otkr));
     p.os_tn = reinterpret_cast<const int64_t*>(at(i_otn));
+}
+
+namespace {
+
+Status upload(Plan& p)
+{
+    PlanImage img;
+    build_image(p, img);
+    void* d = nullptr;
+    hipError_t e = hipMalloc(&d, img.bytes.size());
+    if (e != hipSuccess) return Status::error(MISTRA_ERR_NOMEM, std::string("plan allocation: ") + hipGetErrorString(e));
+    e = hipMemcpy(d, img.bytes.data(), img.bytes.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess)
+    {
+        (void)hipFree(d);
+        return Status::error(MISTRA_ERR_HIP, std::string("plan upload: ") + hipGetErrorString(e));
+    }
+    p.dmem = d;
+    bind_image(p, img, d);
     return Status();
 }
 
 }  // namespace
 
-Status get_plan(const tblis::tblis_tensor* A, const label_type* idx_A, const tblis::tblis_tensor* B,
-                const label_type* idx_B, const tblis::tblis_tensor* C, const label_type* idx_C, int level,
-                int impl, int tile, unsigned flags, std::shared_ptr<Plan>& out, bool capturing)
+std::string plan_key(const tblis::tblis_tensor* A, const label_type* idx_A, const tblis::tblis_tensor* B,
+                     const label_type* idx_B, const tblis::tblis_tensor* C, const label_type* idx_C, int level,
+                     int impl, int tile, unsigned flags, int dev)
 {
-    int dev = 0;
-    HIP_TRY(hipGetDevice(&dev));
     std::ostringstream ks;
     ks << dev << '|' << int(A->type) << '|' << level << '|' << impl << '|' << tile << '|'
        << (flags & (MISTRA_FLAG_REF_FRINGE | MISTRA_FLAG_FUSED_SUMS | MISTRA_FLAG_OUTER_STAGED |
@@ -912,6 +938,16 @@ Status get_plan(const tblis::tblis_tensor* A, const label_type* idx_A, const tbl
     append_key(key, A, idx_A);
     append_key(key, B, idx_B);
     append_key(key, C, idx_C);
+    return key;
+}
+
+Status get_plan(const tblis::tblis_tensor* A, const label_type* idx_A, const tblis::tblis_tensor* B,
+                const label_type* idx_B, const tblis::tblis_tensor* C, const label_type* idx_C, int level,
+                int impl, int tile, unsigned flags, std::shared_ptr<Plan>& out, bool capturing)
+{
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    const std::string key = plan_key(A, idx_A, B, idx_B, C, idx_C, level, impl, tile, flags, dev);
     {
         std::lock_guard<std::mutex> lk(g_plan_mutex);
         auto f = g_plan_index.find(key);
@@ -1108,6 +1144,23 @@ hipError_t fork_end(hipStream_t main, const SideFork& f)
     return e;
 }
 
+// Workspace of a call: from its arena when it has one (hipErrorOutOfMemory when the block is too small),
+// else from the stream-ordered pool.
+hipError_t ws_alloc(const Call& c, void** out, size_t bytes, hipStream_t s)
+{
+    if (!c.ws) return hipMallocAsync(out, bytes, s);
+    const size_t off = align256(c.ws->used);
+    if (off > c.ws->size || bytes > c.ws->size - off) return hipErrorOutOfMemory;
+    *out = c.ws->base + off;
+    c.ws->used = off + bytes;
+    return hipSuccess;
+}
+
+hipError_t ws_free(const Call& c, void* p, hipStream_t s)
+{
+    return c.ws ? hipSuccess : hipFreeAsync(p, s);
+}
+
 // Outer-staged ABC (Plan::outer_staged): the outer sums into S and T, then one launch per stage over the
 // inner sums of S and T (2 terms at level 2, 1 at level 1) with the multi-C epilogue on C.
 hipError_t run_outer_staged(const Plan& p, const Call& c, hipStream_t stream)
@@ -1117,12 +1170,12 @@ hipError_t run_outer_staged(const Plan& p, const Call& c, hipStream_t stream)
     const size_t tbytes = nouter * size_t(p.os_k) * size_t(p.os_n) * size_t(p.elem);
     void *S = nullptr, *T = nullptr;
     // an allocation failure returns before anything is launched (the caller then takes the fused form)
-    hipError_t e = hipMallocAsync(&S, sbytes, stream);
+    hipError_t e = ws_alloc(c, &S, sbytes, stream);
     if (e != hipSuccess) return e;
-    e = hipMallocAsync(&T, tbytes, stream);
+    e = ws_alloc(c, &T, tbytes, stream);
     if (e != hipSuccess)
     {
-        (void)hipFreeAsync(S, stream);
+        (void)ws_free(c, S, stream);
         return e;
     }
     // sums of one operand: one pass over its source sub-blocks when they are translates and K (the rows
@@ -1173,7 +1226,7 @@ hipError_t run_outer_staged(const Plan& p, const Call& c, hipStream_t stream)
     };
     // B's sums on the side stream, A's on the caller's, joined before the stage launches
     SideFork fk;
-    const bool forked = fork_begin(stream, fk);
+    const bool forked = !c.ws && fork_begin(stream, fk);
     e = sums(0, stream);
     const hipError_t eb = sums(1, forked ? fk.side : stream);
     if (e == hipSuccess) e = eb;
@@ -1214,7 +1267,7 @@ hipError_t run_outer_staged(const Plan& p, const Call& c, hipStream_t stream)
                              false, stream);
         timing_end(stream, leaf_flops(p, n, p.ms, p.ns, p.ks));
     }
-    const hipError_t f1 = hipFreeAsync(S, stream), f2 = hipFreeAsync(T, stream);
+    const hipError_t f1 = ws_free(c, S, stream), f2 = ws_free(c, T, stream);
     if (e != hipSuccess) return e;
     return f1 != hipSuccess ? f1 : f2;
 }
@@ -1267,7 +1320,11 @@ Status run_plan(const Plan& p, const Call& c, hipStream_t stream)
     {
         const size_t mbytes = size_t(p.nbatch) * size_t(p.nprod) * size_t(p.ms) * size_t(p.ns) * size_t(p.elem);
         void* M = nullptr;
-        HIP_TRY(hipMallocAsync(&M, mbytes, stream));
+        {
+            const hipError_t ea = ws_alloc(c, &M, mbytes, stream);
+            if (ea != hipSuccess)
+                return Status::error(MISTRA_ERR_NOMEM, std::string("AB workspace: ") + hipGetErrorString(ea));
+        }
         g.ms = p.ms;
         g.ns = p.ns;
         g.ks = p.ks;
@@ -1303,7 +1360,7 @@ Status run_plan(const Plan& p, const Call& c, hipStream_t stream)
             ca.batch_c = g.batch_c;
             e = launch_combine(p.elem, ca, stream);
         }
-        hipError_t f = hipFreeAsync(M, stream);
+        hipError_t f = ws_free(c, M, stream);
         HIP_TRY(e);
         HIP_TRY(f);
     }
@@ -1314,13 +1371,13 @@ Status run_plan(const Plan& p, const Call& c, hipStream_t stream)
         const size_t mbytes = size_t(p.nprod) * size_t(p.ms) * size_t(p.ns) * size_t(p.elem);
         void *S = nullptr, *T = nullptr, *M = nullptr;
         // on a failed allocation the ones already made go back to the pool before the error returns
-        hipError_t e = hipMallocAsync(&S, sbytes, stream);
-        if (e == hipSuccess) e = hipMallocAsync(&T, tbytes, stream);
-        if (e == hipSuccess) e = hipMallocAsync(&M, mbytes, stream);
+        hipError_t e = ws_alloc(c, &S, sbytes, stream);
+        if (e == hipSuccess) e = ws_alloc(c, &T, tbytes, stream);
+        if (e == hipSuccess) e = ws_alloc(c, &M, mbytes, stream);
         if (e != hipSuccess)
         {
             for (void* x : {S, T, M})
-                if (x) (void)hipFreeAsync(x, stream);
+                if (x) (void)ws_free(c, x, stream);
             return Status::error(MISTRA_ERR_NOMEM, std::string("Naive workspace: ") + hipGetErrorString(e));
         }
         // one-pass sums (every source sub-block read once) when the sub-blocks are translates and K has
@@ -1363,7 +1420,7 @@ Status run_plan(const Plan& p, const Call& c, hipStream_t stream)
             sa.zoff = p.h_batch_a[size_t(z)];
             // B's sums on the side stream, A's on the caller's, joined before the dense GEMM
             SideFork fk;
-            const bool forked = fork_begin(stream, fk);
+            const bool forked = !c.ws && fork_begin(stream, fk);
             const hipStream_t sb = forked ? fk.side : stream;
             e = s_k ? multisums(true, stream) : launch_operand_sums(p.elem, sa, stream);
             sa.X = c.B;
@@ -1431,7 +1488,7 @@ Status run_plan(const Plan& p, const Call& c, hipStream_t stream)
             ca.nbatch = 1;
             e = launch_combine(p.elem, ca, stream);
         }
-        hipError_t f1 = hipFreeAsync(S, stream), f2 = hipFreeAsync(T, stream), f3 = hipFreeAsync(M, stream);
+        hipError_t f1 = ws_free(c, S, stream), f2 = ws_free(c, T, stream), f3 = ws_free(c, M, stream);
         HIP_TRY(e);
         HIP_TRY(f1);
         HIP_TRY(f2);

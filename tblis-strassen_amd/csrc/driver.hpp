@@ -145,13 +145,33 @@ struct Plan
     ~Plan();
 };
 
+// Device workspace handed out from one fixed block (bump allocation, nothing freed) instead of the
+// stream-ordered pool: callers that must not allocate pass one.
+struct Arena
+{
+    char* base = nullptr;
+    size_t size = 0;
+    size_t used = 0;
+};
+
 struct Call
 {
     const void* A;
     const void* B;
     void* C;
     double alpha, beta;
+    Arena* ws = nullptr;  // null: workspace from hipMallocAsync; else from this arena (and no side stream)
 };
+
+// Device image of a plan: every table in one byte block (256-byte aligned parts) and the offset of each.
+struct PlanImage
+{
+    static constexpr int kParts = 43;
+    std::vector<unsigned char> bytes;
+    size_t off[kParts];
+};
+void build_image(Plan& p, PlanImage& img);
+void bind_image(Plan& p, const PlanImage& img, void* base);
 
 // Build (or fetch from the cache) the plan of a contraction.  capturing: the call is being recorded
 // into a HIP graph; then the plan must already exist (an error otherwise) and is pinned for the life of
@@ -159,6 +179,16 @@ struct Call
 Status get_plan(const tblis::tblis_tensor* A, const label_type* idx_A, const tblis::tblis_tensor* B,
                 const label_type* idx_B, const tblis::tblis_tensor* C, const label_type* idx_C, int level,
                 int impl, int tile, unsigned flags, std::shared_ptr<Plan>& out, bool capturing = false);
+
+// Cache key of a plan: device, type, level, impl, tile, the flags that change tables, and every tensor's
+// labels, lengths and strides.
+std::string plan_key(const tblis::tblis_tensor* A, const label_type* idx_A, const tblis::tblis_tensor* B,
+                     const label_type* idx_B, const tblis::tblis_tensor* C, const label_type* idx_C, int level,
+                     int impl, int tile, unsigned flags, int dev);
+
+// The allocation-free entry point's device scratch block on the current device (noalloc.cpp): its
+// address (from hipGetSymbolAddress, looked up once per device) and size.
+Status noalloc_scratch(void** ptr, size_t* bytes);
 
 // Build a plan without touching the device (index analysis and tables only; for tests and tuning).
 Status build_plan_host(const tblis::tblis_tensor* A, const label_type* idx_A, const tblis::tblis_tensor* B,

@@ -174,6 +174,21 @@ def tensor_mult(alpha, A, idx_A, B, idx_B, beta, C, idx_C, level=1, impl="abc", 
                                   _stream_ptr(stream)))
 
 
+def tensor_mult_noalloc(alpha, A, idx_A, B, idx_B, beta, C, idx_C, level=1, impl="abc", precision="native",
+                        stream=None, flags=0):
+    """mistra_tensor_mult_noalloc: the same contraction with no device allocation (tables and workspace in
+    the library's fixed 64 MiB scratch block per GPU; device tensors, explicit level and impl)."""
+    lib = load()
+    fn = lib.mistra_tensor_mult_noalloc
+    fn.argtypes = lib.mistra_tensor_mult.argtypes
+    fn.restype = ctypes.c_int
+    tA, tB, tC = make_tensor(A, alpha), make_tensor(B, 1.0), make_tensor(C, beta)
+    impl_c = IMPL[impl] if isinstance(impl, str) else int(impl)
+    prec_c = PREC[precision] if isinstance(precision, str) else int(precision)
+    _check(fn(ctypes.byref(tA), idx_A.encode(), ctypes.byref(tB), idx_B.encode(), ctypes.byref(tC), idx_C.encode(),
+              level, impl_c, prec_c, flags, _stream_ptr(stream)))
+
+
 def tensor_mult_backward(alpha, A, idx_A, B, idx_B, dC, idx_C, dA=None, dB=None, beta=0.0, level=1, impl="abc",
                          precision="native", stream=None):
     """Adjoints of C = alpha*A.B: dA = alpha*dC.B + beta*dA and dB = alpha*A.dC + beta*dB over the same
