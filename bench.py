@@ -17,7 +17,13 @@ meets its tolerance (1e-12 at L1, 5e-12 at L2); every variant, ABC included, is 
 roofline: FP64 MFMA bound of the leaf kernel of the reported variant: achieved = leaf flops per launch /
 average launch duration (HIP events recorded by the library on the launch stream); traffic comes from the
 committed counter summary profiles/traffic_<workload>.json when present.
-cpu_baseline: the CPU oracle (a port of the reference algorithm) on a bounded sample of the workload.
+precision (default run): the split-precision target on the same contraction -- FP64 classical against
+Ozaki BF16 / INT8 digit slicing at L1 and L2 and the 3-product BF16 split, each with its error against
+the FP64 classical result and its leaf kernel's fraction of that dtype's MFMA peak.
+configs (default run): the c1, c3, c4 and c5 lines below, each without the fields this line carries.
+cpu_baseline: the oracle's blocked CPU port of the reference's ABC loop nest on the measured
+configuration itself (one 8192^3 L1 call, minimum of 3 with a cache flush between), on the host's
+physical cores up to the 16 CPUs this process may use, with lscpu's socket and core counts.
 
 Other workloads (one JSON line each, same fields; --workload NAME):
   c1  the same measurement at m=n=k=1024 (2 indices of 32 per group);
@@ -100,35 +106,63 @@ def run_variant(ts, torch, case, A, B, C, level, impl, steps, warmup, stream, fl
     return ms, kms / max(1, launches), launches, kflops / max(1, launches)
 
 
-def cpu_baseline(threads, min_seconds=10.0, max_calls=40):
-    """The oracle's 1-level ABC Strassen on a bounded sample: the same index structure at 4096^3, called
-    repeatedly until about `min_seconds` of CPU work (at most `max_calls` calls)."""
+def host_cpu():
+    """lscpu's model, sockets, cores per socket and threads per core, and the CPUs this process may use."""
+    info = {}
+    try:
+        import subprocess
+
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            key, _, val = line.partition(":")
+            info[key.strip()] = val.strip()
+    except Exception:  # noqa: BLE001 - lscpu is informational only
+        pass
+    sockets = int(info.get("Socket(s)", "0") or 0)
+    cores = int(info.get("Core(s) per socket", "0") or 0)
+    return {"model": info.get("Model name", "unknown"), "sockets": sockets, "cores_per_socket": cores,
+            "threads_per_core": int(info.get("Thread(s) per core", "0") or 0),
+            "physical_cores": sockets * cores, "usable_cpus": len(os.sched_getaffinity(0))}
+
+
+def cpu_baseline(case, threads, repeats=3, level=1):
+    """The oracle's blocked 1-level ABC Strassen (a CPU port of the reference's loop nest: packed
+    operand-sum panels, register micro-kernel, multi-C update; oracle_tensor_mult_blocked) on the measured
+    configuration itself: one call = the whole contraction, timed as the reference's harness does
+    (test/test_strassen.cxx:74-95: the minimum over `repeats` runs, caches flushed between runs by writing
+    a 256 MiB buffer)."""
     import numpy as np
 
     import cases
     import oracle
 
     oracle.load()
-    lens = (16, 16, 16)
-    case = cases.make_case(777, m=lens, n=lens, k=lens)
     m, n, k = cases.mnk(case)
     a, b = oracle.fill_uniform(m * k, 1), oracle.fill_uniform(k * n, 2)
     c = np.zeros(m * n)
     A, B, C = (cases.host_tensor(case[f"shape_{x}"], v) for x, v in zip("ABC", (a, b, c)))
-    calls = 0
-    t0 = time.perf_counter()
-    while calls < max_calls and (calls == 0 or time.perf_counter() - t0 < min_seconds):
-        oracle.tensor_mult(1.0, A, case["idx_A"], B, case["idx_B"], 1.0, C, case["idx_C"], level=1, impl="abc",
-                           threads=threads)
-        calls += 1
-    dt = time.perf_counter() - t0
+    trash = np.empty(32 << 20)
+    best = float("inf")
+    for _ in range(repeats):
+        trash.fill(1.0)
+        t0 = time.perf_counter()
+        oracle.tensor_mult_blocked(1.0, A, case["idx_A"], B, case["idx_B"], 1.0, C, case["idx_C"], level=level,
+                                   threads=threads)
+        best = min(best, time.perf_counter() - t0)
+        log(f"cpu baseline: one L{level} ABC call {time.perf_counter() - t0:.2f} s")
+    hw = host_cpu()
     return {
-        "value": round(2.0 * m * n * k * calls / dt / 1e12, 5),
+        "value": round(2.0 * m * n * k / best / 1e12, 5),
         "unit": "TFLOP/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"oracle 1-level ABC Strassen, FP64, m=n=k=4096 (3 indices of 16 per group, seeded "
-                  f"permutations), {calls} calls in {dt:.2f} s with {threads} OpenMP threads",
+        "sample": f"the measured workload itself: one {level}-level ABC Strassen call of the oracle's blocked CPU "
+                  f"port of the reference loop nest, FP64, m=n=k={m} with the same index layout, minimum of {repeats} "
+                  f"calls with a 256 MiB cache flush between them, {threads} OpenMP threads; host "
+                  f"{hw['model']}, {hw['sockets']} socket(s) x {hw['cores_per_socket']} cores "
+                  f"({hw['physical_cores']} physical), {hw['usable_cpus']} CPUs usable by this process",
+        "seconds_per_call": round(best, 3),
+        "host": hw,
     }
 
 
@@ -339,6 +373,106 @@ def bench_c5(ts, torch, args, stream):
                    "frac": round(head["tflops"] / PEAK["f32"], 4), "traffic": None})
 
 
+def bench_tc(ts, torch, args, stream, workload, variant_spec):
+    """FP64 Strassen TC at levels 0-2 x ABC / AB / Naive on a c1 / c2 workload; the JSON line of the run."""
+    case = make_workload(workload)
+    import cases
+
+    m, n, k = cases.mnk(case)
+    eff_flops = 2.0 * m * n * k
+    log(f"workload {workload}: C[{case['idx_C']}] += A[{case['idx_A']}] B[{case['idx_B']}]  m=n=k={m}")
+
+    A, abuf = device_tensor(case["shape_A"], torch.float64, torch)
+    B, bbuf = device_tensor(case["shape_B"], torch.float64, torch)
+    ts.fill_uniform(abuf, 1, stream=stream)
+    ts.fill_uniform(bbuf, 2, stream=stream)
+    Cref, crefbuf = device_tensor(case["shape_C"], torch.float64, torch)
+    crefbuf.zero_()
+    ts.tensor_mult(1.0, A, case["idx_A"], B, case["idx_B"], 1.0, Cref, case["idx_C"], level=0, stream=stream)
+    C, cbuf = device_tensor(case["shape_C"], torch.float64, torch)
+
+    variants = {}
+    forms = {"": 0, "fused": ts.FLAG_FUSED_SUMS, "staged": ts.FLAG_OUTER_STAGED, "staged_all": ts.FLAG_ALL_STAGED}
+    for spec in variant_spec.split(","):
+        lv, impl, *form = spec.split(":")
+        form = form[0] if form else ""
+        flags = forms[form]
+        level = int(lv)
+        name = f"L{level}_{impl}" + (f"_{form}" if form else "")
+        cbuf.zero_()
+        ts.tensor_mult(1.0, A, case["idx_A"], B, case["idx_B"], 1.0, C, case["idx_C"], level=level, impl=impl,
+                       stream=stream, flags=flags)
+        err = ts.rel_error(cbuf, crefbuf, stream=stream)
+        cbuf.zero_()
+        ms, kms, launches, kflops = run_variant(ts, torch, case, A, B, C, level, impl, args.steps, args.warmup,
+                                                stream, flags)
+        variants[name] = {
+            "level": level, "impl": impl, "form": form or "default", "ms_per_step": round(ms, 4),
+            "eff_tflops": round(eff_flops / ms / 1e9, 3), "rel_err_vs_L0": err,
+            "tol": TOL[level], "ok": err <= TOL[level] if level else True,
+            "leaf_kernel_ms": round(kms, 4), "leaf_kernel_tflops": round(kflops / kms / 1e9, 3) if kms else None,
+            "leaf_flops_per_launch": kflops, "leaf_launches_per_step": launches / args.steps,
+        }
+        log(f"{name:10s} {ms:8.3f} ms  {eff_flops / ms / 1e9:7.2f} eff TFLOP/s  err {err:.2e}  "
+            f"leaf {kms:.3f} ms/launch {kflops / kms / 1e9 if kms else 0:.2f} TFLOP/s")
+
+    cands = [v for v in variants.values() if v["level"] > 0 and v["ok"]]
+    head = max(cands, key=lambda v: v["eff_tflops"]) if cands else variants.get("L0_abc")
+    classical = variants.get("L0_abc")
+    head_name = next(k for k, v in variants.items() if v is head)
+
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"traffic_{workload}_{head_name}.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    achieved = head["leaf_kernel_tflops"]
+    lens = "{" + ",".join(str(x) for x in WORKLOADS[workload][0]) + "}"
+    return {
+        "metric": "effective TFLOP/s (2mnk/t) Strassen L1/L2 vs classical TC, same GPU, at rel-err",
+        "value": head["eff_tflops"],
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (splitmix64 uniform[-1,1), seeded index permutations)",
+        "config": {
+            "workload": f"{workload}: FP64 TC m=n=k={m}, groups of {len(WORKLOADS[workload][0])} indices {lens} in "
+                        f"seeded order, seeded index permutations; reported variant {head_name}",
+            "m": m, "n": n, "k": k, "idx": f"{case['idx_C']} += {case['idx_A']} * {case['idx_B']}",
+            "reported_variant": head_name,
+            "rel_err_vs_L0": head["rel_err_vs_L0"],
+            "classical_eff_tflops": classical["eff_tflops"] if classical else None,
+            "best_abc_eff_tflops": max((v["eff_tflops"] for v in variants.values()
+                                        if v["level"] > 0 and v["impl"] == "abc" and v["ok"]), default=None),
+            "speedup_vs_classical": round(head["eff_tflops"] / classical["eff_tflops"], 4) if classical else None,
+        },
+        "roofline": {
+            "bound": "mfma",
+            "achieved": achieved,
+            "peak": PEAK_FP64_MFMA,
+            "unit": "TFLOP/s",
+            "frac": round(achieved / PEAK_FP64_MFMA, 4) if achieved else None,
+            "traffic": traffic,
+        },
+        "variants": variants,
+        "cpu_baseline": None,
+        "_case": case,
+    }
+
+
+def summary(line):
+    """A configuration's JSON line without the fields the headline line already carries."""
+    drop = {"n_gpus", "steps", "warmup", "higher_is_better", "scaling", "vs_baseline", "data", "cpu_baseline",
+            "_case"}
+    return {k: v for k, v in line.items() if k not in drop}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -351,6 +485,11 @@ def main():
                          "outer level's sums staged in HBM) or staged_all (every level's sums staged); default: "
                          "the library's choice")
     ap.add_argument("--modes", default="", help="c2p / c3: precision:level list (default: all modes)")
+    ap.add_argument("--precision-modes", default="fp64:0,ozaki_bf16:1,ozaki_bf16:2,ozaki_int8:1,ozaki_int8:2,"
+                                                 "bf16x3:1",
+                    help="c2: the precision modes of the headline line's `precision` block")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="c2: leave out the `precision` block and the c1 / c3 / c4 / c5 summaries")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run one device copy of MIB MiB (a kernel of known traffic for counter scaling)")
@@ -382,97 +521,39 @@ def main():
             out = {"c3": bench_c3, "c4": bench_c4, "c5": bench_c5}[args.workload](ts, torch, args, stream)
         print(json.dumps(out))
         return
-    case = make_workload(args.workload)
-    import cases
-
-    m, n, k = cases.mnk(case)
-    eff_flops = 2.0 * m * n * k
-    log(f"workload {args.workload}: C[{case['idx_C']}] += A[{case['idx_A']}] B[{case['idx_B']}]  m=n=k={m}")
-
-    A, abuf = device_tensor(case["shape_A"], torch.float64, torch)
-    B, bbuf = device_tensor(case["shape_B"], torch.float64, torch)
-    ts.fill_uniform(abuf, 1, stream=stream)
-    ts.fill_uniform(bbuf, 2, stream=stream)
-    Cref, crefbuf = device_tensor(case["shape_C"], torch.float64, torch)
-    crefbuf.zero_()
-    ts.tensor_mult(1.0, A, case["idx_A"], B, case["idx_B"], 1.0, Cref, case["idx_C"], level=0, stream=stream)
-    C, cbuf = device_tensor(case["shape_C"], torch.float64, torch)
-
-    variants = {}
-    forms = {"": 0, "fused": ts.FLAG_FUSED_SUMS, "staged": ts.FLAG_OUTER_STAGED, "staged_all": ts.FLAG_ALL_STAGED}
-    for spec in args.variants.split(","):
-        lv, impl, *form = spec.split(":")
-        form = form[0] if form else ""
-        flags = forms[form]
-        level = int(lv)
-        name = f"L{level}_{impl}" + (f"_{form}" if form else "")
-        cbuf.zero_()
-        ts.tensor_mult(1.0, A, case["idx_A"], B, case["idx_B"], 1.0, C, case["idx_C"], level=level, impl=impl,
-                       stream=stream, flags=flags)
-        err = ts.rel_error(cbuf, crefbuf, stream=stream)
-        cbuf.zero_()
-        ms, kms, launches, kflops = run_variant(ts, torch, case, A, B, C, level, impl, args.steps, args.warmup,
-                                                stream, flags)
-        variants[name] = {
-            "level": level, "impl": impl, "form": form or "default", "ms_per_step": round(ms, 4),
-            "eff_tflops": round(eff_flops / ms / 1e9, 3), "rel_err_vs_L0": err,
-            "tol": TOL[level], "ok": err <= TOL[level] if level else True,
-            "leaf_kernel_ms": round(kms, 4), "leaf_kernel_tflops": round(kflops / kms / 1e9, 3) if kms else None,
-            "leaf_flops_per_launch": kflops, "leaf_launches_per_step": launches / args.steps,
+    out = bench_tc(ts, torch, args, stream, args.workload, args.variants)
+    case = out.pop("_case")
+    torch.cuda.empty_cache()
+    if args.workload == "c2" and not args.no_extras:
+        # the split-precision target on the same contraction: FP64 classical against the Ozaki modes
+        pargs = argparse.Namespace(**vars(args))
+        pargs.modes = args.precision_modes
+        prec = bench_c3(ts, torch, pargs, stream, "c2p")
+        modes = prec["modes"]
+        fp64 = modes["fp64_L0"]["eff_tflops"]
+        split = {k: v for k, v in modes.items() if v["precision"] == "ozaki_bf16"}
+        met = {k: bool(v["eff_tflops"] > fp64 and v["rel_err_vs_fp64"] <= 1e-10) for k, v in split.items()}
+        out["precision"] = {
+            "workload": "the same contraction (m=n=k=8192), A, B uniform[-1,1); errors vs the FP64 classical "
+                        "result; leaf_frac = leaf MFMA rate / that dtype's dense peak",
+            "fp64_classical_eff_tflops": fp64,
+            "split_bf16_above_fp64_classical_at_1e-10": met,
+            "modes": modes,
         }
-        log(f"{name:10s} {ms:8.3f} ms  {eff_flops / ms / 1e9:7.2f} eff TFLOP/s  err {err:.2e}  "
-            f"leaf {kms:.3f} ms/launch {kflops / kms / 1e9 if kms else 0:.2f} TFLOP/s")
-
-    cands = [v for v in variants.values() if v["level"] > 0 and v["ok"]]
-    head = max(cands, key=lambda v: v["eff_tflops"]) if cands else variants.get("L0_abc")
-    classical = variants.get("L0_abc")
-    head_name = next(k for k, v in variants.items() if v is head)
-
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"traffic_{args.workload}_{head_name}.json")
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
-    achieved = head["leaf_kernel_tflops"]
-    out = {
-        "metric": "effective TFLOP/s (2mnk/t) Strassen L1/L2 vs classical TC, same GPU, at rel-err",
-        "value": head["eff_tflops"],
-        "unit": "TFLOP/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": head["ms_per_step"],
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic (splitmix64 uniform[-1,1), seeded index permutations)",
-        "config": {
-            "workload": f"{args.workload}: FP64 TC m=n=k={m}, groups of 3 indices {{16,16,32}} in seeded order, "
-                        f"seeded index permutations; reported variant {head_name}",
-            "m": m, "n": n, "k": k, "idx": f"{case['idx_C']} += {case['idx_A']} * {case['idx_B']}",
-            "reported_variant": head_name,
-            "rel_err_vs_L0": head["rel_err_vs_L0"],
-            "classical_eff_tflops": classical["eff_tflops"] if classical else None,
-            "best_abc_eff_tflops": max((v["eff_tflops"] for v in variants.values()
-                                        if v["level"] > 0 and v["impl"] == "abc" and v["ok"]), default=None),
-            "speedup_vs_classical": round(head["eff_tflops"] / classical["eff_tflops"], 4) if classical else None,
-        },
-        "roofline": {
-            "bound": "mfma",
-            "achieved": achieved,
-            "peak": PEAK_FP64_MFMA,
-            "unit": "TFLOP/s",
-            "frac": round(achieved / PEAK_FP64_MFMA, 4) if achieved else None,
-            "traffic": traffic,
-        },
-        "variants": variants,
-    }
+        torch.cuda.empty_cache()
+        out["configs"] = {}
+        c1 = bench_tc(ts, torch, args, stream, "c1", "0:abc,1:abc,1:ab,1:naive,2:abc,2:naive")
+        c1.pop("_case")
+        out["configs"]["c1"] = summary(c1)
+        torch.cuda.empty_cache()
+        for name, fn in (("c3", bench_c3), ("c4", bench_c4), ("c5", bench_c5)):
+            cargs = argparse.Namespace(**vars(args))
+            cargs.modes = ""
+            out["configs"][name] = summary(fn(ts, torch, cargs, stream))
+            torch.cuda.empty_cache()
     if not args.no_cpu_baseline:
-        threads = min(16, os.cpu_count() or 1)
-        out["cpu_baseline"] = cpu_baseline(threads)
-    else:
-        out["cpu_baseline"] = None
+        threads = max(1, min(16, host_cpu()["physical_cores"] or 16, len(os.sched_getaffinity(0))))
+        out["cpu_baseline"] = cpu_baseline(case, threads)
     print(json.dumps(out))
 
 

@@ -18,6 +18,8 @@
  *   strassen tables       src/stra_internal/3t/strassen:1-14 (U,V,W), products in the order of
  *                         src/stra_internal/3t/stra_mult.cxx:426-591; 2-level = Kronecker square in
  *                         the order of src/stra_internal/3t/stra_mult_2level.cxx:341-1458
+ *   oracle_tensor_mult_blocked  the ABC impl through the reference's blocked loop nest (see below); the
+ *                         timed CPU baseline
  *   oracle_tensor_mult    internal::stra_mult dispatch (src/stra_internal/3t/stra_mult.cxx:1701-1846)
  *                         with the three impls:
  *                           ABC    sums fused into packing, per-KC-panel C_i += coeff*AB
@@ -625,6 +627,187 @@ int oracle_tensor_mult(int level, int impl, int ref_fringe, int nthreads, int nd
     free(S);
     free(T);
     free(M);
+    flat_free(&f);
+    return 0;
+}
+
+/* ---------------------------------------------------------------------------------------------- */
+/* Blocked ABC: the reference's CPU loop nest, for timing the CPU baseline at full size            */
+/* ---------------------------------------------------------------------------------------------- */
+/*
+ * oracle_tensor_mult_blocked restates the reference's ABC path at the level of its loop nest rather than
+ * its definition: per product, the 5-loop partition NC / KC / MC / NR / MR (stra_partition,
+ * src/stra_nodes/stra_partm.hpp:12-133; blocksizes of src/configs/sandybridge/config.hpp:30-35 with the
+ * register tile widened to 8 x 6), packing that sums the product's 2^L sub-blocks of A and of B into the
+ * packed panels (stra_pack_row_panel, src/stra_nodes/stra_packm.hpp:367-683), a register micro-kernel
+ * over the packed panels (mul then add, as the AVX kernels of src/configs/sandybridge/bli_stra_*asm_d8x4.c)
+ * and the multi-C update C_t += coeff_t * alpha * AB of every C target, beta at the first KC panel of the
+ * first product that writes C_t (stra_gemm_micro_kernel, src/stra_nodes/stra_gemm_ukr.hpp:626-930).
+ * The remainder of the 2^L split is computed by classical passes.  Only the timing of bench.py's CPU
+ * baseline uses it; tests pin it to oracle_tensor_mult.
+ */
+#include <immintrin.h>
+
+#define BL_MR 8
+#define BL_NR 6
+#define BL_MC 96
+#define BL_KC 256
+#define BL_NC 4096
+
+/* acc (8 x 6, column-major) = sum over kc of a[k][0..7] * b[k][0..5]; a, b packed panels */
+__attribute__((target("avx2"))) static void ukr_8x6(long kc, const double* a, const double* b, double* acc)
+{
+    __m256d c0[BL_NR], c1[BL_NR];
+    for (int j = 0; j < BL_NR; j++) c0[j] = c1[j] = _mm256_setzero_pd();
+    for (long p = 0; p < kc; p++)
+    {
+        const __m256d a0 = _mm256_loadu_pd(a + BL_MR * p), a1 = _mm256_loadu_pd(a + BL_MR * p + 4);
+        for (int j = 0; j < BL_NR; j++)
+        {
+            const __m256d bj = _mm256_broadcast_sd(b + BL_NR * p + j);
+            c0[j] = _mm256_add_pd(c0[j], _mm256_mul_pd(a0, bj));
+            c1[j] = _mm256_add_pd(c1[j], _mm256_mul_pd(a1, bj));
+        }
+    }
+    for (int j = 0; j < BL_NR; j++)
+    {
+        _mm256_storeu_pd(acc + BL_MR * j, c0[j]);
+        _mm256_storeu_pd(acc + BL_MR * j + 4, c1[j]);
+    }
+}
+
+/* One product of a Strassen level on batch entry z through the blocked loop nest. */
+static void blocked_product(const flat_t* f, long z, const double* A, const double* B, double* C, const prod_t* P,
+                            long d, long ms, long ns, long ks, double alpha, const int* first, double beta,
+                            double* Bp)
+{
+    const double* Az = A + f->ba[z];
+    const double* Bz = B + f->bb[z];
+    double* Cz = C + f->bc[z];
+    long ar[4], ac[4], br[4], bc[4], cr[4], cc[4];
+    for (int t = 0; t < P->A.n; t++) ar[t] = (P->A.sub[t] / d) * ms, ac[t] = (P->A.sub[t] % d) * ks;
+    for (int t = 0; t < P->B.n; t++) br[t] = (P->B.sub[t] / d) * ks, bc[t] = (P->B.sub[t] % d) * ns;
+    for (int t = 0; t < P->C.n; t++) cr[t] = (P->C.sub[t] / d) * ms, cc[t] = (P->C.sub[t] % d) * ns;
+    for (long jc = 0; jc < ns; jc += BL_NC)
+    {
+        const long nc = ns - jc < BL_NC ? ns - jc : BL_NC;
+        const long nsl = (nc + BL_NR - 1) / BL_NR;
+        for (long pc = 0; pc < ks; pc += BL_KC)
+        {
+            const long kc = ks - pc < BL_KC ? ks - pc : BL_KC;
+#pragma omp parallel
+            {
+                /* the B panel: sum of the product's B sub-blocks, NR-wide slivers of [k][NR] */
+#pragma omp for schedule(static)
+                for (long sl = 0; sl < nsl; sl++)
+                    for (long p = 0; p < kc; p++)
+                        for (int jj = 0; jj < BL_NR; jj++)
+                        {
+                            const long j = jc + sl * BL_NR + jj;
+                            double v = 0.0;
+                            if (j < jc + nc)
+                                for (int t = 0; t < P->B.n; t++)
+                                    v += P->B.coef[t] * Bz[f->bk[br[t] + pc + p] + f->bn[bc[t] + j]];
+                            Bp[(sl * kc + p) * BL_NR + jj] = v;
+                        }
+                double* Ap = (double*)malloc(sizeof(double) * BL_MC * BL_KC);
+                double acc[BL_MR * BL_NR];
+#pragma omp for schedule(dynamic, 1)
+                for (long ic = 0; ic < ms; ic += BL_MC)
+                {
+                    const long mc = ms - ic < BL_MC ? ms - ic : BL_MC;
+                    const long msl = (mc + BL_MR - 1) / BL_MR;
+                    /* the A block: sum of the product's A sub-blocks, MR-tall slivers of [k][MR] */
+                    for (long sl = 0; sl < msl; sl++)
+                        for (long p = 0; p < kc; p++)
+                            for (int ii = 0; ii < BL_MR; ii++)
+                            {
+                                const long i = ic + sl * BL_MR + ii;
+                                double v = 0.0;
+                                if (i < ic + mc)
+                                    for (int t = 0; t < P->A.n; t++)
+                                        v += P->A.coef[t] * Az[f->am[ar[t] + i] + f->ak[ac[t] + pc + p]];
+                                Ap[(sl * kc + p) * BL_MR + ii] = v;
+                            }
+                    for (long jr = 0; jr < nsl; jr++)
+                        for (long ir = 0; ir < msl; ir++)
+                        {
+                            ukr_8x6(kc, Ap + ir * kc * BL_MR, Bp + jr * kc * BL_NR, acc);
+                            const long i0 = ic + ir * BL_MR, j0 = jc + jr * BL_NR;
+                            const long mr = ic + mc - i0 < BL_MR ? ic + mc - i0 : BL_MR;
+                            const long nr = jc + nc - j0 < BL_NR ? jc + nc - j0 : BL_NR;
+                            for (int t = 0; t < P->C.n; t++)
+                            {
+                                const double cf = alpha * P->C.coef[t];
+                                const int apply_beta = first[t] && pc == 0;
+                                for (long j = 0; j < nr; j++)
+                                {
+                                    double* cj = Cz + f->cn[cc[t] + j0 + j];
+                                    for (long i = 0; i < mr; i++)
+                                    {
+                                        double* cp = cj + f->cm[cr[t] + i0 + i];
+                                        const double v = cf * acc[BL_MR * j + i];
+                                        if (apply_beta) *cp = (beta == 0.0 ? 0.0 : beta * *cp) + v;
+                                        else *cp += v;
+                                    }
+                                }
+                            }
+                        }
+                }
+                free(Ap);
+            }
+        }
+    }
+}
+
+int oracle_tensor_mult_blocked(int level, int nthreads, int ndA, const long* lenA, const long* strideA,
+                               const char* idxA, const double* A, int ndB, const long* lenB, const long* strideB,
+                               const char* idxB, const double* B, int ndC, const long* lenC, const long* strideC,
+                               const char* idxC, double* C, double alpha, double beta)
+{
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+    flat_t f;
+    groups_t G;
+    memset(&f, 0, sizeof(f));
+    if (flatten(ndA, lenA, strideA, idxA, ndB, lenB, strideB, idxB, ndC, lenC, strideC, idxC, &f, &G)) return -1;
+    if (G.Ao.n || G.Bo.n || G.Co.n)
+    {
+        flat_free(&f);
+        return -2; /* pure contractions only */
+    }
+    const long d = 1L << level, ms = f.m / d, ns = f.n / d, ks = f.k / d;
+    const int core = ms > 0 && ns > 0 && ks > 0;
+    prod_t P[49];
+    const int np = core ? make_products(level, P) : 0;
+    double* Bp = (double*)malloc(sizeof(double) * BL_KC * (BL_NC + BL_NR));
+    for (long z = 0; z < f.nbat; z++)
+    {
+        unsigned touched = 0;
+        for (int p = 0; p < np; p++)
+        {
+            int first[4];
+            for (int t = 0; t < P[p].C.n; t++)
+            {
+                first[t] = !(touched & (1u << P[p].C.sub[t]));
+                touched |= 1u << P[p].C.sub[t];
+            }
+            blocked_product(&f, z, A, B, C, &P[p], d, ms, ns, ks, alpha, first, beta, Bp);
+        }
+        if (!core)
+            classical_pass(&f, z, A, B, C, 0, f.m, 0, f.n, 0, f.k, alpha, beta, 1);
+        else
+        {
+            const long dm = d * ms, dn = d * ns, dk = d * ks;
+            if (dm < f.m) classical_pass(&f, z, A, B, C, dm, f.m - dm, 0, f.n, 0, f.k, alpha, beta, 1);
+            if (dn < f.n) classical_pass(&f, z, A, B, C, 0, dm, dn, f.n - dn, 0, f.k, alpha, beta, 1);
+            if (dk < f.k) classical_pass(&f, z, A, B, C, 0, dm, 0, dn, dk, f.k - dk, alpha, beta, 0);
+        }
+    }
+    free(Bp);
     flat_free(&f);
     return 0;
 }

@@ -33,6 +33,7 @@ def load():
     tens = [ctypes.c_int, L, L, s, D]
     lib.oracle_contract_ref.argtypes = tens * 3 + [ctypes.c_double, ctypes.c_double]
     lib.oracle_tensor_mult.argtypes = [ctypes.c_int] * 4 + tens * 3 + [ctypes.c_double, ctypes.c_double]
+    lib.oracle_tensor_mult_blocked.argtypes = [ctypes.c_int] * 2 + tens * 3 + [ctypes.c_double, ctypes.c_double]
     lib.oracle_products.argtypes = [ctypes.c_int, D]
     lib.oracle_describe.argtypes = [ctypes.c_int, L, L, s] * 3 + [L, L, ctypes.c_long]
     lib.oracle_fill_uniform.argtypes = [D, ctypes.c_long, ctypes.c_ulonglong, ctypes.c_int]
@@ -65,6 +66,16 @@ def tensor_mult(alpha, A, idx_A, B, idx_B, beta, C, idx_C, level=1, impl="abc", 
                                 *_tens(C, idx_C), alpha, beta)
     if rc:
         raise ValueError("oracle_tensor_mult: bad labels/lengths")
+
+
+def tensor_mult_blocked(alpha, A, idx_A, B, idx_B, beta, C, idx_C, level=1, threads=0):
+    """ABC Strassen through the reference's blocked loop nest (packed operand-sum panels, register
+    micro-kernel, multi-C update), in place on C; pure contractions only.  The timed CPU baseline."""
+    lib = load()
+    rc = lib.oracle_tensor_mult_blocked(level, threads, *_tens(A, idx_A), *_tens(B, idx_B), *_tens(C, idx_C),
+                                        alpha, beta)
+    if rc:
+        raise ValueError("oracle_tensor_mult_blocked: bad labels/lengths or not a pure contraction")
 
 
 def products(level):

@@ -224,3 +224,33 @@ def test_uniform_generator():
         assert np.array_equal(oracle.fill_uniform(1000, seed, emax), cases.uniform(1000, seed, emax))
     v = cases.uniform(100000, 5)
     assert v.min() >= -1.0 and v.max() < 1.0 and abs(v.mean()) < 0.01
+
+
+@pytest.mark.parametrize("groups", [dict(m=(16, 16), n=(8, 16), k=(32, 4)), dict(m=(13, 7), n=(11,), k=(9, 5), batch=(2,)),
+                                    dict(m=(100,), n=(90,), k=(300,)), dict(m=(200,), n=(7, 9), k=(6, 50))])
+@pytest.mark.parametrize("level", [0, 1, 2])
+def test_blocked_port_equals_the_oracle(groups, level):
+    """The blocked ABC port (the timed CPU baseline: packed panels of summed sub-blocks, 8 x 6 register
+    tiles, fringes of the micro-tile, KC panels, multi-C updates) computes the oracle's contraction:
+    bit for bit on small integers, within 1e-14 on uniform inputs, with alpha and beta."""
+    case = cases.make_case(5 + level, **groups)
+    n = [int(np.prod(case[f"shape_{x}"])) for x in "ABC"]
+    for gen, tol in ((cases.small_ints, 0.0), (cases.uniform, 1e-14)):
+        a, b, c = gen(n[0], 1), gen(n[1], 2), gen(n[2], 3)
+        A, B = cases.host_tensor(case["shape_A"], a), cases.host_tensor(case["shape_B"], b)
+        C1, C2 = cases.host_tensor(case["shape_C"], c.copy()), cases.host_tensor(case["shape_C"], c.copy())
+        oracle.tensor_mult(2.0, A, case["idx_A"], B, case["idx_B"], -0.5, C1, case["idx_C"], level=level, impl="abc")
+        oracle.tensor_mult_blocked(2.0, A, case["idx_A"], B, case["idx_B"], -0.5, C2, case["idx_C"], level=level)
+        if tol == 0.0:
+            assert np.array_equal(C1, C2)
+        else:
+            assert cases.rel_err(C2, C1) <= tol
+
+
+def test_blocked_port_rejects_general_classes():
+    case = cases.make_case(9, m=(6,), n=(5,), k=(7,), a_only=(3,))
+    n = [int(np.prod(case[f"shape_{x}"])) for x in "ABC"]
+    A, B = cases.host_tensor(case["shape_A"], np.ones(n[0])), cases.host_tensor(case["shape_B"], np.ones(n[1]))
+    C = cases.host_tensor(case["shape_C"], np.zeros(n[2]))
+    with pytest.raises(ValueError):
+        oracle.tensor_mult_blocked(1.0, A, case["idx_A"], B, case["idx_B"], 0.0, C, case["idx_C"], level=1)
