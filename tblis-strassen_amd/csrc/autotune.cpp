@@ -38,17 +38,20 @@ std::string shape_key(const tblis::tblis_tensor* A, const label_type* idx_A, con
 }
 
 // Leaf-kernel rates (TFLOP/s) measured on MI355X with the 8192^3 FP64 workload, by register tile and
-// number of sum terms (1, 2, 4); FP32 runs the same kernels at about 1.8 times the rate.  Single-term
-// launches over K-fast operands (K-tiles straight into LDS) run at 72.5 - 7645 / ks: 71.6 at a leaf
-// depth of 8192 (the classical kernel), 68.8 at 2048 (the L=2 Naive dense launch) -- the per-tile
-// start and end weigh more on shorter K loops.
+// number of sum terms (1, 2, 4).  Single-term launches over K-fast operands (K-tiles straight into LDS)
+// run at 72.5 - 7645 / ks: 71.6 at a leaf depth of 8192 (the classical kernel), 68.8 at 2048 (the L=2
+// Naive dense launch) -- the per-tile start and end weigh more on shorter K loops.  FP32 runs the
+// single-term kernels at about 1.8 times the rate (FP32 MFMA has twice the FP64 rate); the multi-term
+// loaders' vector work is the same in both types and does not overlap the MFMAs, so FP32 multi-term
+// launches gain less: 2 terms measured 87 TFLOP/s (1.5x) at m = n = k = 4096 (c5 L1), 4 terms taken
+// at 1.3x.
 double leaf_rate(int elem, int tile, int terms, bool dma, int64_t ks)
 {
-    static const double r128[3] = {61.0, 57.5, 46.0}, r64[3] = {57.0, 46.0, 38.0};
+    static const double r128[3] = {61.0, 57.5, 46.0}, r64[3] = {57.0, 46.0, 38.0}, f32[3] = {1.8, 1.5, 1.3};
     const int ti = terms <= 1 ? 0 : (terms == 2 ? 1 : 2);
     double r = tile == TILE_64 ? r64[ti] : r128[ti];
     if (tile != TILE_64 && ti == 0 && dma) r = std::max(r, 72.5 - 7645.0 / double(std::max<int64_t>(ks, 512)));
-    return elem == 8 ? r : 1.8 * r;
+    return elem == 8 ? r : f32[ti] * r;
 }
 
 constexpr double kHbm = 2.7e12;       // bytes/s of the gathering sum / combine kernels (strided gathers)

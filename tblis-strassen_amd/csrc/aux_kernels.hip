@@ -313,7 +313,7 @@ hipError_t launch_operand_sums(int elem_size, const SumArgs& a, hipStream_t s)
 constexpr int kCombineRows = 64;
 constexpr int kCombineCols = 8;
 
-template <typename T>
+template <typename T, typename TM>
 __global__ __launch_bounds__(256) void combine_kernel(CombineArgs a)
 {
     const int s = blockIdx.y;
@@ -330,7 +330,7 @@ __global__ __launch_bounds__(256) void combine_kernel(CombineArgs a)
     for (int64_t z = blockIdx.z; z < a.nbatch; z += gridDim.z)
     {
         T* C = static_cast<T*>(a.C) + (a.batch_c ? a.batch_c[z] : 0) + cm;
-        const T* M = static_cast<const T*>(a.M) + z * int64_t(a.nprod) * nel + c0 * a.ms + r;
+        const TM* M = static_cast<const TM*>(a.M) + z * int64_t(a.nprod) * nel + c0 * a.ms + r;
         double acc[kCombineCols];
 #pragma unroll
         for (int j = 0; j < kCombineCols; j++) acc[j] = 0.0;
@@ -339,8 +339,8 @@ __global__ __launch_bounds__(256) void combine_kernel(CombineArgs a)
             for (int t = tb; t < te; t++)
             {
                 const double w = a.terms[t].coeff;
-                const T* Mp = M + int64_t(a.terms[t].prod) * nel;
-                T v[kCombineCols];
+                const TM* Mp = M + int64_t(a.terms[t].prod) * nel;
+                TM v[kCombineCols];
 #pragma unroll
                 for (int j = 0; j < kCombineCols; j++) v[j] = Mp[int64_t(j) * a.ms];
 #pragma unroll
@@ -352,7 +352,7 @@ __global__ __launch_bounds__(256) void combine_kernel(CombineArgs a)
             for (int t = tb; t < te; t++)
             {
                 const double w = a.terms[t].coeff;
-                const T* Mp = M + int64_t(a.terms[t].prod) * nel;
+                const TM* Mp = M + int64_t(a.terms[t].prod) * nel;
 #pragma unroll
                 for (int j = 0; j < kCombineCols; j++)
                     if (j < ncol) acc[j] += w * double(Mp[int64_t(j) * a.ms]);
@@ -376,7 +376,7 @@ __global__ __launch_bounds__(256) void combine_kernel(CombineArgs a)
 // sub-block sums the same terms in the same order as the per-sub-block form.  The next product's values
 // are loaded before the current ones are used (the last product re-reads itself), with no branch around
 // the loads.
-template <typename T, int NSUB, int NC>
+template <typename T, int NSUB, int NC, typename TM>
 __global__ __launch_bounds__(256) void combine_all_kernel(CombineArgs a)
 {
     const int64_t nel = a.ms * a.ns;
@@ -389,7 +389,7 @@ __global__ __launch_bounds__(256) void combine_all_kernel(CombineArgs a)
     const int last = a.nprod - 1;
     for (int64_t z = blockIdx.z; z < a.nbatch; z += gridDim.z)
     {
-        const T* M = static_cast<const T*>(a.M) + z * int64_t(a.nprod) * nel + c0 * a.ms + r;
+        const TM* M = static_cast<const TM*>(a.M) + z * int64_t(a.nprod) * nel + c0 * a.ms + r;
         double acc[NSUB][NC];
 #pragma unroll
         for (int s = 0; s < NSUB; s++)
@@ -399,12 +399,12 @@ __global__ __launch_bounds__(256) void combine_all_kernel(CombineArgs a)
         int64_t coff_j[NC];
 #pragma unroll
         for (int j = 0; j < NC; j++) coff_j[j] = (j < ncol ? j : 0) * a.ms;
-        T v[NC], vn[NC];
+        TM v[NC], vn[NC];
 #pragma unroll
         for (int j = 0; j < NC; j++) v[j] = M[coff_j[j]];
         for (int p = 0; p < a.nprod; p++)
         {
-            const T* Mn = M + int64_t(p < last ? p + 1 : last) * nel;
+            const TM* Mn = M + int64_t(p < last ? p + 1 : last) * nel;
 #pragma unroll
             for (int j = 0; j < NC; j++) vn[j] = Mn[coff_j[j]];
             const unsigned pm = a.pmask[p];
@@ -443,6 +443,9 @@ hipError_t launch_combine(int elem_size, const CombineArgs& a, hipStream_t s)
 {
     const int64_t nel = a.ms * a.ns;
     if (nel == 0 || a.nsub == 0 || a.nbatch == 0 || a.nprod == 0) return hipSuccess;
+    // M in FP32 beside an FP64 C (the FP8 mode's products), else M has C's type
+    if (a.m_elem != 0 && a.m_elem != elem_size && !(a.m_elem == 4 && elem_size == 8)) return hipErrorInvalidValue;
+    const bool m_f32 = a.m_elem == 4 && elem_size == 8;
     const int64_t row_tiles = (a.ms + kCombineRows - 1) / kCombineRows;
     const int64_t zb = a.nbatch < 65535 ? a.nbatch : 65535;
     auto blocks_for = [&](int cols_per_lane) {
@@ -454,11 +457,15 @@ hipError_t launch_combine(int elem_size, const CombineArgs& a, hipStream_t s)
         const int64_t blocks = blocks_for(nc);
         if (blocks > 0x7fffffff) return hipErrorInvalidValue;
         const dim3 grid(static_cast<unsigned>(blocks), 1u, static_cast<unsigned>(zb));
-#define MISTRA_COMBINE_ALL(NSUB, NC)                                                                         \
-    do                                                                                                       \
-    {                                                                                                        \
-        if (elem_size == 8) hipLaunchKernelGGL((combine_all_kernel<double, NSUB, NC>), grid, dim3(256), 0, s, a); \
-        else hipLaunchKernelGGL((combine_all_kernel<float, NSUB, NC>), grid, dim3(256), 0, s, a);            \
+#define MISTRA_COMBINE_ALL(NSUB, NC)                                                                            \
+    do                                                                                                          \
+    {                                                                                                           \
+        if (m_f32 && elem_size == 8)                                                                            \
+            hipLaunchKernelGGL((combine_all_kernel<double, NSUB, NC, float>), grid, dim3(256), 0, s, a);       \
+        else if (elem_size == 8)                                                                                \
+            hipLaunchKernelGGL((combine_all_kernel<double, NSUB, NC, double>), grid, dim3(256), 0, s, a);      \
+        else                                                                                                    \
+            hipLaunchKernelGGL((combine_all_kernel<float, NSUB, NC, float>), grid, dim3(256), 0, s, a);        \
     } while (0)
         if (a.nsub == 16) MISTRA_COMBINE_ALL(16, 2);
         else if (a.nsub == 4) MISTRA_COMBINE_ALL(4, 8);
@@ -469,8 +476,9 @@ hipError_t launch_combine(int elem_size, const CombineArgs& a, hipStream_t s)
     const int64_t blocks = blocks_for(kCombineCols);
     if (blocks > 0x7fffffff || a.nsub > 65535) return hipErrorInvalidValue;
     const dim3 grid(static_cast<unsigned>(blocks), static_cast<unsigned>(a.nsub), static_cast<unsigned>(zb));
-    if (elem_size == 8) hipLaunchKernelGGL((combine_kernel<double>), grid, dim3(256), 0, s, a);
-    else hipLaunchKernelGGL((combine_kernel<float>), grid, dim3(256), 0, s, a);
+    if (m_f32) hipLaunchKernelGGL((combine_kernel<double, float>), grid, dim3(256), 0, s, a);
+    else if (elem_size == 8) hipLaunchKernelGGL((combine_kernel<double, double>), grid, dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((combine_kernel<float, float>), grid, dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

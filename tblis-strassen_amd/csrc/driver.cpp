@@ -1533,7 +1533,7 @@ Status run_plan_lowp(const Plan& p, const Call& c, int precision, int nslices, h
         const bool fp8 = precision == MISTRA_PREC_FP8_BLOCK, i8 = precision == MISTRA_PREC_OZAKI_INT8;
         const int64_t ms = p.ms, ns = p.ns, ks = p.ks, nq = p.nprod;
         auto up = [](int64_t x, int64_t m) { return (x + m - 1) / m * m; };
-        const int64_t mp = up(ms, 128), np = up(ns, 64), kp = up(ks, fp8 ? 128 : 64);
+        const int64_t mp = up(ms, 128), np = up(ns, fp8 ? 128 : 64), kp = up(ks, fp8 ? 128 : 64);
         const int nsl = fp8 ? 1 : nslices;
         const size_t esz = fp8 || i8 ? 1 : 2;
         struct Buf
@@ -1544,10 +1544,12 @@ Status run_plan_lowp(const Plan& p, const Call& c, int precision, int nslices, h
         // Strassen core: the products are staged in M and combined into C in one pass (measured faster
         // than updating the C targets in the GEMM epilogue stage by stage: 70.8 against 65.2 TFLOP/s
         // for INT8 digits at 8192^3, 55.4 against 42.3 at m = n = 16384, k = 1024)
+        // (FP8 products are staged in FP32: their E4M3 operands carry about 2^-4 relative error)
         const size_t nS = size_t(nq * ms * ks) * 8, nT = size_t(nq * ks * ns) * 8,
-                     nM = direct ? 8 : size_t(nq * ms * ns) * 8;
-        const size_t nEA = fp8 ? size_t(nq * mp * (kp / 128)) * 4 : size_t(nq * ms) * 4;
-        const size_t nEB = fp8 ? size_t(nq * np * (kp / 128)) * 4 : size_t(nq * ns) * 4;
+                     nM = direct ? 8 : size_t(nq * ms * ns) * (fp8 ? 4 : 8);
+        // FP8: one scale byte per row and 128-long K block; digits: one exponent per row
+        const size_t nEA = fp8 ? size_t(nq * mp * (kp / 128)) : size_t(nq * ms) * 4;
+        const size_t nEB = fp8 ? size_t(nq * np * (kp / 128)) : size_t(nq * ns) * 4;
         const size_t nXA = size_t(nsl) * size_t(nq * mp * kp) * esz, nXB = size_t(nsl) * size_t(nq * np * kp) * esz;
         hipError_t e = hipSuccess;
         for (auto pr : {std::make_pair(&S, nS), std::make_pair(&T, nT), std::make_pair(&M, nM), std::make_pair(&EA, nEA),
@@ -1639,7 +1641,7 @@ Status run_plan_lowp(const Plan& p, const Call& c, int precision, int nslices, h
             la.outB = XB.p;
             la.s_kfast = s_kfast;
             LowpGemmArgs lg{ms, ns, mp, np, kp, p.nprod, nullptr, nullptr, nullptr, c.alpha, c.beta, 0, p.nprod,
-                            g_dbg_i8_fold.load()};
+                            g_dbg_i8_fold.load(), fp8 && !direct};
             if (direct)
             {
                 lg.C = reinterpret_cast<double*>(static_cast<char*>(c.C) + p.h_batch_c[size_t(z)] * p.elem);
@@ -1651,7 +1653,7 @@ Status run_plan_lowp(const Plan& p, const Call& c, int precision, int nslices, h
                 timing_begin(stream);
                 hipError_t r;
                 if (fp8)
-                    r = launch_fp8_gemm(XA.p, XB.p, static_cast<const float*>(EA.p), static_cast<const float*>(EB.p),
+                    r = launch_fp8_gemm(XA.p, XB.p, static_cast<const uint32_t*>(EA.p), static_cast<const uint32_t*>(EB.p),
                                         static_cast<double*>(M.p), g, stream);
                 else if (i8)
                     r = launch_ozaki_i8_gemm(nsl, XA.p, XB.p, la.ea, la.eb, static_cast<double*>(M.p), g, stream);
@@ -1661,7 +1663,7 @@ Status run_plan_lowp(const Plan& p, const Call& c, int precision, int nslices, h
                 return r;
             };
             if (fp8)
-                e = launch_fp8_slices(la, static_cast<float*>(EA.p), static_cast<float*>(EB.p), stream);
+                e = launch_fp8_slices(la, static_cast<uint32_t*>(EA.p), static_cast<uint32_t*>(EB.p), stream);
             else
             {
                 e = launch_lowp_exponents(la, stream);
@@ -1690,6 +1692,7 @@ Status run_plan_lowp(const Plan& p, const Call& c, int precision, int nslices, h
             ca.alpha = c.alpha;
             ca.beta = c.beta;
             ca.nbatch = 1;
+            ca.m_elem = fp8 ? 4 : 0;
             e = launch_combine(8, ca, stream);
         }
         for (Buf* b : {&S, &T, &M, &EA, &EB, &XA, &XB})

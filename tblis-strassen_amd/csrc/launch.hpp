@@ -123,6 +123,8 @@ struct CombineArgs
     // Every M_p is then read once and added to all its sub-blocks.
     int dense;
     unsigned pmask[kCombineMaxProds];
+    // element size of M: 0 = C's; 4 with an FP64 C = products staged in FP32 (the FP8 mode)
+    int m_elem;
 };
 hipError_t launch_combine(int elem_size, const CombineArgs& a, hipStream_t stream);
 

@@ -39,6 +39,8 @@ struct LowpGemmArgs
     // INT8 GEMM: fold the exact INT32 sums into FP64 every fold_steps K-steps of 64 (0: the default,
     // 1024, and only when K needs it; at most 1024)
     int fold_steps;
+    // FP8 GEMM: stage M_q in FP32 (M is then a float array of the same layout)
+    bool m_f32;
 };
 
 // Scale exponents, then digits: launch_lowp_exponents fills ea for an S with rows fastest; the slice
@@ -50,8 +52,10 @@ hipError_t launch_ozaki_gemm(int nslices, const void* As, const void* Bs, const 
 hipError_t launch_ozaki_i8_slices(int nslices, const LowpSliceArgs& a, hipStream_t s);
 hipError_t launch_ozaki_i8_gemm(int nslices, const void* As, const void* Bs, const int* ea, const int* eb, double* M,
                                 const LowpGemmArgs& g, hipStream_t s);
-hipError_t launch_fp8_slices(const LowpSliceArgs& a, float* sa, float* sb, hipStream_t s);
-hipError_t launch_fp8_gemm(const void* As, const void* Bs, const float* sa, const float* sb, double* M,
+// FP8: E4M3 codes and E8M0 block scales (one per row and 128-long K block, four 16-row blocks per word);
+// the GEMM needs mp, np and kp multiples of 128.
+hipError_t launch_fp8_slices(const LowpSliceArgs& a, uint32_t* sa, uint32_t* sb, hipStream_t s);
+hipError_t launch_fp8_gemm(const void* As, const void* Bs, const uint32_t* sa, const uint32_t* sb, double* M,
                            const LowpGemmArgs& g, hipStream_t s);
 
 }  // namespace mistra

@@ -711,9 +711,15 @@ __global__ __launch_bounds__(512) void ozaki_i8_gemm_kernel(const int8_t* __rest
     tile_out_128x64(g, q, acc, reinterpret_cast<double*>(&sa[0][0][0]), row0, col0, wm, wn, ea, eb, M);
 }
 
-// ---- FP8 E4M3 with per-128 K-block scales -------------------------------------------------------------
+// ---- FP8 E4M3 with per-128 K-block power-of-two scales (MX-style) ------------------------------------
 
-constexpr float kE4M3Max = 240.0f;  // largest finite E4M3 value common to the OCP and FNUZ encodings
+// Codes: E4M3 values of x / 2^e with 2^e the smallest power of two that brings the block's largest
+// magnitude to at most 240 (the largest finite E4M3 value common to the OCP and FNUZ encodings), one e
+// per (row, 128-long K block).  Scales: E8M0 bytes (e + 127) packed so that the four 16-row blocks of a
+// 64-row band share one 32-bit word: byte (r % 64) / 16 of word [(q * nkb + kb) * (rp / 4) + (r / 64) * 16
+// + r % 16].  A lane of the GEMM then loads one word per operand and K-step, and the MFMA's operand
+// select picks the byte of its 16-row block.
+constexpr double kE4M3Max = 240.0;
 
 __device__ __forceinline__ uint32_t fp8x4(float a, float b, float c, float d)
 {
@@ -721,10 +727,25 @@ __device__ __forceinline__ uint32_t fp8x4(float a, float b, float c, float d)
     return uint32_t(__builtin_amdgcn_cvt_pk_fp8_f32(c, d, lo, true));
 }
 
-// S (rows fastest) -> codes[q][i][k] (k fastest, kp multiple of 128), scales sa[q][i][kb]: one thread
-// per (q, i, kb) (reads strided by `rows`, writes 128 contiguous bytes)
+// exponent e of the block scale: amax / 2^e <= 240 with e as small as possible, e in [-127, 127]
+__device__ __forceinline__ int fp8_block_exp(double amax)
+{
+    if (!(amax > 0.0)) return 0;
+    int x = 0;
+    const double f = frexp(amax / kE4M3Max, &x);  // amax / 240 = f 2^x, f in [0.5, 1)
+    const int e = f == 0.5 ? x - 1 : x;
+    return e < -127 ? -127 : (e > 127 ? 127 : e);
+}
+
+__device__ __forceinline__ int64_t fp8_scale_byte(int64_t q, int64_t kb, int64_t nkb, int64_t r, int64_t rp)
+{
+    return ((q * nkb + kb) * (rp / 4) + (r / 64) * 16 + (r % 16)) * 4 + (r % 64) / 16;
+}
+
+// S (rows fastest) -> codes[q][i][k] (k fastest, kp a multiple of 128) and scale bytes: one thread per
+// (q, i, kb) (reads strided by `rows`, coalesced across threads; writes 128 contiguous bytes)
 __global__ __launch_bounds__(256) void fp8_rows_kernel(const double* __restrict__ S, int64_t rows, int64_t cols,
-                                                       uint8_t* __restrict__ out, float* __restrict__ sc,
+                                                       uint8_t* __restrict__ out, uint8_t* __restrict__ sc,
                                                        int64_t rp, int64_t kp, int nmat)
 {
     const int64_t nkb = kp / 128, n = int64_t(nmat) * rp * nkb;
@@ -735,8 +756,8 @@ __global__ __launch_bounds__(256) void fp8_rows_kernel(const double* __restrict_
         double m = 0.0;
         if (i < rows)
             for (int64_t k = kb * 128; k < kb * 128 + 128 && k < cols; k++) m = fmax(m, fabs(s[k * rows]));
-        const float scale = m > 0.0 ? float(m) / kE4M3Max : 1.0f;
-        sc[(q * rp + i) * nkb + kb] = scale;
+        const int e = fp8_block_exp(m);
+        sc[fp8_scale_byte(q, kb, nkb, i, rp)] = uint8_t(e + 127);
         uint32_t* o = reinterpret_cast<uint32_t*>(out + (q * rp + i) * kp + kb * 128);
         for (int c = 0; c < 128; c += 4)
         {
@@ -744,16 +765,16 @@ __global__ __launch_bounds__(256) void fp8_rows_kernel(const double* __restrict_
             for (int u = 0; u < 4; u++)
             {
                 const int64_t k = kb * 128 + c + u;
-                v[u] = (i < rows && k < cols) ? float(s[k * rows] / double(scale)) : 0.0f;
+                v[u] = (i < rows && k < cols) ? float(ldexp(s[k * rows], -e)) : 0.0f;
             }
             o[c / 4] = fp8x4(v[0], v[1], v[2], v[3]);
         }
     }
 }
 
-// T (k fastest, [q][j][k]) -> codes[q][j][k], scales sb[q][j][kb]: one wave per (q, j, kb)
+// T (k fastest, [q][j][k]) -> codes[q][j][k] and scale bytes: one wave per (q, j, kb)
 __global__ __launch_bounds__(256) void fp8_cols_kernel(const double* __restrict__ T, int64_t kk, int64_t cols,
-                                                       uint8_t* __restrict__ out, float* __restrict__ sc,
+                                                       uint8_t* __restrict__ out, uint8_t* __restrict__ sc,
                                                        int64_t cp, int64_t kp, int nmat)
 {
     const int lane = threadIdx.x & 63;
@@ -766,84 +787,166 @@ __global__ __launch_bounds__(256) void fp8_cols_kernel(const double* __restrict_
         const double v0 = (j < cols && k0 < kk) ? t[k0] : 0.0, v1 = (j < cols && k0 + 1 < kk) ? t[k0 + 1] : 0.0;
         double m = fmax(fabs(v0), fabs(v1));
         for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
-        const float scale = m > 0.0 ? float(m) / kE4M3Max : 1.0f;
-        if (lane == 0) sc[(q * cp + j) * nkb + kb] = scale;
-        const int pk = __builtin_amdgcn_cvt_pk_fp8_f32(float(v0 / double(scale)), float(v1 / double(scale)), 0, false);
+        const int e = fp8_block_exp(m);
+        if (lane == 0) sc[fp8_scale_byte(q, kb, nkb, j, cp)] = uint8_t(e + 127);
+        const int pk = __builtin_amdgcn_cvt_pk_fp8_f32(float(ldexp(v0, -e)), float(ldexp(v1, -e)), 0, false);
         reinterpret_cast<uint16_t*>(out + (q * cp + j) * kp + kb * 128)[lane] = uint16_t(pk & 0xffff);
     }
 }
 
-__global__ __launch_bounds__(256) void fp8_gemm_kernel(const uint8_t* __restrict__ As, const uint8_t* __restrict__ Bs,
-                                                       const float* __restrict__ sa, const float* __restrict__ sb,
-                                                       double* __restrict__ M, LowpGemmArgs g)
+// 256 threads (4 waves as 2 x 2, 64 x 64 each = 4 x 4 blocks of 16 x 16) per 128 x 128 tile of M_q,
+// K-steps of 128 (one v_mfma_scale_f32_16x16x128_f8f6f4 per block and step, the block scales applied by
+// the MFMA itself, FP32 accumulation across blocks).  The code tiles are double-buffered in LDS as
+// [row][128 bytes] with the eight 16-byte chunks of row r XOR-swizzled by r & 7, so the operand reads (16
+// consecutive rows at one chunk pair) spread over the chunk slots; the next step's tiles and scale words
+// are loaded into registers while the current step's MFMAs run.  Output through LDS (the 64 KiB of tile
+// buffers, free after the loop): consecutive threads write consecutive rows of M_q (its unit stride).
+// Needs mp % 128 == 0, np % 128 == 0, kp % 128 == 0.
+typedef int i32x8 __attribute__((ext_vector_type(8)));
+typedef int i32x4v __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(256, 2) void fp8_gemm_kernel(const uint8_t* __restrict__ As, const uint8_t* __restrict__ Bs,
+                                                          const uint32_t* __restrict__ sa, const uint32_t* __restrict__ sb,
+                                                          double* __restrict__ M, LowpGemmArgs g)
 {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int tiles_m = int(g.mp / 64);
+    __shared__ __attribute__((aligned(16))) uint8_t lds[2][2][128 * 128];  // [buffer][A, B][row][128 B]
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tiles_m = int(g.mp / 128);
     const int tm = blockIdx.x % tiles_m, tn = blockIdx.x / tiles_m;
     const int q = g.q0 + int(blockIdx.y);  // product (the launch covers [q0, q0 + nlaunch))
-    const int64_t row0 = int64_t(tm) * 64 + (wave & 1) * 32, col0 = int64_t(tn) * 64 + (wave >> 1) * 32;
+    const int64_t row0 = int64_t(tm) * 128, col0 = int64_t(tn) * 128;
+    const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
     const int64_t nkb = g.kp / 128;
-    const uint8_t* pa = As + (int64_t(q) * g.mp + row0 + (lane & 15)) * g.kp + 8 * (lane >> 4);
-    const uint8_t* pb = Bs + (int64_t(q) * g.np + col0 + (lane & 15)) * g.kp + 8 * (lane >> 4);
-    double acc[2][2][4];
+    const uint8_t* ta = As + (int64_t(q) * g.mp + row0) * g.kp;
+    const uint8_t* tb = Bs + (int64_t(q) * g.np + col0) * g.kp;
+    // loader: 1024 chunks of 16 bytes per operand and step, 4 per thread; 8 threads cover one 128-byte row
+    int64_t goff[4];
+    int soff[4];
 #pragma unroll
-    for (int i = 0; i < 2; i++)
+    for (int u = 0; u < 4; u++)
+    {
+        const int c = tid + 256 * u, r = c >> 3, ch = c & 7;
+        goff[u] = int64_t(r) * g.kp + ch * 16;
+        soff[u] = r * 128 + ((ch ^ (r & 7)) << 4);
+    }
+    // scale words of this lane's 16-row blocks: rows wm + 16 i + lane % 16, byte i of the word
+    const uint32_t* wa = sa + int64_t(q) * nkb * (g.mp / 4) + ((row0 + wm) / 64) * 16 + (lane & 15);
+    const uint32_t* wb = sb + int64_t(q) * nkb * (g.np / 4) + ((col0 + wn) / 64) * 16 + (lane & 15);
+    // operand reads: row (16-row block base + lane % 16), chunks 2 (lane / 16) and 2 (lane / 16) + 1
+    const int fr = lane & 15, fg = lane >> 4;
+    int aoff[4][2], boff[4][2];
 #pragma unroll
-        for (int j = 0; j < 2; j++)
+    for (int i = 0; i < 4; i++)
 #pragma unroll
-            for (int v = 0; v < 4; v++) acc[i][j][v] = 0.0;
+        for (int h = 0; h < 2; h++)
+        {
+            const int ra = wm + 16 * i + fr, rb = wn + 16 * i + fr;
+            aoff[i][h] = ra * 128 + (((2 * fg + h) ^ (ra & 7)) << 4);
+            boff[i][h] = rb * 128 + (((2 * fg + h) ^ (rb & 7)) << 4);
+        }
+
+    f32x4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = f32x4{0, 0, 0, 0};
+
+    i32x4v ra[4], rb[4];
+    uint32_t swa = 0, swb = 0;
+    auto gload = [&](int64_t kb) {
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+        {
+            ra[u] = *reinterpret_cast<const i32x4v*>(ta + goff[u] + kb * 128);
+            rb[u] = *reinterpret_cast<const i32x4v*>(tb + goff[u] + kb * 128);
+        }
+        swa = wa[kb * (g.mp / 4)];
+        swb = wb[kb * (g.np / 4)];
+    };
+    auto lstore = [&](int buf) {
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+        {
+            *reinterpret_cast<i32x4v*>(&lds[buf][0][soff[u]]) = ra[u];
+            *reinterpret_cast<i32x4v*>(&lds[buf][1][soff[u]]) = rb[u];
+        }
+    };
+
+    uint32_t cwa = 0, cwb = 0;  // scale words of the step being computed
+    if (nkb > 0)
+    {
+        gload(0);
+        lstore(0);
+        cwa = swa;
+        cwb = swb;
+    }
+    __syncthreads();
     for (int64_t kb = 0; kb < nkb; kb++)
     {
-        f32x4 a32[2][2];
+        const int cur = int(kb & 1);
+        if (kb + 1 < nkb) gload(kb + 1);
+        i32x8 fa[4], fb[4];
 #pragma unroll
-        for (int i = 0; i < 2; i++)
-#pragma unroll
-            for (int j = 0; j < 2; j++) a32[i][j] = f32x4{0, 0, 0, 0};
-#pragma unroll
-        for (int st = 0; st < 4; st++)
+        for (int i = 0; i < 4; i++)
         {
-            long fa[2], fb[2];
-#pragma unroll
-            for (int i = 0; i < 2; i++)
-            {
-                fa[i] = *reinterpret_cast<const long*>(pa + int64_t(i) * 16 * g.kp + kb * 128 + st * 32);
-                fb[i] = *reinterpret_cast<const long*>(pb + int64_t(i) * 16 * g.kp + kb * 128 + st * 32);
-            }
-#pragma unroll
-            for (int i = 0; i < 2; i++)
-#pragma unroll
-                for (int j = 0; j < 2; j++)
-                    a32[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(fa[i], fb[j], a32[i][j], 0, 0, 0);
+            const i32x4v a0 = *reinterpret_cast<const i32x4v*>(&lds[cur][0][aoff[i][0]]);
+            const i32x4v a1 = *reinterpret_cast<const i32x4v*>(&lds[cur][0][aoff[i][1]]);
+            const i32x4v b0 = *reinterpret_cast<const i32x4v*>(&lds[cur][1][boff[i][0]]);
+            const i32x4v b1 = *reinterpret_cast<const i32x4v*>(&lds[cur][1][boff[i][1]]);
+            fa[i] = i32x8{a0[0], a0[1], a0[2], a0[3], a1[0], a1[1], a1[2], a1[3]};
+            fb[i] = i32x8{b0[0], b0[1], b0[2], b0[3], b1[0], b1[1], b1[2], b1[3]};
         }
-        // FP64 accumulation across blocks with the row and column scales of this block
-#pragma unroll
-        for (int i = 0; i < 2; i++)
-#pragma unroll
-            for (int j = 0; j < 2; j++)
-            {
-                const double cs = sb[(int64_t(q) * g.np + col0 + j * 16 + (lane & 15)) * nkb + kb];
-#pragma unroll
-                for (int v = 0; v < 4; v++)
-                {
-                    const double rs = sa[(int64_t(q) * g.mp + row0 + i * 16 + 4 * (lane >> 4) + v) * nkb + kb];
-                    acc[i][j][v] += double(a32[i][j][v]) * rs * cs;
-                }
-            }
-    }
-    double* Mq = M + int64_t(q) * g.ms * g.ns;
-#pragma unroll
-    for (int i = 0; i < 2; i++)
-#pragma unroll
-        for (int j = 0; j < 2; j++)
+#define MISTRA_FP8_MFMA(I, J)                                                                               \
+    acc[I][J] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(fa[I], fb[J], acc[I][J], 0, 0, I, int(cwa), J, \
+                                                                 int(cwb))
+#define MISTRA_FP8_ROW(I) \
+    MISTRA_FP8_MFMA(I, 0); \
+    MISTRA_FP8_MFMA(I, 1); \
+    MISTRA_FP8_MFMA(I, 2); \
+    MISTRA_FP8_MFMA(I, 3)
+        MISTRA_FP8_ROW(0);
+        MISTRA_FP8_ROW(1);
+        MISTRA_FP8_ROW(2);
+        MISTRA_FP8_ROW(3);
+#undef MISTRA_FP8_ROW
+#undef MISTRA_FP8_MFMA
+        if (kb + 1 < nkb)
         {
-            const int64_t col = col0 + j * 16 + (lane & 15);
-            if (col >= g.ns) continue;
+            lstore(cur ^ 1);
+            cwa = swa;
+            cwb = swb;
+        }
+        __syncthreads();
+    }
+
+    // output: the tile as [col][row] floats in the tile buffers, row r of column c at r ^ (c % 32) (the
+    // writes of one instruction -- 16 columns at 4 rows -- then fall on distinct banks), then rows-fastest
+    float* stage = reinterpret_cast<float*>(&lds[0][0][0]);
+    static_assert(sizeof(lds) >= 128 * 128 * sizeof(float), "output stage fits the tile buffers");
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++)
 #pragma unroll
             for (int v = 0; v < 4; v++)
             {
-                const int64_t row = row0 + i * 16 + 4 * (lane >> 4) + v;
-                if (row < g.ms) put_out(g, q, Mq, row, col, acc[i][j][v]);
+                const int c = wn + 16 * j + (lane & 15), r = wm + 16 * i + 4 * (lane >> 4) + v;
+                stage[c * 128 + (r ^ (c & 31))] = acc[i][j][v];
             }
+    __syncthreads();
+    double* Mq = M + int64_t(q) * g.ms * g.ns;
+    float* Mf = reinterpret_cast<float*>(M) + int64_t(q) * g.ms * g.ns;
+    const int r = tid & 127;
+    const int64_t row = row0 + r;
+    if (row < g.ms)
+        for (int c = tid >> 7; c < 128; c += 2)
+        {
+            const int64_t col = col0 + c;
+            if (col >= g.ns) continue;
+            const float v = stage[c * 128 + (r ^ (c & 31))];
+            if (g.m_f32 && !g.C)
+                Mf[row + col * g.ms] = v;
+            else
+                put_out(g, q, Mq, row, col, double(v));
         }
 }
 
@@ -992,20 +1095,24 @@ hipError_t launch_ozaki_gemm(int nsl, const void* As, const void* Bs, const int*
     }
 }
 
-hipError_t launch_fp8_slices(const LowpSliceArgs& a, float* sa, float* sb, hipStream_t s)
+hipError_t launch_fp8_slices(const LowpSliceArgs& a, uint32_t* sa, uint32_t* sb, hipStream_t s)
 {
+    if (a.mp % 64 || a.np % 64 || a.kp % 128) return hipErrorInvalidValue;
     hipLaunchKernelGGL(fp8_rows_kernel, dim3(grid_of(int64_t(a.nprod) * a.mp * (a.kp / 128), 256)), dim3(256), 0, s,
-                       a.S, a.ms, a.ks, static_cast<uint8_t*>(a.outA), sa, a.mp, a.kp, a.nprod);
+                       a.S, a.ms, a.ks, static_cast<uint8_t*>(a.outA), reinterpret_cast<uint8_t*>(sa), a.mp, a.kp,
+                       a.nprod);
     hipLaunchKernelGGL(fp8_cols_kernel, dim3(grid_of(int64_t(a.nprod) * a.np * (a.kp / 128), 4)), dim3(256), 0, s,
-                       a.T, a.ks, a.ns, static_cast<uint8_t*>(a.outB), sb, a.np, a.kp, a.nprod);
+                       a.T, a.ks, a.ns, static_cast<uint8_t*>(a.outB), reinterpret_cast<uint8_t*>(sb), a.np, a.kp,
+                       a.nprod);
     return hipGetLastError();
 }
 
-hipError_t launch_fp8_gemm(const void* As, const void* Bs, const float* sa, const float* sb, double* M,
+hipError_t launch_fp8_gemm(const void* As, const void* Bs, const uint32_t* sa, const uint32_t* sb, double* M,
                            const LowpGemmArgs& g, hipStream_t s)
 {
-    hipLaunchKernelGGL(fp8_gemm_kernel, dim3(unsigned((g.mp / 64) * (g.np / 64)), unsigned(g.nlaunch)), dim3(256), 0, s,
-                       static_cast<const uint8_t*>(As), static_cast<const uint8_t*>(Bs), sa, sb, M, g);
+    if (g.mp % 128 || g.np % 128 || g.kp % 128) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(fp8_gemm_kernel, dim3(unsigned((g.mp / 128) * (g.np / 128)), unsigned(g.nlaunch)), dim3(256), 0,
+                       s, static_cast<const uint8_t*>(As), static_cast<const uint8_t*>(Bs), sa, sb, M, g);
     return hipGetLastError();
 }
 
