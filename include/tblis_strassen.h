@@ -139,10 +139,12 @@ enum
                                          A and of B in HBM workspace (7/4 of A's and of B's size) and form
                                          only the inner sums (2 terms at level 2, none at level 1) in the
                                          leaf kernel, whenever the shape allows it (one batch entry).   */
-    MISTRA_FLAG_ALL_STAGED = 1u << 5   /* ABC: stage the sums of every level (at level 2 the 49 sums of A
+    MISTRA_FLAG_ALL_STAGED = 1u << 5,  /* ABC: stage the sums of every level (at level 2 the 49 sums of A
                                          and of B: 49/16 of A's and of B's size); each leaf product then
                                          reads one dense operand pair.  Without any of the three flags
                                          the cost model decides.                                         */
+    MISTRA_FLAG_BATCH_GROUPED = 1u << 6 /* mistra_batch_mult: one grouped launch per (Strassen stage round,
+                                         kernel variant) instead of the single persistent launch        */
 };
 
 /* Error codes (0 = success). */
@@ -227,13 +229,19 @@ typedef struct mistra_contraction
     int impl;    /* -1 = autotuner's choice (only ABC is available inside a batch) */
 } mistra_contraction;
 
-/* Run count independent contractions with few launches: the ABC launches of all entries are grouped
- * by Strassen stage (round) and kernel variant, and each group is one grouped launch whose workgroups
- * find their contraction in a device job table (ABC in its fused form, MISTRA_FLAG_FUSED_SUMS, so that
- * every launch is a leaf launch).  Entries must not write overlapping C memory.
- * level/impl -1 take the autotuner's choice; entries that are not device-resident ABC contractions
- * (host data, general index classes, AB/Naive) run as individual calls. Asynchronous on stream unless
- * MISTRA_FLAG_SYNC. No counterpart in the reference (its harness loops over contractions). */
+/* Run count independent contractions in one persistent launch: a device job queue first packs every
+ * leaf product's operand sums (one term for a classical product) into dense K-fast panels, then runs
+ * the GEMM tiles of every product, in Strassen stage order per contraction (device counters with
+ * bounded waits), with the multi-C epilogue on each contraction's C.  The packed panels and tables
+ * are built on the first call with a given set of entries (same pointers, shapes, strides, scalars,
+ * level and impl) and kept for later calls (a few batches are cached; mistra_release_workspace drops
+ * them).  MISTRA_FLAG_GRAPH records the launch in a HIP graph and replays it.  MISTRA_FLAG_SYNC also
+ * reports waits that gave up (MISTRA_ERR_HIP).  MISTRA_FLAG_BATCH_GROUPED selects the older form: one
+ * grouped launch per (stage round, kernel variant), ABC in its fused form.  Entries must not write
+ * overlapping C memory.  level/impl -1 take the autotuner's choice; entries that are not
+ * device-resident ABC contractions of the batch's element type without batch (ABC-class) indices
+ * (host data, general index classes, AB/Naive) run as individual calls.  Asynchronous on stream unless
+ * MISTRA_FLAG_SYNC.  No counterpart in the reference (its harness loops over contractions). */
 int mistra_batch_mult(int count, const mistra_contraction* entries, unsigned flags, void* stream);
 
 /* ---- Adjoints of C = alpha*A.B (+beta*C) ------------------------------------------------------ */

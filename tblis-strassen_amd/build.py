@@ -23,6 +23,7 @@ SOURCES = [
     "autotune.cpp",
     "batch.cpp",
     "noalloc.cpp",
+    "batch_persist.hip",
     "leaf_gemm.cpp",
     "leaf_f64_multi.hip",
     "leaf_f64_store.hip",

@@ -66,6 +66,24 @@ int mistra_debug_set_combine_form(int per_sub_block)
     return mistra::g_combine_per_sub.exchange(per_sub_block ? 1 : 0);
 }
 
+int mistra_debug_batch_check(int count, const mistra_contraction* entries, char* msg, int msg_len)
+{
+    const mistra::Status st = mistra::batch_check_host(count, entries);
+    if (msg && msg_len > 0)
+    {
+        std::strncpy(msg, st.msg.c_str(), size_t(msg_len) - 1);
+        msg[msg_len - 1] = 0;
+    }
+    return st.code;
+}
+
+int mistra_debug_set_batch_wait(long long spin_limit, unsigned dep_bias)
+{
+    mistra::g_dbg_batch_spin.store(spin_limit > 0 ? spin_limit : 0);
+    mistra::g_dbg_batch_bias.store(dep_bias);
+    return 0;
+}
+
 int mistra_debug_set_lowp(int bf16_slices, int int8_slices, int int8_fold_ksteps)
 {
     if (bf16_slices != 0 && (bf16_slices < 2 || bf16_slices > 7)) return MISTRA_ERR_ARG;

@@ -734,6 +734,7 @@ void build_image(Plan& p, PlanImage& img)
     for (const ProdDesc& pd : p.h_prods) fast_prods.push_back(shifted(p, pd));
     std::vector<int> order;
     make_stages(p, order);
+    p.h_stage_order = order;
     std::vector<ProdDesc> staged, fast_staged;
     std::vector<ProdDesc> inner_staged;
     for (size_t j = 0; j < order.size(); j++)

@@ -84,6 +84,7 @@ struct Plan
     // launch; their C targets are disjoint.  Every C sub-block sees its products in stage order, and
     // h_stage_first holds the first-writer masks (beta) for that order, per staged product.
     std::vector<int> h_stage_begin, h_stage_terms, h_stage_first;
+    std::vector<int> h_stage_order;  // product index of each staged position (make_stages' order)
     const ProdDesc* prods_staged = nullptr;
     const ProdDesc* prods_fast_staged = nullptr;
 
@@ -242,6 +243,9 @@ double model_ms(const Plan& p);
 
 // Batched contractions (batch.cpp).
 Status batch_mult(int count, const mistra_contraction* entries, unsigned flags, hipStream_t stream);
+// The persistent launch's tables for these entries, built on the host alone and checked by replaying the
+// job queue in order (no device work; for tests).
+Status batch_check_host(int count, const mistra_contraction* entries);
 void release_batch_graphs();
 
 // 1: the combine pass always takes its per-sub-block form (mistra_debug_set_combine_form)
@@ -250,6 +254,10 @@ extern std::atomic<int> g_combine_per_sub;
 // and the INT8 GEMM's fold period in K-steps
 extern std::atomic<int> g_dbg_ozaki_slices, g_dbg_i8_slices, g_dbg_i8_fold;
 double scalar_value(const tblis::tblis_tensor* t);
+void set_scalar(tblis::tblis_tensor* t, double v);
+// test overrides of the persistent batch launch's waits (mistra_debug_set_batch_wait)
+extern std::atomic<long long> g_dbg_batch_spin;
+extern std::atomic<unsigned> g_dbg_batch_bias;
 
 // Kernel timing hook of the dominant leaf kernel.
 void timing_begin(hipStream_t s);

@@ -77,6 +77,13 @@ struct GemmArgs
     // fast K mode (chosen by the host per launch): every term's K-tiles are affine with these strides
     // and every element offset fits in 31 bits
     int64_t ka_stride, kb_stride;
+    // direct-to-LDS body: 0 = C targets are shifts (ProdDesc base) of the first C sub-block; 1 = they are
+    // addressed through (roff, coff) into c_m / c_n (sub-blocks that are not translates of each other)
+    int c_scatter;
+    // multi-C epilogue: C is read and written with device-coherent (agent-scope) accesses, because
+    // workgroups on other XCDs read or write the same C during the same launch (the rounds of one
+    // contraction in the persistent batch launch); 0 = ordinary cached accesses
+    int c_coherent;
 };
 
 }  // namespace mistra

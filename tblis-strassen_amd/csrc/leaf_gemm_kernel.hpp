@@ -79,6 +79,32 @@ __device__ __forceinline__ const __attribute__((address_space(4))) T* cptr(const
     return (const __attribute__((address_space(4))) T*)(p);
 }
 
+// Operand and result pointers taken from tables in memory reach the kernel as generic (flat) pointers.
+// Flat loads may target LDS, so the compiler must wait for both vmcnt and lgkmcnt to reach zero before
+// using any of them, which leaves one load in flight per wave.  The data lives in device memory, so it
+// is addressed through the global address space instead (in-order vmcnt: many loads in flight).
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* gptr(T* p)
+{
+    return (__attribute__((address_space(1))) T*)(p);
+}
+
+// Device-coherent element access (agent scope, `sc1`): the load cannot hit a line of this XCD's L2 that
+// another XCD has since written, and the store writes through this XCD's L2 to memory, so once it has
+// completed (vmcnt) every XCD reads the new value.  The whole-L2 alternative, a release/acquire fence,
+// writes back and invalidates the entire L2 of the XCD (buffer_wbl2 / buffer_inv), which stalls every
+// workgroup on it.
+template <typename P>
+__device__ __forceinline__ auto coherent_load(P p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename P, typename T>
+__device__ __forceinline__ void coherent_store(P p, T v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 __device__ __forceinline__ int64_t lmin(int64_t a, int64_t b) { return a < b ? a : b; }
 constexpr int cmax(int a, int b) { return a > b ? a : b; }
 
@@ -138,6 +164,20 @@ __device__ __forceinline__ int64_t readlane64(int64_t x, int l)
     return int64_t((uint64_t(uint32_t(hi)) << 32) | uint64_t(uint32_t(lo)));
 }
 
+// A value the same in every lane, held as such (in SGPRs): a no-op for values the compiler already knows
+// to be uniform (kernel arguments and scalar loads); for ones it loaded with vector loads (arguments
+// read from a table after a fence) it keeps buffer resources built from them out of waterfall loops.
+__device__ __forceinline__ int64_t uniform64(int64_t x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    const int lo = __builtin_amdgcn_readfirstlane(int(uint64_t(x) & 0xffffffffu));
+    const int hi = __builtin_amdgcn_readfirstlane(int(uint64_t(x) >> 32));
+    return int64_t((uint64_t(uint32_t(hi)) << 32) | uint64_t(uint32_t(lo)));
+#else
+    return x;
+#endif
+}
+
 // The body of one workgroup: register tile bx of the launch's tile grid, product by, batch entry bz.
 template <typename T, int TM, int TN, int TK, int TERMS, bool A_MFAST, bool B_NFAST, int EPI, int KMODE>
 __device__ __forceinline__ void leaf_gemm_body(const GemmArgs& g, int bx, int by, int64_t bz)
@@ -189,7 +229,7 @@ __device__ __forceinline__ void leaf_gemm_body(const GemmArgs& g, int bx, int by
     const int64_t z = bz;
     const T* __restrict__ Ag = static_cast<const T*>(g.A) + (g.batch_a ? cptr(g.batch_a)[z] : 0);
     const T* __restrict__ Bg = static_cast<const T*>(g.B) + (g.batch_b ? cptr(g.batch_b)[z] : 0);
-    T* __restrict__ Cg = static_cast<T*>(g.C) + (g.batch_c ? cptr(g.batch_c)[z] : 0);
+    __attribute__((address_space(1))) T* __restrict__ Cg = gptr(static_cast<T*>(g.C)) + (g.batch_c ? cptr(g.batch_c)[z] : 0);
     const T* __restrict__ zero = static_cast<const T*>(g.zero);
 
     // Loader positions inside the tile.  A: M-fast -> consecutive lanes walk m (coalesced when m has
@@ -222,6 +262,7 @@ __device__ __forceinline__ void leaf_gemm_body(const GemmArgs& g, int bx, int by
             const double alpha = g.alpha, beta = g.beta;
             const int nC = P.nC;
             const int first_mask = P.first_mask;
+            const bool coherent = g.c_coherent != 0;
             // column offsets of every target, for this tile (one shared row in translate mode)
             for (int x = tid; x < (translate ? 1 : nC) * TN; x += NT)
             {
@@ -256,18 +297,32 @@ __device__ __forceinline__ void leaf_gemm_body(const GemmArgs& g, int bx, int by
                     const bool first = (first_mask >> c) & 1;
                     const bool read_c = !(first && beta == 0.0);
                     const T bet = first ? T(beta) : T(1);
-                    T* Crow = Cg + P.c[c].base + (translate ? rbase : g.c_m[P.c[c].roff + growc]);
+                    __attribute__((address_space(1))) T* Crow = Cg + P.c[c].base + (translate ? rbase : g.c_m[P.c[c].roff + growc]);
                     const int64_t* cc = ccol[translate ? 0 : c];
                     T old[CPER];
+                    if (coherent)
+                    {
 #pragma unroll
-                    for (int e = 0; e < CPER; e++) old[e] = read_c ? Crow[cc[cg + e * CSTEP]] : T(0);
+                        for (int e = 0; e < CPER; e++) old[e] = read_c ? coherent_load(&Crow[cc[cg + e * CSTEP]]) : T(0);
+                    }
+                    else
+                    {
+#pragma unroll
+                        for (int e = 0; e < CPER; e++) old[e] = read_c ? Crow[cc[cg + e * CSTEP]] : T(0);
+                    }
 #pragma unroll
                     for (int e = 0; e < CPER; e++)
                     {
                         const int col = cg + e * CSTEP;
                         const T val = coef * stage[rl * LDE + col];
                         const T out = read_c ? bet * old[e] + val : val;
-                        if (rok && col0 + col < ns) Crow[cc[col]] = out;
+                        if (rok && col0 + col < ns)
+                        {
+                            if (coherent)
+                                coherent_store(&Crow[cc[col]], out);
+                            else
+                                Crow[cc[col]] = out;
+                        }
                     }
                 }
                 __syncthreads();
@@ -351,8 +406,8 @@ __device__ __forceinline__ void leaf_gemm_body(const GemmArgs& g, int bx, int by
         {
             acoef[t] = t < nA ? T(P.a[t].coeff) : T(0);
             bcoef[t] = t < nB ? T(P.b[t].coeff) : T(0);
-            ash[t] = t < nA ? P.a[t].base : 0;
-            bsh[t] = t < nB ? P.b[t].base : 0;
+            ash[t] = uniform64(t < nA ? P.a[t].base : 0);
+            bsh[t] = uniform64(t < nB ? P.b[t].base : 0);
         }
 
         const int64_t nkt = (ks + TK - 1) / TK;
@@ -684,7 +739,7 @@ __device__ __forceinline__ void leaf_gemm_body(const GemmArgs& g, int bx, int by
                     __syncthreads();
                 }
             }
-            epilogue(P, p, acc, true, double(acoef[0]) * double(bcoef[0]));
+            epilogue(P, p, acc, g.c_scatter == 0, double(acoef[0]) * double(bcoef[0]));
             return;
         }
         else if constexpr (!kTwoSets)
