@@ -405,11 +405,16 @@ def bench_tc(ts, torch, args, stream, workload, variant_spec):
     C, cbuf = device_tensor(case["shape_C"], torch.float64, torch)
 
     variants = {}
-    forms = {"": 0, "fused": ts.FLAG_FUSED_SUMS, "staged": ts.FLAG_OUTER_STAGED, "staged_all": ts.FLAG_ALL_STAGED}
+    # a form is one or more of these joined by "+" ("persist": every stage product in one persistent
+    # launch instead of one leaf launch per stage)
+    forms = {"": 0, "fused": ts.FLAG_FUSED_SUMS, "staged": ts.FLAG_OUTER_STAGED, "staged_all": ts.FLAG_ALL_STAGED,
+             "persist": ts.FLAG_STAGE_PERSIST}
     for spec in variant_spec.split(","):
         lv, impl, *form = spec.split(":")
         form = form[0] if form else ""
-        flags = forms[form]
+        flags = 0
+        for f in form.split("+"):
+            flags |= forms[f]
         level = int(lv)
         name = f"L{level}_{impl}" + (f"_{form}" if form else "")
         cbuf.zero_()

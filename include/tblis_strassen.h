@@ -143,8 +143,13 @@ enum
                                          and of B: 49/16 of A's and of B's size); each leaf product then
                                          reads one dense operand pair.  Without any of the three flags
                                          the cost model decides.                                         */
-    MISTRA_FLAG_BATCH_GROUPED = 1u << 6 /* mistra_batch_mult: one grouped launch per (Strassen stage round,
+    MISTRA_FLAG_BATCH_GROUPED = 1u << 6, /* mistra_batch_mult: one grouped launch per (Strassen stage round,
                                          kernel variant) instead of the single persistent launch        */
+    MISTRA_FLAG_STAGE_PERSIST = 1u << 7 /* ABC with every sum staged (level 1 staged, level 2 all
+                                         staged): all products in one persistent launch, whose tiles
+                                         wait only for earlier products updating the same C tile,
+                                         instead of one leaf launch per stage (products with disjoint C
+                                         targets) in stream order                                        */
 };
 
 /* Error codes (0 = success). */

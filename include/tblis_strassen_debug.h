@@ -41,9 +41,10 @@ int mistra_debug_set_combine_form(int per_sub_block);
  * 0 restores a default.  Returns 0, or MISTRA_ERR_ARG for a value out of range. */
 int mistra_debug_set_lowp(int bf16_slices, int int8_slices, int int8_fold_ksteps);
 
-/* Waits of the persistent batch launch: spin_limit polls before a wait gives up (0 = the default) and
- * dep_bias added to every wait's target (0 in normal use; a positive bias makes every wait unreachable,
- * so each gives up after spin_limit polls and the launch still ends).  Process-wide; returns 0. */
+/* Waits of the persistent launches (the batch launch and the single launch over a staged plan's
+ * products): spin_limit polls before a wait gives up (0 = the default) and dep_bias added to every
+ * wait's target (0 in normal use; a positive bias makes every wait unreachable, so each gives up after
+ * spin_limit polls and the launch still ends).  Process-wide; returns 0. */
 int mistra_debug_set_batch_wait(long long spin_limit, unsigned dep_bias);
 
 /* Builds the persistent batch launch's tables for these entries on the host alone (plans without device

@@ -609,6 +609,7 @@ Status tensor_mult(const tblis::tblis_tensor* A, const label_type* idx_A, const 
             else
             {
                 Call call{dA, dB, dC, alpha, beta};
+                call.flags = flags;
                 if (plan->transposed) std::swap(call.A, call.B);
                 if (precision == MISTRA_PREC_NATIVE)
                     st = run_plan(*plan, call, stream);
@@ -637,6 +638,8 @@ Status tensor_mult(const tblis::tblis_tensor* A, const label_type* idx_A, const 
     else if (synchronous || (flags & MISTRA_FLAG_SYNC))
     {
         TRY_HIP(hipStreamSynchronize(stream));
+        const Status ws = stage_wait_status();
+        if (!ws.ok()) return ws;
     }
 #undef TRY_HIP
     set_scalar_one(C);
@@ -778,12 +781,13 @@ Status tensor_mult_noalloc(const tblis::tblis_tensor* A, const label_type* idx_A
     ws.base = static_cast<char*>(base) + sp->table_bytes;
     ws.size = room;
     Call call{A->data, B->data, C->data, alpha, beta, &ws};
+    call.flags = flags;
     if (p.transposed) std::swap(call.A, call.B);
     st = run_plan(p, call, stream);
     if (st.ok() && (flags & MISTRA_FLAG_SYNC))
     {
         e = hipStreamSynchronize(stream);
-        if (e != hipSuccess) st = Status::error(MISTRA_ERR_HIP, hipGetErrorString(e));
+        st = e != hipSuccess ? Status::error(MISTRA_ERR_HIP, hipGetErrorString(e)) : stage_wait_status();
     }
     if (st.ok()) set_scalar_one(C);
     return st;

@@ -246,46 +246,61 @@ struct HostItem
     double alpha, beta;
 };
 
-// Replays the launch's queue discipline on the host tables, one job after the other in queue order (a
-// job waits only for jobs before it, so this is the order every wait must already be met in): every
-// pack chunk advances its product's counter, every tile's pack and round targets must be reached when
-// its turn comes, and at the end every product's counter equals its target.
-Status check_queue(const PackDesc* packs, int npacks, const TileItem* items, int nitems, int nentries,
-                   int64_t njobs_pack, int64_t njobs)
+// Replays the launch's queue discipline on the host tables, segment by segment in queue order (a job
+// waits only for jobs before it, so this is the order every wait must already be met in): the segments
+// tile the queue without gaps, every panel and every product appears exactly once, every pack chunk
+// advances its product's counter, every product's pack and round targets are reached when its first
+// tile's turn comes, and at the end every product's counter equals its target.
+Status check_queue(const Segment* segs, int nsegs, const PackDesc* packs, int npacks, const TileItem* items, int nitems,
+                   int nentries, int64_t njobs)
 {
     std::vector<int64_t> packed(size_t(nitems), 0), done(size_t(nentries), 0);
-    int pi = 0, ti = 0;
-    for (int64_t j = 0; j < njobs; j++)
+    std::vector<char> seen_pack(size_t(npacks), 0), seen_item(size_t(nitems), 0);
+    int64_t pos = 0;
+    for (int s = 0; s < nsegs; s++)
     {
-        if (j < njobs_pack)
+        const Segment& S = segs[s];
+        const std::string where = "queue: segment " + std::to_string(s) + " at " + std::to_string(pos);
+        if (S.first_job != pos) return Status::error(MISTRA_ERR_ARG, where + ": segments leave a gap or overlap");
+        if (S.kind == 0)
         {
-            while (pi + 1 < npacks && packs[pi + 1].first_job <= j) pi++;
-            const PackDesc& P = packs[pi];
-            if (j - P.first_job >= pack_jobs_of(P.rows, P.ld) || P.item < 0 || P.item >= nitems)
-                return Status::error(MISTRA_ERR_ARG, "queue: pack job " + std::to_string(j) + " outside its panel");
-            packed[size_t(P.item)]++;
+            if (S.index < 0 || S.index >= npacks || seen_pack[size_t(S.index)]++)
+                return Status::error(MISTRA_ERR_ARG, where + ": panel index out of range or repeated");
+            const PackDesc& P = packs[S.index];
+            if (P.first_job != pos || P.item < 0 || P.item >= nitems)
+                return Status::error(MISTRA_ERR_ARG, where + ": panel position or product out of step");
+            const int64_t n = pack_jobs_of(P.rows, P.ld);
+            packed[size_t(P.item)] += n;
+            pos += n;
             continue;
         }
-        const int64_t t = j - njobs_pack;
-        while (ti + 1 < nitems && items[ti + 1].first_job <= t) ti++;
-        const TileItem& it = items[ti];
-        if (t - it.first_job >= int64_t(it.g.tiles_m) * it.g.tiles_n)
-            return Status::error(MISTRA_ERR_ARG, "queue: tile job " + std::to_string(j) + " outside its product");
-        if (packed[size_t(ti)] < it.need_pack)
-            return Status::error(MISTRA_ERR_ARG, "queue: tile job " + std::to_string(j) + " of product " +
-                                                     std::to_string(ti) + " waits for " + std::to_string(it.need_pack) +
-                                                     " pack chunks, " + std::to_string(packed[size_t(ti)]) + " done");
+        if (S.kind != 1 || S.index < 0 || S.index >= nitems || seen_item[size_t(S.index)]++)
+            return Status::error(MISTRA_ERR_ARG, where + ": product index out of range or repeated");
+        const TileItem& it = items[S.index];
+        if (it.first_job != pos) return Status::error(MISTRA_ERR_ARG, where + ": product position out of step");
+        if (packed[size_t(S.index)] < it.need_pack)
+            return Status::error(MISTRA_ERR_ARG, where + ": product " + std::to_string(S.index) + " waits for " +
+                                                     std::to_string(it.need_pack) + " pack chunks, " +
+                                                     std::to_string(packed[size_t(S.index)]) + " queued before it");
         if (it.entry < 0 || it.entry >= nentries || done[size_t(it.entry)] < it.need_entry)
-            return Status::error(MISTRA_ERR_ARG, "queue: tile job " + std::to_string(j) + " waits for " +
-                                                     std::to_string(it.need_entry) + " earlier tiles of contraction " +
-                                                     std::to_string(it.entry));
-        done[size_t(it.entry)]++;
+            return Status::error(MISTRA_ERR_ARG, where + ": product waits for " + std::to_string(it.need_entry) +
+                                                     " earlier tiles of contraction " + std::to_string(it.entry));
+        const int64_t n = int64_t(it.g.tiles_m) * it.g.tiles_n;
+        done[size_t(it.entry)] += n;
+        pos += n;
     }
+    if (pos != njobs) return Status::error(MISTRA_ERR_ARG, "queue: segments cover " + std::to_string(pos) + " of " +
+                                                              std::to_string(njobs) + " positions");
+    for (int i = 0; i < npacks; i++)
+        if (!seen_pack[size_t(i)]) return Status::error(MISTRA_ERR_ARG, "queue: panel " + std::to_string(i) + " never packed");
     for (int i = 0; i < nitems; i++)
+    {
+        if (!seen_item[size_t(i)]) return Status::error(MISTRA_ERR_ARG, "queue: product " + std::to_string(i) + " never queued");
         if (packed[size_t(i)] != items[i].need_pack)
             return Status::error(MISTRA_ERR_ARG, "queue: product " + std::to_string(i) + " packs " +
                                                      std::to_string(packed[size_t(i)]) + " chunks, expects " +
                                                      std::to_string(items[i].need_pack));
+    }
     return Status();
 }
 
@@ -462,7 +477,9 @@ Status prepare_persistent(int count, const mistra_contraction* e, hipStream_t st
     }
     const int npacks = 2 * nitems;
     const size_t o_packs = 0, o_items = align256(size_t(npacks) * sizeof(PackDesc)) + o_packs;
-    const size_t o_prods = o_items + align256(size_t(nitems) * sizeof(TileItem));
+    const int nsegs = npacks + nitems;
+    const size_t o_segs = o_items + align256(size_t(nitems) * sizeof(TileItem));
+    const size_t o_prods = o_segs + align256(size_t(nsegs) * sizeof(Segment));
     const size_t o_cnt = o_prods + align256(size_t(nitems) * sizeof(ProdDesc));
     const size_t ncnt = size_t(kPersistHeader + nitems + nentries);
     const size_t o_iota = o_cnt + align256(ncnt * sizeof(unsigned));
@@ -492,6 +509,7 @@ Status prepare_persistent(int count, const mistra_contraction* e, hipStream_t st
     std::vector<unsigned char> host(table_bytes, 0);  // counters start at zero
     PackDesc* hp = reinterpret_cast<PackDesc*>(host.data() + o_packs);
     TileItem* hi = reinterpret_cast<TileItem*>(host.data() + o_items);
+    Segment* hs = reinterpret_cast<Segment*>(host.data() + o_segs);
     ProdDesc* hd = reinterpret_cast<ProdDesc*>(host.data() + o_prods);
     int64_t* hiota = reinterpret_cast<int64_t*>(host.data() + o_iota);
     for (int64_t k = 0; k < max_kpad; k++) hiota[k] = k;
@@ -503,7 +521,6 @@ Status prepare_persistent(int count, const mistra_contraction* e, hipStream_t st
     const int64_t* d_iota = reinterpret_cast<const int64_t*>(T0 + o_iota);
     std::vector<int64_t> entry_tiles(size_t(nentries), 0), entry_round_start(size_t(nentries), 0);
     std::vector<int> entry_round(size_t(nentries), -1);
-    int64_t tile_job = 0, pack_job = 0;
     for (int i = 0; i < nitems; i++)
     {
         const HostItem& it = items[size_t(i)];
@@ -532,8 +549,6 @@ Status prepare_persistent(int count, const mistra_contraction* e, hipStream_t st
             }
             d.rows_fast = isA ? int(p.a_mfast) : int(p.b_nfast);
             d.item = i;
-            d.first_job = pack_job;
-            pack_job += pack_jobs_of(d.rows, d.ld);
         }
         const int64_t need_pack = pack_jobs_of(it.ms, it.kpad) + pack_jobs_of(it.ns, it.kpad);
         // the product table of the tile jobs: one packed term per operand, the plan's C targets
@@ -572,7 +587,6 @@ Status prepare_persistent(int count, const mistra_contraction* e, hipStream_t st
         g.c_scatter = 1;
         ti.entry = it.entry;
         ti.round = it.round;
-        ti.first_job = tile_job;
         ti.need_pack = need_pack;
         // tiles of the contraction's earlier rounds (items arrive in round order)
         const size_t en = size_t(it.entry);
@@ -582,9 +596,38 @@ Status prepare_persistent(int count, const mistra_contraction* e, hipStream_t st
             entry_round_start[en] = entry_tiles[en];
         }
         ti.need_entry = entry_round_start[en];
-        const int64_t nt = int64_t(g.tiles_m) * g.tiles_n;
-        entry_tiles[en] += nt;
-        tile_job += nt;
+        entry_tiles[en] += int64_t(g.tiles_m) * g.tiles_n;
+    }
+    // Queue positions.  The products' tiles go in the order above; each product's two panels are queued
+    // before its tiles, and early enough that about one grid of tile jobs lies between a panel and the
+    // first tile that reads it: by the time that tile is taken, its panels are long packed, and from the
+    // start the workgroups that pack (memory-bound) run beside workgroups that multiply (MFMA-bound).
+    int nseg = 0;
+    int64_t pos = 0;
+    {
+        const int64_t lead = slots;  // tile jobs
+        int next = 0;                // next product whose panels are queued
+        int64_t covered = 0, queued = 0;
+        auto tiles_of = [&](int i) { return int64_t(hi[i].g.tiles_m) * hi[i].g.tiles_n; };
+        for (int i = 0; i < nitems; i++)
+        {
+            while (next < nitems && (next <= i || covered - queued < lead))
+            {
+                for (int w = 0; w < 2; w++)
+                {
+                    PackDesc& d = hp[2 * next + w];
+                    d.first_job = pos;
+                    hs[nseg++] = Segment{pos, 0, 2 * next + w};
+                    pos += pack_jobs_of(d.rows, d.ld);
+                }
+                covered += tiles_of(next);
+                next++;
+            }
+            hi[i].first_job = pos;
+            hs[nseg++] = Segment{pos, 1, i};
+            pos += tiles_of(i);
+            queued += tiles_of(i);
+        }
     }
     // a tile's C writes are fenced when a later round of its contraction reads them
     {
@@ -598,11 +641,16 @@ Status prepare_persistent(int count, const mistra_contraction* e, hipStream_t st
             hi[i].g.c_coherent = last > 0 ? 1 : 0;
         }
     }
-    if (pack_job + tile_job + (int64_t(1) << 20) >= (int64_t(1) << 32))
+    if (pos + (int64_t(1) << 20) >= (int64_t(1) << 32))
         return Status::error(MISTRA_ERR_UNSUPPORTED, "persistent batch: more than 2^32 jobs");
     for (int64_t n : entry_tiles)
         if (n >= int64_t(1) << 31) return Status::error(MISTRA_ERR_UNSUPPORTED, "persistent batch: 2^31 tiles in one contraction");
-    if (host_only) return check_queue(hp, npacks, hi, nitems, nentries, pack_job, pack_job + tile_job);
+    // every launch's queue is replayed on the host before it is uploaded (a few microseconds per thousand
+    // segments; a queue that fails it could leave a wait unmet)
+    {
+        Status qs = check_queue(hs, nseg, hp, npacks, hi, nitems, nentries, pos);
+        if (!qs.ok() || host_only) return qs;
+    }
     err = hipMemcpy(pb.tables, host.data(), table_bytes, hipMemcpyHostToDevice);
     if (err != hipSuccess) return Status::error(MISTRA_ERR_HIP, std::string("persistent batch upload: ") + hipGetErrorString(err));
     PersistArgs& a = pb.args;
@@ -610,9 +658,10 @@ Status prepare_persistent(int count, const mistra_contraction* e, hipStream_t st
     a.npacks = npacks;
     a.items = reinterpret_cast<const TileItem*>(T0 + o_items);
     a.nitems = nitems;
+    a.segs = reinterpret_cast<const Segment*>(T0 + o_segs);
+    a.nsegs = nseg;
     a.nentries = nentries;
-    a.njobs_pack = pack_job;
-    a.njobs = pack_job + tile_job;
+    a.njobs = pos;
     a.counters = reinterpret_cast<unsigned*>(T0 + o_cnt);
     a.spin_limit = kDefaultSpin;
     a.dep_bias = 0;

@@ -1,11 +1,12 @@
 // One persistent launch for a batch of contractions (mistra_batch_mult).
 //
-// The launch works through one device job queue: first the pack jobs, which write every leaf product's
-// operand sums (sum_t c_t A_t, sum_t c_t B_t -- one term for a classical product or a fringe pass) as
-// dense K-fast panels, then the GEMM tile jobs of every product, ordered by (round, contraction,
-// product).  Every tile job therefore runs the same kernel variant: a single-term leaf over K-fast
-// operands with its K-tiles loaded straight into LDS, and the multi-C epilogue on the contraction's own
-// C.  A tile waits (bounded) until its product's panels are packed and its contraction's earlier rounds
+// The launch works through one device job queue of two kinds of jobs: pack jobs, which write every leaf
+// product's operand sums (sum_t c_t A_t, sum_t c_t B_t -- one term for a classical product or a fringe
+// pass) as dense K-fast panels, and the GEMM tile jobs of every product.  The host orders the products
+// longest remaining chain first and queues each panel's pack jobs well ahead of the product's tiles, so
+// memory-bound packing runs beside MFMA-bound tiles on the same CUs.  Every tile job runs the same kernel
+// variant: a single-term leaf over K-fast operands with its K-tiles loaded straight into LDS, and the
+// multi-C epilogue on the contraction's own C.  A tile waits (bounded) until its product's panels are packed and its contraction's earlier rounds
 // are written: the products of one round write disjoint parts of C, rounds that write the same part
 // follow each other, as the Strassen stages of one contraction do (src/stra_internal/3t/stra_mult.cxx:
 // 445-592 runs the 7 products one after another).  A job waits only for jobs with smaller indices, which
@@ -13,8 +14,17 @@
 // the counters for the next launch (a HIP-graph replay needs no memset).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "batch_persist.hpp"
 #include "leaf_gemm_kernel.hpp"
+
+#include <map>
+#include <mutex>
+
+// Waits of the stage launches that gave up (every launch on the device adds to it; read and cleared by
+// the host).  Host code reaches it only through the address hipGetSymbolAddress returns per device.
+static __device__ unsigned stage_timeouts;
 
 namespace mistra {
 
@@ -259,7 +269,7 @@ __device__ __forceinline__ int find_record(const R* __restrict__ recs, int n, in
 // kernel of its own.
 template <typename T, int TM>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void batch_persistent_kernel(
-    PersistArgs a, const PackDesc* __restrict__ packs, const TileItem* __restrict__ items)
+    PersistArgs a, const PackDesc* __restrict__ packs, const TileItem* __restrict__ items, const Segment* __restrict__ segs)
 {
     constexpr int TK = sizeof(T) == 8 ? 16 : 32;
     __shared__ int64_t s_job;
@@ -280,10 +290,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void b
         const int64_t j = int64_t(__builtin_amdgcn_readfirstlane(unsigned(s_job)));
         __syncthreads();  // every thread has read s_job before wave 0 takes the next job
         if (j >= a.njobs) break;
-        if (j < a.njobs_pack)
+        const Segment& S = segs[find_record(segs, a.nsegs, j)];
+        if (S.kind == 0)
         {
-            const PackDesc& P = packs[find_record(packs, a.npacks, j)];
-            pack_job<T>(P, j - P.first_job, tile);
+            const PackDesc& P = packs[S.index];
+            pack_job<T>(P, j - S.first_job, tile);
             // every thread's panel stores have completed (they are device-coherent) before the count says so
             __builtin_amdgcn_s_waitcnt(0);
             __syncthreads();
@@ -291,8 +302,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void b
         }
         else
         {
-            const int64_t t = j - a.njobs_pack;
-            const int i = find_record(items, a.nitems, t);
+            const int i = S.index;
             const TileItem& it = items[i];
             if (wave == 0)
             {
@@ -307,7 +317,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void b
             // 128-byte lines that no other job shares, and read only after its count is complete, so no L2
             // holds an older copy of it.
             const GemmArgs g = uniform_args(it.g);
-            leaf_gemm_body<T, TM, TM, TK, 1, false, false, EPI_MULTI_C, KMODE_AFFINE_VEC>(g, int(t - it.first_job), 0, 0);
+            leaf_gemm_body<T, TM, TM, TK, 1, false, false, EPI_MULTI_C, KMODE_AFFINE_VEC>(g, int(j - S.first_job), 0, 0);
             // C writes a later round reads (device-coherent stores) have completed before the count says so
             if (it.release) __builtin_amdgcn_s_waitcnt(0);
             __syncthreads();
@@ -330,10 +340,108 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void b
     }
 }
 
+// Every stage product of one staged ABC plan in one launch (StageArgs).  g0 holds the arguments the
+// products share; a job swaps in its product's table entry.
+template <typename T, int TM>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void stage_persistent_kernel(
+    StageArgs a, GemmArgs g0, const StageProd* __restrict__ prods)
+{
+    constexpr int TK = sizeof(T) == 8 ? 16 : 32;
+    __shared__ int s_pick[3];
+    unsigned* const queue = a.counters;
+    unsigned* const timeouts = a.timeouts;
+    unsigned* const picks = a.counters + kStageHeader;
+    unsigned* const tiles = picks + int64_t(kStageXcds) * a.nprod;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
+    const int xcd = int(blockIdx.x % kStageXcds);
+    const unsigned ntiles = unsigned(a.tiles_m) * unsigned(a.tiles_n);
+    const unsigned njobs = ntiles * unsigned(a.nprod);
+    for (;;)
+    {
+        if (wave == 0)
+        {
+            const unsigned j = bump(queue, lane);
+            int prod = -1, tm = 0, tn = 0;
+            if (j < njobs)
+            {
+                prod = int(j / ntiles);
+                // a tile of this product: from this XCD's block, else from the next block with tiles left
+                // (the product has as many jobs as tiles, so some block always has one)
+                for (int k = 0; k < kStageXcds; k++)
+                {
+                    const int x = (xcd + k) % kStageXcds;
+                    const int r0 = a.rb[x / a.pc], r1 = a.rb[x / a.pc + 1];
+                    const int c0 = a.cb[x % a.pc], c1 = a.cb[x % a.pc + 1];
+                    const unsigned len = unsigned(r1 - r0) * unsigned(c1 - c0);
+                    if (len == 0) continue;
+                    const unsigned v = bump(&picks[int64_t(prod) * kStageXcds + x], lane);
+                    if (v < len)
+                    {
+                        tm = r0 + int(v / unsigned(c1 - c0));
+                        tn = c0 + int(v % unsigned(c1 - c0));
+                        break;
+                    }
+                }
+                const StageProd& P = prods[prod];
+                const int64_t at = int64_t(tm) * a.tiles_n + tn;
+                for (int c = 0; c < P.nC; c++)
+                    wait_count(&tiles[P.cbase[c] + at], P.need[c] + a.dep_bias, a.spin_limit, timeouts, lane);
+            }
+            s_pick[0] = prod;
+            s_pick[1] = tm;
+            s_pick[2] = tn;
+        }
+        __syncthreads();
+        const int prod = __builtin_amdgcn_readfirstlane(s_pick[0]);
+        const int tm = __builtin_amdgcn_readfirstlane(s_pick[1]);
+        const int tn = __builtin_amdgcn_readfirstlane(s_pick[2]);
+        __syncthreads();  // every thread has read the pick before wave 0 takes the next job
+        if (prod < 0) break;
+        const StageProd& P = prods[prod];
+        GemmArgs g = g0;
+        g.prods = g0.prods + prod;
+        leaf_gemm_body<T, TM, TM, TK, 1, false, false, EPI_MULTI_C, KMODE_AFFINE_VEC>(g, 0, 0, 0, tm, tn);
+        // C stores a later product reads (device-coherent stores) have completed before the counts say so
+        if (P.release) __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        if (wave == 0)
+        {
+            const int64_t at = int64_t(tm) * a.tiles_n + tn;
+            for (int c = 0; c < P.nC; c++) (void)bump(&tiles[P.cbase[c] + at], lane);
+        }
+    }
+}
+
+__global__ void zero_u32_kernel(unsigned* p, int64_t n)
+{
+    for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) p[i] = 0u;
+}
+
+template <typename T, int TM>
+hipError_t launch_stage_t(const StageArgs& a, const GemmArgs& g, int grid, hipStream_t s)
+{
+    hipLaunchKernelGGL((stage_persistent_kernel<T, TM>), dim3(unsigned(grid)), dim3(256), 0, s, a, g, a.prods);
+    return hipGetLastError();
+}
+
+template <typename T, int TM>
+int stage_occupancy_t()
+{
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void*>(&stage_persistent_kernel<T, TM>),
+                                                     256, 0) != hipSuccess)
+    {
+        (void)hipGetLastError();
+        return 1;
+    }
+    return n > 0 ? n : 1;
+}
+
 template <typename T, int TM>
 hipError_t launch_t(const PersistArgs& a, int grid, hipStream_t s)
 {
-    hipLaunchKernelGGL((batch_persistent_kernel<T, TM>), dim3(unsigned(grid)), dim3(256), 0, s, a, a.packs, a.items);
+    hipLaunchKernelGGL((batch_persistent_kernel<T, TM>), dim3(unsigned(grid)), dim3(256), 0, s, a, a.packs, a.items, a.segs);
     return hipGetLastError();
 }
 
@@ -356,6 +464,59 @@ hipError_t launch_batch_persistent(int elem, int tile, const PersistArgs& a, int
 {
     if (elem == 8) return tile == TILE_64 ? launch_t<double, 64>(a, grid, s) : launch_t<double, 128>(a, grid, s);
     return tile == TILE_64 ? launch_t<float, 64>(a, grid, s) : launch_t<float, 128>(a, grid, s);
+}
+
+hipError_t launch_stage_persistent(int elem, int tile, const StageArgs& a, const GemmArgs& g, int grid, hipStream_t s)
+{
+    if (elem == 8) return tile == TILE_64 ? launch_stage_t<double, 64>(a, g, grid, s) : launch_stage_t<double, 128>(a, g, grid, s);
+    return tile == TILE_64 ? launch_stage_t<float, 64>(a, g, grid, s) : launch_stage_t<float, 128>(a, g, grid, s);
+}
+
+int stage_persistent_occupancy(int elem, int tile)
+{
+    if (elem == 8) return tile == TILE_64 ? stage_occupancy_t<double, 64>() : stage_occupancy_t<double, 128>();
+    return tile == TILE_64 ? stage_occupancy_t<float, 64>() : stage_occupancy_t<float, 128>();
+}
+
+hipError_t stage_timeouts_word(unsigned** p)
+{
+    static std::mutex mu;
+    static std::map<int, unsigned*> at;  // device -> address of its copy of stage_timeouts
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = at.find(dev);
+    if (it == at.end())
+    {
+        void* q = nullptr;
+        e = hipGetSymbolAddress(&q, HIP_SYMBOL(stage_timeouts));
+        if (e != hipSuccess) return e;
+        it = at.emplace(dev, static_cast<unsigned*>(q)).first;
+    }
+    *p = it->second;
+    return hipSuccess;
+}
+
+hipError_t stage_timeouts_take(unsigned* n)
+{
+    unsigned* p = nullptr;
+    hipError_t e = stage_timeouts_word(&p);
+    if (e == hipSuccess) e = hipMemcpy(n, p, sizeof(unsigned), hipMemcpyDeviceToHost);
+    if (e == hipSuccess && *n)
+    {
+        const unsigned zero = 0;
+        e = hipMemcpy(p, &zero, sizeof(unsigned), hipMemcpyHostToDevice);
+    }
+    return e;
+}
+
+hipError_t launch_zero_u32(unsigned* p, int64_t n, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    const int64_t blocks = std::min<int64_t>((n + 255) / 256, 1024);
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(unsigned(blocks)), dim3(256), 0, s, p, n);
+    return hipGetLastError();
 }
 
 int batch_persistent_occupancy(int elem, int tile)

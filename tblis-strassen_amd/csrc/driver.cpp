@@ -20,6 +20,7 @@
 #include <map>
 #include <mutex>
 #include <sstream>
+#include <tuple>
 #include <unordered_map>
 
 namespace mistra {
@@ -721,6 +722,123 @@ void make_stages(Plan& p, std::vector<int>& order)
     }
 }
 
+// Tables of the single persistent launch over every staged product (StageArgs): the products in launch
+// order with their first-writer masks for that order (stage_pdesc), per product the counter block of
+// each C target (one counter per tile of the grid), how many earlier products update that target (the
+// tile waits until they all have), and whether a later one reads it; the tile grid split into one block
+// per XCD, as square as the grid allows (an XCD's operand panel reads grow with its block's rows plus
+// columns).
+//
+// Launch order: a tile that has to wait holds its workgroup idle, so the products are ordered greedily
+// by when their targets are free in a simple model of the launch (the resident workgroups take tiles in
+// queue order at a steady rate; every tile takes one unit of time): next comes the product whose last
+// earlier writer of any of its targets finishes first, ties in stage order.
+void build_stage_persist(Plan& p, const std::vector<ProdDesc>& staged)
+{
+    p.stage_persist = false;
+    p.h_stage_prods.clear();
+    p.h_stage_pdesc.clear();
+    p.stage_targets = 0;
+    std::memset(&p.h_stage_args, 0, sizeof(p.h_stage_args));
+    const int n = int(staged.size());
+    if (!p.outer_staged || n < 2 || p.kf_inner.mode != KMODE_AFFINE_VEC) return;
+    for (int t : p.h_inner_stage_terms)
+        if (t != 1) return;
+    const int64_t te = tile_extent(p.tile);
+    const int64_t tm = (p.ms + te - 1) / te, tn = (p.ns + te - 1) / te;
+    // jobs and tile-counter indices stay far below 2^31 (the queue also counts one failed take per
+    // workgroup at the end)
+    if (tm * tn * n >= (int64_t(1) << 30)) return;
+    // targets of every product
+    std::vector<std::tuple<int64_t, int64_t, int64_t>> keys;
+    std::vector<std::vector<int>> tgt(static_cast<size_t>(n));
+    for (int j = 0; j < n; j++)
+    {
+        const ProdDesc& pd = staged[size_t(j)];
+        for (int t = 0; t < pd.nC; t++)
+        {
+            const auto key = std::make_tuple(pd.c[t].base, pd.c[t].roff, pd.c[t].coff);
+            size_t i = 0;
+            while (i < keys.size() && keys[i] != key) i++;
+            if (i == keys.size()) keys.push_back(key);
+            tgt[size_t(j)].push_back(int(i));
+        }
+    }
+    // launch order (see above); 512 resident workgroups (two per CU on 256 CUs)
+    std::vector<int> order;
+    {
+        const double step = double(tm * tn) / 512.0;  // time to take one product's tiles
+        std::vector<double> free_at(keys.size(), 0.0);
+        std::vector<char> used(size_t(n), 0);
+        double now = 0.0;
+        for (int k = 0; k < n; k++)
+        {
+            int best = -1;
+            double best_ready = 0.0;
+            for (int j = 0; j < n; j++)
+            {
+                if (used[size_t(j)]) continue;
+                double ready = 0.0;
+                for (int i : tgt[size_t(j)]) ready = std::max(ready, free_at[size_t(i)]);
+                if (best < 0 || ready < best_ready)
+                {
+                    best = j;
+                    best_ready = ready;
+                }
+            }
+            used[size_t(best)] = 1;
+            order.push_back(best);
+            const double end = std::max(now, best_ready) + step + 1.0;
+            for (int i : tgt[size_t(best)]) free_at[size_t(i)] = end;
+            now += step;
+        }
+    }
+    // products in launch order, first writers and dependencies for that order
+    std::vector<unsigned> writers(keys.size(), 0);
+    std::vector<int> last(keys.size(), -1);
+    for (int k = 0; k < n; k++)
+    {
+        const int j = order[size_t(k)];
+        ProdDesc pd = staged[size_t(j)];
+        StageProd sp;
+        std::memset(&sp, 0, sizeof(sp));
+        sp.nC = pd.nC;
+        pd.first_mask = 0;
+        for (int t = 0; t < pd.nC; t++)
+        {
+            const int i = tgt[size_t(j)][size_t(t)];
+            if (writers[size_t(i)] == 0) pd.first_mask |= 1 << t;
+            sp.cbase[t] = int64_t(i) * tm * tn;
+            sp.need[t] = writers[size_t(i)]++;
+            last[size_t(i)] = k;
+        }
+        p.h_stage_pdesc.push_back(pd);
+        p.h_stage_prods.push_back(sp);
+    }
+    for (int k = 0; k < n; k++)
+        for (int i : tgt[size_t(order[size_t(k)])]) p.h_stage_prods[size_t(k)].release |= last[size_t(i)] > k ? 1 : 0;
+    StageArgs& a = p.h_stage_args;
+    a.nprod = n;
+    a.tiles_m = int(tm);
+    a.tiles_n = int(tn);
+    int64_t best = -1;
+    for (int pr = 1; pr <= kStageXcds; pr *= 2)
+    {
+        const int pc = kStageXcds / pr;
+        const int64_t cost = (tm + pr - 1) / pr + (tn + pc - 1) / pc;
+        if (best < 0 || cost < best)
+        {
+            best = cost;
+            a.pr = pr;
+            a.pc = pc;
+        }
+    }
+    for (int i = 0; i <= a.pr; i++) a.rb[i] = int(tm * i / a.pr);
+    for (int i = 0; i <= a.pc; i++) a.cb[i] = int(tn * i / a.pc);
+    p.stage_targets = int(keys.size());
+    p.stage_persist = true;
+}
+
 // Builds the device image of a plan (the byte layout of every table) on the host; bind_image then points
 // the plan's device-table members into a copy of it at `base`.
 void build_image(Plan& p, PlanImage& img)
@@ -752,6 +870,7 @@ void build_image(Plan& p, PlanImage& img)
     for (size_t st = 0; p.outer_staged && st + 1 < p.h_stage_begin.size(); st++)
         p.h_inner_stage_terms.push_back(
             max_terms(inner_staged.data() + p.h_stage_begin[st], p.h_stage_begin[st + 1] - p.h_stage_begin[st]));
+    build_stage_persist(p, inner_staged);
     // dense K-fast index vectors of the outer-staged buffers
     std::vector<int64_t> os_sm, os_sk, os_tk, os_tn, os_skrun, os_tkrun;
     for (int64_t i = 0; i < p.os_m; i++) os_sm.push_back(i * p.os_k);
@@ -834,7 +953,9 @@ void build_image(Plan& p, PlanImage& img)
          i_ins = add(inner_staged.data(), inner_staged.size() * sizeof(ProdDesc)),
          i_osm = add(os_sm.data(), os_sm.size() * 8), i_osk = add(os_sk.data(), os_sk.size() * 8),
          i_oskr = add(os_skrun.data(), os_skrun.size() * 8), i_otk = add(os_tk.data(), os_tk.size() * 8),
-         i_otkr = add(os_tkrun.data(), os_tkrun.size() * 8), i_otn = add(os_tn.data(), os_tn.size() * 8);
+         i_otkr = add(os_tkrun.data(), os_tkrun.size() * 8), i_otn = add(os_tn.data(), os_tn.size() * 8),
+         i_spp = add(p.h_stage_prods.data(), p.h_stage_prods.size() * sizeof(StageProd)),
+         i_spd = add(p.h_stage_pdesc.data(), p.h_stage_pdesc.size() * sizeof(ProdDesc));
 
     img.bytes.assign(total, 0);
     for (const Item& it : items)
@@ -842,7 +963,7 @@ void build_image(Plan& p, PlanImage& img)
     const size_t idx[] = {i_a_m, i_a_k, i_a_kr, i_b_k, i_b_kr, i_b_n, i_c_m, i_c_n, i_ba, i_bb, i_bc, i_pr, i_prf,
                           i_pst, i_pfst, i_sr, i_sc, i_tb, i_te, i_sm, i_sk, i_skm, i_skr, i_tk, i_tkr, i_tn, i_dp,
                           i_zero, i_os, i_mca, i_mcb, i_msa, i_mc1a, i_mc1b, i_msb, i_osf, i_ins, i_osm, i_osk,
-                          i_oskr, i_otk, i_otkr, i_otn};
+                          i_oskr, i_otk, i_otkr, i_otn, i_spp, i_spd};
     static_assert(sizeof(idx) / sizeof(idx[0]) == PlanImage::kParts, "one offset per table");
     for (int i = 0; i < PlanImage::kParts; i++) img.off[i] = items[idx[i]].off;
 }
@@ -855,7 +976,7 @@ void bind_image(Plan& p, const PlanImage& img, void* d)
     {
         i_a_m, i_a_k, i_a_kr, i_b_k, i_b_kr, i_b_n, i_c_m, i_c_n, i_ba, i_bb, i_bc, i_pr, i_prf, i_pst, i_pfst,
         i_sr, i_sc, i_tb, i_te, i_sm, i_sk, i_skm, i_skr, i_tk, i_tkr, i_tn, i_dp, i_zero, i_os, i_mca, i_mcb,
-        i_msa, i_mc1a, i_mc1b, i_msb, i_osf, i_ins, i_osm, i_osk, i_oskr, i_otk, i_otkr, i_otn, n_parts
+        i_msa, i_mc1a, i_mc1b, i_msb, i_osf, i_ins, i_osm, i_osk, i_oskr, i_otk, i_otkr, i_otn, i_spp, i_spd, n_parts
     };
     static_assert(n_parts == PlanImage::kParts, "one offset per table");
     auto at = [&](size_t i) { return reinterpret_cast<unsigned char*>(d) + img.off[i]; };
@@ -902,6 +1023,8 @@ void bind_image(Plan& p, const PlanImage& img, void* d)
     p.os_tk = reinterpret_cast<const int64_t*>(at(i_otk));
     p.os_tkrun = reinterpret_cast<const int64_t*>(at(i_otkr));
     p.os_tn = reinterpret_cast<const int64_t*>(at(i_otn));
+    p.h_stage_args.prods = reinterpret_cast<const StageProd*>(at(i_spp));
+    p.stage_pdesc = reinterpret_cast<const ProdDesc*>(at(i_spd));
 }
 
 namespace {
@@ -1162,8 +1285,28 @@ hipError_t ws_free(const Call& c, void* p, hipStream_t s)
     return c.ws ? hipSuccess : hipFreeAsync(p, s);
 }
 
-// Outer-staged ABC (Plan::outer_staged): the outer sums into S and T, then one launch per stage over the
-// inner sums of S and T (2 terms at level 2, 1 at level 1) with the multi-C epilogue on C.
+}  // namespace
+
+Status stage_wait_status()
+{
+    unsigned n = 0;
+    const hipError_t e = stage_timeouts_take(&n);
+    if (e != hipSuccess) return Status::error(MISTRA_ERR_HIP, std::string("stage launch wait count: ") + hipGetErrorString(e));
+    if (n)
+        return Status::error(MISTRA_ERR_HIP, "persistent stage launch: " + std::to_string(n) +
+                                                 " tile waits gave up (results are not valid)");
+    return Status();
+}
+
+namespace {
+
+// Polls a stage-launch wait makes before it gives up (each a coherent load and a short sleep: seconds)
+constexpr long long kStageSpin = 1ll << 22;
+
+// Outer-staged ABC (Plan::outer_staged): the outer sums into S and T, then the products over the inner
+// sums of S and T (2 terms at level 2 with one staged level, else 1) with the multi-C epilogue on C: one
+// launch per stage, or with MISTRA_FLAG_STAGE_PERSIST (when every product has one term per operand,
+// Plan::stage_persist) all of them in one persistent launch.
 hipError_t run_outer_staged(const Plan& p, const Call& c, hipStream_t stream)
 {
     const size_t nouter = p.h_outer_sums.size();
@@ -1258,7 +1401,48 @@ hipError_t run_outer_staged(const Plan& p, const Call& c, hipStream_t stream)
     g.zero = p.zero;
     g.ka_stride = p.kf_inner.sa;
     g.kb_stride = p.kf_inner.sb;
-    for (size_t st = 0; e == hipSuccess && st + 1 < p.h_stage_begin.size(); st++)
+    // every stage product in one persistent launch; its counters come from the workspace (a failed
+    // allocation leaves the per-stage launches below, which need none)
+    bool stages_done = false;
+    if (e == hipSuccess && p.stage_persist && (c.flags & MISTRA_FLAG_STAGE_PERSIST))
+    {
+        const StageArgs& t = p.h_stage_args;
+        const int64_t ncnt = stage_counters(t.nprod, p.stage_targets, t.tiles_m, t.tiles_n);
+        void* cnt = nullptr;
+        if (ws_alloc(c, &cnt, size_t(ncnt) * sizeof(unsigned), stream) == hipSuccess)
+        {
+            StageArgs a = t;
+            a.counters = static_cast<unsigned*>(cnt);
+            const long long spin = g_dbg_batch_spin.load();
+            a.spin_limit = spin > 0 ? spin : kStageSpin;
+            a.dep_bias = g_dbg_batch_bias.load();
+            e = stage_timeouts_word(&a.timeouts);
+            if (e == hipSuccess) e = launch_zero_u32(a.counters, ncnt, stream);
+            int dev = 0, cus = 256;
+            if (e == hipSuccess) e = hipGetDevice(&dev);
+            if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+            const int64_t jobs = int64_t(t.tiles_m) * t.tiles_n * t.nprod;
+            const int grid = int(std::max<int64_t>(1, std::min<int64_t>(int64_t(stage_persistent_occupancy(p.elem, p.tile)) * cus, jobs)));
+            GemmArgs gs = g;
+            gs.prods = p.stage_pdesc;
+            gs.nprod = 1;
+            gs.tiles_m = t.tiles_m;
+            gs.tiles_n = t.tiles_n;
+            gs.c_coherent = 1;
+            if (e == hipSuccess)
+            {
+                timing_begin(stream);
+                e = launch_stage_persistent(p.elem, p.tile, a, gs, grid, stream);
+                timing_end(stream, leaf_flops(p, t.nprod, p.ms, p.ns, p.ks));
+            }
+            const hipError_t f = ws_free(c, cnt, stream);
+            if (e == hipSuccess) e = f;
+            stages_done = true;
+        }
+        else
+            (void)hipGetLastError();
+    }
+    for (size_t st = 0; !stages_done && e == hipSuccess && st + 1 < p.h_stage_begin.size(); st++)
     {
         const int b = p.h_stage_begin[st], n = p.h_stage_begin[st + 1] - b;
         g.prods = p.inner_staged + b;

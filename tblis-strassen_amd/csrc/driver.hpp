@@ -10,6 +10,7 @@
 #include <string>
 #include <vector>
 
+#include "batch_persist.hpp"
 #include "kernels.hpp"
 #include "launch.hpp"
 #include "plan.hpp"
@@ -121,6 +122,16 @@ struct Plan
     const ProdDesc* inner_staged = nullptr;
     const int64_t *os_sm = nullptr, *os_sk = nullptr, *os_skrun = nullptr, *os_tk = nullptr, *os_tkrun = nullptr,
                   *os_tn = nullptr;
+    // The staged products in one persistent launch (stage_persistent_kernel, MISTRA_FLAG_STAGE_PERSIST)
+    // instead of one launch per stage: when every staged product has one term per operand and whole
+    // K-tiles.  h_stage_args holds
+    // everything but the per-call counters; stage_targets is the number of distinct C targets.
+    bool stage_persist = false;
+    std::vector<ProdDesc> h_stage_pdesc;   // the staged products in launch order (first writers for it)
+    const ProdDesc* stage_pdesc = nullptr;
+    std::vector<StageProd> h_stage_prods;
+    StageArgs h_stage_args{};
+    int stage_targets = 0;
 
     // device mirror (one allocation)
     void* dmem = nullptr;
@@ -162,12 +173,13 @@ struct Call
     void* C;
     double alpha, beta;
     Arena* ws = nullptr;  // null: workspace from hipMallocAsync; else from this arena (and no side stream)
+    unsigned flags = 0;   // the call's MISTRA_FLAG_* (MISTRA_FLAG_STAGE_PERSIST is read at run time)
 };
 
 // Device image of a plan: every table in one byte block (256-byte aligned parts) and the offset of each.
 struct PlanImage
 {
-    static constexpr int kParts = 43;
+    static constexpr int kParts = 45;
     std::vector<unsigned char> bytes;
     size_t off[kParts];
 };
@@ -186,6 +198,10 @@ Status get_plan(const tblis::tblis_tensor* A, const label_type* idx_A, const tbl
 std::string plan_key(const tblis::tblis_tensor* A, const label_type* idx_A, const tblis::tblis_tensor* B,
                      const label_type* idx_B, const tblis::tblis_tensor* C, const label_type* idx_C, int level,
                      int impl, int tile, unsigned flags, int dev);
+
+// After the stream has drained: an error if a persistent stage launch on this device gave up a wait
+// since the last check (its results are then not valid); clears the count.
+Status stage_wait_status();
 
 // The allocation-free entry point's device scratch block on the current device (noalloc.cpp): its
 // address (from hipGetSymbolAddress, looked up once per device) and size.
@@ -255,7 +271,7 @@ extern std::atomic<int> g_combine_per_sub;
 extern std::atomic<int> g_dbg_ozaki_slices, g_dbg_i8_slices, g_dbg_i8_fold;
 double scalar_value(const tblis::tblis_tensor* t);
 void set_scalar(tblis::tblis_tensor* t, double v);
-// test overrides of the persistent batch launch's waits (mistra_debug_set_batch_wait)
+// test overrides of the persistent launches' waits (mistra_debug_set_batch_wait)
 extern std::atomic<long long> g_dbg_batch_spin;
 extern std::atomic<unsigned> g_dbg_batch_bias;
 

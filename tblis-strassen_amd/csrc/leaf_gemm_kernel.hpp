@@ -179,8 +179,10 @@ __device__ __forceinline__ int64_t uniform64(int64_t x)
 }
 
 // The body of one workgroup: register tile bx of the launch's tile grid, product by, batch entry bz.
+// at_tm >= 0: the tile at (at_tm, at_tn) of the grid instead (callers that choose the tile themselves).
 template <typename T, int TM, int TN, int TK, int TERMS, bool A_MFAST, bool B_NFAST, int EPI, int KMODE>
-__device__ __forceinline__ void leaf_gemm_body(const GemmArgs& g, int bx, int by, int64_t bz)
+__device__ __forceinline__ void leaf_gemm_body(const GemmArgs& g, int bx, int by, int64_t bz, int at_tm = -1,
+                                               int at_tn = 0)
 {
     constexpr int NT = 256;
     constexpr int WM = TM / 2, WN = TN / 2;  // 2x2 waves
@@ -220,8 +222,8 @@ __device__ __forceinline__ void leaf_gemm_body(const GemmArgs& g, int bx, int by
     const int wm0 = (wave & 1) * WM;
     const int wn0 = (wave >> 1) * WN;
 
-    int tm, tn;
-    tile_coords(bx, g.tiles_m, g.tiles_n, tm, tn);
+    int tm = at_tm, tn = at_tn;
+    if (at_tm < 0) tile_coords(bx, g.tiles_m, g.tiles_n, tm, tn);
     const int64_t row0 = int64_t(tm) * TM;
     const int64_t col0 = int64_t(tn) * TN;
     const int64_t ms = g.ms, ns = g.ns, ks = g.ks;

@@ -25,6 +25,7 @@ FLAG_FUSED_SUMS = 8
 FLAG_OUTER_STAGED = 16
 FLAG_ALL_STAGED = 32
 FLAG_BATCH_GROUPED = 64
+FLAG_STAGE_PERSIST = 128
 
 
 class MistraError(RuntimeError):
@@ -354,9 +355,9 @@ def debug_set_lowp(bf16_slices=0, int8_slices=0, int8_fold_ksteps=0):
 
 
 def debug_set_batch_wait(spin_limit=0, dep_bias=0):
-    """Test hook (include/tblis_strassen_debug.h): polls before a wait of the persistent batch launch
-    gives up (0 = default) and a bias added to every wait's target (a positive one makes every wait
-    unreachable, so each gives up)."""
+    """Test hook (include/tblis_strassen_debug.h): polls before a wait of a persistent launch (batch, or
+    a staged plan's products) gives up (0 = default) and a bias added to every wait's target (a positive
+    one makes every wait unreachable, so each gives up)."""
     lib = load()
     lib.mistra_debug_set_batch_wait.argtypes = [ctypes.c_longlong, ctypes.c_uint]
     lib.mistra_debug_set_batch_wait(spin_limit, dep_bias)
