@@ -693,7 +693,9 @@ Status batch_persistent(int count, const mistra_contraction* entries, unsigned f
     int dev = 0;
     (void)hipGetDevice(&dev);
     std::ostringstream k;
-    k << dev << "#p" << count;
+    // per stream: the cached queue counters and panels serve one launch at a time, and calls on one stream
+    // are ordered, so calls of the same batch on different streams get their own
+    k << dev << "#p" << count << '@' << reinterpret_cast<uintptr_t>(stream);
     for (int i = 0; i < count; i++)
     {
         const mistra_contraction& x = entries[i];
