@@ -47,6 +47,20 @@ int mistra_debug_set_lowp(int bf16_slices, int int8_slices, int int8_fold_ksteps
  * spin_limit polls and the launch still ends).  Process-wide; returns 0. */
 int mistra_debug_set_batch_wait(long long spin_limit, unsigned dep_bias);
 
+/* The autotuner's candidates for a contraction shape (host-side model only, no device work): level, impl,
+ * register tile and modelled milliseconds of each, fastest first, up to cap entries.  Returns the number
+ * of candidates (0 for contractions that take the general path), or a negative error code. */
+int mistra_debug_tune_candidates(const tblis_tensor* A, const label_type* idx_A, const tblis_tensor* B,
+                                 const label_type* idx_B, const tblis_tensor* C, const label_type* idx_C, int* level,
+                                 int* impl, int* tile, double* model_ms, int cap);
+
+/* Register tile of the leaf launches of every later call that does not name one (plans built from then
+ * on; cached plans keep their tile): 0 = 128x128, 1 = 64x64, 2 = 128x64 (leaf rows x columns), 3 = 64x128
+ * (the rectangular tiles serve single-term launches; the plan's other launches take 128x128), -1 = the
+ * plan's own choice.  For tuning sweeps and tests.  Process-wide; returns 0, or MISTRA_ERR_ARG for an
+ * unknown tile. */
+int mistra_debug_set_tile(int tile);
+
 /* Builds the persistent batch launch's tables for these entries on the host alone (plans without device
  * tables, placeholder addresses; data pointers are not dereferenced) and replays its job queue in order:
  * every tile's waits must be met by the jobs before it and every product's pack count must come out

@@ -513,6 +513,8 @@ Status tensor_mult(const tblis::tblis_tensor* A, const label_type* idx_A, const 
         if (impl == MISTRA_IMPL_AUTO) impl = ch.impl;
         if (tile < 0) tile = ch.tile;
     }
+    if (tile < 0 && g_dbg_tile.load() >= 0) tile = g_dbg_tile.load();
+    if (tile >= kNumTiles) return Status::error(MISTRA_ERR_ARG, "unknown register tile");
     if (level > 2) return Status::error(MISTRA_ERR_ARG, "level must be 0, 1 or 2");
     if (impl < 0 || impl > 4) return Status::error(MISTRA_ERR_ARG, "impl must be in 0..4");
     // low-precision leaves always stage the operand sums (the Naive tables of the plan)

@@ -39,19 +39,49 @@ std::string shape_key(const tblis::tblis_tensor* A, const label_type* idx_A, con
 
 // Leaf-kernel rates (TFLOP/s) measured on MI355X with the 8192^3 FP64 workload, by register tile and
 // number of sum terms (1, 2, 4).  Single-term launches over K-fast operands (K-tiles straight into LDS)
-// run at 72.5 - 7645 / ks: 71.6 at a leaf depth of 8192 (the classical kernel), 68.8 at 2048 (the L=2
-// Naive dense launch) -- the per-tile start and end weigh more on shorter K loops.  FP32 runs the
-// single-term kernels at about 1.8 times the rate (FP32 MFMA has twice the FP64 rate); the multi-term
-// loaders' vector work is the same in both types and does not overlap the MFMAs, so FP32 multi-term
-// launches gain less: 2 terms measured 87 TFLOP/s (1.5x) at m = n = k = 4096 (c5 L1), 4 terms taken
-// at 1.3x.
-double leaf_rate(int elem, int tile, int terms, bool dma, int64_t ks)
+// run at 72.5 - 7645 / ks with 128 x 128 tiles: 71.6 at a leaf depth of 8192 (the classical kernel),
+// 68.8 at 2048 (the L=2 Naive dense launch) -- the per-tile start and end weigh more on shorter K loops.
+// FP32 runs the single-term kernels at about 1.8 times the rate (FP32 MFMA has twice the FP64 rate); the
+// multi-term loaders' vector work is the same in both types and does not overlap the MFMAs, so FP32
+// multi-term launches gain less: 2 terms measured 87 TFLOP/s (1.5x) at m = n = k = 4096 (c5 L1), 4 terms
+// taken at 1.3x.
+// The rectangular tiles (single-term launches only; rates net of the wave quantisation launch_s adds)
+// against 128 x 128: FP64 about 1.5% above it (64 x 128) or level (128 x 64) when the epilogue stores the
+// product (Naive's dense GEMM at 8192^3), 1.4% / 2.2% below it with the multi-C epilogue (the classical
+// launch), 2% above it in the register-staged form (an operand that is not K-fast); FP32 (classical at
+// m = n = k = 4096) 1.23x / 1.16x with both operands K-fast, 1.04x / 1.03x otherwise.
+double leaf_rate(int elem, int tile, int terms, bool dma, int64_t ks, bool store)
 {
     static const double r128[3] = {61.0, 57.5, 46.0}, r64[3] = {57.0, 46.0, 38.0}, f32[3] = {1.8, 1.5, 1.3};
     const int ti = terms <= 1 ? 0 : (terms == 2 ? 1 : 2);
+    const double dma128 = 72.5 - 7645.0 / double(std::max<int64_t>(ks, 512));
     double r = tile == TILE_64 ? r64[ti] : r128[ti];
-    if (tile != TILE_64 && ti == 0 && dma) r = std::max(r, 72.5 - 7645.0 / double(std::max<int64_t>(ks, 512)));
+    if (tile != TILE_64 && ti == 0 && dma) r = std::max(r, dma128);
+    if (tile_is_rect(tile) && ti == 0)
+    {
+        const bool wide = tile == TILE_64x128;
+        if (elem == 4) return f32[0] * r * (dma ? (wide ? 1.23 : 1.16) : (wide ? 1.037 : 1.034));
+        return r * (!dma ? 1.02 : store ? (wide ? 1.015 : 1.003) : (wide ? 0.986 : 0.978));
+    }
     return elem == 8 ? r : f32[ti] * r;
+}
+
+// Share of an ABC stage launch that the multi-C epilogue costs (C targets per product over the leaf
+// depth): the stages of an ABC launch sequence end all their workgroups' read-modify-writes of C at once.
+// Measured at m = n = k = 8192: -9% at L=2 (about 3 targets per product, 2048-deep leaves), nothing
+// visible at L=1 (1.7 targets, 4096 deep); with the rectangular tiles -21% and -3% (fewer columns share
+// each row's C read).
+double epilogue_penalty(int tile, double targets_per_product, int64_t ks)
+{
+    const double w = tile_is_rect(tile) ? 160.0 : 80.0;
+    return std::max(0.0, std::min(0.3, w * targets_per_product / double(std::max<int64_t>(ks, 1)) - 0.033));
+}
+
+// Resident workgroups per CU (LDS and registers of the kernel variants): 128 x 128 two, the rectangular
+// tiles three, 64 x 64 four.
+int64_t resident_slots(int tile, int cus = 256)
+{
+    return int64_t(tile == TILE_128 ? 2 : tile == TILE_64 ? 4 : 3) * cus;
 }
 
 constexpr double kHbm = 2.7e12;       // bytes/s of the gathering sum / combine kernels (strided gathers)
@@ -59,21 +89,20 @@ constexpr double kHbmSums = 4.5e12;   // the one-pass sum kernels (translate sub
 constexpr double kHbmComb = 4.0e12;   // the one-pass combine (every staged product read once)
 constexpr double kLaunch = 4.0e-6;    // seconds per launch (dispatch and tail)
 
-// `cpen`: the multi-C epilogue's share of the launch (C targets per product over the leaf depth): the
-// stages of an ABC launch sequence end all their workgroups' read-modify-writes of C at once; measured
-// -9% at L=2 (about 3 targets per product, 2048-deep leaves), nothing visible at L=1.
-double launch_s(int elem, int tile, int terms, bool dma, int64_t tiles, double flops, int64_t ks, double cpen = 0.0)
+// `cpen`: epilogue_penalty of an ABC stage launch; `store`: the epilogue stores the product (AB, Naive).
+double launch_s(int elem, int tile, int terms, bool dma, int64_t tiles, double flops, int64_t ks, double cpen,
+                bool store)
 {
     if (tiles <= 0 || flops <= 0) return 0.0;
-    const int64_t slots = tile == TILE_64 ? 1024 : 512;  // resident workgroups on 256 CUs
+    const int64_t slots = resident_slots(tile);
     const double waves = std::ceil(double(tiles) / double(slots)) * double(slots) / double(tiles);
-    return flops / (leaf_rate(elem, tile, terms, dma, ks) * 1e12 * (1.0 - cpen)) * waves + kLaunch;
+    return flops / (leaf_rate(elem, tile, terms, dma, ks, store) * 1e12 * (1.0 - cpen)) * waves + kLaunch;
 }
 
 int64_t ntiles(int tile, int64_t ms, int64_t ns)
 {
-    const int64_t t = tile_extent(tile);
-    return ((ms + t - 1) / t) * ((ns + t - 1) / t);
+    const int64_t tm = tile_m(tile), tn = tile_n(tile);
+    return ((ms + tm - 1) / tm) * ((ns + tn - 1) / tn);
 }
 
 }  // namespace
@@ -87,17 +116,23 @@ double model_ms(const Plan& p0)
     const int64_t nb = p.nbatch;
     // single-term launches over K-fast operands (both K-fast, or staged copies) load straight into LDS
     const bool kfast = p.outer_staged || (!p.a_mfast && !p.b_nfast);
-    auto gemm = [&](int64_t ms, int64_t ns, int64_t ks, int nprod, int terms, int t, bool dma, double cpen) {
+    // `per`: C targets per product of an ABC stage launch (0: no stage epilogue penalty)
+    auto gemm = [&](int64_t ms, int64_t ns, int64_t ks, int nprod, int terms, int t, bool dma, double per,
+                    bool store) {
+        // rectangular tiles serve single-term launches with affine K only (else the launch takes 128 x 128)
+        if (tile_is_rect(t) && terms > 1) t = TILE_128;
         // work of whole register tiles (a partly filled tile costs as much as a full one)
         const int64_t nt = ntiles(t, ms, ns) * nprod * nb;
-        const double te = double(tile_extent(t));
-        return launch_s(e, t, terms, dma, nt, 2.0 * te * te * double(ks) * double(nt), ks, cpen);
+        const double te = double(tile_m(t)) * double(tile_n(t));
+        const double cpen = per > 0.0 ? epilogue_penalty(t, per, ks) : 0.0;
+        return launch_s(e, t, terms, dma, nt, 2.0 * te * double(ks) * double(nt), ks, cpen, store);
     };
-    for (const Pass& f : p.fringe) s += gemm(f.ms, f.ns, f.ks, 1, 1, tile, kfast, 0.0);
+    // fringe passes (classical launches with 64 x 64 tiles; modelled with the plan's square tile)
+    for (const Pass& f : p.fringe) s += gemm(f.ms, f.ns, f.ks, 1, 1, tile_is_rect(tile) ? TILE_64 : tile, kfast, 0.0, false);
     if (p.nprod == 0) return s * 1e3;
     if (!p.core)
     {
-        s += gemm(p.ms, p.ns, p.ks, 1, 1, tile, kfast, 0.0);
+        s += gemm(p.ms, p.ns, p.ks, 1, 1, tile, kfast, 0.0, false);
         if (p.outer_staged)  // packed copies of A and B: read and written once each
             s += 2.0 * (double(p.ms) * p.ks + double(p.ks) * p.ns) * e / kHbm + 2 * kLaunch;
         return s * 1e3;
@@ -108,7 +143,7 @@ double model_ms(const Plan& p0)
                                               : (double(p.nprod) + 2.0 * p.nsub) * cbytes / kHbmComb;
     if (p.impl == IMPL_AB)
     {
-        s += gemm(p.ms, p.ns, p.ks, p.nprod, max_terms(p.h_prods.data(), p.nprod), tile, false, 0.0);
+        s += gemm(p.ms, p.ns, p.ks, p.nprod, max_terms(p.h_prods.data(), p.nprod), tile, false, 0.0, true);
         s += comb + kLaunch;
     }
     else if (p.impl == IMPL_NAIVE)
@@ -128,8 +163,9 @@ double model_ms(const Plan& p0)
         s += (one_a ? (double(p.d * p.d) + p.nprod) * sa * e * nb / kHbmSums : (ta + p.nprod) * sa * e * nb / kHbm);
         s += (one_b ? (double(p.d * p.d) + p.nprod) * sb * e * nb / kHbmSums : (tb + p.nprod) * sb * e * nb / kHbm);
         s += 2 * kLaunch;
-        s += gemm(p.ms, p.ns, p.ks, p.nprod, 1, p.ms * p.ns >= int64_t(1024) * 1024 ? TILE_128 : TILE_64,
-                  one_a && one_b, 0.0);
+        s += gemm(p.ms, p.ns, p.ks, p.nprod, 1,
+                  p.tile_given ? p.tile : (p.ms * p.ns >= int64_t(1024) * 1024 ? TILE_128 : TILE_64), one_a && one_b,
+                  0.0, true);
         s += comb + kLaunch;
     }
     else
@@ -148,11 +184,8 @@ double model_ms(const Plan& p0)
             int nct = 0;
             for (int j = p.h_stage_begin[st]; j < p.h_stage_begin[st + 1]; j++) nct += p.h_prods[size_t(order[size_t(j)])].nC;
             const double per = double(nct) / double(p.h_stage_begin[st + 1] - p.h_stage_begin[st]);
-            // (80 per / ks - 0.033: 0 at L=1 of 8192^3, about 1.7 targets per product over 4096; 0.08 at
-            // L=2, about 3 over 2048)
-            const double cpen = std::min(0.3, 80.0 * per / double(std::max<int64_t>(p.ks, 1)) - 0.033);
             s += gemm(p.ms, p.ns, p.ks, p.h_stage_begin[st + 1] - p.h_stage_begin[st], terms, tile, kfast && terms == 1,
-                      std::max(0.0, cpen));
+                      per, false);
         }
         if (p.outer_staged)
         {
@@ -176,6 +209,49 @@ double model_ms(const Plan& p0)
     return s * 1e3;
 }
 
+// The classical launch, staged ABC stages, Naive's dense GEMM.
+bool has_single_term_launch(const Plan& p)
+{
+    if (!p.core) return true;
+    if (p.impl == IMPL_NAIVE) return true;
+    if (p.impl == IMPL_ABC && p.outer_staged)
+    {
+        for (int t : p.h_inner_stage_terms)
+            if (t != 1) return false;
+        return true;
+    }
+    return false;
+}
+
+Status tune_candidates(const tblis::tblis_tensor* A, const label_type* idx_A, const tblis::tblis_tensor* B,
+                       const label_type* idx_B, const tblis::tblis_tensor* C, const label_type* idx_C,
+                       std::vector<TuneCand>& cands, bool& general)
+{
+    cands.clear();
+    general = false;
+    for (int level = 0; level <= 2; level++)
+        for (int impl : {int(IMPL_ABC), int(IMPL_AB), int(IMPL_NAIVE)})
+        {
+            if (level == 0 && impl != IMPL_ABC) continue;
+            for (int tile : {int(TILE_128), int(TILE_64), int(TILE_128x64), int(TILE_64x128)})
+            {
+                Plan p;
+                Status st = build_plan_host(A, idx_A, B, idx_B, C, idx_C, level, impl, tile, 0, p);
+                if (!st.ok()) return st;
+                if (p.general)
+                {
+                    general = true;
+                    return Status();
+                }
+                if (level > 0 && !p.core) continue;  // would only repeat the classical launch
+                if (tile_is_rect(tile) && !has_single_term_launch(p)) continue;  // the same launches as 128 x 128
+                cands.push_back(TuneCand{level, impl, tile, model_ms(p)});
+            }
+        }
+    std::sort(cands.begin(), cands.end(), [](const TuneCand& a, const TuneCand& b) { return a.ms < b.ms; });
+    return Status();
+}
+
 Status autotune(const tblis::tblis_tensor* A, const label_type* idx_A, const tblis::tblis_tensor* B,
                 const label_type* idx_B, const tblis::tblis_tensor* C, const label_type* idx_C, bool measure,
                 hipStream_t stream, TuneChoice& out)
@@ -190,32 +266,16 @@ Status autotune(const tblis::tblis_tensor* A, const label_type* idx_A, const tbl
             return Status();
         }
     }
-    struct Cand
+    std::vector<TuneCand> cands;
+    bool general = false;
+    Status st0 = tune_candidates(A, idx_A, B, idx_B, C, idx_C, cands, general);
+    if (!st0.ok()) return st0;
+    if (general)
     {
-        int level, impl, tile;
-        double ms;
-    };
-    std::vector<Cand> cands;
-    for (int level = 0; level <= 2; level++)
-        for (int impl : {int(IMPL_ABC), int(IMPL_AB), int(IMPL_NAIVE)})
-        {
-            if (level == 0 && impl != IMPL_ABC) continue;
-            for (int tile : {int(TILE_128), int(TILE_64)})
-            {
-                Plan p;
-                Status st = build_plan_host(A, idx_A, B, idx_B, C, idx_C, level, impl, tile, 0, p);
-                if (!st.ok()) return st;
-                if (p.general)
-                {
-                    // the general path reduces operands first; its inner contraction takes the default
-                    out = TuneChoice{1, IMPL_ABC, -1, 0.0, false};
-                    return Status();
-                }
-                if (level > 0 && !p.core) continue;  // would only repeat the classical launch
-                cands.push_back(Cand{level, impl, tile, model_ms(p)});
-            }
-        }
-    std::sort(cands.begin(), cands.end(), [](const Cand& a, const Cand& b) { return a.ms < b.ms; });
+        // the general path reduces operands first; its inner contraction takes the default
+        out = TuneChoice{1, IMPL_ABC, -1, 0.0, false};
+        return Status();
+    }
     TuneChoice best{cands[0].level, cands[0].impl, cands[0].tile, cands[0].ms, false};
 
     if (measure)
@@ -252,7 +312,7 @@ Status autotune(const tblis::tblis_tensor* A, const label_type* idx_A, const tbl
         Cs.data = static_cast<char*>(scratch) - lo * elem;
         double best_ms = 1e30;
         Status st;
-        for (const Cand& c : cands)
+        for (const TuneCand& c : cands)
         {
             double ms_min = 1e30;
             for (int r = 0; r < 4 && st.ok(); r++)

@@ -95,5 +95,31 @@ int mistra_debug_set_lowp(int bf16_slices, int int8_slices, int int8_fold_ksteps
     return 0;
 }
 
+int mistra_debug_tune_candidates(const tblis_tensor* A, const label_type* idx_A, const tblis_tensor* B,
+                                 const label_type* idx_B, const tblis_tensor* C, const label_type* idx_C, int* level,
+                                 int* impl, int* tile, double* model_ms, int cap)
+{
+    std::vector<mistra::TuneCand> cands;
+    bool general = false;
+    const mistra::Status st = mistra::tune_candidates(A, idx_A, B, idx_B, C, idx_C, cands, general);
+    if (!st.ok()) return st.code;
+    const int n = int(cands.size());
+    for (int i = 0; i < n && i < cap; i++)
+    {
+        level[i] = cands[size_t(i)].level;
+        impl[i] = cands[size_t(i)].impl;
+        tile[i] = cands[size_t(i)].tile;
+        model_ms[i] = cands[size_t(i)].ms;
+    }
+    return n;
+}
+
+int mistra_debug_set_tile(int tile)
+{
+    if (tile < -1 || tile >= mistra::kNumTiles) return MISTRA_ERR_ARG;
+    mistra::g_dbg_tile.store(tile);
+    return 0;
+}
+
 }  // extern "C"
 }  // namespace tblis

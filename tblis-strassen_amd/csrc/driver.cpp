@@ -119,7 +119,9 @@ int forced_tile()
     static const int forced = [] {
         const char* e = std::getenv("MISTRA_TILE");
         if (!e) return -1;
-        return std::atoi(e) == 64 ? int(TILE_64) : std::atoi(e) == 128 ? int(TILE_128) : -1;
+        const std::string v(e);
+        return v == "64" ? int(TILE_64) : v == "128" ? int(TILE_128) : v == "128x64" ? int(TILE_128x64)
+                                                           : v == "64x128" ? int(TILE_64x128) : -1;
     }();
     return forced;
 }
@@ -137,6 +139,7 @@ int choose_tile(const Plan& p)
 int dense_tile(const Plan& p)
 {
     if (forced_tile() >= 0) return forced_tile();
+    if (p.tile_given) return p.tile;
     return p.ms * p.ns >= int64_t(1024) * 1024 ? TILE_128 : TILE_64;
 }
 
@@ -402,8 +405,8 @@ void choose_outer_staged(Plan& p, unsigned flags)
     }
     // small leaves: launches that do not fill the 256 CUs are latency-bound, so fewer sum terms gain
     // little against the extra sum launches and the workspace
-    const int64_t te = tile_extent(p.tile);
-    if (((p.ms + te - 1) / te) * ((p.ns + te - 1) / te) < 256) return;
+    const int64_t tm = tile_m(p.tile), tn = tile_n(p.tile);
+    if (((p.ms + tm - 1) / tm) * ((p.ns + tn - 1) / tn) < 256) return;
     int best = -1;
     double best_ms = model_ms(p);
     for (int depth = lo; depth <= hi; depth++)
@@ -460,6 +463,7 @@ void set_comb_mask(const Plan& p, CombineArgs& ca)
 
 std::atomic<int> g_combine_per_sub{0};
 std::atomic<int> g_dbg_ozaki_slices{0}, g_dbg_i8_slices{0}, g_dbg_i8_fold{0};
+std::atomic<int> g_dbg_tile{-1};
 
 Plan::~Plan()
 {
@@ -482,6 +486,7 @@ Status build_plan_host(const tblis::tblis_tensor* A, const label_type* idx_A, co
     p.level = level;
     p.impl = impl;
     p.tile = tile;
+    p.tile_given = tile >= 0;
     p.flags = flags;
 
     Grouping g;
@@ -741,7 +746,8 @@ void build_stage_persist(Plan& p, const std::vector<ProdDesc>& staged)
     p.stage_targets = 0;
     std::memset(&p.h_stage_args, 0, sizeof(p.h_stage_args));
     const int n = int(staged.size());
-    if (!p.outer_staged || n < 2 || p.kf_inner.mode != KMODE_AFFINE_VEC) return;
+    // (the persistent stage kernel has the square tiles only)
+    if (!p.outer_staged || n < 2 || p.kf_inner.mode != KMODE_AFFINE_VEC || tile_is_rect(p.tile)) return;
     for (int t : p.h_inner_stage_terms)
         if (t != 1) return;
     const int64_t te = tile_extent(p.tile);
@@ -1173,6 +1179,7 @@ Status plan_abc_jobs(const Plan& p, const Call& c, std::vector<LaunchJob>& out)
     if (p.general) return Status::error(MISTRA_ERR_UNSUPPORTED, "general index classes run outside a batch");
     if (p.outer_staged) return Status::error(MISTRA_ERR_UNSUPPORTED, "outer-staged plans run outside a batch");
     auto push = [&](GemmArgs g, int tile, int terms, int kmode, int round, double flops) {
+        if (tile_is_rect(tile)) tile = TILE_128;  // grouped launches have the square tiles only
         tile = effective_tile(tile, kmode, terms, p.a_mfast, p.b_nfast);
         const int te = tile_extent(tile);
         g.tiles_m = int((g.ms + te - 1) / te);

@@ -57,6 +57,7 @@ struct Plan
     int level = 0;
     int impl = IMPL_ABC;
     int tile = TILE_128;
+    bool tile_given = false;  // the tile came from the caller or the autotuner (also for Naive's dense GEMM)
     unsigned flags = 0;
     int64_t m = 0, n = 0, k = 0, nbatch = 1;
     int64_t d = 1, ms = 0, ns = 0, ks = 0;  // split factor and leaf extents
@@ -256,6 +257,19 @@ int best_level(const tblis::tblis_tensor* A, const label_type* idx_A, const tbli
                const label_type* idx_B, const tblis::tblis_tensor* C, const label_type* idx_C, int impl);
 // Modelled milliseconds of one candidate (host only).
 double model_ms(const Plan& p);
+// Whether a plan makes a single-term leaf launch with affine K addressing, the only launches the
+// rectangular tiles serve.
+bool has_single_term_launch(const Plan& p);
+// Every (level, impl, tile) the autotuner considers for a shape, with its modelled time, fastest first;
+// `general`: the contraction takes the general path (no candidates).
+struct TuneCand
+{
+    int level, impl, tile;
+    double ms;
+};
+Status tune_candidates(const tblis::tblis_tensor* A, const label_type* idx_A, const tblis::tblis_tensor* B,
+                       const label_type* idx_B, const tblis::tblis_tensor* C, const label_type* idx_C,
+                       std::vector<TuneCand>& cands, bool& general);
 
 // Batched contractions (batch.cpp).
 Status batch_mult(int count, const mistra_contraction* entries, unsigned flags, hipStream_t stream);
@@ -269,6 +283,8 @@ extern std::atomic<int> g_combine_per_sub;
 // test overrides of the Ozaki modes (mistra_debug_set_lowp; 0 = default): BF16 and INT8 digit counts,
 // and the INT8 GEMM's fold period in K-steps
 extern std::atomic<int> g_dbg_ozaki_slices, g_dbg_i8_slices, g_dbg_i8_fold;
+// register tile of every later plan built without an explicit one (mistra_debug_set_tile; -1 = off)
+extern std::atomic<int> g_dbg_tile;
 double scalar_value(const tblis::tblis_tensor* t);
 void set_scalar(tblis::tblis_tensor* t, double v);
 // test overrides of the persistent launches' waits (mistra_debug_set_batch_wait)

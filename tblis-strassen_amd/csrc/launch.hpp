@@ -11,12 +11,20 @@ namespace mistra {
 
 enum TileCfg
 {
-    TILE_128 = 0,  // 128x128 workgroup tile (2x2 waves of 64x64)
-    TILE_64 = 1    // 64x64 workgroup tile (2x2 waves of 32x32), for small or skinny leaves
+    TILE_128 = 0,     // 128x128 workgroup tile (2x2 waves of 64x64)
+    TILE_64 = 1,      // 64x64 workgroup tile (2x2 waves of 32x32), for small or skinny leaves
+    TILE_128x64 = 2,  // 128 rows (leaf m) x 64 columns (leaf n): 2x2 waves of 64x32, single-term launches
+    TILE_64x128 = 3,  // 64 x 128: 2x2 waves of 32x64, single-term launches
+    kNumTiles = 4
 };
 
+// Rows (leaf m) and columns (leaf n) of a workgroup tile; tile_extent is the side of the square tiles
+// (TILE_128, TILE_64), the only ones the batched and persistent launches use.
+int tile_m(int tile);
+int tile_n(int tile);
 int tile_extent(int tile);
 int tile_kdepth(int tile, int elem_size);
+bool tile_is_rect(int tile);
 
 // Leaf GEMM over scatter vectors with fused operand sums (leaf_gemm_kernel.hpp); `terms` is the
 // largest number of sub-blocks in any operand sum of the launch (1, 2 or 4).

@@ -5,17 +5,17 @@
 
 namespace mistra {
 
-template <typename T, int TM, int TK, int TERMS, int EPI, int KF>
+template <typename T, int TM, int TN, int TK, int TERMS, int EPI, int KF>
 hipError_t launch_variant(const GemmArgs& g, bool a_mfast, bool b_nfast, hipStream_t s, dim3 grid)
 {
     if (a_mfast && b_nfast)
-        hipLaunchKernelGGL((leaf_gemm_kernel<T, TM, TM, TK, TERMS, true, true, EPI, KF>), grid, dim3(256), 0, s, g);
+        hipLaunchKernelGGL((leaf_gemm_kernel<T, TM, TN, TK, TERMS, true, true, EPI, KF>), grid, dim3(256), 0, s, g);
     else if (a_mfast)
-        hipLaunchKernelGGL((leaf_gemm_kernel<T, TM, TM, TK, TERMS, true, false, EPI, KF>), grid, dim3(256), 0, s, g);
+        hipLaunchKernelGGL((leaf_gemm_kernel<T, TM, TN, TK, TERMS, true, false, EPI, KF>), grid, dim3(256), 0, s, g);
     else if (b_nfast)
-        hipLaunchKernelGGL((leaf_gemm_kernel<T, TM, TM, TK, TERMS, false, true, EPI, KF>), grid, dim3(256), 0, s, g);
+        hipLaunchKernelGGL((leaf_gemm_kernel<T, TM, TN, TK, TERMS, false, true, EPI, KF>), grid, dim3(256), 0, s, g);
     else
-        hipLaunchKernelGGL((leaf_gemm_kernel<T, TM, TM, TK, TERMS, false, false, EPI, KF>), grid, dim3(256), 0, s, g);
+        hipLaunchKernelGGL((leaf_gemm_kernel<T, TM, TN, TK, TERMS, false, false, EPI, KF>), grid, dim3(256), 0, s, g);
     return hipGetLastError();
 }
 
@@ -67,9 +67,18 @@ hipError_t launch_grouped_typed(int tile, int terms, int kmode, const Grouped& q
 template <typename T, int TM, int TK, int EPI, int KF>
 hipError_t launch_terms(const GemmArgs& g, int terms, bool a_mfast, bool b_nfast, hipStream_t s, dim3 grid)
 {
-    if (terms <= 1) return launch_variant<T, TM, TK, 1, EPI, KF>(g, a_mfast, b_nfast, s, grid);
-    if (terms == 2) return launch_variant<T, TM, TK, 2, EPI, KF>(g, a_mfast, b_nfast, s, grid);
-    return launch_variant<T, TM, TK, 4, EPI, KF>(g, a_mfast, b_nfast, s, grid);
+    if (terms <= 1) return launch_variant<T, TM, TM, TK, 1, EPI, KF>(g, a_mfast, b_nfast, s, grid);
+    if (terms == 2) return launch_variant<T, TM, TM, TK, 2, EPI, KF>(g, a_mfast, b_nfast, s, grid);
+    return launch_variant<T, TM, TM, TK, 4, EPI, KF>(g, a_mfast, b_nfast, s, grid);
+}
+
+// Rectangular tiles (single-term launches with affine K addressing only; effective_tile maps every other
+// launch to a square tile).
+template <typename T, int TK, int EPI, int KF>
+hipError_t launch_rect(int tile, const GemmArgs& g, bool a_mfast, bool b_nfast, hipStream_t s, dim3 grid)
+{
+    if (tile == TILE_128x64) return launch_variant<T, 128, 64, TK, 1, EPI, KF>(g, a_mfast, b_nfast, s, grid);
+    return launch_variant<T, 64, 128, TK, 1, EPI, KF>(g, a_mfast, b_nfast, s, grid);
 }
 
 // One element type and epilogue; `kmode` selects the K addressing (KMode).  The general (per-term
@@ -79,6 +88,12 @@ hipError_t launch_leaf_typed(int tile, int terms, int kmode, const GemmArgs& g, 
                              hipStream_t s, dim3 grid)
 {
     constexpr int TK = sizeof(T) == 8 ? 16 : 32;
+    if (tile_is_rect(tile))
+    {
+        if (terms > 1 || (kmode != KMODE_AFFINE_VEC && kmode != KMODE_AFFINE)) return hipErrorInvalidValue;
+        return kmode == KMODE_AFFINE_VEC ? launch_rect<T, TK, EPI, KMODE_AFFINE_VEC>(tile, g, a_mfast, b_nfast, s, grid)
+                                         : launch_rect<T, TK, EPI, KMODE_AFFINE>(tile, g, a_mfast, b_nfast, s, grid);
+    }
     if (kmode == KMODE_GENERAL) return launch_terms<T, 64, TK, EPI, KMODE_GENERAL>(g, terms, a_mfast, b_nfast, s, grid);
     if (kmode == KMODE_GATHER)
         return tile == TILE_64 ? launch_terms<T, 64, TK, EPI, KMODE_GATHER>(g, terms, a_mfast, b_nfast, s, grid)

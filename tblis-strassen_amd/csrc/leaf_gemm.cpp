@@ -20,6 +20,8 @@ hipError_t launch_leaf_f32_grouped(int tile, int terms, int kmode, const Grouped
 int effective_tile(int tile, int kmode, int terms, bool a_mfast, bool b_nfast)
 {
     if (kmode == KMODE_GENERAL) return TILE_64;  // the general variant exists with the 64x64 tile only
+    // rectangular variants: single-term launches with affine K; a plan's other launches take 128 x 128
+    if (tile_is_rect(tile) && (terms > 1 || (kmode != KMODE_AFFINE_VEC && kmode != KMODE_AFFINE))) tile = TILE_128;
     // gathered K offsets cost registers per lane: these 128x128 variants would spill
     if (kmode == KMODE_GATHER && (terms >= 4 || (terms >= 2 && a_mfast && b_nfast))) return TILE_64;
     return tile;
@@ -34,7 +36,10 @@ hipError_t launch_leaf_grouped(int elem_size, int tile, int terms, int kmode, co
                           : launch_leaf_f32_grouped(tile, terms, kmode, q, a_mfast, b_nfast, stream);
 }
 
+int tile_m(int tile) { return tile == TILE_64 || tile == TILE_64x128 ? 64 : 128; }
+int tile_n(int tile) { return tile == TILE_64 || tile == TILE_128x64 ? 64 : 128; }
 int tile_extent(int tile) { return tile == TILE_64 ? 64 : 128; }
+bool tile_is_rect(int tile) { return tile == TILE_128x64 || tile == TILE_64x128; }
 
 int tile_kdepth(int tile, int elem_size)
 {
@@ -57,9 +62,9 @@ hipError_t launch_leaf_gemm(int elem_size, int epi, int tile, int terms, int kmo
                             bool b_nfast, hipStream_t stream)
 {
     tile = effective_tile(tile, kmode, terms, a_mfast, b_nfast);
-    const int te = tile_extent(tile);
-    g.tiles_m = int((g.ms + te - 1) / te);
-    g.tiles_n = int((g.ns + te - 1) / te);
+    const int tm = tile_m(tile), tn = tile_n(tile);
+    g.tiles_m = int((g.ms + tm - 1) / tm);
+    g.tiles_n = int((g.ns + tn - 1) / tn);
     if (g.tiles_m == 0 || g.tiles_n == 0 || g.nbatch == 0 || g.nprod == 0) return hipSuccess;
     if (g.ks <= 0) return hipErrorInvalidValue;  // callers route empty K to a scale of C
     if (g.ks / 16 >= int64_t(INT32_MAX)) return hipErrorInvalidValue;  // the kernels count K-tiles in 32 bits

@@ -203,9 +203,12 @@ __device__ __forceinline__ void leaf_gemm_body(const GemmArgs& g, int bx, int by
     constexpr int ROWB = TK * int(sizeof(T));
     constexpr bool kDma = KMODE == KMODE_AFFINE_VEC && !A_MFAST && !B_NFAST && TERMS == 1 && ROWB == 128 &&
                           (TM * ROWB) % (NT * 16) == 0 && (TN * ROWB) % (NT * 16) == 0;
-    // K buffers; the multi-C epilogue reuses sA as a 32-row staging tile (32 x (TN + 1) elements)
-    constexpr int SAN = cmax(cmax(TK * LDA, 16 * (TN + 1)), kDma ? TERMS * TM * TK : 0);
+    // K buffers; the multi-C epilogue reuses the A buffers as a 32-row staging tile (32 x (TN + 1)
+    // elements), or the B buffers when only those hold it (TN > TM)
+    constexpr int SAN0 = cmax(TK * LDA, kDma ? TERMS * TM * TK : 0);
     constexpr int SBN = cmax(TK * LDB, kDma ? TERMS * TN * TK : 0);
+    constexpr bool kStageB = 16 * (TN + 1) > SAN0 && 16 * (TN + 1) <= SBN;
+    constexpr int SAN = kStageB ? SAN0 : cmax(SAN0, 16 * (TN + 1));
     __shared__ __attribute__((aligned(16))) T sA[2][SAN];
     __shared__ __attribute__((aligned(16))) T sB[2][SBN];
     __shared__ int64_t ccol[EPI == EPI_MULTI_C ? kMaxTerms : 1][TN];  // C column offsets per target
@@ -259,8 +262,8 @@ __device__ __forceinline__ void leaf_gemm_body(const GemmArgs& g, int bx, int by
             // VGPRs, so the main loop keeps a small register budget.
             constexpr int LDE = TN + 1;
             constexpr int RROWS = 32, CSTEP = NT / RROWS, CPER = TN / CSTEP;
-            static_assert(2 * 16 * LDE <= 2 * SAN, "staging tile must fit the A buffers");
-            T* stage = sA[0];  // the K-tile buffers are free after the final barrier of the K loop
+            static_assert(2 * 16 * LDE <= 2 * (kStageB ? SBN : SAN), "staging tile must fit its buffers");
+            T* stage = kStageB ? sB[0] : sA[0];  // the K-tile buffers are free after the K loop's last barrier
             const double alpha = g.alpha, beta = g.beta;
             const int nC = P.nC;
             const int first_mask = P.first_mask;

@@ -221,6 +221,20 @@ def autotune(A, idx_A, B, idx_B, C, idx_C, measure=False, stream=None):
     return lv.value, im.value, ti.value
 
 
+def autotune_candidates(A, idx_A, B, idx_B, C, idx_C):
+    """[(level, impl, tile, model_ms)] the autotuner weighs for this shape, fastest modelled first
+    (mistra_debug_tune_candidates; host-side model, no device work)."""
+    lib = load()
+    tA, tB, tC = make_tensor(A), make_tensor(B), make_tensor(C)
+    cap = 64
+    lv, im, ti = (ctypes.c_int * cap)(), (ctypes.c_int * cap)(), (ctypes.c_int * cap)()
+    ms = (ctypes.c_double * cap)()
+    n = lib.mistra_debug_tune_candidates(ctypes.byref(tA), idx_A.encode(), ctypes.byref(tB), idx_B.encode(),
+                                         ctypes.byref(tC), idx_C.encode(), lv, im, ti, ms, cap)
+    _check(min(n, 0))
+    return [(lv[i], im[i], ti[i], ms[i]) for i in range(min(n, cap))]
+
+
 class Batch:
     """Keeps the tblis_tensor images of a batch alive for as long as the batch is reused (graph replays
     key on the tensors' pointers, shapes, strides and scalars)."""
@@ -352,6 +366,17 @@ def debug_set_lowp(bf16_slices=0, int8_slices=0, int8_fold_ksteps=0):
     rc = lib.mistra_debug_set_lowp(bf16_slices, int8_slices, int8_fold_ksteps)
     if rc != 0:
         raise MistraError(f"mistra_debug_set_lowp: error {rc}")
+
+
+def debug_set_tile(tile=-1):
+    """Register tile of later calls' plans (mistra_debug_set_tile): 0 128x128, 1 64x64, 2 128x64,
+    3 64x128, -1 the plan's own choice."""
+    lib = load()
+    lib.mistra_debug_set_tile.restype = ctypes.c_int
+    lib.mistra_debug_set_tile.argtypes = [ctypes.c_int]
+    rc = lib.mistra_debug_set_tile(int(tile))
+    if rc != 0:
+        raise MistraError(f"mistra_debug_set_tile: error {rc}")
 
 
 def debug_set_batch_wait(spin_limit=0, dep_bias=0):

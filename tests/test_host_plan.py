@@ -254,8 +254,9 @@ def test_autotuner_analytic_model():
 
 def test_autotuner_model_picks_measured_fastest_at_c2():
     # with c2's real (column-major, seeded-permutation) layout the model's costs follow the measured
-    # ones (classical 15.4 ms, L1 ABC staged 14.5, L2 ABC staged 14.5, L2 Naive 14.2 on MI355X) and it
-    # picks L2 Naive with 128 x 128 tiles; only the descriptors are read
+    # ones (classical 15.4 ms, L1 ABC staged 14.6, L2 ABC staged 14.6 (64 x 64 tiles), L2 Naive 13.9
+    # with 64 x 128 tiles, 14.1 with 128 x 128 on MI355X) and it picks L2 Naive with 64 x 128 tiles;
+    # only the descriptors are read
     import bench
 
     case = bench.make_workload("c2")
@@ -263,7 +264,8 @@ def test_autotuner_model_picks_measured_fastest_at_c2():
     A, B, C = (np.lib.stride_tricks.as_strided(buf, shape=case[f"shape_{x}"],
                                                strides=[8 * s for s in cases.colmajor_strides(case[f"shape_{x}"])])
                for x in "ABC")
-    assert ts.autotune(A, case["idx_A"], B, case["idx_B"], C, case["idx_C"]) == (2, ts.IMPL["naive"], 0)
+    choice = ts.autotune(A, case["idx_A"], B, case["idx_B"], C, case["idx_C"])
+    assert choice == (2, ts.IMPL["naive"], 3), choice
 
 
 def test_abc_stage_counts():
