@@ -409,14 +409,24 @@ def bench_tc(ts, torch, args, stream, workload, variant_spec):
     # launch instead of one leaf launch per stage)
     forms = {"": 0, "fused": ts.FLAG_FUSED_SUMS, "staged": ts.FLAG_OUTER_STAGED, "staged_all": ts.FLAG_ALL_STAGED,
              "persist": ts.FLAG_STAGE_PERSIST}
+    tiles = {0: "128x128", 1: "64x64", 2: "128x64", 3: "64x128"}
+    impls = {0: "abc", 3: "naive", 4: "ab"}
     for spec in variant_spec.split(","):
-        lv, impl, *form = spec.split(":")
-        form = form[0] if form else ""
+        choice = None
+        if spec == "auto":
+            # the autotuner's choice of level, impl and register tile (its analytic model), run as a caller
+            # asking for it would: level -1, impl "auto"
+            lv_c, im_c, ti_c = ts.autotune(A, case["idx_A"], B, case["idx_B"], C, case["idx_C"])
+            choice = {"level": lv_c, "impl": impls.get(im_c, im_c), "tile": tiles.get(ti_c, ti_c)}
+            lv, impl, form = "-1", "auto", ""
+        else:
+            lv, impl, *form = spec.split(":")
+            form = form[0] if form else ""
         flags = 0
         for f in form.split("+"):
             flags |= forms[f]
         level = int(lv)
-        name = f"L{level}_{impl}" + (f"_{form}" if form else "")
+        name = "auto" if choice else f"L{level}_{impl}" + (f"_{form}" if form else "")
         cbuf.zero_()
         ts.tensor_mult(1.0, A, case["idx_A"], B, case["idx_B"], 1.0, C, case["idx_C"], level=level, impl=impl,
                        stream=stream, flags=flags)
@@ -424,6 +434,8 @@ def bench_tc(ts, torch, args, stream, workload, variant_spec):
         cbuf.zero_()
         ms, kms, launches, kflops = run_variant(ts, torch, case, A, B, C, level, impl, args.steps, args.warmup,
                                                 stream, flags)
+        if choice:
+            level = choice["level"]
         variants[name] = {
             "level": level, "impl": impl, "form": form or "default", "ms_per_step": round(ms, 4),
             "eff_tflops": round(eff_flops / ms / 1e9, 3), "rel_err_vs_L0": err,
@@ -431,6 +443,8 @@ def bench_tc(ts, torch, args, stream, workload, variant_spec):
             "leaf_kernel_ms": round(kms, 4), "leaf_kernel_tflops": round(kflops / kms / 1e9, 3) if kms else None,
             "leaf_flops_per_launch": kflops, "leaf_launches_per_step": launches / args.steps,
         }
+        if choice:
+            variants[name]["autotuned"] = choice
         log(f"{name:10s} {ms:8.3f} ms  {eff_flops / ms / 1e9:7.2f} eff TFLOP/s  err {err:.2e}  "
             f"leaf {kms:.3f} ms/launch {kflops / kms / 1e9 if kms else 0:.2f} TFLOP/s")
 
@@ -498,10 +512,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS) + ["c2p", "c3", "c4", "c5"])
     ap.add_argument("--variants", default="0:abc,1:abc:staged,1:abc:fused,2:abc:staged_all,2:abc:staged,"
-                                          "2:abc:fused,1:ab,2:ab,1:naive,2:naive",
+                                          "2:abc:fused,1:ab,2:ab,1:naive,2:naive,auto",
                     help="level:impl[:form], form = fused (every operand sum in the leaf kernel), staged (the "
                          "outer level's sums staged in HBM) or staged_all (every level's sums staged); default: "
-                         "the library's choice")
+                         "the library's choice; `auto`: the autotuner's level, impl and register tile")
     ap.add_argument("--modes", default="", help="c2p / c3: precision:level list (default: all modes)")
     ap.add_argument("--precision-modes", default="fp64:0,ozaki_bf16:1,ozaki_bf16:2,ozaki_int8:1,ozaki_int8:2,"
                                                  "bf16x3:1",
