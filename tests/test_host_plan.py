@@ -308,3 +308,23 @@ def test_packed_classical_choice():
     small = cases.make_case(9, m=(64,), n=(64,), k=(64,))
     assert _debug_plan(small, 0, flags=ts.FLAG_OUTER_STAGED)[1][15] == 1
     assert _debug_plan(small, 0, flags=ts.FLAG_FUSED_SUMS)[1][15] == 0
+
+
+def test_tune_candidates_cover_four_tiles_where_launches_allow():
+    # the rectangular tiles serve single-term launches only: classical, Naive's dense GEMM, staged ABC;
+    # AB (operand sums formed in the leaf kernel) is offered with the square tiles alone
+    import bench
+
+    case = bench.make_workload("c2")
+    buf = np.zeros(1)
+    A, B, C = (np.lib.stride_tricks.as_strided(buf, shape=case[f"shape_{x}"],
+                                               strides=[8 * s for s in cases.colmajor_strides(case[f"shape_{x}"])])
+               for x in "ABC")
+    cands = ts.autotune_candidates(A, case["idx_A"], B, case["idx_B"], C, case["idx_C"])
+    got = {(lv, im, ti) for lv, im, ti, _ in cands}
+    naive, ab = ts.IMPL["naive"], ts.IMPL["ab"]
+    assert {(0, 0, t) for t in range(4)} <= got
+    assert {(2, naive, t) for t in range(4)} <= got
+    assert not any(lv > 0 and im == ab and ti >= 2 for lv, im, ti in got)
+    ms = [m for *_, m in cands]
+    assert ms == sorted(ms) and all(m > 0 for m in ms)
