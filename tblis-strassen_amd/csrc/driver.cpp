@@ -254,6 +254,19 @@ void build_msum(Plan& p, int depth, int64_t om, int64_t on, int64_t ok)
             p.h_msum_shift_b[size_t(src)] =
                 (p.h_b_k[size_t(rb * ok)] - p.h_b_k[0]) + (p.h_b_n[size_t(cb * on)] - p.h_b_n[0]);
     }
+    auto pairs = [](const std::vector<int64_t>& rs, int64_t rows, const std::vector<int64_t>& cs, int64_t cols,
+                    const std::vector<int64_t>& shift) {
+        if (rows % 2 != 0) return false;
+        for (int64_t x : shift)
+            if (x % 2 != 0) return false;
+        for (int64_t r = 0; r + 1 < rows; r += 2)
+            if (rs[size_t(r)] % 2 != 0 || rs[size_t(r + 1)] != rs[size_t(r)] + 1) return false;
+        for (int64_t c = 0; c < cols; c++)
+            if (cs[size_t(c)] % 2 != 0) return false;
+        return true;
+    };
+    p.msum_pair_a = p.msum_a && pairs(p.h_a_k, ok, p.h_a_m, om, p.h_msum_shift_a);
+    p.msum_pair_b = p.msum_b && pairs(p.h_b_k, ok, p.h_b_n, on, p.h_msum_shift_b);
     p.h_msum_coef_a.assign(one.size() * size_t(nsrc), 0.0);
     p.h_msum_coef_b.assign(one.size() * size_t(nsrc), 0.0);
     for (size_t o = 0; o < one.size(); o++)
@@ -1595,6 +1608,8 @@ Status run_plan(const Plan& p, const Call& c, hipStream_t stream)
                 set_msum_cv(p, isA, ma);
                 ma.shift = isA ? p.msum_shift_a : p.msum_shift_b;
                 ma.out = isA ? S : T;
+                ma.pair = (isA ? p.msum_pair_a : p.msum_pair_b) && ma.zoff % 2 == 0 &&
+                          (reinterpret_cast<uintptr_t>(ma.X) & uintptr_t(2 * p.elem - 1)) == 0;
                 return launch_operand_multisums(p.elem, ma, st);
             };
             SumArgs sa;
@@ -1780,7 +1795,8 @@ Status run_plan_lowp(const Plan& p, const Call& c, int precision, int nslices, h
                 set_msum_cv(p, isA, ma);
                 ma.shift = isA ? p.msum_shift_a : p.msum_shift_b;
                 ma.out = isA ? S.p : T.p;
-                ma.pair = false;
+                ma.pair = (isA ? p.msum_pair_a : p.msum_pair_b) && ma.zoff % 2 == 0 &&
+                          (reinterpret_cast<uintptr_t>(ma.X) & uintptr_t(15)) == 0;
                 return launch_operand_multisums(8, ma, stream);
             };
             if (s_kfast)

@@ -108,6 +108,9 @@ struct Plan
     // one-pass sums (launch_operand_multisums) when every source sub-block of A (B) is a translate:
     // coefficient matrix [nout][nsrc] over the sub-blocks of the staged split, and their shifts
     bool msum_a = false, msum_b = false;
+    // the one-pass sums may read two K rows per load: K comes in contiguous pairs at even offsets, and
+    // every column offset and source shift is even (the batch offset and base address are checked at launch)
+    bool msum_pair_a = false, msum_pair_b = false;
     int msum_nsrc = 0;
     std::vector<double> h_msum_coef_a, h_msum_coef_b;
     std::vector<double> h_msum_coef1_a, h_msum_coef1_b;  // 1-level 7 x 4 matrices (Kronecker factor, depth 2)
