@@ -216,9 +216,10 @@ bool has_single_term_launch(const Plan& p)
     if (p.impl == IMPL_NAIVE) return true;
     if (p.impl == IMPL_ABC && p.outer_staged)
     {
-        for (int t : p.h_inner_stage_terms)
-            if (t != 1) return false;
-        return true;
+        // the products over the staged sums (host plans have them before any device image is built)
+        for (const ProdDesc& q : p.h_inner)
+            if (q.nA > 1 || q.nB > 1) return false;
+        return !p.h_inner.empty();
     }
     return false;
 }
