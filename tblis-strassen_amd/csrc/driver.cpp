@@ -658,6 +658,19 @@ Status build_plan_host(const tblis::tblis_tensor* A, const label_type* idx_A, co
     p.h_c_m = std::move(c_m);
     p.h_c_n = std::move(c_n);
     choose_outer_staged(p, flags);
+    // Staged ABC whose stage launches are all single-term (level 1 staged, level 2 all staged) over leaves
+    // at most 2048 deep: 64 x 64 tiles, whose four resident workgroups per CU overlap the multi-C epilogues
+    // of short leaves better than two of 128 x 128 (measured eff. TFLOP/s: m = n = 16384, k = 1024 level 1
+    // 61.6 against 56.8 and level 2 39.7 against 35.6; FP32 4096^3 level 1 forward + adjoints 113.3
+    // against 107.5; 8192^3 level 2 level, 74.2 against 74.1; at 4096-deep leaves 128 x 128 stays ahead,
+    // 75.2 against 72.1)
+    if (!p.tile_given && forced_tile() < 0 && p.core && p.impl == IMPL_ABC && p.outer_staged && p.ks <= 2048 &&
+        p.tile == TILE_128)
+    {
+        bool single = !p.h_inner.empty();
+        for (const ProdDesc& q : p.h_inner) single = single && q.nA <= 1 && q.nB <= 1;
+        if (single) p.tile = TILE_64;
+    }
     // the low-precision leaves stage every sum of a Naive plan (depth = level) in one pass when they can
     if (p.core && p.impl == IMPL_NAIVE && !p.outer_staged) build_msum(p, p.level, p.ms, p.ns, p.ks);
     return Status();
