@@ -416,6 +416,7 @@ Status prepare_persistent(int count, const mistra_contraction* e, hipStream_t st
     int64_t t128 = 0;
     for (const HostItem& it : items) t128 += ((it.ms + 127) / 128) * ((it.ns + 127) / 128);
     pb.tile = t128 >= 2 * cus ? TILE_128 : TILE_64;
+    if (tile_override() == TILE_128 || tile_override() == TILE_64) pb.tile = tile_override();  // tuning (MISTRA_TILE)
     const int64_t te = tile_extent(pb.tile);
     const int64_t slots = int64_t(host_only ? 2 : batch_persistent_occupancy(pb.elem, pb.tile)) * cus;
 

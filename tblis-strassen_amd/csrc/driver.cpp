@@ -483,6 +483,8 @@ Plan::~Plan()
     if (dmem) (void)hipFree(dmem);
 }
 
+int tile_override() { return forced_tile(); }
+
 Status build_plan_host(const tblis::tblis_tensor* A, const label_type* idx_A, const tblis::tblis_tensor* B,
                        const label_type* idx_B, const tblis::tblis_tensor* C, const label_type* idx_C,
                        int level, int impl, int tile, unsigned flags, Plan& p)

@@ -260,6 +260,8 @@ int best_level(const tblis::tblis_tensor* A, const label_type* idx_A, const tbli
                const label_type* idx_B, const tblis::tblis_tensor* C, const label_type* idx_C, int impl);
 // Modelled milliseconds of one candidate (host only).
 double model_ms(const Plan& p);
+// MISTRA_TILE (tuning): the register tile every plan is forced to, or -1
+int tile_override();
 // Whether a plan makes a single-term leaf launch with affine K addressing, the only launches the
 // rectangular tiles serve.
 bool has_single_term_launch(const Plan& p);
