@@ -54,6 +54,11 @@ int mistra_debug_tune_candidates(const tblis_tensor* A, const label_type* idx_A,
                                  const label_type* idx_B, const tblis_tensor* C, const label_type* idx_C, int* level,
                                  int* impl, int* tile, double* model_ms, int cap);
 
+/* Low-precision plans of one level whose operand is K-fast: 1 (default) forms the operand's 7 sums and
+ * their digits in one launch; 0 stages the sums in HBM and slices them in a second launch.  Both give the
+ * same digits bit for bit; the switch exists to test that.  Process-wide; returns 0. */
+int mistra_debug_set_lowp_fusion(int fused);
+
 /* Register tile of the leaf launches of every later call that does not name one (plans built from then
  * on; cached plans keep their tile): 0 = 128x128, 1 = 64x64, 2 = 128x64 (leaf rows x columns), 3 = 64x128
  * (the rectangular tiles serve single-term launches; the plan's other launches take 128x128), -1 = the

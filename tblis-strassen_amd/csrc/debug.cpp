@@ -114,6 +114,12 @@ int mistra_debug_tune_candidates(const tblis_tensor* A, const label_type* idx_A,
     return n;
 }
 
+int mistra_debug_set_lowp_fusion(int fused)
+{
+    mistra::g_dbg_unfused_lowp.store(fused ? 0 : 1);
+    return 0;
+}
+
 int mistra_debug_set_tile(int tile)
 {
     if (tile < -1 || tile >= mistra::kNumTiles) return MISTRA_ERR_ARG;

@@ -477,6 +477,7 @@ void set_comb_mask(const Plan& p, CombineArgs& ca)
 std::atomic<int> g_combine_per_sub{0};
 std::atomic<int> g_dbg_ozaki_slices{0}, g_dbg_i8_slices{0}, g_dbg_i8_fold{0};
 std::atomic<int> g_dbg_tile{-1};
+std::atomic<int> g_dbg_unfused_lowp{0};
 
 Plan::~Plan()
 {
@@ -1767,7 +1768,17 @@ Status run_plan_lowp(const Plan& p, const Call& c, int precision, int nslices, h
         // than updating the C targets in the GEMM epilogue stage by stage: 70.8 against 65.2 TFLOP/s
         // for INT8 digits at 8192^3, 55.4 against 42.3 at m = n = 16384, k = 1024)
         // (FP8 products are staged in FP32: their E4M3 operands carry about 2^-4 relative error)
-        const size_t nS = size_t(nq * ms * ks) * 8, nT = size_t(nq * ks * ns) * 8,
+        // Operand sums in FP64.  When every term is a translate (kf_core) and an operand's K has unit
+        // stride, its sums are written with K fastest by the translate-form kernel (S as [q][i][k], like
+        // T), and S's row exponents and digits then use the contiguous-K kernels; a 1-level plan forms
+        // such an operand's 7 sums and their digits in one launch instead (no staged sums).
+        const bool s_kfast = !fp8 && p.kf_core.on && !p.a_mfast, t_shift = p.kf_core.on && !p.b_nfast;
+        const bool ms_a = p.core && p.msum_a && p.msum_nsrc == int(p.d * p.d);
+        const bool ms_b = p.core && p.msum_b && p.msum_nsrc == int(p.d * p.d);
+        const bool fuse = !fp8 && p.msum_nsrc == 4 && p.nprod == 7 && p.h_msum_coef_a.size() == 28 &&
+                          p.h_msum_coef_b.size() == 28 && !g_dbg_unfused_lowp.load();
+        const bool fuse_a = fuse && s_kfast && ms_a, fuse_b = fuse && t_shift && ms_b;
+        const size_t nS = fuse_a ? 8 : size_t(nq * ms * ks) * 8, nT = fuse_b ? 8 : size_t(nq * ks * ns) * 8,
                      nM = direct ? 8 : size_t(nq * ms * ns) * (fp8 ? 4 : 8);
         // FP8: one scale byte per row and 128-long K block; digits: one exponent per row
         const size_t nEA = fp8 ? size_t(nq * mp * (kp / 128)) : size_t(nq * ms) * 4;
@@ -1777,10 +1788,6 @@ Status run_plan_lowp(const Plan& p, const Call& c, int precision, int nslices, h
         for (auto pr : {std::make_pair(&S, nS), std::make_pair(&T, nT), std::make_pair(&M, nM), std::make_pair(&EA, nEA),
                         std::make_pair(&EB, nEB), std::make_pair(&XA, nXA), std::make_pair(&XB, nXB)})
             if (e == hipSuccess) e = hipMallocAsync(&pr.first->p, pr.second ? pr.second : 1, stream);
-        // Operand sums in FP64.  When every term is a translate (kf_core) and an operand's K has unit
-        // stride, its sums are written with K fastest by the translate-form kernel (S as [q][i][k], like
-        // T), and S's row exponents and digits then use the contiguous-K kernels.
-        const bool s_kfast = !fp8 && p.kf_core.on && !p.a_mfast, t_shift = p.kf_core.on && !p.b_nfast;
         for (int64_t z = 0; z < p.nbatch && e == hipSuccess; z++)
         {
             SumArgs sa;
@@ -1792,8 +1799,6 @@ Status run_plan_lowp(const Plan& p, const Call& c, int precision, int nslices, h
             sa.out = S.p;
             sa.zoff = p.h_batch_a[size_t(z)];
             // one pass over the source sub-blocks when the tables are there (Naive core plans)
-            const bool ms_a = p.core && p.msum_a && p.msum_nsrc == int(p.d * p.d);
-            const bool ms_b = p.core && p.msum_b && p.msum_nsrc == int(p.d * p.d);
             auto multisums = [&](bool isA) {
                 MultiSumArgs ma;
                 std::memset(&ma, 0, sizeof(ma));
@@ -1832,7 +1837,26 @@ Status run_plan_lowp(const Plan& p, const Call& c, int precision, int nslices, h
                 sa.cols = ks;
                 sa.rows_fast = p.a_mfast;
             }
-            e = s_kfast && ms_a ? multisums(true) : launch_operand_sums(8, sa, stream);
+            auto sum_slices = [&](bool isA) {
+                SumSliceArgs fa;
+                std::memset(&fa, 0, sizeof(fa));
+                fa.X = static_cast<const double*>(isA ? c.A : c.B);
+                fa.zoff = isA ? p.h_batch_a[size_t(z)] : p.h_batch_b[size_t(z)];
+                fa.rs = isA ? p.a_k : p.b_k;
+                fa.cs = isA ? p.a_m : p.b_n;
+                fa.ks = ks;
+                fa.rows = isA ? ms : ns;
+                fa.cp = isA ? mp : np;
+                fa.kp = kp;
+                const std::vector<int64_t>& sh = isA ? p.h_msum_shift_a : p.h_msum_shift_b;
+                const std::vector<double>& cf = isA ? p.h_msum_coef_a : p.h_msum_coef_b;
+                for (int i = 0; i < 4; i++) fa.shift[i] = sh[size_t(i)];
+                for (int i = 0; i < 28; i++) fa.coef[i] = cf[size_t(i)];
+                fa.e = static_cast<int*>(isA ? EA.p : EB.p);
+                fa.out = isA ? XA.p : XB.p;
+                return launch_ozaki_sum_slices(nsl, i8, fa, stream);
+            };
+            e = fuse_a ? sum_slices(true) : s_kfast && ms_a ? multisums(true) : launch_operand_sums(8, sa, stream);
             if (e != hipSuccess) break;
             sa.X = c.B;
             sa.rs = p.b_k;
@@ -1845,7 +1869,7 @@ Status run_plan_lowp(const Plan& p, const Call& c, int precision, int nslices, h
             sa.prods = t_shift ? p.prods_fast : p.prods;
             sa.out = T.p;
             sa.zoff = p.h_batch_b[size_t(z)];
-            e = t_shift && ms_b ? multisums(false) : launch_operand_sums(8, sa, stream);
+            e = fuse_b ? sum_slices(false) : t_shift && ms_b ? multisums(false) : launch_operand_sums(8, sa, stream);
             if (e != hipSuccess) break;
 
             LowpSliceArgs la;
@@ -1863,6 +1887,8 @@ Status run_plan_lowp(const Plan& p, const Call& c, int precision, int nslices, h
             la.outA = XA.p;
             la.outB = XB.p;
             la.s_kfast = s_kfast;
+            la.skip_a = fuse_a;
+            la.skip_b = fuse_b;
             LowpGemmArgs lg{ms, ns, mp, np, kp, p.nprod, nullptr, nullptr, nullptr, c.alpha, c.beta, 0, p.nprod,
                             g_dbg_i8_fold.load(), fp8 && !direct};
             if (direct)

@@ -290,6 +290,9 @@ extern std::atomic<int> g_combine_per_sub;
 extern std::atomic<int> g_dbg_ozaki_slices, g_dbg_i8_slices, g_dbg_i8_fold;
 // register tile of every later plan built without an explicit one (mistra_debug_set_tile; -1 = off)
 extern std::atomic<int> g_dbg_tile;
+// 1: low-precision plans stage their operand sums before slicing them, even where one launch could do both
+// (mistra_debug_set_lowp_fusion)
+extern std::atomic<int> g_dbg_unfused_lowp;
 double scalar_value(const tblis::tblis_tensor* t);
 void set_scalar(tblis::tblis_tensor* t, double v);
 // test overrides of the persistent launches' waits (mistra_debug_set_batch_wait)

@@ -22,7 +22,28 @@ struct LowpSliceArgs
     void* outA;
     void* outB;
     bool s_kfast;
+    // operands whose digits (and exponents) a fused sum-and-slice launch wrote already
+    bool skip_a, skip_b;
 };
+
+// One-pass operand sums and their digits in one launch, for a 1-level plan whose operand is K-fast with
+// translate sub-blocks: sum q (of 7) of row i at K position k is sum_s coef[q][s] X[zoff + shift[s] + rs[k] +
+// cs[i]] over the 4 sub-blocks s, summed in that order with zero coefficients skipped (the arithmetic of
+// the one-pass sums kernel); then each row's exponent and digits exactly as the K-fast slice kernel
+// derives them from the staged sums.  Rows are padded to cp and K to kp with zero digits.
+struct SumSliceArgs
+{
+    const double* X;
+    int64_t zoff;
+    const int64_t* rs;  // K offsets, ks of them
+    const int64_t* cs;  // row offsets, rows of them
+    int64_t ks, rows, cp, kp;
+    int64_t shift[4];
+    double coef[28];  // [q][s]
+    int* e;           // [q][rows] row exponents
+    void* out;        // digits [slice][q][cp][kp]
+};
+hipError_t launch_ozaki_sum_slices(int nslices, bool int8, const SumSliceArgs& a, hipStream_t s);
 
 struct LowpGemmArgs
 {

@@ -198,6 +198,86 @@ __global__ __launch_bounds__(256) void slice_kfast_kernel(const double* __restri
     }
 }
 
+// One-pass sums of a 1-level plan (7 sums of 4 translate sub-blocks, K-fast) and their digits in one
+// launch (SumSliceArgs): one wave per padded row i, which computes the row's 7 sums twice from the source
+// sub-blocks -- for the 7 largest magnitudes (exponents), then for the digits -- instead of staging the
+// sums in HBM and reading them back.  The sums and digits are bit for bit those of the one-pass sums
+// kernel followed by slice_kfast_kernel.
+template <int NSL, typename D>
+__global__ __launch_bounds__(256) void sum_slice_kfast_kernel(SumSliceArgs a)
+{
+    constexpr int NQ = 7, NS = 4;
+    const int lane = threadIdx.x & 63;
+    const int64_t slab = int64_t(NQ) * a.cp * a.kp;
+    D* __restrict__ out = static_cast<D*>(a.out);
+    for (int64_t i = (int64_t(blockIdx.x) * 256 + threadIdx.x) / 64; i < a.cp; i += int64_t(gridDim.x) * 4)
+    {
+        const bool real = i < a.rows;
+        const double* __restrict__ X = a.X + a.zoff + (real ? a.cs[i] : 0);
+        auto sums = [&](int64_t k, double (&y)[NQ]) {
+            const int64_t rk = a.rs[k];
+            double v[NS];
+#pragma unroll
+            for (int s = 0; s < NS; s++) v[s] = X[a.shift[s] + rk];
+#pragma unroll
+            for (int q = 0; q < NQ; q++)
+            {
+                double acc = 0.0;
+#pragma unroll
+                for (int s = 0; s < NS; s++)
+                {
+                    const double cf = a.coef[q * NS + s];  // uniform
+                    if (cf != 0.0) acc += cf * v[s];
+                }
+                y[q] = acc;
+            }
+        };
+        double mx[NQ];
+#pragma unroll
+        for (int q = 0; q < NQ; q++) mx[q] = 0.0;
+        if (real)
+            for (int64_t k = lane; k < a.ks; k += 64)
+            {
+                double y[NQ];
+                sums(k, y);
+#pragma unroll
+                for (int q = 0; q < NQ; q++) mx[q] = fmax(mx[q], fabs(y[q]));
+            }
+        int ex[NQ];
+#pragma unroll
+        for (int q = 0; q < NQ; q++)
+        {
+            double m = mx[q];
+            for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+            ex[q] = exp_of(m);
+            if (real && lane == 0) a.e[q * a.rows + i] = ex[q];
+        }
+        for (int64_t k = lane; k < a.kp; k += 64)
+        {
+            const bool in = real && k < a.ks;
+            double y[NQ];
+            if (in)
+                sums(k, y);
+            else
+#pragma unroll
+                for (int q = 0; q < NQ; q++) y[q] = 0.0;
+#pragma unroll
+            for (int q = 0; q < NQ; q++)
+            {
+                const double x = in ? ldexp(y[q], -ex[q]) : 0.0;
+                D d[NSL];
+                if constexpr (std::is_same<D, int8_t>::value)
+                    digits_i8<NSL>(x, d);
+                else
+                    digits<NSL>(x, d);
+                D* o = out + (int64_t(q) * a.cp + i) * a.kp + k;
+#pragma unroll
+                for (int sl = 0; sl < NSL; sl++) o[sl * slab] = d[sl];
+            }
+        }
+    }
+}
+
 // ---- epilogue -----------------------------------------------------------------------------------------
 
 // Element (row, col) of product q: staged into M_q[row + col * ms]; or, with the direct epilogue, C =
@@ -959,14 +1039,17 @@ unsigned grid_of(int64_t n, int per)
 template <int NSL>
 hipError_t slices_t(const LowpSliceArgs& a, hipStream_t s)
 {
-    if (a.s_kfast)  // S laid out as T: rows of contiguous K
+    if (a.skip_a)
+        ;
+    else if (a.s_kfast)  // S laid out as T: rows of contiguous K
         hipLaunchKernelGGL((slice_kfast_kernel<NSL, __bf16>), dim3(grid_of(int64_t(a.nprod) * a.mp, 4)), dim3(256), 0,
                            s, a.S, a.ks, a.ms, a.ea, static_cast<__bf16*>(a.outA), a.mp, a.kp, a.nprod);
     else
         hipLaunchKernelGGL(slice_rows_kernel<NSL>, dim3(unsigned(a.mp / 64), unsigned(a.kp / 64), unsigned(a.nprod)),
                            dim3(256), 0, s, a.S, a.ms, a.ks, a.ea, static_cast<__bf16*>(a.outA), a.mp, a.kp, a.nprod);
-    hipLaunchKernelGGL((slice_kfast_kernel<NSL, __bf16>), dim3(grid_of(int64_t(a.nprod) * a.np, 4)), dim3(256), 0, s,
-                       a.T, a.ks, a.ns, a.eb, static_cast<__bf16*>(a.outB), a.np, a.kp, a.nprod);
+    if (!a.skip_b)
+        hipLaunchKernelGGL((slice_kfast_kernel<NSL, __bf16>), dim3(grid_of(int64_t(a.nprod) * a.np, 4)), dim3(256), 0, s,
+                           a.T, a.ks, a.ns, a.eb, static_cast<__bf16*>(a.outB), a.np, a.kp, a.nprod);
     return hipGetLastError();
 }
 
@@ -993,14 +1076,17 @@ hipError_t gemm_t(const void* As, const void* Bs, const int* ea, const int* eb, 
 template <int NSL>
 hipError_t slices_i8_t(const LowpSliceArgs& a, hipStream_t s)
 {
-    if (a.s_kfast)
+    if (a.skip_a)
+        ;
+    else if (a.s_kfast)
         hipLaunchKernelGGL((slice_kfast_kernel<NSL, int8_t>), dim3(grid_of(int64_t(a.nprod) * a.mp, 4)), dim3(256), 0,
                            s, a.S, a.ks, a.ms, a.ea, static_cast<int8_t*>(a.outA), a.mp, a.kp, a.nprod);
     else
         hipLaunchKernelGGL(slice_rows_i8_kernel<NSL>, dim3(unsigned(a.mp / 64), unsigned(a.kp / 64), unsigned(a.nprod)),
                            dim3(256), 0, s, a.S, a.ms, a.ks, a.ea, static_cast<int8_t*>(a.outA), a.mp, a.kp, a.nprod);
-    hipLaunchKernelGGL((slice_kfast_kernel<NSL, int8_t>), dim3(grid_of(int64_t(a.nprod) * a.np, 4)), dim3(256), 0, s,
-                       a.T, a.ks, a.ns, a.eb, static_cast<int8_t*>(a.outB), a.np, a.kp, a.nprod);
+    if (!a.skip_b)
+        hipLaunchKernelGGL((slice_kfast_kernel<NSL, int8_t>), dim3(grid_of(int64_t(a.nprod) * a.np, 4)), dim3(256), 0, s,
+                           a.T, a.ks, a.ns, a.eb, static_cast<int8_t*>(a.outB), a.np, a.kp, a.nprod);
     return hipGetLastError();
 }
 
@@ -1057,13 +1143,40 @@ hipError_t launch_ozaki_i8_gemm(int nsl, const void* As, const void* Bs, const i
 hipError_t launch_lowp_exponents(const LowpSliceArgs& a, hipStream_t s)
 {
     // K-fast operands (T always, S when s_kfast) get their exponents in the slice pass itself
-    if (!a.s_kfast)
+    if (!a.s_kfast && !a.skip_a)
     {
         hipLaunchKernelGGL(row_exp_kernel, dim3(grid_of(int64_t(a.nprod) * a.ms, 256)), dim3(256), 0, s, a.S, a.ms,
                            a.ks, a.nprod, a.ea);
         return hipGetLastError();
     }
     return hipSuccess;
+}
+
+hipError_t launch_ozaki_sum_slices(int nsl, bool int8, const SumSliceArgs& a, hipStream_t s)
+{
+    if (a.cp <= 0 || a.kp <= 0) return hipSuccess;
+    const dim3 grid(grid_of(a.cp, 4));
+#define MISTRA_SUM_SLICES(N)                                                                              \
+    case N:                                                                                               \
+        if (int8)                                                                                         \
+            hipLaunchKernelGGL((sum_slice_kfast_kernel<N, int8_t>), grid, dim3(256), 0, s, a);           \
+        else                                                                                              \
+            hipLaunchKernelGGL((sum_slice_kfast_kernel<N, __bf16>), grid, dim3(256), 0, s, a);           \
+        return hipGetLastError();
+    switch (nsl)
+    {
+        MISTRA_SUM_SLICES(2)
+        MISTRA_SUM_SLICES(3)
+        MISTRA_SUM_SLICES(4)
+        MISTRA_SUM_SLICES(5)
+        MISTRA_SUM_SLICES(6)
+    case 7:
+        if (int8) return hipErrorInvalidValue;
+        hipLaunchKernelGGL((sum_slice_kfast_kernel<7, __bf16>), grid, dim3(256), 0, s, a);
+        return hipGetLastError();
+    default: return hipErrorInvalidValue;
+    }
+#undef MISTRA_SUM_SLICES
 }
 
 hipError_t launch_ozaki_slices(int nsl, const LowpSliceArgs& a, hipStream_t s)

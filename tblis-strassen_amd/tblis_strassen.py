@@ -368,6 +368,14 @@ def debug_set_lowp(bf16_slices=0, int8_slices=0, int8_fold_ksteps=0):
         raise MistraError(f"mistra_debug_set_lowp: error {rc}")
 
 
+def debug_set_lowp_fusion(fused=True):
+    """mistra_debug_set_lowp_fusion: 1-level low-precision operands form their sums and digits in one
+    launch (True, the default) or stage the sums first (False); the digits are the same either way."""
+    lib = load()
+    lib.mistra_debug_set_lowp_fusion.argtypes = [ctypes.c_int]
+    lib.mistra_debug_set_lowp_fusion(1 if fused else 0)
+
+
 def debug_set_tile(tile=-1):
     """Register tile of later calls' plans (mistra_debug_set_tile): 0 128x128, 1 64x64, 2 128x64,
     3 64x128, -1 the plan's own choice."""
