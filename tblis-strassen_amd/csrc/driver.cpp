@@ -1775,8 +1775,12 @@ Status run_plan_lowp(const Plan& p, const Call& c, int precision, int nslices, h
         const bool s_kfast = !fp8 && p.kf_core.on && !p.a_mfast, t_shift = p.kf_core.on && !p.b_nfast;
         const bool ms_a = p.core && p.msum_a && p.msum_nsrc == int(p.d * p.d);
         const bool ms_b = p.core && p.msum_b && p.msum_nsrc == int(p.d * p.d);
-        const bool fuse = !fp8 && p.msum_nsrc == 4 && p.nprod == 7 && p.h_msum_coef_a.size() == 28 &&
-                          p.h_msum_coef_b.size() == 28 && !g_dbg_unfused_lowp.load();
+        // (level 1: 4 sources, 7 sums; level 2: 16 sources, 49 sums through the Kronecker coefficients)
+        const bool fuse1 = p.msum_nsrc == 4 && p.nprod == 7 && p.h_msum_coef_a.size() == 28 &&
+                           p.h_msum_coef_b.size() == 28;
+        const bool fuse2 = p.msum_nsrc == 16 && p.nprod == 49 && p.h_msum_coef1_a.size() == 28 &&
+                           p.h_msum_coef1_b.size() == 28;
+        const bool fuse = !fp8 && (fuse1 || fuse2) && !g_dbg_unfused_lowp.load();
         const bool fuse_a = fuse && s_kfast && ms_a, fuse_b = fuse && t_shift && ms_b;
         const size_t nS = fuse_a ? 8 : size_t(nq * ms * ks) * 8, nT = fuse_b ? 8 : size_t(nq * ks * ns) * 8,
                      nM = direct ? 8 : size_t(nq * ms * ns) * (fp8 ? 4 : 8);
@@ -1848,9 +1852,11 @@ Status run_plan_lowp(const Plan& p, const Call& c, int precision, int nslices, h
                 fa.rows = isA ? ms : ns;
                 fa.cp = isA ? mp : np;
                 fa.kp = kp;
+                fa.level = fuse2 ? 2 : 1;
                 const std::vector<int64_t>& sh = isA ? p.h_msum_shift_a : p.h_msum_shift_b;
-                const std::vector<double>& cf = isA ? p.h_msum_coef_a : p.h_msum_coef_b;
-                for (int i = 0; i < 4; i++) fa.shift[i] = sh[size_t(i)];
+                const std::vector<double>& cf = fuse2 ? (isA ? p.h_msum_coef1_a : p.h_msum_coef1_b)
+                                                      : (isA ? p.h_msum_coef_a : p.h_msum_coef_b);
+                for (int i = 0; i < p.msum_nsrc; i++) fa.shift[i] = sh[size_t(i)];
                 for (int i = 0; i < 28; i++) fa.coef[i] = cf[size_t(i)];
                 fa.e = static_cast<int*>(isA ? EA.p : EB.p);
                 fa.out = isA ? XA.p : XB.p;

@@ -26,11 +26,12 @@ struct LowpSliceArgs
     bool skip_a, skip_b;
 };
 
-// One-pass operand sums and their digits in one launch, for a 1-level plan whose operand is K-fast with
-// translate sub-blocks: sum q (of 7) of row i at K position k is sum_s coef[q][s] X[zoff + shift[s] + rs[k] +
-// cs[i]] over the 4 sub-blocks s, summed in that order with zero coefficients skipped (the arithmetic of
-// the one-pass sums kernel); then each row's exponent and digits exactly as the K-fast slice kernel
-// derives them from the staged sums.  Rows are padded to cp and K to kp with zero digits.
+// One-pass operand sums and their digits in one launch, for a 1- or 2-level plan whose operand is K-fast
+// with translate sub-blocks.  At level 1, sum q (of 7) of row i at K position k is
+// sum_s coef[q][s] X[zoff + shift[s] + rs[k] + cs[i]] over the 4 sub-blocks s, summed in that order with
+// zero coefficients skipped (the arithmetic of the one-pass sums kernel); level 2 follows the one-pass
+// Kronecker sums kernel (below).  Each row's exponent and digits then come out exactly as the K-fast
+// slice kernel derives them from the staged sums.  Rows are padded to cp and K to kp with zero digits.
 struct SumSliceArgs
 {
     const double* X;
@@ -38,10 +39,15 @@ struct SumSliceArgs
     const int64_t* rs;  // K offsets, ks of them
     const int64_t* cs;  // row offsets, rows of them
     int64_t ks, rows, cp, kp;
-    int64_t shift[4];
-    double coef[28];  // [q][s]
-    int* e;           // [q][rows] row exponents
-    void* out;        // digits [slice][q][cp][kp]
+    // level 1: 4 sub-blocks and coef[q][s] for the 7 sums; level 2: 16 sub-blocks and the 7 x 4 one-level
+    // coefficients coef[o][a], whose Kronecker square gives the 49 sums in the one-pass Kronecker sums
+    // kernel's order (per outer product o, U_b = sum_a coef[o][a] src(a, b), then sum 7 o + p =
+    // sum_b coef[p][b] U_b; source (a, b) is sub-block 4 (2 (a / 2) + b / 2) + 2 (a % 2) + b % 2)
+    int level;
+    int64_t shift[16];
+    double coef[28];
+    int* e;      // [q][rows] row exponents
+    void* out;   // digits [slice][q][cp][kp]
 };
 hipError_t launch_ozaki_sum_slices(int nslices, bool int8, const SumSliceArgs& a, hipStream_t s);
 

@@ -206,7 +206,7 @@ __global__ __launch_bounds__(256) void slice_kfast_kernel(const double* __restri
 template <int NSL, typename D>
 __global__ __launch_bounds__(256) void sum_slice_kfast_kernel(SumSliceArgs a)
 {
-    constexpr int NQ = 7, NS = 4;
+    constexpr int NQ = 7, NS = 4;  // (shift[0..3] and coef[q][s] of SumSliceArgs)
     const int lane = threadIdx.x & 63;
     const int64_t slab = int64_t(NQ) * a.cp * a.kp;
     D* __restrict__ out = static_cast<D*>(a.out);
@@ -275,6 +275,118 @@ __global__ __launch_bounds__(256) void sum_slice_kfast_kernel(SumSliceArgs a)
                 for (int sl = 0; sl < NSL; sl++) o[sl * slab] = d[sl];
             }
         }
+    }
+}
+
+// The 2-level form: one wave per padded row i forms all 49 sums of the row from its 16 source values per K
+// position (the one-pass Kronecker sums kernel's arithmetic), twice -- for the 49 exponents, then for the
+// digits -- so every source row is read by one wave only.
+template <int NSL, typename D>
+__global__ __launch_bounds__(256) void sum_slice_kfast_kron_kernel(SumSliceArgs a)
+{
+    __shared__ int sex[4][49];  // the row exponents, per wave
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t slab = int64_t(49) * a.cp * a.kp;
+    D* __restrict__ out = static_cast<D*>(a.out);
+    for (int64_t i = (int64_t(blockIdx.x) * 256 + threadIdx.x) / 64; i < a.cp; i += int64_t(gridDim.x) * 4)
+    {
+        const bool real = i < a.rows;
+        const double* __restrict__ X = a.X + a.zoff + (real ? a.cs[i] : 0);
+        // sums 7 o .. 7 o + 6 at K position k (v: the 16 source values there)
+        auto outer = [&](const double (&v)[16], int o, double (&y)[7]) {
+            double u[4];
+#pragma unroll
+            for (int b = 0; b < 4; b++)
+            {
+                u[b] = 0.0;
+#pragma unroll
+                for (int q = 0; q < 4; q++)  // outer sub-block
+                {
+                    const double cf = a.coef[o * 4 + q];  // uniform
+                    if (cf != 0.0) u[b] += cf * v[4 * (2 * (q / 2) + b / 2) + 2 * (q % 2) + b % 2];
+                }
+            }
+#pragma unroll
+            for (int p = 0; p < 7; p++)
+            {
+                double acc = 0.0;
+#pragma unroll
+                for (int b = 0; b < 4; b++)
+                {
+                    const double cf = a.coef[p * 4 + b];
+                    if (cf != 0.0) acc += cf * u[b];
+                }
+                y[p] = acc;
+            }
+        };
+        auto load = [&](int64_t k, double (&v)[16]) {
+            const int64_t rk = a.rs[k];
+#pragma unroll
+            for (int s = 0; s < 16; s++) v[s] = X[a.shift[s] + rk];
+        };
+        double mx[49];
+#pragma unroll
+        for (int q = 0; q < 49; q++) mx[q] = 0.0;
+        if (real)
+            for (int64_t k = lane; k < a.ks; k += 64)
+            {
+                double v[16];
+                load(k, v);
+#pragma unroll
+                for (int o = 0; o < 7; o++)
+                {
+                    double y[7];
+                    outer(v, o, y);
+#pragma unroll
+                    for (int p = 0; p < 7; p++) mx[7 * o + p] = fmax(mx[7 * o + p], fabs(y[p]));
+                }
+            }
+#pragma unroll
+        for (int q = 0; q < 49; q++)
+        {
+            double m = mx[q];
+            for (int sh = 32; sh > 0; sh >>= 1) m = fmax(m, __shfl_xor(m, sh));
+            const int ex = exp_of(m);
+            if (lane == 0) sex[wave][q] = ex;
+            if (real && lane == 0) a.e[q * a.rows + i] = ex;
+        }
+        __builtin_amdgcn_wave_barrier();
+        for (int64_t k = lane; k < a.kp; k += 64)
+        {
+            const bool in = real && k < a.ks;
+            double v[16];
+            if (in)
+                load(k, v);
+            else
+#pragma unroll
+                for (int s = 0; s < 16; s++) v[s] = 0.0;
+            // one outer product at a time (a loop, not unrolled: unrolled, the digits of all 49 sums stayed
+            // live and spilled), and one sum of it at a time, its exponent from LDS
+#pragma unroll 1
+            for (int o = 0; o < 7; o++)
+            {
+                double y[7];
+                outer(v, o, y);
+#pragma unroll 1
+                for (int p = 0; p < 7; p++)
+                {
+                    double yp = y[0];
+#pragma unroll
+                    for (int j = 1; j < 7; j++) yp = p == j ? y[j] : yp;
+                    const int q = 7 * o + p;
+                    const double x = in ? ldexp(yp, -sex[wave][q]) : 0.0;
+                    D d[NSL];
+                    if constexpr (std::is_same<D, int8_t>::value)
+                        digits_i8<NSL>(x, d);
+                    else
+                        digits<NSL>(x, d);
+                    D* op = out + (int64_t(q) * a.cp + i) * a.kp + k;
+#pragma unroll
+                    for (int sl = 0; sl < NSL; sl++) op[sl * slab] = d[sl];
+                }
+            }
+        }
+        __builtin_amdgcn_wave_barrier();  // the next row's exponents overwrite sex[wave]
     }
 }
 
@@ -1152,31 +1264,29 @@ hipError_t launch_lowp_exponents(const LowpSliceArgs& a, hipStream_t s)
     return hipSuccess;
 }
 
+template <int NSL, typename D>
+hipError_t sum_slices_t(const SumSliceArgs& a, hipStream_t s)
+{
+    const dim3 grid(grid_of(a.cp, 4));
+    if (a.level == 1) hipLaunchKernelGGL((sum_slice_kfast_kernel<NSL, D>), grid, dim3(256), 0, s, a);
+    else if (a.level == 2) hipLaunchKernelGGL((sum_slice_kfast_kron_kernel<NSL, D>), grid, dim3(256), 0, s, a);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
 hipError_t launch_ozaki_sum_slices(int nsl, bool int8, const SumSliceArgs& a, hipStream_t s)
 {
     if (a.cp <= 0 || a.kp <= 0) return hipSuccess;
-    const dim3 grid(grid_of(a.cp, 4));
-#define MISTRA_SUM_SLICES(N)                                                                              \
-    case N:                                                                                               \
-        if (int8)                                                                                         \
-            hipLaunchKernelGGL((sum_slice_kfast_kernel<N, int8_t>), grid, dim3(256), 0, s, a);           \
-        else                                                                                              \
-            hipLaunchKernelGGL((sum_slice_kfast_kernel<N, __bf16>), grid, dim3(256), 0, s, a);           \
-        return hipGetLastError();
     switch (nsl)
     {
-        MISTRA_SUM_SLICES(2)
-        MISTRA_SUM_SLICES(3)
-        MISTRA_SUM_SLICES(4)
-        MISTRA_SUM_SLICES(5)
-        MISTRA_SUM_SLICES(6)
-    case 7:
-        if (int8) return hipErrorInvalidValue;
-        hipLaunchKernelGGL((sum_slice_kfast_kernel<7, __bf16>), grid, dim3(256), 0, s, a);
-        return hipGetLastError();
+    case 2: return int8 ? sum_slices_t<2, int8_t>(a, s) : sum_slices_t<2, __bf16>(a, s);
+    case 3: return int8 ? sum_slices_t<3, int8_t>(a, s) : sum_slices_t<3, __bf16>(a, s);
+    case 4: return int8 ? sum_slices_t<4, int8_t>(a, s) : sum_slices_t<4, __bf16>(a, s);
+    case 5: return int8 ? sum_slices_t<5, int8_t>(a, s) : sum_slices_t<5, __bf16>(a, s);
+    case 6: return int8 ? sum_slices_t<6, int8_t>(a, s) : sum_slices_t<6, __bf16>(a, s);
+    case 7: return int8 ? hipErrorInvalidValue : sum_slices_t<7, __bf16>(a, s);
     default: return hipErrorInvalidValue;
     }
-#undef MISTRA_SUM_SLICES
 }
 
 hipError_t launch_ozaki_slices(int nsl, const LowpSliceArgs& a, hipStream_t s)

@@ -1,7 +1,7 @@
-"""Low-precision 1-level plans whose operands are K-fast form each operand's 7 Strassen sums and their
-digits in one launch.  The sums never reach HBM, but the digits must be exactly those of the staged path.
-That path runs the one-pass sums kernel, then the K-fast slice kernel, and mistra_debug_set_lowp_fusion(0)
-forces it.
+"""Low-precision 1- and 2-level plans whose operands are K-fast form each operand's Strassen sums (7, or
+49 through the Kronecker coefficients) and their digits in one launch.  The sums never reach HBM, but the
+digits must be exactly those of the staged path.  That path runs the one-pass sums kernel, then the K-fast
+slice kernel, and mistra_debug_set_lowp_fusion(0) forces it.
 
 Checks, on shapes with padded rows and K, multi-index groups, a batch index, and one operand K-fast and
 the other not:
@@ -59,14 +59,15 @@ def _restore():
     ts.debug_set_lowp(0, 0, 0)
 
 
+@pytest.mark.parametrize("level", [1, 2])
 @pytest.mark.parametrize("si", range(len(SHAPES)))
 @pytest.mark.parametrize("mi", range(len(MODES)))
-def test_fused_sum_slices_equal_staged_path(si, mi):
+def test_fused_sum_slices_equal_staged_path(si, mi, level):
     prec, nsl, _ = MODES[mi]
     shape = dict(SHAPES[si])
     b_kfast = shape.pop("b_kfast", True)
     case = kfast_case(1500 + si, b_kfast=b_kfast, **shape)
-    say(f"fused sums+digits: {case['idx_C']} = {case['idx_A']} * {case['idx_B']}, {prec} with {nsl} digits")
+    say(f"fused sums+digits L{level}: {case['idx_C']} = {case['idx_A']} * {case['idx_B']}, {prec} with {nsl} digits")
     n = [int(np.prod(case[f"shape_{x}"])) for x in "ABC"]
     a, b, c = cases.uniform(n[0], 11, emax=8), cases.uniform(n[1], 12, emax=8), cases.uniform(n[2], 13)
     if prec == "ozaki_bf16":
@@ -74,9 +75,9 @@ def test_fused_sum_slices_equal_staged_path(si, mi):
     else:
         ts.debug_set_lowp(0, nsl, 0)
     ts.debug_set_lowp_fusion(True)
-    fused = np.array(run(case, a, b, c, 1, prec, 1.0, 0.5))
+    fused = np.array(run(case, a, b, c, level, prec, 1.0, 0.5))
     ts.debug_set_lowp_fusion(False)
-    staged = np.array(run(case, a, b, c, 1, prec, 1.0, 0.5))
+    staged = np.array(run(case, a, b, c, level, prec, 1.0, 0.5))
     assert np.array_equal(fused, staged)
     want = cases.einsum_ref(case, cases.host_tensor(case["shape_A"], a), cases.host_tensor(case["shape_B"], b),
                             cases.host_tensor(case["shape_C"], c), 1.0, 0.5)
