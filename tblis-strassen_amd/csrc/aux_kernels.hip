@@ -373,9 +373,11 @@ __global__ __launch_bounds__(256) void combine_kernel(CombineArgs a)
 // per product and adds it to every sub-block the product feeds, so M is read once rather than once per C
 // target (12 of 7 products' worth at one level, 144 of 49 at two).  Products are visited in order and a
 // sub-block the product does not feed is skipped (a uniform branch on the kernel-argument mask), so each
-// sub-block sums the same terms in the same order as the per-sub-block form.  The next product's values
-// are loaded before the current ones are used (the last product re-reads itself), with no branch around
-// the loads.
+// sub-block sums the same terms in the same order as the per-sub-block form.  The loads of kCombineChunk
+// products are issued together before any is used (a chunk past the last product re-reads the last one:
+// in bounds, never added), so a wave keeps kCombineChunk * NC loads in flight.  With one product in flight
+// the 2-level combine ran at 3.9 TB/s, latency-bound.
+constexpr int kCombineChunk = 7;
 template <typename T, int NSUB, int NC, typename TM>
 __global__ __launch_bounds__(256) void combine_all_kernel(CombineArgs a)
 {
@@ -399,27 +401,32 @@ __global__ __launch_bounds__(256) void combine_all_kernel(CombineArgs a)
         int64_t coff_j[NC];
 #pragma unroll
         for (int j = 0; j < NC; j++) coff_j[j] = (j < ncol ? j : 0) * a.ms;
-        TM v[NC], vn[NC];
-#pragma unroll
-        for (int j = 0; j < NC; j++) v[j] = M[coff_j[j]];
-        for (int p = 0; p < a.nprod; p++)
+        for (int p0 = 0; p0 < a.nprod; p0 += kCombineChunk)
         {
-            const TM* Mn = M + int64_t(p < last ? p + 1 : last) * nel;
+            TM v[kCombineChunk][NC];
 #pragma unroll
-            for (int j = 0; j < NC; j++) vn[j] = Mn[coff_j[j]];
-            const unsigned pm = a.pmask[p];
-#pragma unroll
-            for (int s = 0; s < NSUB; s++)
+            for (int u = 0; u < kCombineChunk; u++)
             {
-                if (pm & (1u << s))
-                {
-                    const double ws = (pm & (1u << (16 + s))) ? -1.0 : 1.0;
+                const TM* Mp = M + int64_t(p0 + u < last ? p0 + u : last) * nel;
 #pragma unroll
-                    for (int j = 0; j < NC; j++) acc[s][j] += ws * double(v[j]);
-                }
+                for (int j = 0; j < NC; j++) v[u][j] = Mp[coff_j[j]];
             }
 #pragma unroll
-            for (int j = 0; j < NC; j++) v[j] = vn[j];
+            for (int u = 0; u < kCombineChunk; u++)
+            {
+                if (p0 + u > last) break;
+                const unsigned pm = a.pmask[p0 + u];
+#pragma unroll
+                for (int s = 0; s < NSUB; s++)
+                {
+                    if (pm & (1u << s))
+                    {
+                        const double ws = (pm & (1u << (16 + s))) ? -1.0 : 1.0;
+#pragma unroll
+                        for (int j = 0; j < NC; j++) acc[s][j] += ws * double(v[u][j]);
+                    }
+                }
+            }
         }
         T* C = static_cast<T*>(a.C) + (a.batch_c ? a.batch_c[z] : 0);
 #pragma unroll
