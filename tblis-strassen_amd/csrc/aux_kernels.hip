@@ -313,11 +313,33 @@ hipError_t launch_operand_sums(int elem_size, const SumArgs& a, hipStream_t s)
 constexpr int kCombineRows = 64;
 constexpr int kCombineCols = 8;
 
-template <typename T, typename TM>
+// Offset of element (r, c) of a product, and the product stride, in M's layout (CombineArgs::m_tiled);
+// columns c .. c + j of one tile (c % kCombineCols == 0, j < kCombineCols) are kMTileRows apart.
+template <bool TILED>
+__device__ __forceinline__ int64_t m_offset(const CombineArgs& a, int64_t r, int64_t c)
+{
+    if constexpr (TILED)
+        return ((c / kMTileCols) * (a.m_mp / kMTileRows) + r / kMTileRows) * (kMTileRows * kMTileCols) +
+               (c % kMTileCols) * kMTileRows + r % kMTileRows;
+    else
+        return r + c * a.ms;
+}
+template <bool TILED>
+__device__ __forceinline__ int64_t m_colstep(const CombineArgs& a)
+{
+    return TILED ? int64_t(kMTileRows) : a.ms;
+}
+template <bool TILED>
+__device__ __forceinline__ int64_t m_slab(const CombineArgs& a)
+{
+    return TILED ? a.m_mp * a.m_np : a.ms * a.ns;
+}
+
+template <typename T, typename TM, bool TILED>
 __global__ __launch_bounds__(256) void combine_kernel(CombineArgs a)
 {
     const int s = blockIdx.y;
-    const int64_t nel = a.ms * a.ns;
+    const int64_t nel = m_slab<TILED>(a), cstep = m_colstep<TILED>(a);
     const int64_t row_tiles = (a.ms + kCombineRows - 1) / kCombineRows;
     const int64_t rt = int64_t(blockIdx.x) % row_tiles, ct = int64_t(blockIdx.x) / row_tiles;
     const int64_t r = rt * kCombineRows + (threadIdx.x & (kCombineRows - 1));
@@ -330,7 +352,7 @@ __global__ __launch_bounds__(256) void combine_kernel(CombineArgs a)
     for (int64_t z = blockIdx.z; z < a.nbatch; z += gridDim.z)
     {
         T* C = static_cast<T*>(a.C) + (a.batch_c ? a.batch_c[z] : 0) + cm;
-        const TM* M = static_cast<const TM*>(a.M) + z * int64_t(a.nprod) * nel + c0 * a.ms + r;
+        const TM* M = static_cast<const TM*>(a.M) + z * int64_t(a.nprod) * nel + m_offset<TILED>(a, r, c0);
         double acc[kCombineCols];
 #pragma unroll
         for (int j = 0; j < kCombineCols; j++) acc[j] = 0.0;
@@ -342,7 +364,7 @@ __global__ __launch_bounds__(256) void combine_kernel(CombineArgs a)
                 const TM* Mp = M + int64_t(a.terms[t].prod) * nel;
                 TM v[kCombineCols];
 #pragma unroll
-                for (int j = 0; j < kCombineCols; j++) v[j] = Mp[int64_t(j) * a.ms];
+                for (int j = 0; j < kCombineCols; j++) v[j] = Mp[int64_t(j) * cstep];
 #pragma unroll
                 for (int j = 0; j < kCombineCols; j++) acc[j] += w * double(v[j]);
             }
@@ -355,7 +377,7 @@ __global__ __launch_bounds__(256) void combine_kernel(CombineArgs a)
                 const TM* Mp = M + int64_t(a.terms[t].prod) * nel;
 #pragma unroll
                 for (int j = 0; j < kCombineCols; j++)
-                    if (j < ncol) acc[j] += w * double(Mp[int64_t(j) * a.ms]);
+                    if (j < ncol) acc[j] += w * double(Mp[int64_t(j) * cstep]);
             }
         }
 #pragma unroll
@@ -378,10 +400,10 @@ __global__ __launch_bounds__(256) void combine_kernel(CombineArgs a)
 // in bounds, never added), so a wave keeps kCombineChunk * NC loads in flight.  With one product in flight
 // the 2-level combine ran at 3.9 TB/s, latency-bound.
 constexpr int kCombineChunk = 7;
-template <typename T, int NSUB, int NC, typename TM>
+template <typename T, int NSUB, int NC, typename TM, bool TILED>
 __global__ __launch_bounds__(256) void combine_all_kernel(CombineArgs a)
 {
-    const int64_t nel = a.ms * a.ns;
+    const int64_t nel = m_slab<TILED>(a);
     const int64_t row_tiles = (a.ms + kCombineRows - 1) / kCombineRows;
     const int64_t rt = int64_t(blockIdx.x) % row_tiles, ct = int64_t(blockIdx.x) / row_tiles;
     const int64_t r = rt * kCombineRows + (threadIdx.x & (kCombineRows - 1));
@@ -391,7 +413,7 @@ __global__ __launch_bounds__(256) void combine_all_kernel(CombineArgs a)
     const int last = a.nprod - 1;
     for (int64_t z = blockIdx.z; z < a.nbatch; z += gridDim.z)
     {
-        const TM* M = static_cast<const TM*>(a.M) + z * int64_t(a.nprod) * nel + c0 * a.ms + r;
+        const TM* M = static_cast<const TM*>(a.M) + z * int64_t(a.nprod) * nel + m_offset<TILED>(a, r, c0);
         double acc[NSUB][NC];
 #pragma unroll
         for (int s = 0; s < NSUB; s++)
@@ -400,7 +422,7 @@ __global__ __launch_bounds__(256) void combine_all_kernel(CombineArgs a)
         // columns past the edge read column c0 again (in bounds) and are never stored
         int64_t coff_j[NC];
 #pragma unroll
-        for (int j = 0; j < NC; j++) coff_j[j] = (j < ncol ? j : 0) * a.ms;
+        for (int j = 0; j < NC; j++) coff_j[j] = (j < ncol ? j : 0) * m_colstep<TILED>(a);
         for (int p0 = 0; p0 < a.nprod; p0 += kCombineChunk)
         {
             TM v[kCombineChunk][NC];
@@ -453,6 +475,8 @@ hipError_t launch_combine(int elem_size, const CombineArgs& a, hipStream_t s)
     // M in FP32 beside an FP64 C (the FP8 mode's products), else M has C's type
     if (a.m_elem != 0 && a.m_elem != elem_size && !(a.m_elem == 4 && elem_size == 8)) return hipErrorInvalidValue;
     const bool m_f32 = a.m_elem == 4 && elem_size == 8;
+    if (a.m_tiled && (!m_f32 || a.m_mp % kMTileRows || a.m_np % kMTileCols || a.m_mp < a.ms || a.m_np < a.ns))
+        return hipErrorInvalidValue;
     const int64_t row_tiles = (a.ms + kCombineRows - 1) / kCombineRows;
     const int64_t zb = a.nbatch < 65535 ? a.nbatch : 65535;
     auto blocks_for = [&](int cols_per_lane) {
@@ -467,12 +491,14 @@ hipError_t launch_combine(int elem_size, const CombineArgs& a, hipStream_t s)
 #define MISTRA_COMBINE_ALL(NSUB, NC)                                                                            \
     do                                                                                                          \
     {                                                                                                           \
-        if (m_f32 && elem_size == 8)                                                                            \
-            hipLaunchKernelGGL((combine_all_kernel<double, NSUB, NC, float>), grid, dim3(256), 0, s, a);       \
+        if (m_f32 && a.m_tiled)                                                                                 \
+            hipLaunchKernelGGL((combine_all_kernel<double, NSUB, NC, float, true>), grid, dim3(256), 0, s, a);  \
+        else if (m_f32)                                                                                         \
+            hipLaunchKernelGGL((combine_all_kernel<double, NSUB, NC, float, false>), grid, dim3(256), 0, s, a); \
         else if (elem_size == 8)                                                                                \
-            hipLaunchKernelGGL((combine_all_kernel<double, NSUB, NC, double>), grid, dim3(256), 0, s, a);      \
+            hipLaunchKernelGGL((combine_all_kernel<double, NSUB, NC, double, false>), grid, dim3(256), 0, s, a);\
         else                                                                                                    \
-            hipLaunchKernelGGL((combine_all_kernel<float, NSUB, NC, float>), grid, dim3(256), 0, s, a);        \
+            hipLaunchKernelGGL((combine_all_kernel<float, NSUB, NC, float, false>), grid, dim3(256), 0, s, a);  \
     } while (0)
         if (a.nsub == 16) MISTRA_COMBINE_ALL(16, 2);
         else if (a.nsub == 4) MISTRA_COMBINE_ALL(4, 8);
@@ -483,9 +509,10 @@ hipError_t launch_combine(int elem_size, const CombineArgs& a, hipStream_t s)
     const int64_t blocks = blocks_for(kCombineCols);
     if (blocks > 0x7fffffff || a.nsub > 65535) return hipErrorInvalidValue;
     const dim3 grid(static_cast<unsigned>(blocks), static_cast<unsigned>(a.nsub), static_cast<unsigned>(zb));
-    if (m_f32) hipLaunchKernelGGL((combine_kernel<double, float>), grid, dim3(256), 0, s, a);
-    else if (elem_size == 8) hipLaunchKernelGGL((combine_kernel<double, double>), grid, dim3(256), 0, s, a);
-    else hipLaunchKernelGGL((combine_kernel<float, float>), grid, dim3(256), 0, s, a);
+    if (m_f32 && a.m_tiled) hipLaunchKernelGGL((combine_kernel<double, float, true>), grid, dim3(256), 0, s, a);
+    else if (m_f32) hipLaunchKernelGGL((combine_kernel<double, float, false>), grid, dim3(256), 0, s, a);
+    else if (elem_size == 8) hipLaunchKernelGGL((combine_kernel<double, double, false>), grid, dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((combine_kernel<float, float, false>), grid, dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

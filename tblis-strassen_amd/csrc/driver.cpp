@@ -1756,7 +1756,7 @@ Status run_plan_lowp(const Plan& p, const Call& c, int precision, int nslices, h
         const bool fp8 = precision == MISTRA_PREC_FP8_BLOCK, i8 = precision == MISTRA_PREC_OZAKI_INT8;
         const int64_t ms = p.ms, ns = p.ns, ks = p.ks, nq = p.nprod;
         auto up = [](int64_t x, int64_t m) { return (x + m - 1) / m * m; };
-        const int64_t mp = up(ms, 128), np = up(ns, fp8 ? 128 : 64), kp = up(ks, fp8 ? 128 : 64);
+        const int64_t mp = up(ms, fp8 ? 256 : 128), np = up(ns, fp8 ? 128 : 64), kp = up(ks, fp8 ? 128 : 64);
         const int nsl = fp8 ? 1 : nslices;
         const size_t esz = fp8 || i8 ? 1 : 2;
         struct Buf
@@ -1783,7 +1783,7 @@ Status run_plan_lowp(const Plan& p, const Call& c, int precision, int nslices, h
         const bool fuse = !fp8 && (fuse1 || fuse2) && !g_dbg_unfused_lowp.load();
         const bool fuse_a = fuse && s_kfast && ms_a, fuse_b = fuse && t_shift && ms_b;
         const size_t nS = fuse_a ? 8 : size_t(nq * ms * ks) * 8, nT = fuse_b ? 8 : size_t(nq * ks * ns) * 8,
-                     nM = direct ? 8 : size_t(nq * ms * ns) * (fp8 ? 4 : 8);
+                     nM = direct ? 8 : fp8 ? size_t(nq * mp * np) * 4 : size_t(nq * ms * ns) * 8;
         // FP8: one scale byte per row and 128-long K block; digits: one exponent per row
         const size_t nEA = fp8 ? size_t(nq * mp * (kp / 128)) : size_t(nq * ms) * 4;
         const size_t nEB = fp8 ? size_t(nq * np * (kp / 128)) : size_t(nq * ns) * 4;
@@ -1948,6 +1948,9 @@ Status run_plan_lowp(const Plan& p, const Call& c, int precision, int nslices, h
             ca.beta = c.beta;
             ca.nbatch = 1;
             ca.m_elem = fp8 ? 4 : 0;
+            ca.m_tiled = fp8 ? 1 : 0;  // (the FP8 GEMM's tiles)
+            ca.m_mp = mp;
+            ca.m_np = np;
             e = launch_combine(8, ca, stream);
         }
         for (Buf* b : {&S, &T, &M, &EA, &EB, &XA, &XB})

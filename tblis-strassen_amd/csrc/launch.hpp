@@ -6,6 +6,7 @@
 #include <cstdint>
 
 #include "kernels.hpp"
+#include "lowp.hpp"  // kMTileRows, kMTileCols
 
 namespace mistra {
 
@@ -133,6 +134,11 @@ struct CombineArgs
     unsigned pmask[kCombineMaxProds];
     // element size of M: 0 = C's; 4 with an FP64 C = products staged in FP32 (the FP8 mode)
     int m_elem;
+    // M_p in tiles (m_tiled != 0, FP32 M only): kMTileRows x kMTileCols tiles, tile (i, j) of M_p at
+    // (j * (m_mp / kMTileRows) + i) kMTileRows kMTileCols, rows fastest inside a tile; M_p padded to
+    // m_mp x m_np (multiples of the tile), products m_mp m_np apart
+    int m_tiled;
+    int64_t m_mp, m_np;
 };
 hipError_t launch_combine(int elem_size, const CombineArgs& a, hipStream_t stream);
 

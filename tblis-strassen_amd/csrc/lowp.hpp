@@ -51,6 +51,9 @@ struct SumSliceArgs
 };
 hipError_t launch_ozaki_sum_slices(int nslices, bool int8, const SumSliceArgs& a, hipStream_t s);
 
+// Tile of the FP8 products staged in FP32 (LowpGemmArgs::m_f32, CombineArgs::m_tiled)
+constexpr int kMTileRows = 256, kMTileCols = 128;
+
 struct LowpGemmArgs
 {
     int64_t ms, ns, mp, np, kp;
@@ -66,7 +69,8 @@ struct LowpGemmArgs
     // INT8 GEMM: fold the exact INT32 sums into FP64 every fold_steps K-steps of 64 (0: the default,
     // 1024, and only when K needs it; at most 1024)
     int fold_steps;
-    // FP8 GEMM: stage M_q in FP32 (M is then a float array of the same layout)
+    // FP8 GEMM: stage M_q in FP32 as mp x np in 256 x 128 tiles (the layout of CombineArgs::m_tiled;
+    // M is then a float array, products mp np apart)
     bool m_f32;
 };
 

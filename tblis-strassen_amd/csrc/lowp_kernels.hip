@@ -23,6 +23,7 @@
 // MFMA operand of lane l is row (l % 16), k = 8 (l / 16) + [0, 8) -- 16 (BF16) or 8 (FP8) contiguous bytes.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <type_traits>
 
@@ -986,43 +987,69 @@ __global__ __launch_bounds__(256) void fp8_cols_kernel(const double* __restrict_
     }
 }
 
-// 256 threads (4 waves as 2 x 2, 64 x 64 each = 4 x 4 blocks of 16 x 16) per 128 x 128 tile of M_q,
+// 512 threads (8 waves as 4 x 2, 64 x 64 each = 4 x 4 blocks of 16 x 16) per 256 x 128 tile of M_q,
 // K-steps of 128 (one v_mfma_scale_f32_16x16x128_f8f6f4 per block and step, the block scales applied by
-// the MFMA itself, FP32 accumulation across blocks).  The code tiles are double-buffered in LDS as
-// [row][128 bytes] with the eight 16-byte chunks of row r XOR-swizzled by r & 7, so the operand reads (16
-// consecutive rows at one chunk pair) spread over the chunk slots; the next step's tiles and scale words
-// are loaded into registers while the current step's MFMAs run.  Output through LDS (the 64 KiB of tile
-// buffers, free after the loop): consecutive threads write consecutive rows of M_q (its unit stride).
-// Needs mp % 128 == 0, np % 128 == 0, kp % 128 == 0.
+// the MFMA itself, FP32 accumulation across blocks).  The code tiles and their scale words go from HBM/L2
+// straight into LDS (buffer loads with the LDS destination, no staging registers), three K-steps deep:
+// the loads of step st + 2 are issued while step st computes (three 50 KiB stages: one workgroup per CU,
+// two waves per SIMD).  No load in the loop writes registers, so the compiler adds no waits of its own
+// for loads.  LDS holds each tile as [row][128 bytes] with the eight 16-byte chunks of row r XOR-swizzled
+// by r & 7, so the operand reads (16 consecutive rows at one chunk pair) spread over the chunk slots.
+// Products staged in FP32 (m_f32) leave straight from the accumulators into M_q's 256 x 128 tile layout
+// (CombineArgs::m_tiled): a tile is 128 KiB of contiguous memory, in which a lane holds 4 consecutive
+// rows of one column per block, one 16-byte store each.  The C update of a single product goes through
+// LDS.  Needs mp % 256 == 0, np % 128 == 0, kp % 128 == 0, 256 kp < 2^31 and (kp / 128) max(mp, np) <
+// 2^31 (host-checked).
 typedef int i32x8 __attribute__((ext_vector_type(8)));
 typedef int i32x4v __attribute__((ext_vector_type(4)));
-__global__ __launch_bounds__(256, 2) void fp8_gemm_kernel(const uint8_t* __restrict__ As, const uint8_t* __restrict__ Bs,
-                                                          const uint32_t* __restrict__ sa, const uint32_t* __restrict__ sb,
-                                                          double* __restrict__ M, LowpGemmArgs g)
+__global__ __launch_bounds__(512) void fp8_gemm_kernel(const uint8_t* __restrict__ As, const uint8_t* __restrict__ Bs,
+                                                      const uint32_t* __restrict__ sa, const uint32_t* __restrict__ sb,
+                                                      double* __restrict__ M, LowpGemmArgs g)
 {
-    __shared__ __attribute__((aligned(16))) uint8_t lds[2][2][128 * 128];  // [buffer][A, B][row][128 B]
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int tiles_m = int(g.mp / 128);
-    const int tm = blockIdx.x % tiles_m, tn = blockIdx.x / tiles_m;
+    // One array per stage, indexed by compile-time stage numbers: the A tile (32 KiB), the B tile (16 KiB),
+    // then 256 bytes of scale words per wave (wave 0: the tile's 64 A words, wave 1: its 32 B words and 32
+    // zeros, the others zeros).  Three arrays keep the compiler's tracking of loads into LDS exact (with
+    // more, it lost track and waited for loads into other stages before every read).
+    constexpr int kB = 256 * 128, kW = kB + 128 * 128, kStage = kW + 8 * 256;
+    static_assert(kMTileRows == 256 && kMTileCols == 128, "the FP8 GEMM tile is M's tile");
+    __shared__ __attribute__((aligned(16))) uint8_t ls0[kStage], ls1[kStage], ls2[kStage];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wm = (wave & 3) * 64, wn = (wave >> 2) * 64;
+    const int64_t tiles_m = g.mp / 256;
+    const int tm = int(blockIdx.x % tiles_m), tn = int(blockIdx.x / tiles_m);
     const int q = g.q0 + int(blockIdx.y);  // product (the launch covers [q0, q0 + nlaunch))
-    const int64_t row0 = int64_t(tm) * 128, col0 = int64_t(tn) * 128;
-    const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
+    const int64_t row0 = int64_t(tm) * 256, col0 = int64_t(tn) * 128;
     const int64_t nkb = g.kp / 128;
-    const uint8_t* ta = As + (int64_t(q) * g.mp + row0) * g.kp;
-    const uint8_t* tb = Bs + (int64_t(q) * g.np + col0) * g.kp;
-    // loader: 1024 chunks of 16 bytes per operand and step, 4 per thread; 8 threads cover one 128-byte row
-    int64_t goff[4];
-    int soff[4];
+    const int nsteps = int(nkb);
+
+    // Loads: scalar resources at the tile's first row (code tiles) and first scale word, fixed lane
+    // offsets and the K-step as the scalar offset.  Wave w fills A rows 32 w .. 32 w + 31 (four 1 KiB
+    // loads) and B rows 16 w .. 16 w + 15 (two): LDS slot `lane` of a 1 KiB load is row 8 u + lane / 8 of
+    // the load's block at position lane % 8, which holds chunk (lane % 8) ^ (lane / 8) of the row (row & 7
+    // = lane / 8).  Scale words: those of rows row0 .. row0 + 255 are 64 consecutive words of a K-step's
+    // block (4 groups of 16 rows), those of columns col0 .. col0 + 127 32 words.  Wave 0 loads the A words
+    // of a step, wave 1 the B words (lanes past 32 read 0), the other waves read 0: one load per wave and
+    // step into the wave's own slot, no branch.
+    const __amdgpu_buffer_rsrc_t rsa = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<uint8_t*>(As + (int64_t(q) * g.mp + row0) * g.kp), (short)0, int(kBufOob), 0x00020000);
+    const __amdgpu_buffer_rsrc_t rsb = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<uint8_t*>(Bs + (int64_t(q) * g.np + col0) * g.kp), (short)0, int(kBufOob), 0x00020000);
+    const uint32_t* w = wave == 1 ? sb + int64_t(q) * nkb * (g.np / 4) + (col0 / 64) * 16
+                                  : sa + int64_t(q) * nkb * (g.mp / 4) + (row0 / 64) * 16;
+    const __amdgpu_buffer_rsrc_t rw =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t*>(w), (short)0, int(kBufOob), 0x00020000);
+    const int lrow = lane >> 3, lch = (lane & 7) ^ (lane >> 3);
+    uint32_t voa[4], vob[2];
 #pragma unroll
-    for (int u = 0; u < 4; u++)
-    {
-        const int c = tid + 256 * u, r = c >> 3, ch = c & 7;
-        goff[u] = int64_t(r) * g.kp + ch * 16;
-        soff[u] = r * 128 + ((ch ^ (r & 7)) << 4);
-    }
-    // scale words of this lane's 16-row blocks: rows wm + 16 i + lane % 16, byte i of the word
-    const uint32_t* wa = sa + int64_t(q) * nkb * (g.mp / 4) + ((row0 + wm) / 64) * 16 + (lane & 15);
-    const uint32_t* wb = sb + int64_t(q) * nkb * (g.np / 4) + ((col0 + wn) / 64) * 16 + (lane & 15);
+    for (int u = 0; u < 4; u++) voa[u] = uint32_t((32 * wave + 8 * u + lrow) * g.kp + lch * 16);
+#pragma unroll
+    for (int u = 0; u < 2; u++) vob[u] = uint32_t((16 * wave + 8 * u + lrow) * g.kp + lch * 16);
+    const uint32_t vow = wave == 0 || (wave == 1 && lane < 32) ? uint32_t(lane * 4) : kBufOob;
+    const int64_t wstep = wave == 1 ? g.np : g.mp;  // bytes of scale words per K-step
+    // this lane's scale words: rows wm + 16 i + lane % 16 (byte i), at (wm / 64) 16 + lane % 16 of the
+    // step's 64, columns the same with wn
+    const int wsel = (wm / 64) * 16 + (lane & 15), wselb = (wn / 64) * 16 + (lane & 15);
     // operand reads: row (16-row block base + lane % 16), chunks 2 (lane / 16) and 2 (lane / 16) + 1
     const int fr = lane & 15, fg = lane >> 4;
     int aoff[4][2], boff[4][2];
@@ -1042,54 +1069,74 @@ __global__ __launch_bounds__(256, 2) void fp8_gemm_kernel(const uint8_t* __restr
 #pragma unroll
         for (int j = 0; j < 4; j++) acc[i][j] = f32x4{0, 0, 0, 0};
 
-    i32x4v ra[4], rb[4];
-    uint32_t swa = 0, swb = 0;
-    auto gload = [&](int64_t kb) {
+    typedef __attribute__((address_space(3))) void* lds_ptr;
+    // The seven loads into LDS of K-step kb (the wait below counts them).  Past the last step (kb >=
+    // nsteps) every offset is out of range: the loads read zeros into a stage that no later step reads,
+    // so every step issues seven loads and waits the same way, with no branch inside a step.
+    auto issue = [&](uint8_t* buf, int kb) {
+        __builtin_amdgcn_sched_barrier(0);
+#if defined(__HIP_DEVICE_COMPILE__)  // the builtin exists for the gfx950 target only, not the host pass
+        const bool real = kb < nsteps;
+        // (the offsets are uniform; said explicitly, or the compiler's induction variable for one can land
+        // in a vector register and the load becomes a loop over its distinct values)
+        const int so = __builtin_amdgcn_readfirstlane(real ? kb * 128 : 0);
+        const int sw = __builtin_amdgcn_readfirstlane(real ? int(kb * wstep) : 0);
 #pragma unroll
         for (int u = 0; u < 4; u++)
-        {
-            ra[u] = *reinterpret_cast<const i32x4v*>(ta + goff[u] + kb * 128);
-            rb[u] = *reinterpret_cast<const i32x4v*>(tb + goff[u] + kb * 128);
-        }
-        swa = wa[kb * (g.mp / 4)];
-        swb = wb[kb * (g.np / 4)];
-    };
-    auto lstore = [&](int buf) {
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(rsa, (lds_ptr)(buf + (4 * wave + u) * 1024), 16,
+                                                     real ? voa[u] : kBufOob, so, 0, 0);
 #pragma unroll
-        for (int u = 0; u < 4; u++)
-        {
-            *reinterpret_cast<i32x4v*>(&lds[buf][0][soff[u]]) = ra[u];
-            *reinterpret_cast<i32x4v*>(&lds[buf][1][soff[u]]) = rb[u];
-        }
+        for (int u = 0; u < 2; u++)
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(rsb, (lds_ptr)(buf + kB + (2 * wave + u) * 1024), 16,
+                                                     real ? vob[u] : kBufOob, so, 0, 0);
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(rw, (lds_ptr)(buf + kW + wave * 256), 4, real ? vow : kBufOob, sw, 0,
+                                                 0);
+#else
+        (void)buf;
+        (void)kb;
+#endif
+        __builtin_amdgcn_sched_barrier(0);
     };
+    // Before the barrier that publishes the next step's tiles: this wave's loads of that step done; the
+    // newest 7 (those of the step after, issued after everything the barrier publishes) may stay in
+    // flight.  The compiler's wait insertion does not see LDS reads depending on loads into LDS.
+    auto landed = [&]() {
+#if defined(__HIP_DEVICE_COMPILE__)
+        __builtin_amdgcn_s_waitcnt(0x0F70 | 7);  // vmcnt(7); expcnt and lgkmcnt unconstrained
+#endif
+    };
+    // this wave's LDS reads done, then s_barrier with no memory fence (the loads of the step after next
+    // stay in flight); the "memory" clobber keeps LDS accesses on their side of it
+    auto step_barrier = [&]() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); };
 
-    uint32_t cwa = 0, cwb = 0;  // scale words of the step being computed
-    if (nkb > 0)
-    {
-        gload(0);
-        lstore(0);
-        cwa = swa;
-        cwb = swb;
-    }
-    __syncthreads();
-    for (int64_t kb = 0; kb < nkb; kb++)
-    {
-        const int cur = int(kb & 1);
-        if (kb + 1 < nkb) gload(kb + 1);
+    // One K-step on stage S (tiles and scale words in ls<S>), loading step st + 2.  A step is one basic
+    // block: with a branch between the MFMAs and the barrier, the compiler sank the MFMAs to their next
+    // use, a step later, and the fragments of several steps spilled.
+    auto step = [&](int st, auto stage_c) {
+        constexpr int S = decltype(stage_c)::value;
+        constexpr int N2 = (S + 2) % 3;  // stage of step st + 2 (last read in step st - 1)
+        const uint8_t* ca = S == 0 ? ls0 : S == 1 ? ls1 : ls2;
+        const uint8_t* cb = ca + kB;
+        const int ws_a = reinterpret_cast<const int*>(ca + kW)[wsel];
+        const int ws_b = reinterpret_cast<const int*>(ca + kW + 256)[wselb];
         i32x8 fa[4], fb[4];
 #pragma unroll
         for (int i = 0; i < 4; i++)
         {
-            const i32x4v a0 = *reinterpret_cast<const i32x4v*>(&lds[cur][0][aoff[i][0]]);
-            const i32x4v a1 = *reinterpret_cast<const i32x4v*>(&lds[cur][0][aoff[i][1]]);
-            const i32x4v b0 = *reinterpret_cast<const i32x4v*>(&lds[cur][1][boff[i][0]]);
-            const i32x4v b1 = *reinterpret_cast<const i32x4v*>(&lds[cur][1][boff[i][1]]);
-            fa[i] = i32x8{a0[0], a0[1], a0[2], a0[3], a1[0], a1[1], a1[2], a1[3]};
+            const i32x4v b0 = *reinterpret_cast<const i32x4v*>(&cb[boff[i][0]]);
+            const i32x4v b1 = *reinterpret_cast<const i32x4v*>(&cb[boff[i][1]]);
             fb[i] = i32x8{b0[0], b0[1], b0[2], b0[3], b1[0], b1[1], b1[2], b1[3]};
         }
-#define MISTRA_FP8_MFMA(I, J)                                                                               \
-    acc[I][J] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(fa[I], fb[J], acc[I][J], 0, 0, I, int(cwa), J, \
-                                                                 int(cwb))
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+        {
+            const i32x4v a0 = *reinterpret_cast<const i32x4v*>(&ca[aoff[i][0]]);
+            const i32x4v a1 = *reinterpret_cast<const i32x4v*>(&ca[aoff[i][1]]);
+            fa[i] = i32x8{a0[0], a0[1], a0[2], a0[3], a1[0], a1[1], a1[2], a1[3]};
+        }
+        issue(N2 == 0 ? ls0 : N2 == 1 ? ls1 : ls2, st + 2);
+#define MISTRA_FP8_MFMA(I, J) \
+    acc[I][J] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(fa[I], fb[J], acc[I][J], 0, 0, I, ws_a, J, ws_b)
 #define MISTRA_FP8_ROW(I) \
     MISTRA_FP8_MFMA(I, 0); \
     MISTRA_FP8_MFMA(I, 1); \
@@ -1101,19 +1148,56 @@ __global__ __launch_bounds__(256, 2) void fp8_gemm_kernel(const uint8_t* __restr
         MISTRA_FP8_ROW(3);
 #undef MISTRA_FP8_ROW
 #undef MISTRA_FP8_MFMA
-        if (kb + 1 < nkb)
-        {
-            lstore(cur ^ 1);
-            cwa = swa;
-            cwb = swb;
-        }
-        __syncthreads();
-    }
+        __builtin_amdgcn_sched_barrier(0);
+        landed();
+        step_barrier();
+        __builtin_amdgcn_sched_barrier(0);
+    };
+    typedef std::integral_constant<int, 0> I0;
+    typedef std::integral_constant<int, 1> I1;
+    typedef std::integral_constant<int, 2> I2;
 
-    // output: the tile as [col][row] floats in the tile buffers, row r of column c at r ^ (c % 32) (the
-    // writes of one instruction -- 16 columns at 4 rows -- then fall on distinct banks), then rows-fastest
-    float* stage = reinterpret_cast<float*>(&lds[0][0][0]);
-    static_assert(sizeof(lds) >= 128 * 128 * sizeof(float), "output stage fits the tile buffers");
+    issue(ls0, 0);  // (no K: only out-of-range loads; the tile is written as zeros)
+    issue(ls1, 1);
+    landed();
+    step_barrier();
+    // three steps per trip, one per stage, then the last nsteps % 3 (no exit inside a trip: with one, the
+    // compiler waited for every load into LDS at the loop's head)
+    int st = 0;
+    for (; st + 3 <= nsteps; st += 3)
+    {
+        step(st, I0());
+        step(st + 1, I1());
+        step(st + 2, I2());
+    }
+    if (st < nsteps) step(st, I0());
+    if (st + 1 < nsteps) step(st + 1, I1());
+    // the loads of the two steps past the end may still be writing zeros into LDS: none may outlive the
+    // workgroup (its LDS goes to the next one)
+#if defined(__HIP_DEVICE_COMPILE__)
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+#endif
+
+    if (!g.C)
+    {
+        // into the tile of M_q (padded rows and columns included): element (r, c) of the tile at
+        // c kMTileRows + r, block (i, j) of the lane at its base + 64 i + 16 kMTileRows 4 j bytes
+        float* mt = reinterpret_cast<float*>(M) + int64_t(q) * g.mp * g.np + (int64_t(tn) * tiles_m + tm) * (256 * 128);
+        const __amdgpu_buffer_rsrc_t rm = __builtin_amdgcn_make_buffer_rsrc(mt, (short)0, 256 * 128 * 4, 0x00020000);
+        const uint32_t mbase = uint32_t(((wn + (lane & 15)) * 256 + wm + 4 * (lane >> 4)) * 4);
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(i32x4v, acc[i][j]), rm, mbase,
+                                                       64 * i + 16384 * j, 0);
+        return;
+    }
+    // The C update of a single product: through this wave's 16 KiB of the idle tile buffers, as
+    // [column][row] with row r of column c at r ^ (c % 32), then one row per lane and column.  The
+    // barrier: every wave's loads into LDS done (each waited for its own above).
+    __syncthreads();
+    float* stg = reinterpret_cast<float*>((wave < 3 ? ls0 : wave < 6 ? ls1 : ls2) + (wave % 3) * 16384);
 #pragma unroll
     for (int i = 0; i < 4; i++)
 #pragma unroll
@@ -1121,25 +1205,18 @@ __global__ __launch_bounds__(256, 2) void fp8_gemm_kernel(const uint8_t* __restr
 #pragma unroll
             for (int v = 0; v < 4; v++)
             {
-                const int c = wn + 16 * j + (lane & 15), r = wm + 16 * i + 4 * (lane >> 4) + v;
-                stage[c * 128 + (r ^ (c & 31))] = acc[i][j][v];
+                const int c = 16 * j + (lane & 15), r = 16 * i + 4 * (lane >> 4) + v;
+                stg[c * 64 + (r ^ (c & 31))] = acc[i][j][v];
             }
-    __syncthreads();
     double* Mq = M + int64_t(q) * g.ms * g.ns;
-    float* Mf = reinterpret_cast<float*>(M) + int64_t(q) * g.ms * g.ns;
-    const int r = tid & 127;
-    const int64_t row = row0 + r;
-    if (row < g.ms)
-        for (int c = tid >> 7; c < 128; c += 2)
-        {
-            const int64_t col = col0 + c;
-            if (col >= g.ns) continue;
-            const float v = stage[c * 128 + (r ^ (c & 31))];
-            if (g.m_f32 && !g.C)
-                Mf[row + col * g.ms] = v;
-            else
-                put_out(g, q, Mq, row, col, double(v));
-        }
+    const int64_t row = row0 + wm + lane;
+#pragma unroll 1
+    for (int c = 0; c < 64; c++)
+    {
+        const int64_t col = col0 + wn + c;
+        if (col >= g.ns || row >= g.ms) continue;
+        put_out(g, q, Mq, row, col, double(stg[c * 64 + (lane ^ (c & 31))]));
+    }
 }
 
 unsigned grid_of(int64_t n, int per)
@@ -1333,9 +1410,13 @@ hipError_t launch_fp8_slices(const LowpSliceArgs& a, uint32_t* sa, uint32_t* sb,
 hipError_t launch_fp8_gemm(const void* As, const void* Bs, const uint32_t* sa, const uint32_t* sb, double* M,
                            const LowpGemmArgs& g, hipStream_t s)
 {
-    if (g.mp % 128 || g.np % 128 || g.kp % 128) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(fp8_gemm_kernel, dim3(unsigned((g.mp / 128) * (g.np / 128)), unsigned(g.nlaunch)), dim3(256), 0,
-                       s, static_cast<const uint8_t*>(As), static_cast<const uint8_t*>(Bs), sa, sb, M, g);
+    if (g.mp % 256 || g.np % 128 || g.kp % 128 || g.kp >= (int64_t(1) << 23)) return hipErrorInvalidValue;
+    if ((g.kp / 128) * std::max(g.mp, g.np) >= (int64_t(1) << 31)) return hipErrorInvalidValue;
+    if (!g.C != g.m_f32) return hipErrorInvalidValue;  // products into M in FP32 tiles, or one into C
+    const int64_t tiles = (g.mp / 256) * (g.np / 128);
+    if (tiles > 0x7fffffff || g.nlaunch > 65535) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(fp8_gemm_kernel, dim3(unsigned(tiles), unsigned(g.nlaunch)), dim3(512), 0, s,
+                       static_cast<const uint8_t*>(As), static_cast<const uint8_t*>(Bs), sa, sb, M, g);
     return hipGetLastError();
 }
 

@@ -79,6 +79,34 @@ def test_bf16_modes_exact_on_small_integers(level, prec):
     assert np.array_equal(got, want)
 
 
+FP8_EXACT = [  # leaf K-steps of 128 at L0: 1, 2, 3, 4, 5, 7 (the FP8 GEMM's loop and its tails)
+    dict(m=(40, 3), n=(50,), k=(100,)),
+    dict(m=(300,), n=(130,), k=(250,)),
+    dict(m=(257,), n=(64,), k=(12, 25)),
+    dict(m=(8, 32), n=(129,), k=(500,)),
+    dict(m=(128,), n=(16, 9), k=(600,)),
+    dict(m=(30,), n=(40,), k=(900,), batch=(2,)),
+]
+
+
+@pytest.mark.parametrize("level", [0, 1, 2])
+@pytest.mark.parametrize("ci", range(len(FP8_EXACT)))
+def test_fp8_exact_on_small_integers(ci, level):
+    """Integers in [-3, 3]: every Strassen sum (|x| <= 12) has at most 4 significant bits, so its E4M3 code
+    under the block's power-of-two scale is exact, the MFMA's products and FP32 sums stay exact (below
+    2^24), and the FP32 products and their FP64 combine reproduce numpy.einsum bit for bit.  The shapes
+    cover padded rows (M not a multiple of 256, odd M, whose products take the LDS-staged store), padded
+    K and 1 to 7 K-steps per leaf."""
+    case = cases.make_case(1400 + ci, **FP8_EXACT[ci])
+    say("fp8 exact", ci, level)
+    n = [int(np.prod(case[f"shape_{x}"])) for x in "ABC"]
+    a, b, c = cases.small_ints(n[0], 40 + ci), cases.small_ints(n[1], 50 + ci), cases.small_ints(n[2], 60 + ci)
+    got = run(case, a, b, c, level, "fp8_block", 2.0, -1.0)
+    want = cases.einsum_ref(case, cases.host_tensor(case["shape_A"], a), cases.host_tensor(case["shape_B"], b),
+                            cases.host_tensor(case["shape_C"], c), 2.0, -1.0)
+    assert np.array_equal(got, want), cases.rel_err(got, want)
+
+
 @pytest.mark.parametrize("level", [0, 1, 2])
 @pytest.mark.parametrize("prec", ["ozaki_bf16", "ozaki_int8", "bf16x3", "fp8_block"])
 def test_precision_on_float_tensors(level, prec):
