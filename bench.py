@@ -39,6 +39,7 @@ Other workloads (one JSON line each, same fields; --workload NAME):
       BF16 split on FP64 copies); value = forward+backward TFLOP/s.
 """
 import argparse
+import gc
 import json
 import os
 import sys
@@ -84,6 +85,19 @@ def device_tensor(shape, dtype, torch):
     return buf.as_strided(tuple(shape), tuple(cases.colmajor_strides(shape))), buf
 
 
+class no_gc:
+    """Python's garbage collector held off over a timed region (collected just before it): a full collection
+    of this interpreter's heap takes about 60 ms, and one that fell inside the timed steps of a 14 ms call
+    once added 4 ms to each of its 10 steps."""
+
+    def __enter__(self):
+        gc.collect()
+        gc.disable()
+
+    def __exit__(self, *exc):
+        gc.enable()
+
+
 def run_variant(ts, torch, case, A, B, C, level, impl, steps, warmup, stream, flags=0):
     """Time `steps` calls after `warmup` calls; returns (ms/step, kernel ms/launch, launches, leaf flops/launch)."""
     for _ in range(warmup):
@@ -93,13 +107,14 @@ def run_variant(ts, torch, case, A, B, C, level, impl, steps, warmup, stream, fl
     ts.kernel_timing(True)
     ts.kernel_timing_read(reset=True)
     start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    start.record(stream)
-    for _ in range(steps):
-        ts.tensor_mult(1.0, A, case["idx_A"], B, case["idx_B"], 1.0, C, case["idx_C"], level=level, impl=impl,
-                       stream=stream, flags=flags)
-    stop.record(stream)
-    torch.cuda.synchronize()
+    with no_gc():
+        torch.cuda.synchronize()
+        start.record(stream)
+        for _ in range(steps):
+            ts.tensor_mult(1.0, A, case["idx_A"], B, case["idx_B"], 1.0, C, case["idx_C"], level=level, impl=impl,
+                           stream=stream, flags=flags)
+        stop.record(stream)
+        torch.cuda.synchronize()
     ms = start.elapsed_time(stop) / steps
     kms, launches, kflops = ts.kernel_timing_read(reset=True)
     ts.kernel_timing(False)
@@ -169,13 +184,14 @@ def cpu_baseline(case, threads, repeats=3, level=1):
 def timed(torch, fn, steps, warmup, stream):
     for _ in range(warmup):
         fn()
-    torch.cuda.synchronize()
     start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    start.record(stream)
-    for _ in range(steps):
-        fn()
-    stop.record(stream)
-    torch.cuda.synchronize()
+    with no_gc():
+        torch.cuda.synchronize()
+        start.record(stream)
+        for _ in range(steps):
+            fn()
+        stop.record(stream)
+        torch.cuda.synchronize()
     return start.elapsed_time(stop) / steps
 
 

@@ -323,7 +323,7 @@ Status general_mult(const tblis::tblis_tensor* A, const label_type* idx_A, const
         for (void* p : temps) (void)hipFreeAsync(p, stream);
     };
     auto alloc = [&](size_t bytes, void** out) -> Status {
-        hipError_t e = hipMallocAsync(out, bytes ? bytes : 1, stream);
+        hipError_t e = mistra::stream_alloc(out, bytes ? bytes : 1, stream);
         if (e != hipSuccess) return Status::error(MISTRA_ERR_NOMEM, std::string("general path: ") + hipGetErrorString(e));
         temps.push_back(*out);
         return Status();
@@ -458,7 +458,7 @@ Status lowp_float(const tblis::tblis_tensor* A, const label_type* idx_A, const t
         int64_t lo = 0, hi = 0;
         span(src[i], lo, hi);
         const int64_t n = hi - lo + 1;
-        hipError_t e = hipMallocAsync(&buf[i], size_t(n) * 8, stream);
+        hipError_t e = mistra::stream_alloc(&buf[i], size_t(n) * 8, stream);
         if (e == hipSuccess)
             e = launch_f32_to_f64(static_cast<const float*>(src[i]->data) + lo, static_cast<double*>(buf[i]), n,
                                   stream);
@@ -580,9 +580,9 @@ Status tensor_mult(const tblis::tblis_tensor* A, const label_type* idx_A, const 
         const size_t nA = empty_tensor(A) ? 0 : size_t(hiA - loA + 1) * elem;
         const size_t nB = empty_tensor(B) ? 0 : size_t(hiB - loB + 1) * elem;
         const size_t nC = size_t(hiC - loC + 1) * elem;
-        if (nA) TRY_HIP(hipMallocAsync(&bufA, nA, stream));
-        if (nB) TRY_HIP(hipMallocAsync(&bufB, nB, stream));
-        TRY_HIP(hipMallocAsync(&bufC, nC, stream));
+        if (nA) TRY_HIP(mistra::stream_alloc(&bufA, nA, stream));
+        if (nB) TRY_HIP(mistra::stream_alloc(&bufB, nB, stream));
+        TRY_HIP(mistra::stream_alloc(&bufC, nC, stream));
         if (nA)
             TRY_HIP(hipMemcpyAsync(bufA, static_cast<const char*>(A->data) + loA * elem, nA, hipMemcpyHostToDevice,
                                    stream));
@@ -945,6 +945,7 @@ int mistra_release_workspace(void)
 {
     mistra::release_batch_graphs();
     mistra::release_all();
+    mistra::trim_stream_pool();
     return 0;
 }
 
@@ -1003,7 +1004,7 @@ int mistra_rel_error(int type, const void* X, const void* Y, long long n, double
     if (type != TYPE_FLOAT && type != TYPE_DOUBLE) return fail(Status::error(MISTRA_ERR_ARG, "type"));
     hipStream_t s = static_cast<hipStream_t>(stream);
     double* buf = nullptr;
-    hipError_t e = hipMallocAsync(reinterpret_cast<void**>(&buf), sizeof(double) * (2 + 2 * 1024), s);
+    hipError_t e = mistra::stream_alloc(reinterpret_cast<void**>(&buf), sizeof(double) * (2 + 2 * 1024), s);
     if (e != hipSuccess) return fail(Status::error(MISTRA_ERR_NOMEM, hipGetErrorString(e)));
     e = mistra::launch_rel_error(type == TYPE_DOUBLE ? 8 : 4, X, Y, n, buf, s);
     double h[2] = {0, 0};

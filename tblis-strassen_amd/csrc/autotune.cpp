@@ -302,7 +302,7 @@ Status autotune(const tblis::tblis_tensor* A, const label_type* idx_A, const tbl
         }
         const size_t bytes = size_t(hi - lo + 1) * elem;
         void* scratch = nullptr;
-        hipError_t e = hipMallocAsync(&scratch, bytes, stream);
+        hipError_t e = stream_alloc(&scratch, bytes, stream);
         if (e != hipSuccess) return Status::error(MISTRA_ERR_NOMEM, hipGetErrorString(e));
         e = hipMemcpyAsync(scratch, static_cast<const char*>(C->data) + lo * elem, bytes, hipMemcpyDeviceToDevice,
                            stream);

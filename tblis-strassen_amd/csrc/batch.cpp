@@ -173,7 +173,7 @@ Status upload_tables(const std::vector<GemmArgs>& jobs, const std::vector<int64_
     std::vector<unsigned char> host(total, 0);
     if (jb) std::memcpy(host.data(), jobs.data(), jb);
     if (!first.empty()) std::memcpy(host.data() + first_base, first.data(), first.size() * sizeof(int64_t));
-    hipError_t e = async ? hipMallocAsync(&buf, total ? total : 1, stream) : hipMalloc(&buf, total ? total : 1);
+    hipError_t e = async ? stream_alloc(&buf, total ? total : 1, stream) : hipMalloc(&buf, total ? total : 1);
     if (e != hipSuccess) return Status::error(MISTRA_ERR_NOMEM, hipGetErrorString(e));
     e = async ? hipMemcpyAsync(buf, host.data(), total, hipMemcpyHostToDevice, stream)
               : hipMemcpy(buf, host.data(), total, hipMemcpyHostToDevice);

@@ -1163,6 +1163,41 @@ void release_all()
     g_plans.clear();
 }
 
+hipError_t stream_alloc(void** out, size_t bytes, hipStream_t stream)
+{
+    static std::mutex m;
+    static std::vector<bool> kept;  // per device: release threshold raised
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    {
+        std::lock_guard<std::mutex> lk(m);
+        if (size_t(dev) >= kept.size()) kept.resize(size_t(dev) + 1, false);
+        if (!kept[size_t(dev)])
+        {
+            hipMemPool_t pool = nullptr;
+            uint64_t keep = UINT64_MAX;
+            if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess)
+                (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+            (void)hipGetLastError();  // a pool without the attribute keeps the runtime's default
+            kept[size_t(dev)] = true;
+        }
+    }
+    return hipMallocAsync(out, bytes, stream);
+}
+
+void trim_stream_pool()
+{
+    int dev = 0;
+    hipMemPool_t pool = nullptr;
+    if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess)
+    {
+        (void)hipDeviceSynchronize();
+        (void)hipMemPoolTrimTo(pool, 0);
+    }
+    (void)hipGetLastError();
+}
+
 namespace {
 
 GemmArgs base_args(const Plan& p, const Call& c)
@@ -1308,7 +1343,7 @@ hipError_t fork_end(hipStream_t main, const SideFork& f)
 // else from the stream-ordered pool.
 hipError_t ws_alloc(const Call& c, void** out, size_t bytes, hipStream_t s)
 {
-    if (!c.ws) return hipMallocAsync(out, bytes, s);
+    if (!c.ws) return stream_alloc(out, bytes, s);
     const size_t off = align256(c.ws->used);
     if (off > c.ws->size || bytes > c.ws->size - off) return hipErrorOutOfMemory;
     *out = c.ws->base + off;
@@ -1791,7 +1826,7 @@ Status run_plan_lowp(const Plan& p, const Call& c, int precision, int nslices, h
         hipError_t e = hipSuccess;
         for (auto pr : {std::make_pair(&S, nS), std::make_pair(&T, nT), std::make_pair(&M, nM), std::make_pair(&EA, nEA),
                         std::make_pair(&EB, nEB), std::make_pair(&XA, nXA), std::make_pair(&XB, nXB)})
-            if (e == hipSuccess) e = hipMallocAsync(&pr.first->p, pr.second ? pr.second : 1, stream);
+            if (e == hipSuccess) e = stream_alloc(&pr.first->p, pr.second ? pr.second : 1, stream);
         for (int64_t z = 0; z < p.nbatch && e == hipSuccess; z++)
         {
             SumArgs sa;

@@ -305,4 +305,12 @@ void timing_end(hipStream_t s, double flops);
 
 void release_all();
 
+// Stream-ordered workspace (hipMallocAsync on the device's default pool).  The pool keeps what calls
+// free rather than handing it back to the driver at every synchronization (its default release
+// threshold is 0): a call after a synchronization would otherwise map its workspace again (about 40 ms
+// for the 3.5 GiB of an Ozaki BF16 1-level call at 8192^3).  trim_stream_pool() gives it back
+// (mistra_release_workspace).
+hipError_t stream_alloc(void** out, size_t bytes, hipStream_t stream);
+void trim_stream_pool();
+
 }  // namespace mistra
