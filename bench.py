@@ -10,7 +10,8 @@ test_strassen does).  A step is one contraction call; inputs are resident in HBM
 Variants: L0 (classical MFMA TC), L1/L2 x {ABC, AB, Naive}; ABC in each of its forms: staged (the outer
 level's 7 operand sums of A and of B written to HBM, the inner sums -- none at L1, 2 terms at L2 -- formed
 in the leaf kernel), staged_all (at L2 all 49 sums written to HBM) and fused (every sum formed in the
-leaf kernel, no workspace).  C is updated in the leaf kernel's epilogue in all of them.  Errors are ||C - C_L0||_F / ||C_L0||_F of a
+leaf kernel, no workspace); +atomic: every product in one launch, adding into C with FP64 atomic adds
+(run-to-run differences in the last bits).  C is updated in the leaf kernel's epilogue in all of them.  Errors are ||C - C_L0||_F / ||C_L0||_F of a
 single call from C = 0.  The reported `value` is the fastest Strassen variant (L1 or L2, any impl) that
 meets its tolerance (1e-12 at L1, 5e-12 at L2); every variant, ABC included, is listed under `variants`.
 
@@ -424,7 +425,7 @@ def bench_tc(ts, torch, args, stream, workload, variant_spec):
     # a form is one or more of these joined by "+" ("persist": every stage product in one persistent
     # launch instead of one leaf launch per stage)
     forms = {"": 0, "fused": ts.FLAG_FUSED_SUMS, "staged": ts.FLAG_OUTER_STAGED, "staged_all": ts.FLAG_ALL_STAGED,
-             "persist": ts.FLAG_STAGE_PERSIST}
+             "persist": ts.FLAG_STAGE_PERSIST, "atomic": ts.FLAG_ATOMIC_C}
     tiles = {0: "128x128", 1: "64x64", 2: "128x64", 3: "64x128"}
     impls = {0: "abc", 3: "naive", 4: "ab"}
     for spec in variant_spec.split(","):
@@ -527,11 +528,13 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS) + ["c2p", "c3", "c4", "c5"])
-    ap.add_argument("--variants", default="0:abc,1:abc:staged,1:abc:fused,2:abc:staged_all,2:abc:staged,"
-                                          "2:abc:fused,1:ab,2:ab,1:naive,2:naive,auto",
+    ap.add_argument("--variants", default="0:abc,1:abc:staged,1:abc:fused,1:abc:staged+atomic,2:abc:staged_all,"
+                                          "2:abc:staged_all+atomic,2:abc:staged,2:abc:fused,1:ab,2:ab,1:naive,"
+                                          "2:naive,auto",
                     help="level:impl[:form], form = fused (every operand sum in the leaf kernel), staged (the "
-                         "outer level's sums staged in HBM) or staged_all (every level's sums staged); default: "
-                         "the library's choice; `auto`: the autotuner's level, impl and register tile")
+                         "outer level's sums staged in HBM) or staged_all (every level's sums staged), optionally "
+                         "+atomic (all products in one launch with atomic C updates) or +persist; default: the "
+                         "library's choice; `auto`: the autotuner's level, impl and register tile")
     ap.add_argument("--modes", default="", help="c2p / c3: precision:level list (default: all modes)")
     ap.add_argument("--precision-modes", default="fp64:0,ozaki_bf16:1,ozaki_bf16:2,ozaki_int8:1,ozaki_int8:2,"
                                                  "bf16x3:1",

@@ -145,11 +145,16 @@ enum
                                          the cost model decides.                                         */
     MISTRA_FLAG_BATCH_GROUPED = 1u << 6, /* mistra_batch_mult: one grouped launch per (Strassen stage round,
                                          kernel variant) instead of the single persistent launch        */
-    MISTRA_FLAG_STAGE_PERSIST = 1u << 7 /* ABC with every sum staged (level 1 staged, level 2 all
+    MISTRA_FLAG_STAGE_PERSIST = 1u << 7, /* ABC with every sum staged (level 1 staged, level 2 all
                                          staged): all products in one persistent launch, whose tiles
                                          wait only for earlier products updating the same C tile,
                                          instead of one leaf launch per stage (products with disjoint C
                                          targets) in stream order                                        */
+    MISTRA_FLAG_ATOMIC_C = 1u << 8      /* ABC with every sum staged, FP64: all products in one launch,
+                                         each adding alpha*coeff*M_p into its C sub-blocks with hardware
+                                         FP64 atomic adds (C scaled by beta first).  The adds into one
+                                         C element come in no fixed order: results may differ from run
+                                         to run in the last bits.  Ignored where it does not apply.      */
 };
 
 /* Error codes (0 = success). */

@@ -27,6 +27,7 @@ SOURCES = [
     "leaf_gemm.cpp",
     "leaf_f64_multi.hip",
     "leaf_f64_store.hip",
+    "leaf_f64_atomic.hip",
     "leaf_f32_multi.hip",
     "leaf_f32_store.hip",
     "aux_kernels.hip",

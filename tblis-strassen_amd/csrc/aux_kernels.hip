@@ -680,6 +680,36 @@ __global__ __launch_bounds__(256) void scale_kernel(T* C, StridedDims d, int64_t
     }
 }
 
+// One workgroup per (64-row x 4-column block, product): every target the product writes first.
+__global__ __launch_bounds__(256) void scale_first_targets_kernel(double* C, const int64_t* c_m, const int64_t* c_n,
+                                                                  int64_t ms, int64_t ns, const ProdDesc* prods,
+                                                                  double beta)
+{
+    const int64_t row_tiles = (ms + 63) / 64;
+    const int64_t r = (int64_t(blockIdx.x) % row_tiles) * 64 + (threadIdx.x & 63);
+    const int64_t col = (int64_t(blockIdx.x) / row_tiles) * 4 + (threadIdx.x >> 6);
+    if (r >= ms || col >= ns) return;
+    const ProdDesc& P = prods[blockIdx.y];
+    const int64_t off = c_m[r] + c_n[col];
+    for (int t = 0; t < P.nC; t++)
+        if ((P.first_mask >> t) & 1)
+        {
+            double* x = C + P.c[t].base + off;
+            *x = beta == 0.0 ? 0.0 : beta * *x;
+        }
+}
+
+hipError_t launch_scale_first_targets(void* C, const int64_t* c_m, const int64_t* c_n, int64_t ms, int64_t ns,
+                                      const ProdDesc* prods, int nprod, double beta, hipStream_t s)
+{
+    if (ms <= 0 || ns <= 0 || nprod <= 0) return hipSuccess;
+    const int64_t blocks = ((ms + 63) / 64) * ((ns + 3) / 4);
+    if (blocks > 0x7fffffff || nprod > 65535) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(scale_first_targets_kernel, dim3(unsigned(blocks), unsigned(nprod)), dim3(256), 0, s,
+                       static_cast<double*>(C), c_m, c_n, ms, ns, prods, beta);
+    return hipGetLastError();
+}
+
 hipError_t launch_scale(int elem_size, void* C, const StridedDims& d, double beta, hipStream_t s)
 {
     int64_t n = 1;

@@ -1374,6 +1374,15 @@ namespace {
 // Polls a stage-launch wait makes before it gives up (each a coherent load and a short sleep: seconds)
 constexpr long long kStageSpin = 1ll << 22;
 
+// Register tile of the atomic one-launch form: the plan's when the caller or the environment named one,
+// else 64x128 (c2, 8192^3: L2 77.3 eff. TFLOP/s against 76.2 with 64x64, 75.6 with 128x128, 75.9 with
+// 128x64; L1 75.1 against 75.9 with 128x128, whose 4096-deep leaves keep the square tile ahead).
+int atomic_tile(const Plan& p)
+{
+    if (p.tile_given || forced_tile() >= 0) return p.tile;
+    return p.ks <= 2048 ? TILE_64x128 : TILE_128;
+}
+
 // Outer-staged ABC (Plan::outer_staged): the outer sums into S and T, then the products over the inner
 // sums of S and T (2 terms at level 2 with one staged level, else 1) with the multi-C epilogue on C: one
 // launch per stage, or with MISTRA_FLAG_STAGE_PERSIST (when every product has one term per operand,
@@ -1512,6 +1521,23 @@ hipError_t run_outer_staged(const Plan& p, const Call& c, hipStream_t stream)
         }
         else
             (void)hipGetLastError();
+    }
+    // MISTRA_FLAG_ATOMIC_C: C's targets scaled by beta once, then every product in one launch adding
+    // alpha*coeff*M_p into its targets with FP64 atomic adds (single-term products only)
+    bool single_term = p.elem == 8 && !p.h_inner_stage_terms.empty();
+    for (int t : p.h_inner_stage_terms) single_term = single_term && t == 1;
+    if (!stages_done && e == hipSuccess && single_term && (c.flags & MISTRA_FLAG_ATOMIC_C))
+    {
+        const int nprod = p.h_stage_begin.back();
+        if (c.beta != 1.0)
+            e = launch_scale_first_targets(g.C, p.c_m, p.c_n, p.ms, p.ns, p.inner_staged, nprod, c.beta, stream);
+        g.prods = p.inner_staged;
+        g.nprod = nprod;
+        timing_begin(stream);
+        if (e == hipSuccess)
+            e = launch_leaf_gemm(p.elem, EPI_ATOMIC_C, atomic_tile(p), 1, p.kf_inner.mode, g, false, false, stream);
+        timing_end(stream, leaf_flops(p, g.nprod, p.ms, p.ns, p.ks));
+        stages_done = true;
     }
     for (size_t st = 0; !stages_done && e == hipSuccess && st + 1 < p.h_stage_begin.size(); st++)
     {

@@ -33,7 +33,9 @@ struct ProdDesc
 enum EpiKind
 {
     EPI_MULTI_C = 0,  // C_c = (first ? beta*C_c : C_c) + alpha*coeff_c*M for each target (ABC, classical)
-    EPI_STORE = 1     // M_p stored dense, column-major, ld = ms, product p at M + p*ms*ns (AB / Naive)
+    EPI_STORE = 1,    // M_p stored dense, column-major, ld = ms, product p at M + p*ms*ns (AB / Naive)
+    EPI_ATOMIC_C = 2  // C_c += alpha*coeff_c*M for each target by floating-point atomic adds: the products of
+                      // one launch may share targets (ABC in one launch; beta applied to C beforehand)
 };
 
 // How the leaf kernel addresses K (chosen per launch by the host).

@@ -157,6 +157,11 @@ struct StridedDims
     int64_t stride[8];
 };
 hipError_t launch_scale(int elem_size, void* C, const StridedDims& d, double beta, hipStream_t stream);
+// C_t = beta*C_t (beta == 0: set to zero) over every C target t that product p writes first (ProdDesc::
+// first_mask), for the launch products prods[0 .. nprod): the beta pass of the atomic multi-C form.  Targets
+// are translates of the first C sub-block: element (r, c) at C + base_t + c_m[r] + c_n[c], r < ms, c < ns.
+hipError_t launch_scale_first_targets(void* C, const int64_t* c_m, const int64_t* c_n, int64_t ms, int64_t ns,
+                                      const ProdDesc* prods, int nprod, double beta, hipStream_t stream);
 // FP32 tensors in the low-precision modes run through FP64 copies: dst[i] = src[i] over n elements, and
 // back, dst[off] = float(src[off]) over the offsets of a strided tensor.
 hipError_t launch_f32_to_f64(const float* src, double* dst, int64_t n, hipStream_t stream);

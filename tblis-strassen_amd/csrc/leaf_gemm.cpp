@@ -11,6 +11,7 @@ MISTRA_DECLARE_LEAF(launch_leaf_f64_multi)
 MISTRA_DECLARE_LEAF(launch_leaf_f64_store)
 MISTRA_DECLARE_LEAF(launch_leaf_f32_multi)
 MISTRA_DECLARE_LEAF(launch_leaf_f32_store)
+MISTRA_DECLARE_LEAF(launch_leaf_f64_atomic)
 #undef MISTRA_DECLARE_LEAF
 hipError_t launch_leaf_f64_grouped(int tile, int terms, int kmode, const Grouped& q, bool a_mfast, bool b_nfast,
                                    hipStream_t s);
@@ -72,6 +73,9 @@ hipError_t launch_leaf_gemm(int elem_size, int epi, int tile, int terms, int kmo
     // grid.y indexes the launch's products; the products of a multi-C launch must write disjoint C
     // sub-blocks (the caller groups them so), since their read-modify-write updates are not ordered
     dim3 grid(unsigned(g.tiles_m) * unsigned(g.tiles_n), unsigned(g.nprod), unsigned(g.nbatch));
+    if (epi == EPI_ATOMIC_C)
+        return elem_size == 8 ? launch_leaf_f64_atomic(tile, terms, kmode, g, a_mfast, b_nfast, stream, grid)
+                              : hipErrorInvalidValue;
     if (elem_size == 8)
         return epi == EPI_STORE ? launch_leaf_f64_store(tile, terms, kmode, g, a_mfast, b_nfast, stream, grid)
                                 : launch_leaf_f64_multi(tile, terms, kmode, g, a_mfast, b_nfast, stream, grid);

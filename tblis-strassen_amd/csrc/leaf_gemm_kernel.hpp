@@ -211,7 +211,8 @@ __device__ __forceinline__ void leaf_gemm_body(const GemmArgs& g, int bx, int by
     constexpr int SAN = kStageB ? SAN0 : cmax(SAN0, 16 * (TN + 1));
     __shared__ __attribute__((aligned(16))) T sA[2][SAN];
     __shared__ __attribute__((aligned(16))) T sB[2][SBN];
-    __shared__ int64_t ccol[EPI == EPI_MULTI_C ? kMaxTerms : 1][TN];  // C column offsets per target
+    constexpr bool kMultiC = EPI == EPI_MULTI_C || EPI == EPI_ATOMIC_C;
+    __shared__ int64_t ccol[kMultiC ? kMaxTerms : 1][TN];  // C column offsets per target
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -253,7 +254,7 @@ __device__ __forceinline__ void leaf_gemm_body(const GemmArgs& g, int bx, int by
     // C sub-block, else they are addressed through (roff, coff) into the scatter vectors.
     // `ab`: a factor (+-1) the operands were not scaled by (the direct-to-LDS loads load them as stored)
     auto epilogue = [&](const ProdDesc& P, int p, auto& acc, bool translate, double ab) {
-        if constexpr (EPI == EPI_MULTI_C)
+        if constexpr (kMultiC)
         {
             // Multi-C update through LDS: for each 16-row block i, the waves write their accumulators
             // (straight from the accumulator registers) into a 32 x TN staging tile, then all 256
@@ -299,6 +300,20 @@ __device__ __forceinline__ void leaf_gemm_body(const GemmArgs& g, int bx, int by
                 for (int c = 0; c < nC; c++)
                 {
                     const T coef = T(alpha * P.c[c].coeff * ab);
+                    if constexpr (EPI == EPI_ATOMIC_C)
+                    {
+                        // C already scaled by beta: one hardware atomic add per element (the products of the
+                        // launch that share this target add in an unspecified order)
+                        T* Crow = (T*)(Cg + P.c[c].base + (translate ? rbase : g.c_m[P.c[c].roff + growc]));
+                        const int64_t* cc = ccol[translate ? 0 : c];
+#pragma unroll
+                        for (int e = 0; e < CPER; e++)
+                        {
+                            const int col = cg + e * CSTEP;
+                            if (rok && col0 + col < ns) unsafeAtomicAdd(&Crow[cc[col]], coef * stage[rl * LDE + col]);
+                        }
+                        continue;
+                    }
                     const bool first = (first_mask >> c) & 1;
                     const bool read_c = !(first && beta == 0.0);
                     const T bet = first ? T(beta) : T(1);

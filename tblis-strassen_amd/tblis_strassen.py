@@ -26,6 +26,7 @@ FLAG_OUTER_STAGED = 16
 FLAG_ALL_STAGED = 32
 FLAG_BATCH_GROUPED = 64
 FLAG_STAGE_PERSIST = 128
+FLAG_ATOMIC_C = 256
 
 
 class MistraError(RuntimeError):
