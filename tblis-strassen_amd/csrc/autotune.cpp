@@ -47,9 +47,11 @@ std::string shape_key(const tblis::tblis_tensor* A, const label_type* idx_A, con
 // taken at 1.3x.
 // The rectangular tiles (single-term launches only; rates net of the wave quantisation launch_s adds)
 // against 128 x 128: FP64 about 1.5% above it (64 x 128) or level (128 x 64) when the epilogue stores the
-// product (Naive's dense GEMM at 8192^3), 1.4% / 2.2% below it with the multi-C epilogue (the classical
-// launch), 2% above it in the register-staged form (an operand that is not K-fast); FP32 (classical at
-// m = n = k = 4096) 1.23x / 1.16x with both operands K-fast, 1.04x / 1.03x otherwise.
+// product (Naive's dense GEMM at 8192^3), with the multi-C epilogue (the classical launch) 2.2% below it
+// (128 x 64) and, for 64 x 128, 0.98 + 45 / ks of it: 1.4% below at a leaf depth of 8192, 2.4% above at
+// 1024 (m = n = 16384, k = 1024), where its three resident workgroups per CU hide more of each tile's
+// start and end; 2% above it in the register-staged form (an operand that is not K-fast); FP32 (classical
+// at m = n = k = 4096) 1.23x / 1.16x with both operands K-fast, 1.04x / 1.03x otherwise.
 double leaf_rate(int elem, int tile, int terms, bool dma, int64_t ks, bool store)
 {
     static const double r128[3] = {61.0, 57.5, 46.0}, r64[3] = {57.0, 46.0, 38.0}, f32[3] = {1.8, 1.5, 1.3};
@@ -61,7 +63,8 @@ double leaf_rate(int elem, int tile, int terms, bool dma, int64_t ks, bool store
     {
         const bool wide = tile == TILE_64x128;
         if (elem == 4) return f32[0] * r * (dma ? (wide ? 1.23 : 1.16) : (wide ? 1.037 : 1.034));
-        return r * (!dma ? 1.02 : store ? (wide ? 1.015 : 1.003) : (wide ? 0.986 : 0.978));
+        const double wide_multi_c = 0.98 + 45.0 / double(std::max<int64_t>(ks, 1024));
+        return r * (!dma ? 1.02 : store ? (wide ? 1.015 : 1.003) : (wide ? wide_multi_c : 0.978));
     }
     return elem == 8 ? r : f32[ti] * r;
 }

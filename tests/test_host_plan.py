@@ -268,6 +268,20 @@ def test_autotuner_model_picks_measured_fastest_at_c2():
     assert choice == (2, ts.IMPL["naive"], 3), choice
 
 
+def test_autotuner_model_picks_measured_fastest_at_c3():
+    # m = n = 16384, k = 1024 (2 x 2 x 2 indices, column-major): classical with 64 x 128 tiles measured
+    # 8.11 / 8.23 ms against 8.28 / 8.32 with 128 x 128 on MI355X (two runs; 128 x 64 8.93, every Strassen
+    # form slower): at a 1024-deep leaf the wide tile's three resident workgroups per CU hide more of each
+    # tile's start and end
+    case = cases.make_case(4242, m=(128, 128), n=(128, 128), k=(32, 32))
+    buf = np.zeros(1)
+    A, B, C = (np.lib.stride_tricks.as_strided(buf, shape=case[f"shape_{x}"],
+                                               strides=[8 * s for s in cases.colmajor_strides(case[f"shape_{x}"])])
+               for x in "ABC")
+    choice = ts.autotune(A, case["idx_A"], B, case["idx_B"], C, case["idx_C"])
+    assert choice == (0, 0, 3), choice
+
+
 def test_abc_stage_counts():
     # first-fit-decreasing packing by C targets: 4 launches at one level, 16 at two (optimal for 49
     # products of equal size: every stage but one holds an even number of products)
