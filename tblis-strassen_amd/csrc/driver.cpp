@@ -154,6 +154,17 @@ bool is_translate(const std::vector<int64_t>& v, int64_t off, int64_t len)
     return true;
 }
 
+// d if v[k] = v[0] + k d for every k < len (len >= 1), else -1 (also for a negative d)
+int64_t affine_step(const std::vector<int64_t>& v, int64_t len)
+{
+    if (len < 1 || int64_t(v.size()) < len) return -1;
+    const int64_t d = len > 1 ? v[1] - v[0] : 0;
+    if (d < 0) return -1;
+    for (int64_t k = 0; k < len; k++)
+        if (v[size_t(k)] != v[0] + k * d) return -1;
+    return d;
+}
+
 // A launch may use the fast kernel body when every sum term's sub-block (A, B and the C targets) is a
 // translate of the first sub-block in its operand's scatter vectors, so that a term is the first
 // sub-block shifted by a uniform offset (the fast product table stores it as the term's base).  Then:
@@ -660,6 +671,8 @@ Status build_plan_host(const tblis::tblis_tensor* A, const label_type* idx_A, co
     p.h_b_n = std::move(b_n);
     p.h_c_m = std::move(c_m);
     p.h_c_n = std::move(c_n);
+    p.a_k_step = affine_step(p.h_a_k, p.ks);
+    p.b_k_step = affine_step(p.h_b_k, p.ks);
     choose_outer_staged(p, flags);
     // Staged ABC whose stage launches are all single-term (level 1 staged, level 2 all staged) over leaves
     // at most 2048 deep: 64 x 64 tiles, whose four resident workgroups per CU overlap the multi-C epilogues
@@ -1908,6 +1921,8 @@ Status run_plan_lowp(const Plan& p, const Call& c, int precision, int nslices, h
                 fa.X = static_cast<const double*>(isA ? c.A : c.B);
                 fa.zoff = isA ? p.h_batch_a[size_t(z)] : p.h_batch_b[size_t(z)];
                 fa.rs = isA ? p.a_k : p.b_k;
+                fa.rs_step = isA ? p.a_k_step : p.b_k_step;
+                fa.rs0 = fa.rs_step >= 0 ? (isA ? p.h_a_k[0] : p.h_b_k[0]) : 0;
                 fa.cs = isA ? p.a_m : p.b_n;
                 fa.ks = ks;
                 fa.rows = isA ? ms : ns;

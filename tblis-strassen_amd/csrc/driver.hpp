@@ -68,6 +68,8 @@ struct Plan
 
     // host copies of the index data (kept for the reduce path and for tests)
     std::vector<int64_t> h_a_m, h_a_k, h_b_k, h_b_n, h_c_m, h_c_n, h_batch_a, h_batch_b, h_batch_c;
+    // step d when the leaf's K offsets are affine (k_off[k] = k_off[0] + k d for k < ks), else -1
+    int64_t a_k_step = -1, b_k_step = -1;
     std::vector<ProdDesc> h_prods;  // [0, nprod) core products, then one per fringe pass
     int nprod = 0;
     std::vector<Pass> fringe;

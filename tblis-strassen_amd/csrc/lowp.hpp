@@ -37,6 +37,7 @@ struct SumSliceArgs
     const double* X;
     int64_t zoff;
     const int64_t* rs;  // K offsets, ks of them
+    int64_t rs0, rs_step;  // rs_step >= 0: rs[k] = rs0 + k rs_step (the kernel computes it), else -1
     const int64_t* cs;  // row offsets, rows of them
     int64_t ks, rows, cp, kp;
     // level 1: 4 sub-blocks and coef[q][s] for the 7 sums; level 2: 16 sub-blocks and the 7 x 4 one-level

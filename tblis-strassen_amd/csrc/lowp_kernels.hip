@@ -204,10 +204,18 @@ __global__ __launch_bounds__(256) void slice_kfast_kernel(const double* __restri
 // sub-blocks -- for the 7 largest magnitudes (exponents), then for the digits -- instead of staging the
 // sums in HBM and reading them back.  The sums and digits are bit for bit those of the one-pass sums
 // kernel followed by slice_kfast_kernel.
-template <int NSL, typename D>
+// In the exponent pass a lane takes 4 K positions (k0 + 64 u) per trip and issues their K-offset loads
+// together, then all their source loads; the digit pass loads the next position's values, and the K
+// offset of the one after, before forming the current one's digits.  With one position per trip and no
+// look-ahead, each trip waited for its offset, then for its values, and a wave (one per row, 16 per CU
+// at most) had one round of loads in flight; an offset load issued after a trip's digit stores would
+// also wait for those stores (gfx950 counts loads and stores on one in-order counter).
+// AFF: the K offsets are affine (SumSliceArgs::rs_step >= 0) and computed instead of loaded.
+template <int NSL, typename D, bool AFF>
 __global__ __launch_bounds__(256) void sum_slice_kfast_kernel(SumSliceArgs a)
 {
     constexpr int NQ = 7, NS = 4;  // (shift[0..3] and coef[q][s] of SumSliceArgs)
+    constexpr int UM = 4;  // K positions per lane and trip in the exponent pass
     const int lane = threadIdx.x & 63;
     const int64_t slab = int64_t(NQ) * a.cp * a.kp;
     D* __restrict__ out = static_cast<D*>(a.out);
@@ -215,11 +223,23 @@ __global__ __launch_bounds__(256) void sum_slice_kfast_kernel(SumSliceArgs a)
     {
         const bool real = i < a.rows;
         const double* __restrict__ X = a.X + a.zoff + (real ? a.cs[i] : 0);
-        auto sums = [&](int64_t k, double (&y)[NQ]) {
-            const int64_t rk = a.rs[k];
-            double v[NS];
+        // the source values of positions k0 + 64 u (k0 < ks); positions at or past ks load those of ks - 1
+        // (in range, not used)
+        auto load = [&](int64_t k0, auto& v) {
+            constexpr int U = std::extent<std::remove_reference_t<decltype(v)>>::value;
+            int64_t rk[U];
 #pragma unroll
-            for (int s = 0; s < NS; s++) v[s] = X[a.shift[s] + rk];
+            for (int u = 0; u < U; u++)
+            {
+                const int64_t k = std::min<int64_t>(k0 + 64 * u, a.ks - 1);
+                rk[u] = AFF ? a.rs0 + k * a.rs_step : a.rs[k];
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++)
+#pragma unroll
+                for (int s = 0; s < NS; s++) v[u][s] = X[a.shift[s] + rk[u]];
+        };
+        auto sums = [&](const double (&v)[NS], double (&y)[NQ]) {
 #pragma unroll
             for (int q = 0; q < NQ; q++)
             {
@@ -237,12 +257,19 @@ __global__ __launch_bounds__(256) void sum_slice_kfast_kernel(SumSliceArgs a)
 #pragma unroll
         for (int q = 0; q < NQ; q++) mx[q] = 0.0;
         if (real)
-            for (int64_t k = lane; k < a.ks; k += 64)
+            for (int64_t k0 = lane; k0 < a.ks; k0 += 64 * UM)
             {
-                double y[NQ];
-                sums(k, y);
+                double v[UM][NS];
+                load(k0, v);
 #pragma unroll
-                for (int q = 0; q < NQ; q++) mx[q] = fmax(mx[q], fabs(y[q]));
+                for (int u = 0; u < UM; u++)
+                {
+                    const bool in = k0 + 64 * u < a.ks;
+                    double y[NQ];
+                    sums(v[u], y);
+#pragma unroll
+                    for (int q = 0; q < NQ; q++) mx[q] = fmax(mx[q], in ? fabs(y[q]) : 0.0);
+                }
             }
         int ex[NQ];
 #pragma unroll
@@ -253,12 +280,32 @@ __global__ __launch_bounds__(256) void sum_slice_kfast_kernel(SumSliceArgs a)
             ex[q] = exp_of(m);
             if (real && lane == 0) a.e[q * a.rows + i] = ex[q];
         }
+        // digit pass: position k's values were loaded in the previous trip, k + 64's go out before k's
+        // digits are formed and stored, and so does the K offset of k + 128 (loaded, not computed): the
+        // next trip's loads then wait for no store of this one
+        auto koff = [&](int64_t k) { return AFF ? a.rs0 + k * a.rs_step : a.rs[k]; };
+        double vn[NS] = {0.0, 0.0, 0.0, 0.0};
+        int64_t rn = 0;  // K offset of position k + 64
+        if (real && lane < a.ks)
+        {
+            const int64_t r0 = koff(lane);
+#pragma unroll
+            for (int s = 0; s < NS; s++) vn[s] = X[a.shift[s] + r0];
+        }
+        if (real && lane + 64 < a.ks) rn = koff(lane + 64);
         for (int64_t k = lane; k < a.kp; k += 64)
         {
             const bool in = real && k < a.ks;
+            double v[NS];
+#pragma unroll
+            for (int s = 0; s < NS; s++) v[s] = vn[s];
+            if (real && k + 64 < a.ks)
+#pragma unroll
+                for (int s = 0; s < NS; s++) vn[s] = X[a.shift[s] + rn];
+            if (real && k + 128 < a.ks) rn = koff(k + 128);
             double y[NQ];
             if (in)
-                sums(k, y);
+                sums(v, y);
             else
 #pragma unroll
                 for (int q = 0; q < NQ; q++) y[q] = 0.0;
@@ -281,8 +328,9 @@ __global__ __launch_bounds__(256) void sum_slice_kfast_kernel(SumSliceArgs a)
 
 // The 2-level form: one wave per padded row i forms all 49 sums of the row from its 16 source values per K
 // position (the one-pass Kronecker sums kernel's arithmetic), twice -- for the 49 exponents, then for the
-// digits -- so every source row is read by one wave only.
-template <int NSL, typename D>
+// digits -- so every source row is read by one wave only.  Both passes load position k + 64's values and
+// the K offset of k + 128 before working on position k (as the 1-level kernel's digit pass does).
+template <int NSL, typename D, bool AFF>
 __global__ __launch_bounds__(256) void sum_slice_kfast_kron_kernel(SumSliceArgs a)
 {
     __shared__ int sex[4][49];  // the row exponents, per wave
@@ -320,19 +368,39 @@ __global__ __launch_bounds__(256) void sum_slice_kfast_kron_kernel(SumSliceArgs 
                 y[p] = acc;
             }
         };
-        auto load = [&](int64_t k, double (&v)[16]) {
-            const int64_t rk = a.rs[k];
+        // look-ahead: vn holds the values of the lane's next position k, rn the K offset of k + 64
+        auto koff = [&](int64_t k) { return AFF ? a.rs0 + k * a.rs_step : a.rs[k]; };
+        double vn[16];
+        int64_t rn = 0;
+        auto start = [&]() {
 #pragma unroll
-            for (int s = 0; s < 16; s++) v[s] = X[a.shift[s] + rk];
+            for (int s = 0; s < 16; s++) vn[s] = 0.0;
+            if (real && lane < a.ks)
+            {
+                const int64_t r0 = koff(lane);
+#pragma unroll
+                for (int s = 0; s < 16; s++) vn[s] = X[a.shift[s] + r0];
+            }
+            if (real && lane + 64 < a.ks) rn = koff(lane + 64);
+        };
+        // v = position k's values; the loads of k + 64's values and k + 128's offset go out
+        auto advance = [&](int64_t k, double (&v)[16]) {
+#pragma unroll
+            for (int s = 0; s < 16; s++) v[s] = vn[s];
+            if (real && k + 64 < a.ks)
+#pragma unroll
+                for (int s = 0; s < 16; s++) vn[s] = X[a.shift[s] + rn];
+            if (real && k + 128 < a.ks) rn = koff(k + 128);
         };
         double mx[49];
 #pragma unroll
         for (int q = 0; q < 49; q++) mx[q] = 0.0;
+        start();
         if (real)
             for (int64_t k = lane; k < a.ks; k += 64)
             {
                 double v[16];
-                load(k, v);
+                advance(k, v);
 #pragma unroll
                 for (int o = 0; o < 7; o++)
                 {
@@ -352,15 +420,12 @@ __global__ __launch_bounds__(256) void sum_slice_kfast_kron_kernel(SumSliceArgs 
             if (real && lane == 0) a.e[q * a.rows + i] = ex;
         }
         __builtin_amdgcn_wave_barrier();
+        start();
         for (int64_t k = lane; k < a.kp; k += 64)
         {
             const bool in = real && k < a.ks;
             double v[16];
-            if (in)
-                load(k, v);
-            else
-#pragma unroll
-                for (int s = 0; s < 16; s++) v[s] = 0.0;
+            advance(k, v);
             // one outer product at a time (a loop, not unrolled: unrolled, the digits of all 49 sums stayed
             // live and spilled), and one sum of it at a time, its exponent from LDS
 #pragma unroll 1
@@ -1345,8 +1410,11 @@ template <int NSL, typename D>
 hipError_t sum_slices_t(const SumSliceArgs& a, hipStream_t s)
 {
     const dim3 grid(grid_of(a.cp, 4));
-    if (a.level == 1) hipLaunchKernelGGL((sum_slice_kfast_kernel<NSL, D>), grid, dim3(256), 0, s, a);
-    else if (a.level == 2) hipLaunchKernelGGL((sum_slice_kfast_kron_kernel<NSL, D>), grid, dim3(256), 0, s, a);
+    if (a.level == 1 && a.rs_step >= 0) hipLaunchKernelGGL((sum_slice_kfast_kernel<NSL, D, true>), grid, dim3(256), 0, s, a);
+    else if (a.level == 1) hipLaunchKernelGGL((sum_slice_kfast_kernel<NSL, D, false>), grid, dim3(256), 0, s, a);
+    else if (a.level == 2 && a.rs_step >= 0)
+        hipLaunchKernelGGL((sum_slice_kfast_kron_kernel<NSL, D, true>), grid, dim3(256), 0, s, a);
+    else if (a.level == 2) hipLaunchKernelGGL((sum_slice_kfast_kron_kernel<NSL, D, false>), grid, dim3(256), 0, s, a);
     else return hipErrorInvalidValue;
     return hipGetLastError();
 }
