@@ -3,8 +3,8 @@
 digits must be exactly those of the staged path.  That path runs the one-pass sums kernel, then the K-fast
 slice kernel, and mistra_debug_set_lowp_fusion(0) forces it.
 
-Checks, on shapes with padded rows and K, multi-index groups, a batch index, and one operand K-fast and
-the other not:
+Checks, on shapes with padded rows and K, multi-index groups, a batch index, one operand K-fast and the
+other not, and K offsets that are not affine (a K index after the M / N indices):
   * results equal to the staged path bit for bit (Ozaki BF16 with 2, 5 and 7 digits, Ozaki INT8 with 3
     and 6);
   * errors against FP64 numpy.einsum within the modes' bars: 1e-10 for the Ozaki modes at their default
@@ -25,8 +25,10 @@ if not torch.cuda.is_available():  # pragma: no cover
 from test_gpu_precision import run, say  # noqa: E402
 
 
-def kfast_case(seed, m, n, k, batch=(), b_kfast=True):
-    """A case whose A (and, with b_kfast, B) has a K index first, i.e. unit-stride K (column-major)."""
+def kfast_case(seed, m, n, k, batch=(), b_kfast=True, k_split=False):
+    """A case whose A (and, with b_kfast, B) has a K index first, i.e. unit-stride K (column-major).  With
+    k_split the other K indices follow the M (N) indices, so the K offsets are not affine in the flat K
+    position (the kernels then load them instead of computing them)."""
     base = cases.make_case(seed, m=m, n=n, k=k, batch=batch, permute=False)
     L = base["lengths"]
     labels = iter(sorted(L))
@@ -34,8 +36,9 @@ def kfast_case(seed, m, n, k, batch=(), b_kfast=True):
     nl = [next(labels) for _ in n]
     kl = [next(labels) for _ in k]
     bl = [next(labels) for _ in batch]
-    idx_A = "".join(kl + ml + bl)
-    idx_B = "".join((kl + nl) if b_kfast else (nl + kl)) + "".join(bl)
+    k1, k2 = (kl[:1], kl[1:]) if k_split else (kl, [])
+    idx_A = "".join(k1 + ml + k2 + bl)
+    idx_B = "".join((k1 + nl + k2) if b_kfast else (nl + kl)) + "".join(bl)
     idx_C = "".join(ml + nl + bl)
     return dict(idx_A=idx_A, idx_B=idx_B, idx_C=idx_C, lengths=L,
                 **{f"shape_{x}": [L[c] for c in ix] for x, ix in zip("ABC", (idx_A, idx_B, idx_C))})
@@ -46,6 +49,7 @@ SHAPES = [
     dict(m=(16, 12), n=(10, 18), k=(8, 26)),
     dict(m=(256,), n=(130,), k=(64,), batch=(2,)),
     dict(m=(64,), n=(100,), k=(96,), b_kfast=False),
+    dict(m=(40, 3), n=(72,), k=(12, 20), k_split=True),
 ]
 MODES = [("ozaki_bf16", 5, 0), ("ozaki_bf16", 2, 0), ("ozaki_bf16", 7, 0), ("ozaki_int8", 6, 0),
          ("ozaki_int8", 3, 0)]
@@ -66,7 +70,8 @@ def test_fused_sum_slices_equal_staged_path(si, mi, level):
     prec, nsl, _ = MODES[mi]
     shape = dict(SHAPES[si])
     b_kfast = shape.pop("b_kfast", True)
-    case = kfast_case(1500 + si, b_kfast=b_kfast, **shape)
+    k_split = shape.pop("k_split", False)
+    case = kfast_case(1500 + si, b_kfast=b_kfast, k_split=k_split, **shape)
     say(f"fused sums+digits L{level}: {case['idx_C']} = {case['idx_A']} * {case['idx_B']}, {prec} with {nsl} digits")
     n = [int(np.prod(case[f"shape_{x}"])) for x in "ABC"]
     a, b, c = cases.uniform(n[0], 11, emax=8), cases.uniform(n[1], 12, emax=8), cases.uniform(n[2], 13)
