@@ -480,10 +480,42 @@ __device__ __forceinline__ void put_out(const LowpGemmArgs& g, int q, double* Mq
 // orientation, and M's), applying the row / column scale exponents.  `stage` (32 KiB) aliases the
 // K-loop's LDS buffers, which are free after the loop.
 constexpr int kStageBytes = 64 * 64 * 8;
+// The scale exponents and (direct epilogue) C offsets of the elements a thread writes out: rows
+// row0 + r and row0 + 64 + r, columns col0 + cg + 8 e.  Loaded before the K loop, whose MFMAs hide their
+// latency: at a short K (32 steps at k = 1024) the epilogue's chain of index, exponent and C accesses
+// after the loop was a large share of each tile's time, with one workgroup per CU to overlap nothing.
+struct TileOutIdx
+{
+    int er[2], ec[8];
+    int64_t cm[2], cn[8];
+};
+__device__ __forceinline__ TileOutIdx tile_out_idx(const LowpGemmArgs& g, int q, int64_t row0, int64_t col0,
+                                                   const int* __restrict__ ea, const int* __restrict__ eb)
+{
+    TileOutIdx x;
+    const int r = threadIdx.x & 63, cg = threadIdx.x >> 6;
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+    {
+        const int64_t row = row0 + 64 * h + r;
+        const bool in = row < g.ms;
+        x.er[h] = in ? ea[int64_t(q) * g.ms + row] : 0;
+        x.cm[h] = in && g.C ? g.c_m[row] : 0;
+    }
+#pragma unroll
+    for (int e = 0; e < 8; e++)
+    {
+        const int64_t col = col0 + cg + 8 * e;
+        const bool in = col < g.ns;
+        x.ec[e] = in ? eb[int64_t(q) * g.ns + col] : 0;
+        x.cn[e] = in && g.C ? g.c_n[col] : 0;
+    }
+    return x;
+}
+
 __device__ __forceinline__ void tile_out_128x64(const LowpGemmArgs& g, int q, const double (&acc)[2][2][4],
                                                 double* stage, int64_t row0, int64_t col0, int wm, int wn,
-                                                const int* __restrict__ ea, const int* __restrict__ eb,
-                                                double* __restrict__ M)
+                                                const TileOutIdx& ix, double* __restrict__ M)
 {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     double* Mq = M + int64_t(q) * g.ms * g.ns;
@@ -509,14 +541,20 @@ __device__ __forceinline__ void tile_out_128x64(const LowpGemmArgs& g, int q, co
         const int64_t row = row0 + 64 * h + r;
         if (row < g.ms)
         {
-            const int er = ea[int64_t(q) * g.ms + row];
 #pragma unroll
             for (int e = 0; e < 8; e++)
             {
                 const int c = cg + 8 * e;
                 const int64_t col = col0 + c;
-                if (col < g.ns)
-                    put_out(g, q, Mq, row, col, ldexp(stage[c * 64 + ((r + c) & 63)], er + eb[int64_t(q) * g.ns + col]));
+                if (col >= g.ns) continue;
+                const double v = ldexp(stage[c * 64 + ((r + c) & 63)], ix.er[h] + ix.ec[e]);
+                if (g.C)
+                {
+                    double* cp = g.C + ix.cm[h] + ix.cn[e];
+                    *cp = g.beta == 0.0 ? g.alpha * v : g.alpha * v + g.beta * *cp;
+                }
+                else
+                    Mq[row + col * g.ms] = v;
             }
         }
     }
@@ -655,6 +693,11 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(NSL <= 2 ? 
     const int tm = blockIdx.x % tiles_m, tn = blockIdx.x / tiles_m;
     const int q = g.q0 + int(blockIdx.y);  // product (the launch covers [q0, q0 + nlaunch))
     const int64_t row0 = int64_t(tm) * 128, col0 = int64_t(tn) * 64;
+    // the epilogue's exponents and C offsets, loaded here so that the K loop hides their latency (not with
+    // 2 slices, whose 128-register budget for four waves per SIMD they would overflow)
+    constexpr bool kEarlyIdx = NSL > 2;
+    TileOutIdx oix;
+    if constexpr (kEarlyIdx) oix = tile_out_idx(g, q, row0, col0, ea, eb);
     const int wm = (wave & 3) * 32, wn = (wave >> 2) * 32;
     const int64_t slab_a = int64_t(g.nprod) * g.mp * g.kp, slab_b = int64_t(g.nprod) * g.np * g.kp;
     // Loads: a scalar resource per slice at the tile's first row, a fixed lane offset and the K-step as
@@ -816,7 +859,8 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(NSL <= 2 ? 
     for (int d = 0; d < NSL; d++) flush(d);
 
     static_assert(sizeof(sa0) >= kStageBytes, "output staging fits the first A buffer");
-    tile_out_128x64(g, q, acc, reinterpret_cast<double*>(&sa0[0][0]), row0, col0, wm, wn, ea, eb, M);
+    if constexpr (!kEarlyIdx) oix = tile_out_idx(g, q, row0, col0, ea, eb);
+    tile_out_128x64(g, q, acc, reinterpret_cast<double*>(&sa0[0][0]), row0, col0, wm, wn, oix, M);
 }
 
 // ---- Ozaki INT8 GEMM ---------------------------------------------------------------------------------
@@ -843,6 +887,11 @@ __global__ __launch_bounds__(512) void ozaki_i8_gemm_kernel(const int8_t* __rest
     const int tm = blockIdx.x % tiles_m, tn = blockIdx.x / tiles_m;
     const int q = g.q0 + int(blockIdx.y);  // product (the launch covers [q0, q0 + nlaunch))
     const int64_t row0 = int64_t(tm) * 128, col0 = int64_t(tn) * 64;
+    // the epilogue's exponents and C offsets, loaded here so that the K loop hides their latency (not in
+    // the folding form, which has no registers left for them)
+    constexpr bool kEarlyIdx = !FOLD;
+    TileOutIdx oix;
+    if constexpr (kEarlyIdx) oix = tile_out_idx(g, q, row0, col0, ea, eb);
     const int wm = (wave & 3) * 32, wn = (wave >> 2) * 32;
     const int64_t slab_a = int64_t(g.nprod) * g.mp * g.kp, slab_b = int64_t(g.nprod) * g.np * g.kp;
     // loader: A tile = 128 rows x 64 bytes = 512 x 16 B per slice (one per thread); B = 64 x 64 = 256 x 16 B.
@@ -966,7 +1015,8 @@ __global__ __launch_bounds__(512) void ozaki_i8_gemm_kernel(const int8_t* __rest
     fold();
 
     static_assert(sizeof(sa) >= kStageBytes, "output staging fits the A buffers");
-    tile_out_128x64(g, q, acc, reinterpret_cast<double*>(&sa[0][0][0]), row0, col0, wm, wn, ea, eb, M);
+    if constexpr (!kEarlyIdx) oix = tile_out_idx(g, q, row0, col0, ea, eb);
+    tile_out_128x64(g, q, acc, reinterpret_cast<double*>(&sa[0][0][0]), row0, col0, wm, wn, oix, M);
 }
 
 // ---- FP8 E4M3 with per-128 K-block power-of-two scales (MX-style) ------------------------------------
