@@ -47,7 +47,9 @@ std::string shape_key(const tblis::tblis_tensor* A, const label_type* idx_A, con
 // taken at 1.3x.
 // The rectangular tiles (single-term launches only; rates net of the wave quantisation launch_s adds)
 // against 128 x 128: FP64 about 1.5% above it (64 x 128) or level (128 x 64) when the epilogue stores the
-// product (Naive's dense GEMM at 8192^3), with the multi-C epilogue (the classical launch) 2.2% below it
+// product (Naive's dense GEMM at 8192^3, as the benchmark times it; from one process to the next the three
+// tiles of that launch trade places by up to 4%, each of them the fastest in some run), with the multi-C
+// epilogue (the classical launch) 2.2% below it
 // (128 x 64) and, for 64 x 128, 0.98 + 45 / ks of it: 1.4% below at a leaf depth of 8192, 2.4% above at
 // 1024 (m = n = 16384, k = 1024), where its three resident workgroups per CU hide more of each tile's
 // start and end; 2% above it in the register-staged form (an operand that is not K-fast); FP32 (classical

@@ -254,9 +254,9 @@ def test_autotuner_analytic_model():
 
 def test_autotuner_model_picks_measured_fastest_at_c2():
     # with c2's real (column-major, seeded-permutation) layout the model's costs follow the measured
-    # ones (classical 15.4 ms, L1 ABC staged 14.6, L2 ABC staged 14.6 (64 x 64 tiles), L2 Naive 13.9
-    # with 64 x 128 tiles, 14.1 with 128 x 128 on MI355X) and it picks L2 Naive with 64 x 128 tiles;
-    # only the descriptors are read
+    # ones (classical 15.3 ms, L1 ABC staged 14.5, L2 ABC staged 14.6, L2 Naive 13.5-14.2 on MI355X) and
+    # it picks L2 Naive with 64 x 128 tiles, on average the fastest of that launch's tiles (they trade
+    # places by up to 4% from one process to the next); only the descriptors are read
     import bench
 
     case = bench.make_workload("c2")
