@@ -9,10 +9,12 @@
 //   * the register tile is accumulated with v_mfma_f64_16x16x4_f64 (or the f32 form for float);
 //   * EPI_MULTI_C scatters the product into every C sub-block it feeds, C_i += alpha*coeff_i*M (the
 //     multi-C micro-kernel of src/stra_nodes/stra_gemm_ukr.hpp:626-930), applying beta exactly once per
-//     C sub-block; EPI_STORE writes the product to a staging buffer (AB and Naive impls).
-// ABC issues the 7^L products as consecutive launches on one stream (the serialized products of
-// src/stra_internal/3t/stra_mult.cxx:445-592), so no two workgroups of a launch write the same C
-// element and no atomics are needed.
+//     C sub-block; EPI_STORE writes the product to a staging buffer (AB and Naive impls);
+//     EPI_ATOMIC_C adds alpha*coeff_i*M into every target with FP64 atomic adds (beta applied by a pass
+//     before the launch), so all products of a plan can share one launch.
+// ABC otherwise issues the products as consecutive stage launches on one stream (the serialized
+// products of src/stra_internal/3t/stra_mult.cxx:445-592), grouped so that no two workgroups of a
+// launch write the same C element and no atomics are needed.
 #pragma once
 
 #include <hip/hip_runtime.h>
