@@ -296,7 +296,9 @@ Status autotune(const tblis::tblis_tensor* A, const label_type* idx_A, const tbl
                 return Status::error(MISTRA_ERR_ARG, "measured tuning needs device tensors");
             }
         }
-        // time every candidate on a scratch copy of C (same strides), 1 warm-up + 3 timed calls each
+        // time every candidate on a scratch copy of C (same strides): one untimed call, then the minimum
+        // over 3 rounds of 3 back-to-back calls (per call); calls timed one at a time, each between its
+        // own events, carried the host's time to issue the call, several percent of a 1 ms call
         const int elem = C->type == tblis::TYPE_DOUBLE ? 8 : 4;
         int64_t lo = 0, hi = 0;
         for (unsigned i = 0; i < C->ndim; i++)
@@ -323,16 +325,20 @@ Status autotune(const tblis::tblis_tensor* A, const label_type* idx_A, const tbl
             double ms_min = 1e30;
             for (int r = 0; r < 4 && st.ok(); r++)
             {
-                tblis::tblis_tensor Ct = Cs;  // tensor_mult resets the scalar of C after each call
+                const int reps = r == 0 ? 1 : 3;
                 (void)hipEventRecord(t0, stream);
-                st = tensor_mult(A, idx_A, B, idx_B, &Ct, idx_C, c.level, c.impl, MISTRA_PREC_NATIVE, 0, stream,
-                                 false, c.tile);
+                for (int i = 0; i < reps && st.ok(); i++)
+                {
+                    tblis::tblis_tensor Ct = Cs;  // tensor_mult resets the scalar of C after each call
+                    st = tensor_mult(A, idx_A, B, idx_B, &Ct, idx_C, c.level, c.impl, MISTRA_PREC_NATIVE, 0, stream,
+                                     false, c.tile);
+                }
                 (void)hipEventRecord(t1, stream);
                 if (!st.ok()) break;
                 (void)hipEventSynchronize(t1);
                 float ms = 0;
                 (void)hipEventElapsedTime(&ms, t0, t1);
-                if (r > 0) ms_min = std::min(ms_min, double(ms));
+                if (r > 0) ms_min = std::min(ms_min, double(ms) / reps);
             }
             if (!st.ok()) break;
             if (ms_min < best_ms)
